@@ -1,0 +1,128 @@
+/* nlfh: MI355X (gfx950) small-strain periodic homogenization, C ABI.
+ *
+ * Drop-in boundary for nlfiber/linear.py (class LinearHomogenization, nlfiber/linear.py:99-264).
+ * The reference has no FFI of its own: its interface is that Python class. Each entry point below
+ * replaces one of its methods (cited per function); the Python class of this package keeps the
+ * reference's signature on top of these calls, and INTEGRATION.md shows the ctypes binding a
+ * maintainer would add.
+ *
+ * Conventions
+ *  - All array arguments are DEVICE pointers (HIP), stream-ordered on `stream` (a hipStream_t;
+ *    NULL = the null stream) unless marked "host".
+ *  - Voxel arrays (rho, s): (nx, ny, nz) C order, per cell, cells concatenated.
+ *  - Nodal fields: component-planar (m, 3, nx, ny, nz) per cell, cells concatenated (cell-major).
+ *    The reference layout (m, nx, ny, nz, 3) (nlfiber/linear.py:80-92) is this memory viewed
+ *    through the axis permutation (0, 2, 3, 4, 1).
+ *  - Voigt order xx yy zz yz xz xy, engineering shear; element e = voxel (i,j,k) spans nodes
+ *    (i..i+1, j..j+1, k..k+1) mod n (nlfiber/linear.py:21-24).
+ *  - Every function returns 0 on success and a nonzero status otherwise; nlfh_last_error()
+ *    gives the message (thread-local). No exception crosses the ABI.
+ *  - dtype: 0 = float64, 1 = float32.
+ */
+#ifndef NLFH_H
+#define NLFH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nlfh_plan nlfh_plan;
+
+#define NLFH_DTYPE_F64 0
+#define NLFH_DTYPE_F32 1
+
+#define NLFH_PREC_FP64 0  /* plain FP64 PCG (the reference's algorithm) */
+#define NLFH_PREC_MIXED 1 /* FP32 inner PCG inside FP64 iterative refinement */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nlfh_last_error(void);
+
+/* ABI version (increases on incompatible change). */
+int nlfh_version(void);
+
+/* Plan over `ncells` periodic cells; shapes = ncells x 3 ints (host) (nx, ny, nz).
+ * Replaces LinearHomogenization.__init__ (nlfiber/linear.py:102-114): builds the element
+ * operator for (E, nu), the preconditioner symbol per distinct shape (nlfiber/linear.py:154-165)
+ * and the launch tables. `m_layout` is the number of right-hand sides per cell in batched field
+ * arrays (ignored for ncells == 1, where any m <= 6 works). The plan owns its workspaces. */
+int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_layout, nlfh_plan** out);
+int nlfh_plan_destroy(nlfh_plan* plan);
+
+/* Tiling of the matvec for distinct shape `shape_index` (in order of first appearance):
+ * ty node rows per tile, xc x-planes per chunk, zpt z values per lane (1, 2, 3 or 4).
+ * Used by the autotuner; defaults come from a heuristic. */
+int nlfh_plan_set_tiling(nlfh_plan* plan, int shape_index, int ty, int xc, int zpt);
+int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* xc, int* zpt);
+
+/* Relative tolerance of each FP32 inner solve of NLFH_PREC_MIXED (default 1e-4).
+ * No counterpart in the reference (which has no mixed precision). */
+int nlfh_plan_set_inner_tol(nlfh_plan* plan, double inner_tol);
+
+/* s = emin + (1 - emin) clip01(rho)^p over all cells' voxels (s nullable), and optionally
+ * dsdrho = p (1 - emin) clip01(rho)^(p-1) inside [0, 1] (bounds included) and 0 outside, the
+ * derivative of the reference's clip01 (nlfiber/linear.py:27-30).
+ * Replaces LinearHomogenization.stiffness_scale (nlfiber/linear.py:137-141). */
+int nlfh_stiffness_scale(nlfh_plan* plan, const double* rho, double emin, double p, double* s, double* dsdrho,
+                         void* stream);
+
+/* Ku = K(s) u for m right-hand sides; u, Ku of `dtype`; s is float64.
+ * Replaces LinearHomogenization.apply_K (nlfiber/linear.py:143-144), vmapped over m as in
+ * nlfiber/linear.py:175. */
+int nlfh_apply_K(nlfh_plan* plan, const double* s, const void* u, void* Ku, int m, int dtype, void* stream);
+
+/* f = assembled a(E_I, .) for the macroscopic strains Es (host, m x 6).
+ * Replaces LinearHomogenization.load (nlfiber/linear.py:146-149). */
+int nlfh_load(nlfh_plan* plan, const double* s, const double* Es_host, int m, double* f, void* stream);
+
+/* r = K(s) u + load(s, Es), not mean-projected (the reference's residual projects;
+ * nlfiber/linear.py:151-152; K u and the load are mean-free analytically). */
+int nlfh_residual(nlfh_plan* plan, const double* s, const double* Es_host, const double* u, int m, double* r,
+                  void* stream);
+
+/* z = irfftn(Kinv . rfftn(r)) per right-hand side and component (zero mean).
+ * Replaces LinearHomogenization.precond (nlfiber/linear.py:167-170). */
+int nlfh_precond(nlfh_plan* plan, const void* r, void* z, int m, int dtype, void* stream);
+
+/* Correctors: K u_I = -load(s, E_I) on the mean-free space by PCG.
+ * Replaces LinearHomogenization.solve / _solve_impl / _cg (nlfiber/linear.py:173-217).
+ * u0: nullable initial guess (mean removed as in nlfiber/linear.py:178). precision:
+ * NLFH_PREC_FP64 runs the reference's PCG and stopping rule (every right-hand side iterates until
+ * all have ||r|| <= tol ||b||, recursive residual; `it` counts iterations as the reference does);
+ * NLFH_PREC_MIXED runs FP32 PCG steps inside FP64 iterative refinement and stops on the true FP64
+ * residual ||b - K u|| <= tol ||b|| (it = total inner iterations). Outputs (host): it_out[ncells],
+ * converged_out[ncells], relres_out[ncells * m] (||r|| / ||b||, true residual in mixed mode,
+ * recursive in FP64 mode). Synchronises with the host while it polls for convergence. */
+int nlfh_solve(nlfh_plan* plan, const double* s, const double* Es_host, int m, const double* u0, double* u_out,
+               double tol, int maxit, int precision, int* it_out, int* converged_out, double* relres_out,
+               void* stream);
+
+/* Same as nlfh_solve with an explicit right-hand side b (m fields) and a zero start:
+ * replaces LinearHomogenization._solve_rhs (nlfiber/linear.py:208-210), used by the adjoint of
+ * the correctors (nlfiber/linear.py:127-131). */
+int nlfh_solve_rhs(nlfh_plan* plan, const double* s, const double* b, int m, double* x_out, double tol, int maxit,
+                   int precision, int* it_out, int* converged_out, double* relres_out, void* stream);
+
+/* C_IJ = mean_e s_e b_e(E_I + u_I, E_J + u_J): C_out (device, ncells x m x m).
+ * dCds_out (nullable, device): per cell (m, m, nx, ny, nz) = b_e,IJ / N = dC_IJ / ds_e.
+ * Replaces LinearHomogenization.bilinear (nlfiber/linear.py:220-229) and gives the gradient that
+ * the reference obtains by autodiff through it (nlfiber/linear.py:10-13, 259-264). */
+int nlfh_bilinear(nlfh_plan* plan, const double* s, const double* Es_host, const double* u, int m, double* C_out,
+                  double* dCds_out, void* stream);
+
+/* v -= per-(cell, rhs, component) spatial mean, in place (m fields).
+ * Replaces _proj (nlfiber/linear.py:95-96). */
+int nlfh_project_mean(nlfh_plan* plan, double* v, int m, void* stream);
+
+/* Host copy of the real symmetric preconditioner symbol of distinct shape `shape_index`:
+ * (nx, ny, nz/2+1, 6) as (k00, k11, k22, k12, k02, k01), not divided by N
+ * (the real part of LinearHomogenization.Kinv, nlfiber/linear.py:164-165). kinv_out is host. */
+int nlfh_symbol(const nlfh_plan* plan, int shape_index, double* kinv_out);
+
+/* Number of bytes of device memory the plan holds. */
+long long nlfh_plan_bytes(const nlfh_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NLFH_H */

@@ -1,0 +1,1197 @@
+// C-ABI of the library (include/nlfh.h): plans, launch tables, the preconditioner symbol, and the
+// PCG / iterative-refinement drivers. Host code; every device step is a kernel of this library.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "nlfh.h"
+#include "nlfh_kernels.h"
+
+namespace nlfh {
+thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+}  // namespace nlfh
+
+using namespace nlfh;
+
+#define CHECK_ARG(cond, msg)        \
+  do {                              \
+    if (!(cond)) {                  \
+      nlfh::set_error(msg);         \
+      return NLFH_EINVAL;           \
+    }                               \
+  } while (0)
+
+#define TRY(expr)                 \
+  do {                            \
+    int _rc = (expr);             \
+    if (_rc != NLFH_OK) return _rc; \
+  } while (0)
+
+namespace {
+
+constexpr int kChunk = 4096;     // voxels per reduction chunk
+constexpr int kMaxRhs = 6;
+constexpr size_t kLdsMax = 160 * 1024 - 1024;
+
+struct Shape {
+  int nx, ny, nz;
+  int ty, xc, zpt;
+  int64_t off_sym;  // in entries of 6 reals
+  int len[3];       // FftLen index per axis
+};
+
+struct MvGroup {
+  int shape;
+  int begin, count;
+  int max_ty;
+};
+
+template <typename T>
+struct Work {
+  T* X = nullptr;
+  T* R = nullptr;
+  T* B = nullptr;
+  T* P[2] = {nullptr, nullptr};
+  T* Ap = nullptr;
+  T* Z = nullptr;
+  T* s = nullptr;
+  cplx<T>* spec = nullptr;
+  double* st = nullptr;  // rz, alpha, beta, bn, rn (5 * ncells * M)
+  int* ist = nullptr;    // it, done (2 * ncells)
+  CgState cs{};
+};
+
+struct GraphKey {
+  int dtype, m, iters;
+  bool operator<(const GraphKey& o) const {
+    if (dtype != o.dtype) return dtype < o.dtype;
+    if (m != o.m) return m < o.m;
+    return iters < o.iters;
+  }
+};
+
+// digit-reversal map of the DIT transform with radices rad[0..nst-1]:
+// position p holds x[idx(p)]
+std::vector<int> digit_reversal(int n, const std::vector<int>& rad) {
+  std::vector<int> idx(n);
+  for (int p = 0; p < n; ++p) {
+    int pp = p, len = n, mult = 1, v = 0;
+    for (int s = (int)rad.size() - 1; s >= 0; --s) {
+      len /= rad[s];
+      const int q = pp / len;
+      pp -= q * len;
+      v += q * mult;
+      mult *= rad[s];
+    }
+    idx[p] = v;
+  }
+  return idx;
+}
+
+bool factor(int n, std::vector<int>& rad) {
+  rad.clear();
+  int v = n;
+  while (v % 4 == 0) { rad.push_back(4); v /= 4; }
+  while (v % 2 == 0) { rad.push_back(2); v /= 2; }
+  while (v % 3 == 0) { rad.push_back(3); v /= 3; }
+  while (v % 5 == 0) { rad.push_back(5); v /= 5; }
+  return v == 1 && (int)rad.size() <= NLFH_FFT_MAX_STAGES;
+}
+
+}  // namespace
+
+struct nlfh_plan {
+  int device = 0;
+  int ncells = 0, m_layout = 6, M = 6;
+  double E = 1.0, nu = 0.3, lam = 0, mu = 0;
+  std::vector<CellDesc> cells;
+  std::vector<int> cell_shape;
+  std::vector<Shape> shapes;
+  int64_t tot_vox = 0, tot_spec = 0, tot_sym = 0;
+
+  std::vector<FftLen> lens;
+  std::vector<MvItem> mv;
+  std::vector<MvGroup> groups;
+  std::vector<PlaneItem> planes;
+  std::vector<LineItem> lines64, lines32;
+  std::vector<ChunkItem> chunks;
+  int rpb64 = 1, rpb32 = 1;
+  size_t lds_plane64 = 0, lds_plane32 = 0, lds_line64 = 0, lds_line32 = 0;
+
+  // device tables
+  CellDesc* d_cells = nullptr;
+  FftLen* d_lens = nullptr;
+  cplx<double>* d_tw64 = nullptr;
+  cplx<float>* d_tw32 = nullptr;
+  int* d_perm = nullptr;
+  int* d_iperm = nullptr;
+  int* d_shape_len = nullptr;
+  double* d_sym64 = nullptr;
+  float* d_sym32 = nullptr;
+  std::vector<double> h_sym;  // unscaled real symbol (6 per frequency), for nlfh_symbol
+  MvItem* d_mv = nullptr;
+  PlaneItem* d_planes = nullptr;
+  LineItem* d_lines64 = nullptr;
+  LineItem* d_lines32 = nullptr;
+  ChunkItem* d_chunks = nullptr;
+  int* d_mv_range = nullptr;
+  int* d_plane_range = nullptr;
+  int* d_chunk_range = nullptr;
+
+  // workspaces
+  Work<double> w64;
+  Work<float> w32;
+  bool w64_ready = false, w32_ready = false;
+  double* d_part_mv = nullptr;
+  double* d_part_a = nullptr;
+  double* d_part_c = nullptr;
+  double* d_part_b = nullptr;
+  double* d_part_e = nullptr;
+  double* d_strain = nullptr;
+  double* d_params = nullptr;  // tol64, tol32
+  int* d_iparams = nullptr;    // maxit64, maxit32
+  double* d_ir = nullptr;      // rn_true (ncells*M), bn (ncells*M), mean (ncells*M*3)
+  int* d_outer_done = nullptr;
+  int* d_maxed = nullptr;      // per cell: iteration budget exhausted
+  double inner_tol = 1e-4;     // relative tolerance of each FP32 inner solve
+  int* h_pinned = nullptr;     // ncells * 2 ints
+  double* h_pinned_d = nullptr;
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::map<GraphKey, hipGraph_t> graph_src;
+
+  template <typename T>
+  int alloc(T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void**)p, n * sizeof(T));
+    if (e != hipSuccess) {
+      set_error(std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+      return NLFH_ENOMEM;
+    }
+    allocs.push_back(*p);
+    bytes += n * sizeof(T);
+    return NLFH_OK;
+  }
+  template <typename T>
+  int upload(T** p, const std::vector<T>& v) {
+    TRY(alloc(p, v.size()));
+    if (!v.empty()) HIP_TRY(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return NLFH_OK;
+  }
+  int64_t field_len() const { return (int64_t)M * 3 * tot_vox; }
+  // values of a caller's field array with m right-hand sides
+  int64_t field_len_m(int m) const { return (int64_t)(ncells == 1 ? m : M) * 3 * tot_vox; }
+  int64_t spec_len() const { return (int64_t)M * 3 * tot_spec; }
+};
+
+namespace {
+
+// Real symmetric symbol of K(1) from the 27-point stencil, inverted with the reference's
+// regularisation (nlfiber/linear.py:154-165): Kh(k) = sum_o S_o exp(-2 pi i k.o / n); Kh(0) = I;
+// Kinv = inv(Kh + 1e-12 |tr Kh| I); Kinv(0) = 0. Stored as (k00, k11, k22, k12, k02, k01).
+void build_symbol(const Shape& sh, double E, double nu, std::vector<double>& out) {
+  // element matrix Ke via the Walsh-mode operator: Ke = W^T L W (unit cube, 2x2x2 Gauss)
+  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+  // dense Ke from B matrices at the Gauss points (the reference's construction, nlfiber/linear.py:57-71,108-112)
+  double gp[2] = {0.5 - 1.0 / (2.0 * std::sqrt(3.0)), 0.5 + 1.0 / (2.0 * std::sqrt(3.0))};
+  double Ke[24][24] = {};
+  double D[6][6] = {};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) D[i][j] = lam;
+  for (int i = 0; i < 3; ++i) D[i][i] = lam + 2 * mu;
+  for (int i = 3; i < 6; ++i) D[i][i] = mu;
+  for (int g = 0; g < 8; ++g) {
+    const double x = gp[g & 1], y = gp[(g >> 1) & 1], z = gp[(g >> 2) & 1];
+    double B[6][24] = {};
+    for (int a = 0; a < 8; ++a) {
+      const int ia = a & 1, ja = (a >> 1) & 1, ka = (a >> 2) & 1;
+      const double fx = ia ? x : 1 - x, fy = ja ? y : 1 - y, fz = ka ? z : 1 - z;
+      const double sx = ia ? 1 : -1, sy = ja ? 1 : -1, sz = ka ? 1 : -1;
+      const double dx = sx * fy * fz, dy = fx * sy * fz, dz = fx * fy * sz;
+      const int c = 3 * a;
+      B[0][c] = dx; B[1][c + 1] = dy; B[2][c + 2] = dz;
+      B[3][c + 1] = dz; B[3][c + 2] = dy;
+      B[4][c] = dz; B[4][c + 2] = dx;
+      B[5][c] = dy; B[5][c + 1] = dx;
+    }
+    double DB[6][24];
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double t = 0;
+        for (int k = 0; k < 6; ++k) t += D[i][k] * B[k][j];
+        DB[i][j] = t;
+      }
+    for (int i = 0; i < 24; ++i)
+      for (int j = 0; j < 24; ++j) {
+        double t = 0;
+        for (int k = 0; k < 6; ++k) t += B[k][i] * DB[k][j];
+        Ke[i][j] += t / 8.0;
+      }
+  }
+  // stencil: S[o][a][c] = response at node offset o (in {-1,0,1}^3) of component a to a unit
+  // displacement of component c at the origin = sum over elements containing both nodes
+  double S[27][3][3] = {};
+  for (int e = 0; e < 8; ++e) {  // element with the origin at local node e
+    const int ex = -(e & 1), ey = -((e >> 1) & 1), ez = -((e >> 2) & 1);  // element origin
+    for (int b = 0; b < 8; ++b) {  // responding node
+      const int ox = ex + (b & 1), oy = ey + ((b >> 1) & 1), oz = ez + ((b >> 2) & 1);
+      const int oi = ((ox + 1) * 3 + (oy + 1)) * 3 + (oz + 1);
+      for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) S[oi][a][c] += Ke[3 * b + a][3 * e + c];
+    }
+  }
+  const int nx = sh.nx, ny = sh.ny, nz = sh.nz, nzh = nz / 2 + 1;
+  out.assign((size_t)nx * ny * nzh * 6, 0.0);
+  const long double two_pi = 6.283185307179586476925286766559L;
+  for (int kx = 0; kx < nx; ++kx)
+    for (int ky = 0; ky < ny; ++ky)
+      for (int kz = 0; kz < nzh; ++kz) {
+        long double K[3][3] = {};
+        for (int ox = -1; ox <= 1; ++ox)
+          for (int oy = -1; oy <= 1; ++oy)
+            for (int oz = -1; oz <= 1; ++oz) {
+              const int oi = ((ox + 1) * 3 + (oy + 1)) * 3 + (oz + 1);
+              // rfftn of the column response r(x) = S_x: Kh(k) = sum_x r(x) e^{-i k.x}; the
+              // stencil is even, so the imaginary part cancels and Kh = sum S_o cos(k.o)
+              const long double ph = two_pi * ((long double)kx * ox / nx + (long double)ky * oy / ny +
+                                               (long double)kz * oz / nz);
+              const long double cs = cosl(ph);
+              for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) K[a][c] += S[oi][a][c] * cs;
+            }
+        if (kx == 0 && ky == 0 && kz == 0) continue;  // Kinv(0) = 0
+        const long double tr = fabsl(K[0][0] + K[1][1] + K[2][2]);
+        for (int a = 0; a < 3; ++a) K[a][a] += 1e-12L * tr;
+        // symmetric 3x3 inverse by cofactors
+        const long double c00 = K[1][1] * K[2][2] - K[1][2] * K[2][1];
+        const long double c01 = K[1][2] * K[2][0] - K[1][0] * K[2][2];
+        const long double c02 = K[1][0] * K[2][1] - K[1][1] * K[2][0];
+        const long double det = K[0][0] * c00 + K[0][1] * c01 + K[0][2] * c02;
+        const long double i00 = c00 / det;
+        const long double i11 = (K[0][0] * K[2][2] - K[0][2] * K[2][0]) / det;
+        const long double i22 = (K[0][0] * K[1][1] - K[0][1] * K[1][0]) / det;
+        const long double i12 = (K[0][2] * K[1][0] - K[0][0] * K[1][2]) / det;
+        const long double i02 = (K[0][1] * K[1][2] - K[0][2] * K[1][1]) / det;
+        const long double i01 = (K[0][2] * K[2][1] - K[0][1] * K[2][2]) / det;
+        double* o = &out[(((size_t)kx * ny + ky) * nzh + kz) * 6];
+        o[0] = (double)i00; o[1] = (double)i11; o[2] = (double)i22;
+        o[3] = (double)i12; o[4] = (double)i02; o[5] = (double)i01;
+      }
+}
+
+// default tiling: one z value per lane up to 64, rows so that a block has <= 512 threads
+void default_tiling(Shape& sh) {
+  int zpt = 1;
+  while (sh.nz / zpt > 64 || sh.nz % zpt) {
+    ++zpt;
+    if (zpt > 4) break;
+  }
+  sh.zpt = zpt;
+  int lz = sh.nz / zpt, lzs = 1;
+  while (lzs < lz) lzs <<= 1;
+  int rows = std::max(3, 512 / lzs);
+  sh.ty = std::max(1, std::min(rows - 2, sh.ny));
+  sh.xc = std::max(1, std::min(8, sh.nx));
+}
+
+int build_mv_items(nlfh_plan* p) {
+  p->mv.clear();
+  p->groups.clear();
+  std::vector<std::vector<int>> cell_items(p->ncells);
+  for (int s = 0; s < (int)p->shapes.size(); ++s) {
+    const Shape& sh = p->shapes[s];
+    MvGroup g{s, (int)p->mv.size(), 0, 0};
+    for (int c = 0; c < p->ncells; ++c) {
+      if (p->cell_shape[c] != s) continue;
+      for (int y0 = 0; y0 < sh.ny; y0 += sh.ty)
+        for (int x0 = 0; x0 < sh.nx; x0 += sh.xc) {
+          MvItem it{c, y0, x0, std::min(sh.xc, sh.nx - x0), std::min(sh.ty, sh.ny - y0), 0};
+          g.max_ty = std::max(g.max_ty, it.ty);
+          cell_items[c].push_back((int)p->mv.size());
+          p->mv.push_back(it);
+        }
+    }
+    g.count = (int)p->mv.size() - g.begin;
+    if (g.count) p->groups.push_back(g);
+  }
+  // items of one cell are contiguous because cells of a shape are grouped; record ranges
+  std::vector<int> range(2 * p->ncells);
+  for (int c = 0; c < p->ncells; ++c) {
+    range[2 * c] = cell_items[c].front();
+    range[2 * c + 1] = cell_items[c].back() + 1;
+    if (range[2 * c + 1] - range[2 * c] != (int)cell_items[c].size()) {
+      set_error("internal: matvec items of a cell are not contiguous");
+      return NLFH_EINVAL;
+    }
+  }
+  if (p->d_mv) { (void)hipFree(p->d_mv); p->allocs.erase(std::find(p->allocs.begin(), p->allocs.end(), (void*)p->d_mv)); }
+  if (p->d_mv_range) { (void)hipFree(p->d_mv_range); p->allocs.erase(std::find(p->allocs.begin(), p->allocs.end(), (void*)p->d_mv_range)); }
+  TRY(p->upload(&p->d_mv, p->mv));
+  TRY(p->upload(&p->d_mv_range, range));
+  return NLFH_OK;
+}
+
+template <typename T>
+size_t line_len_for(int nx, int64_t npos) {
+  const size_t budget = 48 * 1024;
+  size_t len = budget / (3 * (size_t)nx * sizeof(cplx<T>));
+  if (len < 1) len = 1;
+  if ((int64_t)len > npos) len = (size_t)npos;
+  return len;
+}
+
+template <typename T>
+void build_lines(nlfh_plan* p, std::vector<LineItem>& lines, size_t& lds_line) {
+  lines.clear();
+  lds_line = 0;
+  for (int c = 0; c < p->ncells; ++c) {
+    const CellDesc& cd = p->cells[c];
+    const int64_t npos = (int64_t)cd.ny * cd.nzh;
+    const int len = (int)line_len_for<T>(cd.nx, npos);
+    for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
+    lds_line = std::max(lds_line, (size_t)3 * cd.nx * len * sizeof(cplx<T>));
+  }
+}
+
+template <typename T>
+void plane_lds(nlfh_plan* p, int& rpb, size_t& lds) {
+  rpb = 32;
+  for (int c = 0; c < p->ncells; ++c) {
+    const CellDesc& cd = p->cells[c];
+    const size_t plane = (size_t)cd.ny * cd.nzh * sizeof(cplx<T>);
+    const size_t row = (size_t)cd.nz * sizeof(cplx<T>);
+    int fit = plane < kLdsMax ? (int)((kLdsMax - plane) / row) : 0;
+    rpb = std::min(rpb, std::max(1, fit));
+  }
+  lds = 0;
+  for (int c = 0; c < p->ncells; ++c) {
+    const CellDesc& cd = p->cells[c];
+    const int npairs = (cd.ny + 1) / 2;
+    lds = std::max(lds, prec_plane_lds<T>(cd.ny, cd.nz, std::min(rpb, npairs)));
+  }
+}
+
+template <typename T>
+int ensure_work(nlfh_plan* p, Work<T>& w) {
+  const int64_t F = p->field_len();
+  TRY(p->alloc(&w.X, F));
+  TRY(p->alloc(&w.R, F));
+  TRY(p->alloc(&w.B, F));
+  TRY(p->alloc(&w.P[0], F));
+  TRY(p->alloc(&w.P[1], F));
+  TRY(p->alloc(&w.Ap, F));
+  TRY(p->alloc(&w.Z, F));
+  TRY(p->alloc(&w.s, p->tot_vox));
+  TRY(p->alloc(&w.spec, p->spec_len()));
+  const int nk = p->ncells * p->M;
+  TRY(p->alloc(&w.st, 5 * nk));
+  TRY(p->alloc(&w.ist, 2 * p->ncells));
+  w.cs.rz = w.st;
+  w.cs.alpha = w.st + nk;
+  w.cs.beta = w.st + 2 * nk;
+  w.cs.bn = w.st + 3 * nk;
+  w.cs.rn = w.st + 4 * nk;
+  w.cs.it = w.ist;
+  w.cs.done = w.ist + p->ncells;
+  w.cs.skip = nullptr;
+  return NLFH_OK;
+}
+
+int get_ready64(nlfh_plan* p) {
+  if (!p->w64_ready) {
+    TRY(ensure_work(p, p->w64));
+    p->w64.cs.tol = p->d_params;
+    p->w64.cs.maxit = p->d_iparams;
+    p->w64_ready = true;
+  }
+  return NLFH_OK;
+}
+
+int get_ready32(nlfh_plan* p) {
+  if (!p->w32_ready) {
+    TRY(ensure_work(p, p->w32));
+    p->w32.cs.tol = p->d_params + 1;
+    p->w32.cs.maxit = p->d_iparams + 1;
+    p->w32_ready = true;
+  }
+  return NLFH_OK;
+}
+
+template <typename T>
+T* sym_ptr(nlfh_plan* p);
+template <>
+double* sym_ptr<double>(nlfh_plan* p) { return p->d_sym64; }
+template <>
+float* sym_ptr<float>(nlfh_plan* p) { return p->d_sym32; }
+template <typename T>
+const cplx<T>* tw_ptr(nlfh_plan* p);
+template <>
+const cplx<double>* tw_ptr<double>(nlfh_plan* p) { return p->d_tw64; }
+template <>
+const cplx<float>* tw_ptr<float>(nlfh_plan* p) { return p->d_tw32; }
+
+// K u (+ load) for every shape group. u may be null (load only).
+template <typename T>
+int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const T* s, const double* beta,
+               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
+  for (const MvGroup& g : p->groups) {
+    const Shape& sh = p->shapes[g.shape];
+    MatvecArgs<T> a{};
+    a.cells = p->d_cells;
+    a.items = p->d_mv + g.begin;
+    a.u = u;
+    a.p_old = p_old;
+    a.p_new = p_new;
+    a.out = out;
+    a.s = s;
+    a.strain = p->d_strain;
+    a.beta = beta;
+    a.dot_part = dot_part;
+    a.done = done;
+    a.m = m;
+    a.nitems = (int)p->mv.size();
+    a.item_base = g.begin;
+    a.LZ = sh.nz / sh.zpt;
+    int lzs = 1;
+    while (lzs < a.LZ) lzs <<= 1;
+    a.LZS = lzs;
+    a.lam16 = (T)(p->lam / 16.0);
+    a.mu16 = (T)(p->mu / 16.0);
+    a.scale_out = scale;
+    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, st));
+  }
+  return NLFH_OK;
+}
+
+template <typename T>
+PrecArgs<T> prec_args(nlfh_plan* p, int m) {
+  PrecArgs<T> a{};
+  a.cells = p->d_cells;
+  a.planes = p->d_planes;
+  a.lines = sizeof(T) == 8 ? p->d_lines64 : p->d_lines32;
+  a.ft.lens = p->d_lens;
+  a.ft.tw = tw_ptr<T>(p);
+  a.ft.perm = p->d_perm;
+  a.ft.iperm = p->d_iperm;
+  a.ft.shape_len = p->d_shape_len;
+  a.sym = sym_ptr<T>(p);
+  a.m = m;
+  a.nplanes = (int)p->planes.size();
+  a.nlines = (int)(sizeof(T) == 8 ? p->lines64.size() : p->lines32.size());
+  a.rows_per_batch = sizeof(T) == 8 ? p->rpb64 : p->rpb32;
+  return a;
+}
+
+template <typename T>
+int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) {
+  const size_t lp = sizeof(T) == 8 ? p->lds_plane64 : p->lds_plane32;
+  const size_t ll = sizeof(T) == 8 ? p->lds_line64 : p->lds_line32;
+  HIP_TRY(launch_prec<T>(a, update, dot, lp, ll, st));
+  return NLFH_OK;
+}
+
+VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
+  VecArgs va{};
+  va.cells = p->d_cells;
+  va.chunks = p->d_chunks;
+  va.nchunks = (int)p->chunks.size();
+  va.m = m;
+  va.done = done;
+  return va;
+}
+
+// One PCG iteration (nlfiber/linear.py:190-200) with the buffers of `w`; `cur` selects the old
+// search direction.
+template <typename T>
+int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
+  T* p_old = w.P[cur];
+  T* p_new = w.P[1 - cur];
+  TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
+  HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
+  PrecArgs<T> a = prec_args<T>(p, m);
+  a.r = w.R;
+  a.r_out = w.R;
+  a.Ap = w.Ap;
+  a.x = w.X;
+  a.p = p_new;
+  a.alpha = w.cs.alpha;
+  a.spec = w.spec;
+  a.z = w.Z;
+  a.part_a = p->d_part_a;
+  a.part_c = p->d_part_c;
+  a.done = w.cs.done;
+  TRY(run_prec<T>(p, a, 1, 1, st));
+  HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, p->d_part_c, (int)p->planes.size(), p->d_plane_range, st));
+  return NLFH_OK;
+}
+
+constexpr int kItersPerGraph = 16;  // even: the search-direction ping-pong returns to P[0]
+
+template <typename T>
+int get_graph(nlfh_plan* p, Work<T>& w, int m, hipGraphExec_t* out) {
+  const GraphKey key{sizeof(T) == 8 ? 0 : 1, m, kItersPerGraph};
+  auto itg = p->graphs.find(key);
+  if (itg != p->graphs.end()) {
+    *out = itg->second;
+    return NLFH_OK;
+  }
+  hipGraph_t g;
+  HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+  int rc = NLFH_OK;
+  for (int i = 0; i < kItersPerGraph && rc == NLFH_OK; ++i) rc = cg_iteration<T>(p, w, m, i & 1, p->stream);
+  hipError_t e = hipStreamEndCapture(p->stream, &g);
+  if (rc != NLFH_OK) return rc;
+  HIP_TRY(e);
+  hipGraphExec_t ge;
+  HIP_TRY(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  p->graphs[key] = ge;
+  p->graph_src[key] = g;
+  *out = ge;
+  return NLFH_OK;
+}
+
+// PCG on the mean-free space from the state prepared in w (R = b - K X, B holds b for ||b||):
+// nlfiber/linear.py:173-203. Returns after every cell is done; iterations per cell in it_host.
+template <typename T>
+int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use_graph, int maxit) {
+  hipStream_t st = p->stream;
+  w.cs.m = m;  // per-(cell, rhs) scalars use stride m
+  VecArgs va = vec_args(p, m, nullptr);
+  HIP_TRY(launch_reduce<T>(va, bvec, nullptr, p->d_part_b, 0, st));
+  PrecArgs<T> a = prec_args<T>(p, m);
+  a.r = w.R;
+  a.spec = w.spec;
+  a.z = w.Z;
+  a.part_a = p->d_part_a;
+  a.part_c = p->d_part_c;
+  a.done = w.cs.skip;
+  TRY(run_prec<T>(p, a, 0, 1, st));
+  HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, p->d_part_c, (int)p->planes.size(), p->d_plane_range,
+                         p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, st));
+  HIP_TRY(launch_fill<T>(w.P[0], p->field_len(), T(0), st));
+  int* h_done = p->h_pinned;
+  const int nc = p->ncells;
+  int cur = 0;
+  long launched = 0;
+  for (;;) {
+    HIP_TRY(hipMemcpyAsync(h_done, w.cs.done, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    bool all = true;
+    for (int c = 0; c < nc; ++c) all = all && h_done[c];
+    if (all) break;
+    if (use_graph) {
+      hipGraphExec_t ge;
+      TRY(get_graph<T>(p, w, m, &ge));
+      HIP_TRY(hipGraphLaunch(ge, st));
+      launched += kItersPerGraph;
+    } else {
+      for (int i = 0; i < kItersPerGraph; ++i) {
+        TRY(cg_iteration<T>(p, w, m, cur, st));
+        cur ^= 1;
+      }
+      launched += kItersPerGraph;
+    }
+    if (launched > (long)maxit + 2 * kItersPerGraph) {
+      set_error("internal: the solver did not flag completion within maxit iterations");
+      return NLFH_EHIP;
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(it_host, w.cs.it, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return NLFH_OK;
+}
+
+int enter(nlfh_plan* p, void* stream) {
+  HIP_TRY(hipSetDevice(p->device));
+  HIP_TRY(hipEventRecord(p->ev, (hipStream_t)stream));
+  HIP_TRY(hipStreamWaitEvent(p->stream, p->ev, 0));
+  return NLFH_OK;
+}
+
+int leave(nlfh_plan* p, void* stream) {
+  HIP_TRY(hipEventRecord(p->ev, p->stream));
+  HIP_TRY(hipStreamWaitEvent((hipStream_t)stream, p->ev, 0));
+  return NLFH_OK;
+}
+
+int set_strain(nlfh_plan* p, const double* Es_host, int m) {
+  HIP_TRY(hipMemcpyAsync(p->d_strain, Es_host, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, p->stream));
+  return NLFH_OK;
+}
+
+int check_m(nlfh_plan* p, int m) {
+  CHECK_ARG(m >= 1 && m <= kMaxRhs, "m must be in 1..6");
+  CHECK_ARG(p->ncells == 1 || m == p->m_layout, "batched plans take m == m_layout right-hand sides");
+  return NLFH_OK;
+}
+
+// mean projection of a FP64 field (nlfiber/linear.py:95-96): per (cell, rhs, component)
+int project_mean(nlfh_plan* p, double* v, int m, hipStream_t st) {
+  VecArgs va = vec_args(p, m, nullptr);
+  HIP_TRY(launch_reduce<double>(va, v, nullptr, p->d_part_b, 1, st));
+  double* mean = p->d_ir + 2 * p->ncells * p->M;  // [cell][rhs*3 + c], stride 3M
+  HIP_TRY(launch_sum_partials(p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, p->ncells, 3 * m, 1.0, p->d_cells,
+                              3 * p->M, mean, st));
+  va.m = p->M;  // the mean buffer uses the plan's rhs stride
+  va.nchunks = (int)p->chunks.size();
+  VecArgs vs = va;
+  HIP_TRY(launch_sub_mean<double>(vs, v, mean, st));
+  return NLFH_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+extern "C" {
+
+const char* nlfh_last_error(void) { return nlfh::g_err.c_str(); }
+
+int nlfh_version(void) { return 1; }
+
+int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_layout, nlfh_plan** out) {
+  CHECK_ARG(out != nullptr, "out is null");
+  *out = nullptr;
+  CHECK_ARG(shapes != nullptr && ncells >= 1 && ncells <= NLFH_MAX_CELLS, "ncells must be in 1..4096");
+  CHECK_ARG(m_layout >= 1 && m_layout <= kMaxRhs, "m_layout must be in 1..6");
+  CHECK_ARG(nu > -1.0 && nu < 0.5 && E > 0.0, "E must be positive and nu in (-1, 0.5)");
+  nlfh_plan* p = new nlfh_plan();
+  auto fail = [&](int rc) {
+    nlfh_plan_destroy(p);
+    return rc;
+  };
+  HIP_TRY(hipGetDevice(&p->device));
+  p->ncells = ncells;
+  p->m_layout = m_layout;
+  p->M = ncells == 1 ? kMaxRhs : m_layout;
+  p->E = E;
+  p->nu = nu;
+  p->lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+  p->mu = E / (2 * (1 + nu));
+  // distinct shapes in order of first appearance; cells grouped by shape in the tables
+  std::vector<int> lens_n;
+  for (int c = 0; c < ncells; ++c) {
+    const int nx = shapes[3 * c], ny = shapes[3 * c + 1], nz = shapes[3 * c + 2];
+    if (nx < 1 || ny < 1 || nz < 1 || nx > 512 || ny > 512 || nz > 256) {
+      set_error("cell dimensions must be in 1..512 (x, y) and 1..256 (z)");
+      return fail(NLFH_EUNSUPPORTED);
+    }
+    int sid = -1;
+    for (int s = 0; s < (int)p->shapes.size(); ++s)
+      if (p->shapes[s].nx == nx && p->shapes[s].ny == ny && p->shapes[s].nz == nz) sid = s;
+    if (sid < 0) {
+      Shape sh{nx, ny, nz, 0, 0, 0, 0, {0, 0, 0}};
+      default_tiling(sh);
+      if (sh.zpt > 4 || sh.nz % sh.zpt) {
+        set_error("nz must be a multiple of z values per lane with at most 64 lanes (nz <= 256)");
+        return fail(NLFH_EUNSUPPORTED);
+      }
+      const int dims[3] = {nx, ny, nz};
+      for (int a = 0; a < 3; ++a) {
+        std::vector<int> rad;
+        if (!factor(dims[a], rad)) {
+          set_error("every cell dimension must be a product of 2, 3 and 5 (FFT preconditioner)");
+          return fail(NLFH_EUNSUPPORTED);
+        }
+        int li = -1;
+        for (int l = 0; l < (int)lens_n.size(); ++l)
+          if (lens_n[l] == dims[a]) li = l;
+        if (li < 0) {
+          li = (int)lens_n.size();
+          lens_n.push_back(dims[a]);
+        }
+        sh.len[a] = li;
+      }
+      sid = (int)p->shapes.size();
+      p->shapes.push_back(sh);
+    }
+    p->cell_shape.push_back(sid);
+  }
+  // cell order in memory = order given (the caller's arrays); offsets
+  int64_t ov = 0, os = 0;
+  for (int c = 0; c < ncells; ++c) {
+    const Shape& sh = p->shapes[p->cell_shape[c]];
+    CellDesc cd{};
+    cd.nx = sh.nx; cd.ny = sh.ny; cd.nz = sh.nz; cd.nzh = sh.nz / 2 + 1;
+    cd.shape_id = p->cell_shape[c];
+    cd.nvox = (int64_t)sh.nx * sh.ny * sh.nz;
+    cd.nspec = (int64_t)sh.nx * sh.ny * cd.nzh;
+    cd.off_vox = ov;
+    cd.off_field = (int64_t)p->M * 3 * ov;
+    cd.off_spec = (int64_t)p->M * 3 * os;
+    ov += cd.nvox;
+    os += cd.nspec;
+    p->cells.push_back(cd);
+  }
+  p->tot_vox = ov;
+  p->tot_spec = os;
+  // symbols
+  std::vector<double> sym64;
+  for (Shape& sh : p->shapes) {
+    std::vector<double> s;
+    build_symbol(sh, E, nu, s);
+    sh.off_sym = (int64_t)sym64.size() / 6;
+    sym64.insert(sym64.end(), s.begin(), s.end());
+  }
+  p->h_sym = sym64;
+  for (int c = 0; c < ncells; ++c) p->cells[c].off_sym = p->shapes[p->cell_shape[c]].off_sym * 6;
+  // scale by 1 / N of its shape for the device copies
+  std::vector<double> symd(sym64.size());
+  std::vector<float> symf(sym64.size());
+  for (const Shape& sh : p->shapes) {
+    const double inv = 1.0 / ((double)sh.nx * sh.ny * sh.nz);
+    const size_t n = (size_t)sh.nx * sh.ny * (sh.nz / 2 + 1) * 6;
+    for (size_t i = 0; i < n; ++i) {
+      symd[sh.off_sym * 6 + i] = sym64[sh.off_sym * 6 + i] * inv;
+      symf[sh.off_sym * 6 + i] = (float)symd[sh.off_sym * 6 + i];
+    }
+  }
+  p->tot_sym = (int64_t)sym64.size();
+  // FFT tables
+  std::vector<cplx<double>> tw64;
+  std::vector<cplx<float>> tw32;
+  std::vector<int> perm, iperm;
+  for (int n : lens_n) {
+    FftLen fl{};
+    fl.n = n;
+    std::vector<int> rad;
+    factor(n, rad);
+    fl.nst = (int)rad.size();
+    for (int i = 0; i < fl.nst; ++i) fl.rad[i] = rad[i];
+    fl.tw_off = (int)tw64.size();
+    fl.perm_off = (int)perm.size();
+    fl.iperm_off = (int)iperm.size();
+    for (int t = 0; t < n; ++t) {
+      const long double ang = -6.283185307179586476925286766559L * t / n;
+      const double re = (double)cosl(ang), im = (double)sinl(ang);
+      tw64.push_back({re, im});
+      tw32.push_back({(float)re, (float)im});
+    }
+    std::vector<int> idx = digit_reversal(n, rad);
+    std::vector<int> inv(n);
+    for (int q = 0; q < n; ++q) inv[idx[q]] = q;
+    perm.insert(perm.end(), idx.begin(), idx.end());
+    iperm.insert(iperm.end(), inv.begin(), inv.end());
+    p->lens.push_back(fl);
+  }
+  std::vector<int> shape_len;
+  for (const Shape& sh : p->shapes) {
+    shape_len.push_back(sh.len[0]);
+    shape_len.push_back(sh.len[1]);
+    shape_len.push_back(sh.len[2]);
+  }
+  // plane, line, chunk items (cell-major)
+  std::vector<int> prange, crange;
+  for (int c = 0; c < ncells; ++c) {
+    const CellDesc& cd = p->cells[c];
+    prange.push_back((int)p->planes.size());
+    for (int comp = 0; comp < 3; ++comp)
+      for (int x = 0; x < cd.nx; ++x) p->planes.push_back({c, comp, x, 0});
+    prange.push_back((int)p->planes.size());
+    crange.push_back((int)p->chunks.size());
+    for (int64_t b = 0; b < cd.nvox; b += kChunk) p->chunks.push_back({c, 0, b, std::min<int64_t>(b + kChunk, cd.nvox)});
+    crange.push_back((int)p->chunks.size());
+  }
+  build_lines<double>(p, p->lines64, p->lds_line64);
+  build_lines<float>(p, p->lines32, p->lds_line32);
+  plane_lds<double>(p, p->rpb64, p->lds_plane64);
+  plane_lds<float>(p, p->rpb32, p->lds_plane32);
+  if (p->lds_plane64 > kLdsMax || p->lds_plane32 > kLdsMax) {
+    set_error("a y-z plane of the cell does not fit in LDS (ny * (nz/2+1) too large)");
+    return fail(NLFH_EUNSUPPORTED);
+  }
+  int rc = NLFH_OK;
+  if ((rc = p->upload(&p->d_cells, p->cells))) return fail(rc);
+  if ((rc = p->upload(&p->d_lens, p->lens))) return fail(rc);
+  if ((rc = p->upload(&p->d_tw64, tw64))) return fail(rc);
+  if ((rc = p->upload(&p->d_tw32, tw32))) return fail(rc);
+  if ((rc = p->upload(&p->d_perm, perm))) return fail(rc);
+  if ((rc = p->upload(&p->d_iperm, iperm))) return fail(rc);
+  if ((rc = p->upload(&p->d_shape_len, shape_len))) return fail(rc);
+  if ((rc = p->upload(&p->d_sym64, symd))) return fail(rc);
+  if ((rc = p->upload(&p->d_sym32, symf))) return fail(rc);
+  if ((rc = p->upload(&p->d_planes, p->planes))) return fail(rc);
+  if ((rc = p->upload(&p->d_lines64, p->lines64))) return fail(rc);
+  if ((rc = p->upload(&p->d_lines32, p->lines32))) return fail(rc);
+  if ((rc = p->upload(&p->d_chunks, p->chunks))) return fail(rc);
+  if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
+  if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
+  if ((rc = build_mv_items(p))) return fail(rc);
+  // partials: the matvec item count may change with the tiling, so size for the worst case
+  int64_t max_mv = 0;
+  for (int c = 0; c < ncells; ++c) max_mv += (int64_t)p->cells[c].nx * p->cells[c].ny;
+  const int M = p->M;
+  if ((rc = p->alloc(&p->d_part_mv, (size_t)M * max_mv))) return fail(rc);
+  if ((rc = p->alloc(&p->d_part_a, (size_t)M * p->planes.size()))) return fail(rc);
+  if ((rc = p->alloc(&p->d_part_c, (size_t)M * p->planes.size()))) return fail(rc);
+  if ((rc = p->alloc(&p->d_part_b, (size_t)3 * M * p->chunks.size()))) return fail(rc);
+  if ((rc = p->alloc(&p->d_part_e, (size_t)36 * p->chunks.size()))) return fail(rc);
+  if ((rc = p->alloc(&p->d_strain, (size_t)6 * kMaxRhs))) return fail(rc);
+  if ((rc = p->alloc(&p->d_params, 2))) return fail(rc);
+  if ((rc = p->alloc(&p->d_iparams, 2))) return fail(rc);
+  if ((rc = p->alloc(&p->d_ir, (size_t)ncells * M * 5))) return fail(rc);
+  if ((rc = p->alloc(&p->d_outer_done, ncells))) return fail(rc);
+  if ((rc = p->alloc(&p->d_maxed, ncells))) return fail(rc);
+  if (hipHostMalloc((void**)&p->h_pinned, sizeof(int) * 2 * ncells, hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(NLFH_ENOMEM);
+  }
+  if (hipHostMalloc((void**)&p->h_pinned_d, sizeof(double) * ncells * M * 2, hipHostMallocDefault) != hipSuccess) {
+    set_error("hipHostMalloc failed");
+    return fail(NLFH_ENOMEM);
+  }
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev, hipEventDisableTiming) != hipSuccess) {
+    set_error("stream/event creation failed");
+    return fail(NLFH_EHIP);
+  }
+  *out = p;
+  return NLFH_OK;
+}
+
+int nlfh_plan_destroy(nlfh_plan* p) {
+  if (!p) return NLFH_OK;
+  (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : p->graph_src) (void)hipGraphDestroy(kv.second);
+  for (void* a : p->allocs) (void)hipFree(a);
+  if (p->h_pinned) (void)hipHostFree(p->h_pinned);
+  if (p->h_pinned_d) (void)hipHostFree(p->h_pinned_d);
+  if (p->ev) (void)hipEventDestroy(p->ev);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  delete p;
+  return NLFH_OK;
+}
+
+int nlfh_plan_set_tiling(nlfh_plan* p, int shape_index, int ty, int xc, int zpt) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
+  Shape& sh = p->shapes[shape_index];
+  CHECK_ARG(zpt >= 1 && zpt <= 4 && sh.nz % zpt == 0 && sh.nz / zpt <= 64, "zpt must divide nz with nz/zpt <= 64");
+  int lz = sh.nz / zpt, lzs = 1;
+  while (lzs < lz) lzs <<= 1;
+  CHECK_ARG(ty >= 1 && lzs * (std::min(ty, sh.ny) + 2) <= 512, "ty too large for a 512-thread block");
+  CHECK_ARG(xc >= 1, "xc must be positive");
+  const size_t lds = (size_t)2 * (std::min(ty, sh.ny) + 2) * 6 * sh.nz * sizeof(double);
+  CHECK_ARG(lds <= kLdsMax, "tile exceeds LDS");
+  sh.ty = std::min(ty, sh.ny);
+  sh.xc = std::min(xc, sh.nx);
+  sh.zpt = zpt;
+  // cached graphs bake the launch tables: drop them
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : p->graph_src) (void)hipGraphDestroy(kv.second);
+  p->graphs.clear();
+  p->graph_src.clear();
+  return build_mv_items(p);
+}
+
+int nlfh_plan_get_tiling(const nlfh_plan* p, int shape_index, int* ty, int* xc, int* zpt) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
+  const Shape& sh = p->shapes[shape_index];
+  if (ty) *ty = sh.ty;
+  if (xc) *xc = sh.xc;
+  if (zpt) *zpt = sh.zpt;
+  return NLFH_OK;
+}
+
+int nlfh_stiffness_scale(nlfh_plan* p, const double* rho, double emin, double pw, double* s, double* dsdrho,
+                         void* stream) {
+  CHECK_ARG(p && rho && (s || dsdrho), "null argument");
+  TRY(enter(p, stream));
+  HIP_TRY(launch_stiffness_scale(rho, s, nullptr, dsdrho, p->tot_vox, emin, pw, p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_apply_K(nlfh_plan* p, const double* s, const void* u, void* Ku, int m, int dtype, void* stream) {
+  CHECK_ARG(p && s && u && Ku, "null argument");
+  TRY(check_m(p, m));
+  CHECK_ARG(dtype == NLFH_DTYPE_F64 || dtype == NLFH_DTYPE_F32, "dtype must be 0 (f64) or 1 (f32)");
+  TRY(enter(p, stream));
+  if (dtype == NLFH_DTYPE_F64) {
+    TRY(run_matvec<double>(p, (const double*)u, nullptr, nullptr, (double*)Ku, s, nullptr, nullptr, nullptr, m, 0, 0,
+                           0, 1.0, p->stream));
+  } else {
+    TRY(get_ready32(p));
+    HIP_TRY(launch_d2f(s, p->w32.s, p->tot_vox, p->stream));
+    TRY(run_matvec<float>(p, (const float*)u, nullptr, nullptr, (float*)Ku, p->w32.s, nullptr, nullptr, nullptr, m, 0,
+                          0, 0, 1.0f, p->stream));
+  }
+  return leave(p, stream);
+}
+
+int nlfh_load(nlfh_plan* p, const double* s, const double* Es, int m, double* f, void* stream) {
+  CHECK_ARG(p && s && Es && f, "null argument");
+  TRY(check_m(p, m));
+  TRY(enter(p, stream));
+  TRY(set_strain(p, Es, m));
+  TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, f, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_residual(nlfh_plan* p, const double* s, const double* Es, const double* u, int m, double* r, void* stream) {
+  CHECK_ARG(p && s && Es && u && r, "null argument");
+  TRY(check_m(p, m));
+  TRY(enter(p, stream));
+  TRY(set_strain(p, Es, m));
+  TRY(run_matvec<double>(p, u, nullptr, nullptr, r, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_precond(nlfh_plan* p, const void* r, void* z, int m, int dtype, void* stream) {
+  CHECK_ARG(p && r && z, "null argument");
+  TRY(check_m(p, m));
+  CHECK_ARG(dtype == NLFH_DTYPE_F64 || dtype == NLFH_DTYPE_F32, "dtype must be 0 (f64) or 1 (f32)");
+  TRY(enter(p, stream));
+  if (dtype == NLFH_DTYPE_F64) {
+    TRY(get_ready64(p));
+    PrecArgs<double> a = prec_args<double>(p, m);
+    a.r = (const double*)r;
+    a.spec = p->w64.spec;
+    a.z = (double*)z;
+    TRY(run_prec<double>(p, a, 0, 0, p->stream));
+  } else {
+    TRY(get_ready32(p));
+    PrecArgs<float> a = prec_args<float>(p, m);
+    a.r = (const float*)r;
+    a.spec = p->w32.spec;
+    a.z = (float*)z;
+    TRY(run_prec<float>(p, a, 0, 0, p->stream));
+  }
+  return leave(p, stream);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Final per-(cell, rhs) relative residual of the FP64 solver: rn / bn (recursive residual).
+int fetch_fp64_info(nlfh_plan* p, int m, double tol, int* it_out, int* conv_out, double* relres_out, const int* its) {
+  const int nk = p->ncells * m;
+  HIP_TRY(hipMemcpyAsync(p->h_pinned_d, p->w64.cs.rn, nk * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipMemcpyAsync(p->h_pinned_d + nk, p->w64.cs.bn, nk * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (int c = 0; c < p->ncells; ++c) {
+    bool conv = true;
+    for (int t = 0; t < m; ++t) {
+      const double rr = p->h_pinned_d[c * m + t] / p->h_pinned_d[nk + c * m + t];
+      if (relres_out) relres_out[c * m + t] = rr;
+      conv = conv && (p->h_pinned_d[c * m + t] <= tol * p->h_pinned_d[nk + c * m + t]);
+    }
+    if (conv_out) conv_out[c] = conv ? 1 : 0;
+    if (it_out) it_out[c] = its[c];
+  }
+  return NLFH_OK;
+}
+
+int solve_fp64(nlfh_plan* p, int m, const double* u0, double tol, int maxit, int* it_out, int* conv_out,
+               double* relres_out, bool rhs_given) {
+  hipStream_t st = p->stream;
+  Work<double>& w = p->w64;
+  const double params[2] = {tol, tol};
+  const int iparams[2] = {maxit, maxit};
+  HIP_TRY(hipMemcpyAsync(p->d_params, params, sizeof(params), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->d_iparams, iparams, sizeof(iparams), hipMemcpyHostToDevice, st));
+  const int64_t F = p->field_len();
+  if (!rhs_given) {
+    // b = -load(s, Es)  (nlfiber/linear.py:205-206)
+    TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, w.B, w.s, nullptr, nullptr, nullptr, m, 1, 0, 0, -1.0, st));
+  }
+  if (u0) {
+    HIP_TRY(launch_copy_d(u0, w.X, p->field_len_m(m), st));
+    TRY(project_mean(p, w.X, m, st));
+    if (rhs_given) {
+      set_error("internal: rhs solves start from zero");
+      return NLFH_EINVAL;
+    }
+    // r = b - K x = -(K x + load)
+    TRY(run_matvec<double>(p, w.X, nullptr, nullptr, w.R, w.s, nullptr, nullptr, nullptr, m, 1, 0, 0, -1.0, st));
+  } else {
+    HIP_TRY(launch_fill<double>(w.X, F, 0.0, st));
+    HIP_TRY(launch_copy_d(w.B, w.R, F, st));
+  }
+  std::vector<int> its(p->ncells, 0);
+  TRY(run_cg<double>(p, w, m, w.B, its.data(), 1, maxit));
+  return fetch_fp64_info(p, m, tol, it_out, conv_out, relres_out, its.data());
+}
+
+
+// FP32 PCG steps inside FP64 iterative refinement: x += K^-1 r solved in FP32 from the scaled true
+// residual r = b - K x (FP64), until ||b - K x|| <= tol ||b|| per right-hand side (true FP64
+// residual). Scalars of the inner PCG (dots, alpha, beta) are FP64.
+int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, int* it_out, int* conv_out,
+                double* relres_out) {
+  hipStream_t st = p->stream;
+  TRY(get_ready32(p));
+  Work<double>& w = p->w64;
+  Work<float>& v = p->w32;
+  const int nc = p->ncells;
+  const int nk = nc * m;  // per-(cell, rhs) arrays use stride m
+  double* rn_true = p->d_ir;
+  double* bn = p->d_ir + nk;
+  const double params[2] = {tol, p->inner_tol};
+  const int iparams[2] = {maxit, maxit};
+  HIP_TRY(hipMemcpyAsync(p->d_params, params, sizeof(params), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(p->d_iparams, iparams, sizeof(iparams), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_d2f(w.s, v.s, p->tot_vox, st));
+  TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, w.B, w.s, nullptr, nullptr, nullptr, m, 1, 0, 0, -1.0, st));
+  VecArgs va = vec_args(p, m, nullptr);
+  HIP_TRY(launch_reduce<double>(va, w.B, nullptr, p->d_part_b, 0, st));
+  HIP_TRY(launch_norms(p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, nc, m, m, 1e-300, bn, nullptr, 0.0,
+                       nullptr, nullptr, st));
+  if (u0) {
+    HIP_TRY(launch_copy_d(u0, w.X, p->field_len_m(m), st));
+    TRY(project_mean(p, w.X, m, st));
+  } else {
+    HIP_TRY(launch_fill<double>(w.X, p->field_len(), 0.0, st));
+  }
+  HIP_TRY(launch_fill<int>(p->d_maxed, nc, 0, st));
+  v.cs.skip = p->d_outer_done;
+  std::vector<int> total(nc, 0), its(nc, 0);
+  int* h = p->h_pinned;
+  for (int outer = 0; outer < 200; ++outer) {
+    // true residual r = b - K x = -(K x + load), its norms and the outer convergence flags
+    if (outer == 0 && !u0) HIP_TRY(launch_copy_d(w.B, w.R, p->field_len(), st));
+    else TRY(run_matvec<double>(p, w.X, nullptr, nullptr, w.R, w.s, nullptr, nullptr, nullptr, m, 1, 0, 0, -1.0, st));
+    HIP_TRY(launch_reduce<double>(va, w.R, nullptr, p->d_part_b, 0, st));
+    HIP_TRY(launch_norms(p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, nc, m, m, 0.0, rn_true, bn, tol,
+                         p->d_outer_done, p->d_maxed, st));
+    HIP_TRY(hipMemcpyAsync(h, p->d_outer_done, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    std::vector<int> odone(h, h + nc);  // h is reused by run_cg
+    bool all = true;
+    for (int c = 0; c < nc; ++c) all = all && odone[c];
+    if (all) break;
+    // inner FP32 solve of K d = r / ||r|| from d = 0
+    HIP_TRY(launch_ir_down(vec_args(p, m, p->d_outer_done), w.R, v.R, v.X, rn_true, st));
+    TRY(run_cg<float>(p, v, m, v.R, its.data(), 1, maxit));
+    HIP_TRY(launch_ir_acc(vec_args(p, m, p->d_outer_done), w.X, v.X, rn_true, st));
+    std::vector<int> maxed(nc, 0);
+    bool any_max = false;
+    for (int c = 0; c < nc; ++c) {
+      if (!odone[c]) total[c] += its[c];
+      maxed[c] = total[c] >= maxit ? 1 : 0;
+      any_max = any_max || maxed[c];
+    }
+    if (any_max) HIP_TRY(hipMemcpyAsync(p->d_maxed, maxed.data(), nc * sizeof(int), hipMemcpyHostToDevice, st));
+  }
+  // outputs: true relative residual per rhs
+  HIP_TRY(hipMemcpyAsync(p->h_pinned_d, rn_true, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(p->h_pinned_d + nk, bn, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int c = 0; c < nc; ++c) {
+    bool conv = true;
+    for (int t = 0; t < m; ++t) {
+      const double rr = p->h_pinned_d[c * m + t] / p->h_pinned_d[nk + c * m + t];
+      if (relres_out) relres_out[c * m + t] = rr;
+      conv = conv && (rr <= tol);
+    }
+    if (conv_out) conv_out[c] = conv ? 1 : 0;
+    if (it_out) it_out[c] = total[c];
+  }
+  return NLFH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nlfh_solve(nlfh_plan* p, const double* s, const double* Es, int m, const double* u0, double* u_out, double tol,
+               int maxit, int precision, int* it_out, int* converged_out, double* relres_out, void* stream) {
+  CHECK_ARG(p && s && Es && u_out, "null argument");
+  TRY(check_m(p, m));
+  CHECK_ARG(tol >= 0.0, "tol must be >= 0");
+  CHECK_ARG(precision == NLFH_PREC_FP64 || precision == NLFH_PREC_MIXED, "precision must be 0 or 1");
+  TRY(enter(p, stream));
+  TRY(get_ready64(p));
+  TRY(set_strain(p, Es, m));
+  HIP_TRY(launch_copy_d(s, p->w64.s, p->tot_vox, p->stream));
+  if (precision == NLFH_PREC_FP64) {
+    TRY(solve_fp64(p, m, u0, tol, maxit, it_out, converged_out, relres_out, false));
+  } else {
+    TRY(solve_mixed(p, m, u0, tol, maxit, it_out, converged_out, relres_out));
+  }
+  HIP_TRY(launch_copy_d(p->w64.X, u_out, p->field_len_m(m), p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_solve_rhs(nlfh_plan* p, const double* s, const double* b, int m, double* x_out, double tol, int maxit,
+                   int precision, int* it_out, int* converged_out, double* relres_out, void* stream) {
+  CHECK_ARG(p && s && b && x_out, "null argument");
+  TRY(check_m(p, m));
+  CHECK_ARG(precision == NLFH_PREC_FP64, "solve_rhs supports precision 0");
+  TRY(enter(p, stream));
+  TRY(get_ready64(p));
+  HIP_TRY(launch_copy_d(s, p->w64.s, p->tot_vox, p->stream));
+  HIP_TRY(launch_copy_d(b, p->w64.B, p->field_len_m(m), p->stream));
+  // the reference projects b (nlfiber/linear.py:176)
+  TRY(project_mean(p, p->w64.B, m, p->stream));
+  TRY(solve_fp64(p, m, nullptr, tol, maxit, it_out, converged_out, relres_out, true));
+  HIP_TRY(launch_copy_d(p->w64.X, x_out, p->field_len_m(m), p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_bilinear(nlfh_plan* p, const double* s, const double* Es, const double* u, int m, double* C_out,
+                  double* dCds_out, void* stream) {
+  CHECK_ARG(p && s && Es && u && C_out, "null argument");
+  TRY(check_m(p, m));
+  TRY(enter(p, stream));
+  TRY(set_strain(p, Es, m));
+  EnergyArgs a{};
+  a.cells = p->d_cells;
+  a.chunks = p->d_chunks;
+  a.nchunks = (int)p->chunks.size();
+  a.m = m;
+  a.eb = std::max(1, 256 / m);
+  a.u = u;
+  a.s = s;
+  a.strain = p->d_strain;
+  a.part = p->d_part_e;
+  a.dcds = dCds_out;
+  a.lam16 = p->lam / 16.0;
+  a.mu16 = p->mu / 16.0;
+  HIP_TRY(launch_energy(a, p->stream));
+  // C = sum / N per cell: rows = m*m; divide by N per cell (sum_partials takes one scale: loop cells)
+  HIP_TRY(launch_sum_partials(p->d_part_e, (int)p->chunks.size(), p->d_chunk_range, p->ncells, m * m, 1.0, p->d_cells,
+                              m * m, C_out, p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_plan_set_inner_tol(nlfh_plan* p, double inner_tol) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must be in (0, 1)");
+  p->inner_tol = inner_tol;
+  return NLFH_OK;
+}
+
+int nlfh_project_mean(nlfh_plan* p, double* v, int m, void* stream) {
+  CHECK_ARG(p && v, "null argument");
+  TRY(check_m(p, m));
+  TRY(enter(p, stream));
+  TRY(project_mean(p, v, m, p->stream));
+  return leave(p, stream);
+}
+
+int nlfh_symbol(const nlfh_plan* p, int shape_index, double* kinv_out) {
+  CHECK_ARG(p && kinv_out, "null argument");
+  CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
+  const Shape& sh = p->shapes[shape_index];
+  const size_t n = (size_t)sh.nx * sh.ny * (sh.nz / 2 + 1) * 6;
+  std::memcpy(kinv_out, p->h_sym.data() + sh.off_sym * 6, n * sizeof(double));
+  return NLFH_OK;
+}
+
+long long nlfh_plan_bytes(const nlfh_plan* p) { return p ? (long long)p->bytes : 0; }
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Kernel argument blocks and launchers shared by the library's translation units.
+#pragma once
+#include "nlfh_common.h"
+
+namespace nlfh {
+
+#define NLFH_FFT_MAX_STAGES 12
+
+template <typename T>
+struct MatvecArgs {
+  const CellDesc* cells;
+  const MvItem* items;
+  const T* u;            // input field (nullptr: zero field, load only)
+  const T* p_old;        // PUPDATE: previous search direction (u is z then)
+  T* p_new;              // PUPDATE: new search direction p = z + beta p_old (owned nodes)
+  T* out;                // output field
+  const T* s;            // stiffness scale per voxel
+  const double* strain;  // STRAIN: macroscopic strains (m, 6) shared by all cells
+  const double* beta;    // PUPDATE: beta per (cell, rhs)
+  double* dot_part;      // DOT: partial sums of p . out per (rhs, item_base + item)
+  const int* done;       // optional per-cell early-exit flags
+  int m;
+  int nitems;     // total matvec items of the plan (row stride of dot_part)
+  int item_base;  // index of items[0] among all items
+  int LZ;   // lanes holding a row of nz nodes (nz = LZ * ZPT)
+  int LZS;  // lanes reserved per row: power of two >= LZ (lanes LZ..LZS-1 idle)
+  T lam16, mu16;  // lambda / 16, mu / 16
+  T scale_out;    // output multiplier (-1 gives b = -load)
+};
+
+struct FftLen {
+  int n;
+  int nst;
+  int rad[NLFH_FFT_MAX_STAGES];
+  int tw_off;    // offset of w_n^t, t in [0, n), in the twiddle tables
+  int perm_off;  // offset of idx(p) (position p of a DIT input holds x[idx(p)]) in the perm table
+  int iperm_off; // offset of the inverse map
+};
+
+template <typename T>
+struct FftTables {
+  const FftLen* lens;     // per distinct length
+  const cplx<T>* tw;      // twiddles exp(-2 pi i t / n)
+  const int* perm;
+  const int* iperm;
+  const int* shape_len;   // per shape id: index of (nx, ny, nz) in `lens` (3 ints)
+};
+
+template <typename T>
+struct PrecArgs {
+  const CellDesc* cells;
+  const PlaneItem* planes;
+  const LineItem* lines;
+  FftTables<T> ft;
+  const T* sym;          // per shape: [kx][ky][kz][6] real symmetric Kinv / (nx ny nz)
+  const T* r;            // pass A input (and pass C for the r . z dot)
+  T* r_out;              // pass A fused update: r - alpha Ap written here (may equal r)
+  const T* Ap;           // pass A fused update
+  T* x;                  // pass A fused update: x += alpha p
+  const T* p;            // pass A fused update
+  const double* alpha;   // per (cell, rhs)
+  cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
+  T* z;                  // pass C output
+  double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
+  double* part_c;        // pass C: partial r . z per (rhs, plane item)
+  const int* done;       // per-cell early exit (nullable)
+  int m;
+  int nplanes;
+  int nlines;
+  int rows_per_batch;    // row pairs per LDS batch in passes A and C
+};
+
+struct VecArgs {
+  const CellDesc* cells;
+  const ChunkItem* chunks;
+  int nchunks;
+  int m;
+  const int* done;  // per-cell early exit (nullable)
+};
+
+struct CgState {
+  double* rz;     // [cell*m + rhs]
+  double* alpha;
+  double* beta;
+  double* bn;
+  double* rn;
+  int* it;        // [cell]
+  int* done;      // [cell]
+  const double* tol;  // [1]
+  const int* maxit;   // [1]
+  const int* skip;    // nullable [cell]: cells that take no iterations (converged outer loop)
+  int m;
+};
+
+struct EnergyArgs {
+  const CellDesc* cells;
+  const ChunkItem* chunks;
+  int nchunks;
+  int m;
+  int eb;  // elements per block pass (blockDim = eb * m)
+  int pad_;
+  const double* u;       // correctors (SoA)
+  const double* s;       // stiffness scale
+  const double* strain;  // (m, 6)
+  double* part;          // [chunk][m*m]
+  double* dcds;          // nullable: [cell-major] (m, m, nvox) per cell at off_vox * m * m
+  double lam16, mu16;
+};
+
+template <typename T>
+hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
+                         hipStream_t st);
+template <typename T>
+size_t prec_plane_lds(int ny, int nz, int rows_per_batch);
+template <typename T>
+hipError_t launch_prec(const PrecArgs<T>& a, int update, int dot, size_t lds_plane, size_t lds_line, hipStream_t st);
+hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, double* ds, int64_t n, double emin,
+                                  double p, hipStream_t st);
+hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);
+hipError_t launch_copy_d(const double* in, double* out, int64_t n, hipStream_t st);
+template <typename T>
+hipError_t launch_fill(T* out, int64_t n, T v, hipStream_t st);
+template <typename T>
+hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part, int mode, hipStream_t st);
+hipError_t launch_sum_partials(const double* part, int nitems, const int* range, int ncells, int rows, double scale,
+                               const CellDesc* cells, int out_stride, double* out, hipStream_t st);
+template <typename T>
+hipError_t launch_sub_mean(const VecArgs& va, T* v, const double* mean, hipStream_t st);
+hipError_t launch_ir_down(const VecArgs& va, const double* r64, float* r32, float* d32, const double* scale,
+                          hipStream_t st);
+hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, hipStream_t st);
+hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
+                          const double* pb, int nch, const int* cr, hipStream_t st);
+hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int nmv, const int* mr, hipStream_t st);
+hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
+                          hipStream_t st);
+hipError_t launch_norms(const double* part, int nchunks, const int* crange, int ncells, int m, int M, double eps,
+                        double* out, const double* bn, double tol, int* flag, const int* force, hipStream_t st);
+hipError_t launch_energy(const EnergyArgs& a, hipStream_t st);
+
+}  // namespace nlfh

@@ -1,0 +1,297 @@
+// Matrix-free Q1 elasticity operator K(s) u (+ the macroscopic-strain load), gfx950.
+//
+// Restates nlfiber/linear.py:143-149 (apply_K = scatter(s * (gather(u) @ Ke)), load =
+// scatter(s * E.G)) without the 24x24 element matrix. With unnormalised Walsh sums
+// h = W u_e (W = Wz (x) Wy (x) Wx, two-point sum/difference per axis) the unit-cube Q1 stiffness
+// with 2x2x2 Gauss points is Ke = W^T L W with a sparse L: the constant-strain block D/16,
+// the three bilinear modes coupled in pairs/triples with weights (lambda, mu)/48 and the trilinear
+// mode (lambda + 4 mu)/144 (derived from the quadrature-orthogonality of the Walsh modes; checked
+// against the dense Ke to round-off in the tests). The butterflies are shared by neighbouring
+// elements, so one element costs about 140 VALU operations per right-hand side instead of 1152.
+//
+// Tiling: a block owns a y-tile of TY node rows and all nz of one cell, and marches over an
+// x-chunk of XC planes (one extra plane of warm-up). A row of nz nodes is LZ lanes x ZPT
+// consecutive z values in a slot of LZS lanes (a power of two <= 64, so rows never straddle a
+// wavefront); the x
+// butterflies live in registers across the march, z neighbours come from the adjacent lane
+// (__shfl), y neighbours through LDS (two small buffers, two barriers per plane). The tile
+// loads TY + 2 rows and computes TY + 1 element rows (one halo row each side). When nz / ZPT is
+// not a power of two the row is padded to the next power of two lanes (idle lanes), e.g. 48
+// lanes in a slot of 64.
+#include "nlfh_kernels.h"
+
+namespace nlfh {
+
+
+template <typename T>
+__device__ __forceinline__ void modal_op(const T (&h)[3][8], T (&g)[3][8], const T l1, const T m1,
+                                         const T* e4) {
+  T exx = h[0][1], eyy = h[1][2], ezz = h[2][4];
+  T gyz = h[1][4] + h[2][2], gxz = h[0][4] + h[2][1], gxy = h[0][2] + h[1][1];
+  if (e4) {
+    exx += e4[0]; eyy += e4[1]; ezz += e4[2];
+    gyz += e4[3]; gxz += e4[4]; gxy += e4[5];
+  }
+  const T tr = exx + eyy + ezz;
+  const T ltr = l1 * tr;
+  const T m2x = m1 + m1;
+  g[0][1] = ltr + m2x * exx;
+  g[1][2] = ltr + m2x * eyy;
+  g[2][4] = ltr + m2x * ezz;
+  const T tyz = m1 * gyz, txz = m1 * gxz, txy = m1 * gxy;
+  g[1][4] = tyz; g[2][2] = tyz;
+  g[0][4] = txz; g[2][1] = txz;
+  g[0][2] = txy; g[1][1] = txy;
+  const T third = T(1) / T(3);
+  const T l2 = l1 * third, m2 = m1 * third;
+  const T a2 = l2 + T(3) * m2;
+  g[0][3] = a2 * h[0][3] + l2 * h[2][6];
+  g[2][6] = a2 * h[2][6] + l2 * h[0][3];
+  g[1][3] = a2 * h[1][3] + l2 * h[2][5];
+  g[2][5] = a2 * h[2][5] + l2 * h[1][3];
+  g[0][5] = a2 * h[0][5] + l2 * h[1][6];
+  g[1][6] = a2 * h[1][6] + l2 * h[0][5];
+  const T t = h[2][3] + h[1][5] + h[0][6];
+  g[2][3] = m2 * (t + h[2][3]);
+  g[1][5] = m2 * (t + h[1][5]);
+  g[0][6] = m2 * (t + h[0][6]);
+  const T kap = (l1 + T(4) * m1) * (T(1) / T(9));
+  g[0][7] = kap * h[0][7];
+  g[1][7] = kap * h[1][7];
+  g[2][7] = kap * h[2][7];
+  g[0][0] = T(0); g[1][0] = T(0); g[2][0] = T(0);
+}
+
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT>
+__global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
+  extern __shared__ double smem_d[];
+  T* lds = reinterpret_cast<T*>(smem_d);
+  __shared__ double red[8];
+
+  const MvItem it = a.items[blockIdx.x];
+  if (a.done != nullptr && a.done[it.cell]) return;
+  const CellDesc cd = a.cells[it.cell];
+  const int rhs = blockIdx.y;
+  const int nx = cd.nx, ny = cd.ny, nz = cd.nz;
+  const int LZ = a.LZ, LZS = a.LZS;
+  const int ty = it.ty;
+  const int RB = ty + 2;
+  const int tid = threadIdx.x;
+  const int lz = tid % LZS;
+  const int r = tid / LZS;
+  const bool active = r < RB && lz < LZ;
+  const bool erow = r <= ty;              // computes element row yg
+  const bool owns = active && r >= 1 && r <= ty;
+  int yg = it.y0 - 1 + r;
+  yg = ((yg % ny) + ny) % ny;
+  const int z0 = lz * ZPT;
+  const int lane = tid & 63;
+  const int src_next = (lane - lz) + (lz + 1 == LZ ? 0 : lz + 1);
+  const int src_prev = (lane - lz) + (lz == 0 ? LZ - 1 : lz - 1);
+
+  const int64_t nvox = cd.nvox;
+  const int64_t fbase = cd.off_field + (int64_t)rhs * 3 * nvox;
+  const T* __restrict__ U = a.u ? a.u + fbase : nullptr;
+  const T* __restrict__ P = PUPDATE ? a.p_old + fbase : nullptr;
+  T* __restrict__ PN = PUPDATE ? a.p_new + fbase : nullptr;
+  T* __restrict__ O = a.out + fbase;
+  const T* __restrict__ S = a.s + cd.off_vox;
+  T beta = T(0);
+  if (PUPDATE) beta = (T)a.beta[it.cell * a.m + rhs];
+  T e4[6];
+  if (STRAIN) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) e4[j] = (T)(4.0 * a.strain[rhs * 6 + j]);
+  }
+
+  const int plane_sz = 6 * ZPT * LZ;  // values of one row in one LDS buffer
+  T* SD = lds;                          // [RB][6][ZPT][LZ]
+  T* UY = lds + RB * plane_sz;          // [RB][6][ZPT][LZ]
+#define LIDX(rr, f, q) ((((rr) * 6 + (f)) * ZPT + (q)) * LZ + lz)
+
+  auto load_row = [&](int xp, T (&v)[3][ZPT]) {
+    const int64_t o = ((int64_t)xp * ny + yg) * nz + z0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) {
+        T val = U ? U[c * nvox + o + q] : T(0);
+        if (PUPDATE) val += beta * P[c * nvox + o + q];
+        v[c][q] = val;
+      }
+    }
+  };
+
+  T up[3][ZPT], carry[3][ZPT];
+  double dot = 0.0;
+  if (active) load_row(((it.x0 - 1) % nx + nx) % nx, up);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < ZPT; ++q) carry[c][q] = T(0);
+
+  for (int step = 0; step <= it.xc; ++step) {
+    const int xi = ((it.x0 - 1 + step) % nx + nx) % nx;  // element layer / finalised node plane
+    const int xi1 = (xi + 1) % nx;
+    T un[3][ZPT];
+    T pc[3][ZPT];
+    if (active) {
+      load_row(xi1, un);
+      if (PUPDATE && owns && step < it.xc) {
+        const int64_t o = ((int64_t)xi1 * ny + yg) * nz + z0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+          const T sx = un[c][q] + up[c][q];
+          const T dx = un[c][q] - up[c][q];
+          SD[LIDX(r, c, q)] = sx;
+          SD[LIDX(r, 3 + c, q)] = dx;
+          pc[c][q] = up[c][q];
+          up[c][q] = un[c][q];
+        }
+    }
+    __syncthreads();
+
+    // y stage, z stage, modal operator, reverse z: element row yg (rows 0..ty)
+    T Fz[3][4][ZPT];
+    if (active && erow) {
+      T Y[3][4][ZPT];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+          const T s0 = SD[LIDX(r, c, q)], d0 = SD[LIDX(r, 3 + c, q)];
+          const T s1 = SD[LIDX(r + 1, c, q)], d1 = SD[LIDX(r + 1, 3 + c, q)];
+          Y[c][0][q] = s1 + s0;
+          Y[c][2][q] = s1 - s0;
+          Y[c][1][q] = d1 + d0;
+          Y[c][3][q] = d1 - d0;
+        }
+      // z neighbour of the last value: first value of the next lane in the row
+      T Yn[3][4];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = __shfl(Y[c][mm][0], src_next, 64);
+      const int64_t so = ((int64_t)xi * ny + yg) * nz + z0;
+      T hi_last[3][4];
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) {
+        T h[3][8], g[3][8];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) {
+            const T yn = (q + 1 < ZPT) ? Y[c][mm][(q + 1) % ZPT] : Yn[c][mm];
+            h[c][mm] = yn + Y[c][mm][q];
+            h[c][mm + 4] = yn - Y[c][mm][q];
+          }
+        const T sv = S[so + q];
+        modal_op<T>(h, g, sv * a.lam16, sv * a.mu16, STRAIN ? e4 : nullptr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int mm = 0; mm < 4; ++mm) {
+            const T lo = g[c][mm] - g[c][mm + 4];
+            const T hi = g[c][mm] + g[c][mm + 4];
+            if (q == 0) Fz[c][mm][0] = lo;
+            else Fz[c][mm][q] += lo;
+            if (q + 1 < ZPT) Fz[c][mm][(q + 1) % ZPT] = hi;
+            else hi_last[c][mm] = hi;
+          }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int mm = 0; mm < 4; ++mm) Fz[c][mm][0] += __shfl(hi_last[c][mm], src_prev, 64);
+      // reverse y: the upper half goes to row r + 1 through LDS
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+          const T A0 = Fz[c][0][q], B0 = Fz[c][2][q];
+          const T A1 = Fz[c][1][q], B1 = Fz[c][3][q];
+          UY[LIDX(r, 2 * c, q)] = A0 + B0;
+          UY[LIDX(r, 2 * c + 1, q)] = A1 + B1;
+          Fz[c][0][q] = A0 - B0;
+          Fz[c][1][q] = A1 - B1;
+        }
+    }
+    // (all lanes of a row take the same branch, so the shuffles above only read active lanes)
+    __syncthreads();
+
+    if (owns) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+          const T A = Fz[c][0][q] + UY[LIDX(r - 1, 2 * c, q)];
+          const T B = Fz[c][1][q] + UY[LIDX(r - 1, 2 * c + 1, q)];
+          const T lo = A - B;
+          const T hi = A + B;
+          if (step >= 1) {
+            const T v = (carry[c][q] + lo) * a.scale_out;
+            O[c * nvox + ((int64_t)xi * ny + yg) * nz + z0 + q] = v;
+            if (DOT) dot += (double)pc[c][q] * (double)v;
+          }
+          carry[c][q] = hi;
+        }
+    }
+  }
+#undef LIDX
+  if (DOT) {
+    // rows that straddle wavefronts are fine: every thread of the block joins the reduction
+    const double t = block_sum(dot, red);
+    if (threadIdx.x == 0) a.dot_part[(int64_t)rhs * a.nitems + a.item_base + blockIdx.x] = t;
+  }
+}
+
+// Host-side launcher -------------------------------------------------------------------------
+
+template <typename T, int ZPT>
+static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, dim3 grid,
+                                int threads, size_t lds, hipStream_t st) {
+#define NLFH_MV(S_, P_, D_)                                                                           \
+  if (lds > 65536)                                                                                  \
+    (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_>,                       \
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                \
+  hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_>), grid, dim3(threads), lds, st, a);         \
+  return hipGetLastError();
+  if (strain) {
+    NLFH_MV(true, false, false)
+  }
+  if (pupdate) {
+    if (dot) { NLFH_MV(false, true, true) }
+    NLFH_MV(false, true, false)
+  }
+  if (dot) { NLFH_MV(false, false, true) }
+  NLFH_MV(false, false, false)
+#undef NLFH_MV
+}
+
+template <typename T>
+hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
+                         hipStream_t st) {
+  // max_ty + 2 rows of LZS lanes per block
+  const int threads = ((a.LZS * (max_ty + 2) + 63) / 64) * 64;
+  const size_t lds = (size_t)2 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
+  dim3 grid(nblk, a.m, 1);
+  switch (zpt) {
+    case 1: return launch_mv_zpt<T, 1>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 2: return launch_mv_zpt<T, 2>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 3: return launch_mv_zpt<T, 3>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 4: return launch_mv_zpt<T, 4>(a, strain, pupdate, dot, grid, threads, lds, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_matvec<double>(const MatvecArgs<double>&, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_matvec<float>(const MatvecArgs<float>&, int, int, int, int, int, int, hipStream_t);
+
+}  // namespace nlfh

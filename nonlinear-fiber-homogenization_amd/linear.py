@@ -1,0 +1,455 @@
+"""Drop-in `LinearHomogenization` (nlfiber/linear.py:99-264) on the gfx950 library.
+
+Same constructor, attributes, methods, return values and conventions as the reference class; the
+arrays are torch tensors on the current HIP device (float64), except the small homogenized
+stiffness C, returned on the host so `np.asarray(C)` works as in the reference's callers
+(nlfiber/fiber.py:75, scripts/width_pilot.py:43). Every operator runs in libnlfh.so; there is no
+CPU path.
+
+Layouts: nodal fields are (m, nx, ny, nz, 3) as in the reference. The library stores them
+component-planar (m, 3, nx, ny, nz); the fields returned here are views of that storage through the
+axis permutation (0, 2, 3, 4, 1), so handing them back (warm starts, nlfiber/fiber.py:41-56)
+costs no copy.
+
+Differentiability (the reference uses jax.vjp, nlfiber/fiber.py:34-36):
+  * `stiffness_scale` and `bilinear` are torch autograd functions: dC_IJ/ds_e is the element energy
+    divided by N with the correctors held fixed (nlfiber/linear.py:10-13), which the library
+    returns from the same kernel that assembles C, and ds/drho comes from the scale kernel;
+  * `invariants` is therefore differentiable in rho (correctors detached, nlfiber/linear.py:263);
+  * `correctors` has the reference's custom VJP (nlfiber/linear.py:118-134): one adjoint PCG
+    solve K lam = -proj(ubar) and the element cross energies of (lam, u).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+
+# --- element constants (nlfiber/linear.py:21-77), shared with the finite-strain code ----------
+VOIGT = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+NODES = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)])
+_g = 0.5 + np.array([-1.0, 1.0]) / (2 * np.sqrt(3.0))
+GPTS = np.array([[x, y, z] for z in _g for y in _g for x in _g])
+
+
+def clip01(x):
+    """Clamp to [0, 1] with derivative 1 at the bounds (nlfiber/linear.py:27-30)."""
+    if isinstance(x, torch.Tensor):
+        return torch.where(x > 1.0, torch.ones_like(x), torch.where(x < 0.0, torch.zeros_like(x), x))
+    x = np.asarray(x)
+    return np.where(x > 1.0, 1.0, np.where(x < 0.0, 0.0, x))
+
+
+def dN_ref():
+    """Shape-function gradients of the unit Q1 element at the Gauss points: [gauss, node, dim]."""
+    d = np.zeros((8, 8, 3))
+    for gi, (x, y, z) in enumerate(GPTS):
+        for a, (i, j, k) in enumerate(NODES):
+            f = [(x if i else 1 - x), (y if j else 1 - y), (z if k else 1 - z)]
+            sg = [(1 if i else -1), (1 if j else -1), (1 if k else -1)]
+            d[gi, a] = [sg[0] * f[1] * f[2], f[0] * sg[1] * f[2], f[0] * f[1] * sg[2]]
+    return d
+
+
+def lame(E, nu):
+    return E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+
+
+def iso_D(E, nu):
+    lam, mu = lame(E, nu)
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[[0, 1, 2], [0, 1, 2]] = lam + 2 * mu
+    D[[3, 4, 5], [3, 4, 5]] = mu
+    return D
+
+
+def B_matrices():
+    """Strain-displacement matrices (8 Gauss points, 6, 24), engineering shear."""
+    dN = dN_ref()
+    B = np.zeros((8, 6, 24))
+    rows = {0: [(0, 0)], 1: [(1, 1)], 2: [(2, 2)], 3: [(1, 2), (2, 1)], 4: [(0, 2), (2, 0)], 5: [(0, 1), (1, 0)]}
+    for g in range(8):
+        for a in range(8):
+            for r, pairs in rows.items():
+                for comp, der in pairs:  # strain row r takes d u_comp / d x_der
+                    B[g, r, 3 * a + comp] = dN[g, a, der]
+    return B
+
+
+def unit_strain(I):
+    e = np.zeros(6)
+    e[I] = 1.0
+    return e
+
+
+# --- device helpers ------------------------------------------------------------------------
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("nlfh needs a HIP device: torch.cuda.is_available() is False")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _as_f64(x, dev):
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+
+
+def _es_host(Es):
+    if isinstance(Es, torch.Tensor):
+        Es = Es.detach().cpu().numpy()
+    Es = np.ascontiguousarray(np.asarray(Es, dtype=np.float64).reshape(-1, 6))
+    return Es, Es.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Plan:
+    """Owner of one nlfh_plan (cells of given shapes, material E, nu)."""
+
+    def __init__(self, shapes, E=1.0, nu=0.3, m_layout=6):
+        lib = _lib.load()
+        self.device = _device()
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        arr = (ctypes.c_int * (3 * len(self.shapes)))(*[v for s in self.shapes for v in s])
+        h = ctypes.c_void_p()
+        _lib.check(lib.nlfh_plan_create(arr, len(self.shapes), float(E), float(nu), int(m_layout), ctypes.byref(h)))
+        self.handle = h
+        self._lib = lib
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self._lib.nlfh_plan_destroy(h)
+            except Exception:
+                pass
+            self.handle = None
+
+    def set_tiling(self, shape_index, ty, xc, zpt):
+        _lib.check(self._lib.nlfh_plan_set_tiling(self.handle, shape_index, ty, xc, zpt))
+
+    def get_tiling(self, shape_index):
+        ty, xc, zpt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.nlfh_plan_get_tiling(self.handle, shape_index, ctypes.byref(ty), ctypes.byref(xc),
+                                                  ctypes.byref(zpt)))
+        return ty.value, xc.value, zpt.value
+
+    def nbytes(self):
+        return int(self._lib.nlfh_plan_bytes(self.handle))
+
+
+def to_planar(u, shape):
+    """(m, nx, ny, nz, 3) or (nx, ny, nz, 3) -> contiguous (m, 3, nx, ny, nz) float64 on the device."""
+    dev = _device()
+    u = _as_f64(u, dev)
+    if u.dim() == 4:
+        u = u.unsqueeze(0)
+    if tuple(u.shape[1:4]) != tuple(shape) or u.shape[4] != 3:
+        raise ValueError(f"field of shape {tuple(u.shape)} does not match the cell {tuple(shape)} x 3")
+    return u.permute(0, 4, 1, 2, 3).contiguous()
+
+
+def from_planar(x):
+    """(m, 3, nx, ny, nz) storage -> (m, nx, ny, nz, 3) view."""
+    return x.permute(0, 2, 3, 4, 1)
+
+
+class _ScaleFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rho, lh):
+        s = torch.empty_like(rho)
+        ds = torch.empty_like(rho) if rho.requires_grad else None
+        lh._scale_raw(rho, s, ds)
+        ctx.save_for_backward(ds)
+        return s
+
+    @staticmethod
+    def backward(ctx, gs):
+        (ds,) = ctx.saved_tensors
+        return gs * ds, None
+
+
+class _BilinearFn(torch.autograd.Function):
+    """C (host, m x m) from s (device) with the correctors fixed; grad_s = sum_IJ gC_IJ dC_IJ/ds."""
+
+    @staticmethod
+    def forward(ctx, s, lh, Es, u_planar):
+        need = s.requires_grad
+        C, dcds = lh._bilinear_raw(s, Es, u_planar, need)
+        ctx.save_for_backward(dcds if need else torch.empty(0))
+        ctx.need = need
+        return C
+
+    @staticmethod
+    def backward(ctx, gC):
+        if not ctx.need:
+            return None, None, None, None
+        (dcds,) = ctx.saved_tensors
+        g = gC.to(device=dcds.device, dtype=torch.float64)
+        m = g.shape[0]
+        gs = torch.tensordot(g.reshape(m * m), dcds.reshape(m * m, -1), dims=1).reshape(dcds.shape[2:])
+        return gs, None, None, None
+
+
+class _CorrectorsFn(torch.autograd.Function):
+    """u(s) with the adjoint VJP of nlfiber/linear.py:118-134."""
+
+    @staticmethod
+    def forward(ctx, s, lh, Es, u0):
+        u, _ = lh.solve(s.detach(), Es, u0)
+        up = u.permute(0, 4, 1, 2, 3).contiguous()
+        ctx.save_for_backward(s.detach(), up)
+        ctx.lh, ctx.Es = lh, Es
+        return from_planar(up.clone())
+
+    @staticmethod
+    def backward(ctx, ubar):
+        s, up = ctx.saved_tensors
+        lh, Es = ctx.lh, ctx.Es
+        rhs = -to_planar(ubar, lh.shape)
+        lh._project_raw(rhs)
+        lam = lh._solve_rhs_raw(s, rhs)  # K lam = -proj(ubar)
+        lh._project_raw(lam)
+        Es_np, _ = _es_host(Es)
+        g = torch.zeros(lh.shape, dtype=torch.float64, device=s.device)
+        n = float(np.prod(lh.shape))
+        for I in range(Es_np.shape[0]):
+            pair = torch.stack([lam[I], up[I]]).contiguous()
+            E2 = np.stack([np.zeros(6), Es_np[I]])
+            _, dcds = lh._bilinear_raw(s, E2, pair, True)
+            g += dcds[0, 1] * n  # b_e(lam_I, E_I + u_I) = lam_e . Ke u_e + lam_e . G^T E_I
+        return g, None, None, None
+
+
+class LinearHomogenization:
+    """Linear homogenization on a fixed grid. `shape` is (nx, ny, nz) or (n, n) for plane strain.
+
+    Extra keyword (not in the reference): precision = "fp64" (the reference's PCG, default) or
+    "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule)."""
+
+    def __init__(self, shape, E=1.0, nu=0.3, emin=1e-3, p=3.0, dtype=torch.float64, tol=1e-9, maxit=4000,
+                 precision="fp64"):
+        shape = tuple(int(n) for n in shape)
+        self.plane = len(shape) == 2
+        self.shape = shape + (1,) if self.plane else shape
+        self.E, self.nu, self.emin, self.p = E, nu, emin, p
+        self.dtype, self.tol, self.maxit = torch.float64, tol, maxit
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self._dev = _device()
+        B = B_matrices()
+        D = iso_D(E, nu)
+        self.Dm = torch.as_tensor(D, device=self._dev)
+        self.Ke = torch.as_tensor(sum(B[g].T @ D @ B[g] for g in range(8)) / 8.0, device=self._dev)
+        self.G = torch.as_tensor(sum(D @ B[g] for g in range(8)) / 8.0, device=self._dev)
+        self._plan = Plan([self.shape], E, nu)
+        self._lib = _lib.load()
+        self.last_info = {}
+        self.correctors = lambda s, Es, u0: _CorrectorsFn.apply(_as_f64(s, self._dev), self, Es, u0)
+
+    # --- internals ------------------------------------------------------------------------
+    @property
+    def _N(self):
+        return int(np.prod(self.shape))
+
+    def _scale_raw(self, rho, s, ds):
+        _lib.check(self._lib.nlfh_stiffness_scale(self._plan.handle, _ptr(rho), float(self.emin), float(self.p),
+                                                  _ptr(s), _ptr(ds), _stream()))
+
+    def _bilinear_raw(self, s, Es, u_planar, want_grad):
+        Es_np, Es_p = _es_host(Es)
+        m = Es_np.shape[0]
+        s = s.detach().contiguous()
+        C = torch.empty((m, m), dtype=torch.float64, device=self._dev)
+        dcds = torch.empty((m, m) + self.shape, dtype=torch.float64, device=self._dev) if want_grad else None
+        _lib.check(self._lib.nlfh_bilinear(self._plan.handle, _ptr(s), Es_p, _ptr(u_planar), m, _ptr(C), _ptr(dcds),
+                                           _stream()))
+        return C.cpu(), dcds
+
+    def _project_raw(self, v):
+        _lib.check(self._lib.nlfh_project_mean(self._plan.handle, _ptr(v), v.shape[0], _stream()))
+
+    def _solve_rhs_raw(self, s, b_planar):
+        m = b_planar.shape[0]
+        x = torch.empty_like(b_planar)
+        it = (ctypes.c_int * 1)()
+        cv = (ctypes.c_int * 1)()
+        rr = (ctypes.c_double * m)()
+        _lib.check(self._lib.nlfh_solve_rhs(self._plan.handle, _ptr(s.contiguous()), _ptr(b_planar), m, _ptr(x),
+                                            float(self.tol), int(self.maxit), 0, it, cv, rr, _stream()))
+        self.last_info = dict(it=it[0], converged=bool(cv[0]), relres=list(rr))
+        return x
+
+    def _s(self, s):
+        s = _as_f64(s, self._dev)
+        if self.plane and s.dim() == 2:
+            s = s[..., None]
+        return s.contiguous()
+
+    # --- operators (nlfiber/linear.py:137-170) -----------------------------------------------
+    def stiffness_scale(self, rho):
+        rho = _as_f64(rho, self._dev)
+        if self.plane and rho.dim() == 2:
+            rho = rho[..., None]
+        rho = rho.contiguous()
+        if rho.numel() != self._N:
+            flat = rho.reshape(-1, self._N)
+            return torch.stack([self.stiffness_scale(r.reshape(self.shape)) for r in flat]).reshape(rho.shape)
+        if rho.requires_grad:
+            return _ScaleFn.apply(rho, self)
+        s = torch.empty_like(rho)
+        self._scale_raw(rho, s, None)
+        return s
+
+    def apply_K(self, s, u):
+        single = u.dim() == 4 if isinstance(u, torch.Tensor) else np.ndim(u) == 4
+        up = to_planar(u, self.shape)
+        out = torch.empty_like(up)
+        s = self._s(s).detach()
+        for i in range(0, up.shape[0], 6):
+            blk = up[i:i + 6]
+            _lib.check(self._lib.nlfh_apply_K(self._plan.handle, _ptr(s), _ptr(blk), _ptr(out[i:i + 6]),
+                                              blk.shape[0], 0, _stream()))
+        out = from_planar(out)
+        return out[0] if single else out
+
+    def load(self, s, Es):
+        Es_np, Es_p = _es_host(Es)
+        m = Es_np.shape[0]
+        f = torch.empty((m, 3) + self.shape, dtype=torch.float64, device=self._dev)
+        _lib.check(self._lib.nlfh_load(self._plan.handle, _ptr(self._s(s).detach()), Es_p, m, _ptr(f), _stream()))
+        return from_planar(f)
+
+    def residual(self, s, Es, u):
+        Es_np, Es_p = _es_host(Es)
+        up = to_planar(u, self.shape)
+        r = torch.empty_like(up)
+        _lib.check(self._lib.nlfh_residual(self._plan.handle, _ptr(self._s(s).detach()), Es_p, _ptr(up),
+                                           up.shape[0], _ptr(r), _stream()))
+        self._project_raw(r)
+        return from_planar(r)
+
+    @property
+    def Kinv(self):
+        """Preconditioner symbol (nx, ny, nz//2+1, 3, 3), complex128 with zero imaginary part
+        (nlfiber/linear.py:154-165)."""
+        nx, ny, nz = self.shape
+        buf = np.empty((nx, ny, nz // 2 + 1, 6))
+        _lib.check(self._lib.nlfh_symbol(self._plan.handle, 0, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        K = np.empty(buf.shape[:3] + (3, 3))
+        idx = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (1, 2): 3, (2, 1): 3, (0, 2): 4, (2, 0): 4, (0, 1): 5, (1, 0): 5}
+        for (a, c), k in idx.items():
+            K[..., a, c] = buf[..., k]
+        return torch.as_tensor(K.astype(np.complex128), device=self._dev)
+
+    def precond(self, r):
+        single = r.dim() == 4 if isinstance(r, torch.Tensor) else np.ndim(r) == 4
+        rp = to_planar(r, self.shape)
+        z = torch.empty_like(rp)
+        for i in range(0, rp.shape[0], 6):
+            blk = rp[i:i + 6]
+            _lib.check(self._lib.nlfh_precond(self._plan.handle, _ptr(blk), _ptr(z[i:i + 6]), blk.shape[0], 0,
+                                              _stream()))
+        z = from_planar(z)
+        return z[0] if single else z
+
+    # --- solver (nlfiber/linear.py:173-217) -------------------------------------------------
+    def solve(self, s, Es, u0=None):
+        """Correctors for the macroscopic strains Es (m, 6): returns (u (m, nx, ny, nz, 3), it)."""
+        Es_np, Es_p = _es_host(Es)
+        m = Es_np.shape[0]
+        if m > 6:
+            us, its = zip(*[self.solve(s, Es_np[i:i + 6], None if u0 is None else u0[i:i + 6])
+                            for i in range(0, m, 6)])
+            return torch.cat(us), max(its)
+        s = self._s(s).detach()
+        u0p = None if u0 is None else to_planar(u0.detach() if isinstance(u0, torch.Tensor) else u0, self.shape)
+        u = torch.empty((m, 3) + self.shape, dtype=torch.float64, device=self._dev)
+        it = (ctypes.c_int * 1)()
+        cv = (ctypes.c_int * 1)()
+        rr = (ctypes.c_double * m)()
+        prec = 0 if self.precision == "fp64" else 1
+        _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, m, _ptr(u0p), _ptr(u), float(self.tol),
+                                        int(self.maxit), prec, it, cv, rr, _stream()))
+        self.last_info = dict(it=it[0], converged=bool(cv[0]), relres=list(rr), precision=self.precision)
+        return from_planar(u), it[0]
+
+    # --- homogenized stiffness (nlfiber/linear.py:220-264) ------------------------------------
+    def bilinear(self, s, Es, u):
+        """C_IJ = mean_e s_e b_e(E_I + u_I, E_J + u_J) (host tensor, differentiable in s)."""
+        s = self._s(s) if not isinstance(s, torch.Tensor) or s.device != self._dev else s
+        if self.plane and s.dim() == 2:
+            s = s[..., None]
+        up = to_planar(u.detach() if isinstance(u, torch.Tensor) else u, self.shape)
+        return _BilinearFn.apply(s.contiguous(), self, Es, up)
+
+    def C_full(self, rho, u0=None):
+        """6x6 C^H (all six unit strains). Returns (C, u, cg iterations)."""
+        s = self.stiffness_scale(rho)
+        Es = np.eye(6)
+        u, it = self.solve(s.detach(), Es, u0)
+        return self.bilinear(s, Es, u), u, it
+
+    def _shear(self):
+        return 5 if self.plane else 3
+
+    def sym_cases(self):
+        return torch.as_tensor(np.stack([unit_strain(0), unit_strain(self._shear())]), device=self._dev)
+
+    @staticmethod
+    def swap_xy(u):
+        """Corrector of E_yy from the corrector of E_xx of a cell symmetric under x<->y."""
+        return torch.swapaxes(u, -4, -3)[..., [1, 0, 2]]
+
+    def invariants_from(self, s, u2):
+        """(C11, C12, C44) / E (C66 in 2D) from the xx and shear correctors u2."""
+        uyy = self.swap_xy(u2[0])
+        u3 = torch.stack([u2[0], uyy, u2[1]])
+        Es = np.stack([unit_strain(0), unit_strain(1), unit_strain(self._shear())])
+        C = self.bilinear(s, Es, u3)
+        return torch.stack([C[0, 0], C[1, 0], C[2, 2]]) / self.E
+
+    def invariants(self, rho, u0=None):
+        """Normalized invariants of a symmetric cell. Returns (c, u2, cg iterations); c is
+        differentiable in rho (correctors held fixed, which is exact)."""
+        s = self.stiffness_scale(rho)
+        u2, it = self.solve(s.detach(), self.sym_cases(), u0)
+        return self.invariants_from(s, u2.detach()), u2, it
+
+
+# --- host post-processing (nlfiber/linear.py:267-291) ------------------------------------------
+def cubic_invariants(C):
+    """(C11, C12, C44) as averages over the cubic-equivalent entries of a 6x6 C."""
+    C = np.asarray(C.detach().cpu() if isinstance(C, torch.Tensor) else C)
+    return np.array([np.mean([C[0, 0], C[1, 1], C[2, 2]]), np.mean([C[0, 1], C[0, 2], C[1, 2]]),
+                     np.mean([C[3, 3], C[4, 4], C[5, 5]])])
+
+
+def cubic_leakage(C):
+    """Relative distance of C from its cubic projection (full-tensor leakage)."""
+    C = np.asarray(C.detach().cpu() if isinstance(C, torch.Tensor) else C)
+    c11, c12, c44 = cubic_invariants(C)
+    Cc = np.zeros((6, 6))
+    Cc[:3, :3] = c12
+    Cc[[0, 1, 2], [0, 1, 2]] = c11
+    Cc[[3, 4, 5], [3, 4, 5]] = c44
+    return float(np.linalg.norm(C - Cc) / np.linalg.norm(C))
+
+
+def hashin_shtrikman(v, E=1.0, nu=0.3):
+    """HS upper bounds (K_HS, G_HS) of a porous isotropic solid at volume fraction v."""
+    lam, mu = lame(E, nu)
+    K = lam + 2 * mu / 3
+    K_hs = K + (1 - v) / (-1 / K + 3 * v / (3 * K + 4 * mu)) if v < 1 else K
+    G_hs = mu + (1 - v) / (-1 / mu + 6 * v * (K + 2 * mu) / (5 * mu * (3 * K + 4 * mu))) if v < 1 else mu
+    return K_hs, G_hs

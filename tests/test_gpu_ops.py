@@ -1,0 +1,133 @@
+"""Operator parity on the GPU: every C-ABI operator against the FP64 oracle on the same inputs.
+
+Tolerances: float64 results to 1e-12 relative to the largest entry (round-off of different
+summation orders); float32 operators to 2e-6 (matvec) and 1e-5 (preconditioner), relative to the
+largest entry.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import linear_ref as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(4, 4, 4), (3, 4, 5), (6, 5, 4), (8, 8, 1), (12, 10, 6), (16, 16, 16), (32, 24, 48), (20, 18, 64),
+          (8, 6, 128), (5, 9, 3)]
+
+
+def _rand(shape, seed, m=2):
+    rng = np.random.default_rng(seed)
+    s = rng.uniform(1e-3, 1.0, shape)
+    u = rng.standard_normal((m,) + shape + (3,))
+    return s, u
+
+
+def _rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_apply_K_fp64(gpu_pkg, shape):
+    s, u = _rand(shape, 1)
+    lh = gpu_pkg.LinearHomogenization(shape)
+    got = lh.apply_K(s, u).cpu().numpy()
+    ref = np.stack([O.LinearRef(shape).apply_K(s, x) for x in u])
+    assert _rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (32, 24, 48), (8, 6, 128)])
+def test_apply_K_fp32(gpu_pkg, shape):
+    import ctypes
+    s, u = _rand(shape, 2, m=3)
+    lh = gpu_pkg.LinearHomogenization(shape)
+    up = gpu_pkg.to_planar(u, shape).float().contiguous()
+    out = torch.empty_like(up)
+    st = torch.tensor(s, device="cuda")
+    lib = gpu_pkg.load_library()
+    rc = lib.nlfh_apply_K(lh._plan.handle, ctypes.c_void_p(st.data_ptr()), ctypes.c_void_p(up.data_ptr()),
+                          ctypes.c_void_p(out.data_ptr()), 3, 1, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    got = gpu_pkg.from_planar(out).double().cpu().numpy()
+    ref = np.stack([O.LinearRef(shape).apply_K(s, x) for x in u])
+    assert _rel(got, ref) < 2e-6
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (12, 10, 6)])
+def test_load_and_residual(gpu_pkg, shape):
+    s, u = _rand(shape, 3, m=6)
+    lh = gpu_pkg.LinearHomogenization(shape)
+    ref = O.LinearRef(shape)
+    Es = np.random.default_rng(4).standard_normal((6, 6))
+    assert _rel(lh.load(s, Es).cpu().numpy(), ref.load(s, Es)) < 1e-12
+    assert _rel(lh.residual(s, Es, u).cpu().numpy(), ref.residual(s, Es, u)) < 1e-12
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (16, 12, 10), (32, 32, 32), (24, 48, 20),
+                                   (64, 64, 64), (8, 8, 128)])
+def test_precond_fp64(gpu_pkg, shape):
+    rng = np.random.default_rng(5)
+    r = rng.standard_normal((2,) + shape + (3,))
+    lh = gpu_pkg.LinearHomogenization(shape)
+    got = lh.precond(r).cpu().numpy()
+    ref = O.LinearRef(shape).precond(r)
+    assert _rel(got, ref) < 1e-11
+
+
+@pytest.mark.parametrize("shape", [(16, 12, 10), (64, 64, 64)])
+def test_precond_fp32(gpu_pkg, shape):
+    import ctypes
+    rng = np.random.default_rng(6)
+    r = rng.standard_normal((2,) + shape + (3,))
+    lh = gpu_pkg.LinearHomogenization(shape)
+    rp = gpu_pkg.to_planar(r, shape).float().contiguous()
+    z = torch.empty_like(rp)
+    lib = gpu_pkg.load_library()
+    rc = lib.nlfh_precond(lh._plan.handle, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(z.data_ptr()), 2, 1,
+                          ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    got = gpu_pkg.from_planar(z).double().cpu().numpy()
+    ref = O.LinearRef(shape).precond(r)
+    assert _rel(got, ref) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (6, 5, 4), (8, 8, 1)])
+def test_symbol_matches_reference_construction(gpu_pkg, shape):
+    lh = gpu_pkg.LinearHomogenization(shape)
+    got = lh.Kinv.cpu().numpy()
+    ref = O.LinearRef(shape).Kinv
+    assert np.abs(ref.imag).max() < 1e-12 * np.abs(ref.real).max()
+    assert _rel(got.real, ref.real) < 1e-12
+
+
+def test_stiffness_scale_and_projection(gpu_pkg):
+    shape = (6, 5, 4)
+    rng = np.random.default_rng(7)
+    rho = rng.uniform(-0.2, 1.2, shape)
+    lh = gpu_pkg.LinearHomogenization(shape, emin=1e-4, p=3.0)
+    ref = O.LinearRef(shape, emin=1e-4, p=3.0)
+    assert _rel(lh.stiffness_scale(rho).cpu().numpy(), ref.stiffness_scale(rho)) < 1e-15
+    r = torch.tensor(rho, device="cuda", requires_grad=True)
+    s = lh.stiffness_scale(r)
+    s.sum().backward()
+    assert _rel(r.grad.cpu().numpy(), ref.dstiffness_drho(rho)) < 1e-14
+    v = gpu_pkg.to_planar(rng.standard_normal((3,) + shape + (3,)) + 5.0, shape)
+    vref = gpu_pkg.from_planar(v).cpu().numpy().copy()
+    lh._project_raw(v)
+    assert _rel(gpu_pkg.from_planar(v).cpu().numpy(), O.proj(vref)) < 1e-13
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (16, 12, 10)])
+def test_bilinear_and_element_energy(gpu_pkg, shape):
+    s, u = _rand(shape, 8, m=6)
+    lh = gpu_pkg.LinearHomogenization(shape)
+    ref = O.LinearRef(shape)
+    Es = np.eye(6)
+    C = np.asarray(lh.bilinear(s, Es, u).detach())
+    assert _rel(C, ref.bilinear(s, Es, u)) < 1e-12
+    st = torch.tensor(s, device="cuda")
+    _, dcds = lh._bilinear_raw(st, Es, gpu_pkg.to_planar(u, shape), True)
+    b = ref.element_energy(s, Es, u) / np.prod(shape)
+    assert _rel(dcds.cpu().numpy(), b) < 1e-12

@@ -1,0 +1,181 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each check is independent of the formula it checks: an exact tensor-product assembly of Ke, the
+rigid-body null space, brute-force dense assembly of K, a dense direct solve, closed forms
+(uniform solid, laminate), bounds (Reuss/Voigt, Hashin-Shtrikman), symmetry and scaling laws, the
+preconditioner as the exact inverse of K(1), central differences for the two derivatives, and the
+reference's own test properties (tests/test_core.py:77-142 of the reference). A planted mistake in
+the oracle (a dropped term, a wrong sign or index, a transposed operand) fails at least one.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import linear_ref as O
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_Ke_equals_tensor_product_assembly_and_null_space():
+    for E, nu in ((1.0, 0.3), (2.0, 0.25), (0.7, 0.45)):
+        Ke, G, D = O.element_matrices(E, nu)
+        Kt = O.Ke_tensor_product(E, nu)
+        assert np.abs(Ke - Kt).max() < 1e-14 * np.abs(Kt).max()
+        w = np.linalg.eigvalsh(Ke)
+        assert (w > 1e-10).sum() == 18  # 6 rigid modes
+        X = O.NODES.astype(float)
+        for c in range(3):  # translations
+            t = np.zeros(24)
+            t[c::3] = 1.0
+            assert np.abs(Ke @ t).max() < 1e-14
+        for a, b in ((0, 1), (0, 2), (1, 2)):  # infinitesimal rotations
+            r = np.zeros(24)
+            r[a::3] = -X[:, b]
+            r[b::3] = X[:, a]
+            assert np.abs(Ke @ r).max() < 1e-14
+        # G = D times the mean strain-displacement operator: each row sums to zero over nodes
+        assert np.abs(G.reshape(6, 8, 3).sum(axis=1)).max() < 1e-15
+
+
+def test_apply_K_equals_dense_assembly_and_load():
+    rng = np.random.default_rng(0)
+    for shape in ((3, 4, 5), (2, 3, 2), (4, 4, 1)):
+        s = rng.uniform(0.1, 1.0, shape)
+        lr = O.LinearRef(shape)
+        K = O.dense_K(shape, s, O.Ke_tensor_product())
+        u = rng.standard_normal(shape + (3,))
+        assert np.abs(lr.apply_K(s, u).ravel() - K @ u.ravel()).max() < 1e-13
+        assert np.abs(K - K.T).max() < 1e-14
+        assert np.abs(lr.apply_K(s, np.ones(shape + (3,)))).max() < 1e-13  # K const = 0
+        f = lr.load(s, np.eye(6))
+        assert np.abs(f.sum(axis=(1, 2, 3))).max() < 1e-13  # loads are mean-free
+
+
+def test_uniform_solid_is_isotropic_material():
+    C, _, it = O.LinearRef((4, 4, 4), E=2.0, nu=0.25).C_full(np.ones((4, 4, 4)))
+    np.testing.assert_allclose(C, O.iso_D(2.0, 0.25), atol=1e-12)
+    assert it == 0
+
+
+def test_precond_is_exact_inverse_of_reference_medium():
+    rng = np.random.default_rng(1)
+    for shape in ((4, 6, 5), (8, 8, 8), (6, 6, 1)):
+        lr = O.LinearRef(shape)
+        assert np.abs(lr.Kinv.imag).max() < 1e-13 * np.abs(lr.Kinv.real).max()
+        assert np.abs(lr.Kinv - np.swapaxes(lr.Kinv, -1, -2)).max() < 1e-13
+        u = O.proj(rng.standard_normal((1,) + shape + (3,)))
+        Ku = np.stack([lr.apply_K(np.ones(shape), u[0])])
+        assert np.abs(lr.precond(Ku) - u).max() < 1e-9 * np.abs(u).max()
+    # PCG converges in one iteration for a constant stiffness
+    lr = O.LinearRef((6, 6, 6), tol=1e-12)
+    _, _, it = lr.C_full(np.full((6, 6, 6), 0.7))
+    assert it <= 1
+
+
+def test_laminate_closed_form_and_dense_solve():
+    rng = np.random.default_rng(2)
+    sx = rng.uniform(0.05, 1.0, 6)
+    s = np.broadcast_to(sx[:, None, None], (6, 3, 4)).copy()
+    lr = O.LinearRef(s.shape, emin=0.0, p=1.0, tol=1e-13)
+    u, _ = lr.solve(s, np.eye(6))
+    assert np.abs(lr.bilinear(s, np.eye(6), u) - O.laminate_C(sx)).max() < 1e-12
+    s = rng.uniform(0.01, 1.0, (3, 4, 3))
+    lr = O.LinearRef(s.shape, emin=0.0, p=1.0, tol=1e-13)
+    u, _ = lr.solve(s, np.eye(6))
+    C = lr.bilinear(s, np.eye(6), u)
+    Cd, _ = O.dense_C(s.shape, s)
+    assert np.abs(C - Cd).max() < 1e-12
+
+
+def test_bounds_symmetry_scaling_and_shift():
+    rng = np.random.default_rng(3)
+    s = rng.uniform(0.05, 1.0, (5, 4, 6))
+    lr = O.LinearRef(s.shape, emin=0.0, p=1.0, tol=1e-12)
+    C = lr.bilinear(s, np.eye(6), lr.solve(s, np.eye(6))[0])
+    D = O.iso_D(1.0, 0.3)
+    np.testing.assert_allclose(C, C.T, atol=1e-12)
+    assert np.linalg.eigvalsh(np.mean(s) * D - C).min() > -1e-12          # Voigt
+    assert np.linalg.eigvalsh(C - D / np.mean(1.0 / s)).min() > -1e-12     # Reuss
+    C2 = lr.bilinear(2.5 * s, np.eye(6), lr.solve(2.5 * s, np.eye(6))[0])
+    assert np.abs(C2 - 2.5 * C).max() < 1e-11
+    ss = np.roll(s, (2, 1, 3), axis=(0, 1, 2))
+    C3 = lr.bilinear(ss, np.eye(6), lr.solve(ss, np.eye(6))[0])
+    assert np.abs(C3 - C).max() < 1e-11
+    # axis permutation x <-> y swaps the Voigt indices accordingly
+    st = np.transpose(s, (1, 0, 2))
+    lt = O.LinearRef(st.shape, emin=0.0, p=1.0, tol=1e-12)
+    Ct = lt.bilinear(st, np.eye(6), lt.solve(st, np.eye(6))[0])
+    P = [1, 0, 2, 4, 3, 5]
+    assert np.abs(Ct - C[np.ix_(P, P)]).max() < 1e-11
+
+
+def test_cubic_cell_properties_and_hs_bound():
+    g = np.load(os.path.join(GOLDEN, "grey8.npz"))
+    lr = O.LinearRef(g["rho"].shape, emin=float(g["emin"]), tol=1e-11)
+    C, u, it = lr.C_full(g["rho"])
+    assert np.abs(C - g["C"]).max() < 1e-12 * np.abs(C).max()
+    assert O.cubic_leakage(C) < 1e-8
+    c, u2, _ = lr.invariants(g["rho"])
+    np.testing.assert_allclose(c, O.cubic_invariants(C), rtol=1e-8)
+    b = (g["rho"] > 0.5).astype(float)
+    Cb, _, _ = O.LinearRef(b.shape, tol=1e-10).C_full(b)
+    c11, c12, c44 = O.cubic_invariants(Cb)
+    Khs, Ghs = O.hashin_shtrikman(b.mean())
+    assert (c11 + 2 * c12) / 3 < Khs and c44 < Ghs
+
+
+def test_golden_operator_outputs_reproduce():
+    g = np.load(os.path.join(GOLDEN, "grey8.npz"))
+    lr = O.LinearRef(g["rho"].shape, emin=float(g["emin"]))
+    s = lr.stiffness_scale(g["rho"])
+    Kw = np.stack([lr.apply_K(s, x) for x in g["w"]])
+    assert np.abs(Kw - g["Kw"]).max() < 1e-13 * np.abs(g["Kw"]).max()
+    assert np.abs(lr.precond(g["w"]) - g["Mw"]).max() < 1e-13 * np.abs(g["Mw"]).max()
+
+
+def test_invariants_gradient_fd():
+    g = np.load(os.path.join(GOLDEN, "grey8.npz"))
+    rho = g["rho"]
+    lr = O.LinearRef(rho.shape, tol=1e-12)
+    c, u2, _ = lr.invariants(rho)
+    J = lr.invariants_grad(rho, u2)
+    d = np.random.default_rng(5).standard_normal(rho.shape)
+    h = 1e-5
+    fd = (lr.invariants(rho + h * d)[0] - lr.invariants(rho - h * d)[0]) / (2 * h)
+    np.testing.assert_allclose(np.tensordot(J, d, 3), fd, rtol=1e-5, atol=1e-10)
+
+
+def test_corrector_vjp_fd():
+    g = np.load(os.path.join(GOLDEN, "grey8.npz"))
+    lr = O.LinearRef(g["rho"].shape, tol=1e-12)
+    s0 = lr.stiffness_scale(g["rho"])
+    Es = np.eye(6)[:2]
+    f = lambda s: float(np.sum(lr.solve(s, Es)[0] ** 2))  # noqa: E731
+    u, _ = lr.solve(s0, Es)
+    gr = lr.correctors_vjp(s0, Es, u, 2 * u)
+    d = np.random.default_rng(6).standard_normal(s0.shape)
+    h = 1e-6
+    fd = (f(s0 + h * d) - f(s0 - h * d)) / (2 * h)
+    assert abs(float(np.vdot(gr, d)) - fd) < 1e-4 * abs(fd)
+
+
+def test_float32_recursion_option_converges():
+    g = np.load(os.path.join(GOLDEN, "grey8.npz"))
+    lr = O.LinearRef(g["rho"].shape, tol=1e-5)
+    s = lr.stiffness_scale(g["rho"])
+    b = -lr.load(s, np.eye(6))
+    x, it = lr.cg(s, b, np.zeros_like(b), dtype=np.float32)
+    r = b - np.stack([lr.apply_K(s, xi) for xi in x])
+    assert np.sqrt((r ** 2).sum(axis=(1, 2, 3, 4))).max() / np.sqrt((b ** 2).sum(axis=(1, 2, 3, 4))).min() < 1e-4
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_cell_generator_volume_and_symmetry(pkg, n):
+    rho, meta = pkg.cells.random_cell(n, seed=n)
+    assert abs(rho.mean() - meta["v"]) < 1e-12
+    for p in ((1, 0, 2), (2, 1, 0), (0, 2, 1)):
+        assert np.array_equal(rho, np.transpose(rho, p))
+    assert np.array_equal(rho, rho[::-1]) and np.array_equal(rho, rho[:, ::-1]) and np.array_equal(rho, rho[:, :, ::-1])
+    r0 = pkg.cells.schwarz_cell(n)
+    assert abs(r0.mean() - 0.3) < 1e-12
