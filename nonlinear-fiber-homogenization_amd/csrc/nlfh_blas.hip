@@ -134,31 +134,42 @@ __global__ void ir_accumulate_kernel(const VecArgs va, double* __restrict__ x64,
 
 // PCG scalar recurrences (nlfiber/linear.py:173-203), one block per cell, thread t = rhs.
 
-__device__ __forceinline__ double sum_range(const double* part, int nitems, int row, const int* range, int cell) {
+// Sum of part[row][range(cell)] by one wavefront: lane-strided partial sums, then a fixed
+// butterfly, so the result is deterministic; every lane returns the total.
+__device__ __forceinline__ double wave_sum(const double* part, int nitems, int row, const int* range, int cell) {
+  const int lane = threadIdx.x & 63;
   double acc = 0.0;
-  for (int w = range[2 * cell]; w < range[2 * cell + 1]; ++w) acc += part[(int64_t)row * nitems + w];
+  for (int w = range[2 * cell] + lane; w < range[2 * cell + 1]; w += 64) acc += part[(int64_t)row * nitems + w];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   return acc;
 }
+
+// The scalar kernels below run one block per cell and one wavefront per right-hand side
+// (blockDim = 64 m).
 
 // After the first preconditioner application: bn = ||b|| + 1e-300 (from the chunk partials of
 // b), rn (from the ||r||^2 partials of pass A), rz; decide `done` (nlfiber/linear.py:176-188).
 __global__ void cg_init_kernel(CgState st, const double* part_a, const double* part_c, int nplanes,
                                const int* prange, const double* part_b, int nchunks, const int* crange) {
   __shared__ int notconv;
-  const int cell = blockIdx.x, t = threadIdx.x;
-  if (t == 0) notconv = 0;
+  const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) notconv = 0;
   __syncthreads();
   if (t < st.m) {
     const int k = cell * st.m + t;
-    st.bn[k] = sqrt(sum_range(part_b, nchunks, t, crange, cell)) + 1e-300;
-    const double rn = sqrt(sum_range(part_a, nplanes, t, prange, cell));
-    st.rn[k] = rn;
-    st.rz[k] = sum_range(part_c, nplanes, t, prange, cell);
-    st.beta[k] = 0.0;
-    if (rn > st.tol[0] * st.bn[k]) atomicAdd(&notconv, 1);
+    const double bn = sqrt(wave_sum(part_b, nchunks, t, crange, cell)) + 1e-300;
+    const double rn = sqrt(wave_sum(part_a, nplanes, t, prange, cell));
+    const double rz = wave_sum(part_c, nplanes, t, prange, cell);
+    if (lane == 0) {
+      st.bn[k] = bn;
+      st.rn[k] = rn;
+      st.rz[k] = rz;
+      st.beta[k] = 0.0;
+      if (rn > st.tol[0] * bn) atomicAdd(&notconv, 1);
+    }
   }
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     st.it[cell] = 0;
     const int skip = st.skip != nullptr ? st.skip[cell] : 0;
     st.done[cell] = (skip || notconv == 0 || st.maxit[0] <= 0) ? 1 : 0;
@@ -167,11 +178,11 @@ __global__ void cg_init_kernel(CgState st, const double* part_a, const double* p
 
 // alpha = rz / pAp (guard pAp == 0)
 __global__ void cg_alpha_kernel(CgState st, const double* part_mv, int nmv, const int* mrange) {
-  const int cell = blockIdx.x, t = threadIdx.x;
+  const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (st.done[cell] || t >= st.m) return;
   const int k = cell * st.m + t;
-  const double pAp = sum_range(part_mv, nmv, t, mrange, cell);
-  st.alpha[k] = st.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
+  const double pAp = wave_sum(part_mv, nmv, t, mrange, cell);
+  if (lane == 0) st.alpha[k] = st.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
 }
 
 // beta = rz1 / rz (guard), rz = rz1, rn = ||r||, it += 1, done when every rhs has rn <= tol*bn or
@@ -179,22 +190,24 @@ __global__ void cg_alpha_kernel(CgState st, const double* part_mv, int nmv, cons
 __global__ void cg_beta_kernel(CgState st, const double* part_a, const double* part_c, int nplanes,
                                const int* prange) {
   __shared__ int notconv;
-  const int cell = blockIdx.x, t = threadIdx.x;
+  const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (st.done[cell]) return;
-  if (t == 0) notconv = 0;
+  if (threadIdx.x == 0) notconv = 0;
   __syncthreads();
   if (t < st.m) {
     const int k = cell * st.m + t;
-    const double rn = sqrt(sum_range(part_a, nplanes, t, prange, cell));
-    const double rz1 = sum_range(part_c, nplanes, t, prange, cell);
-    const double rz = st.rz[k];
-    st.beta[k] = rz1 / (rz == 0.0 ? 1.0 : rz);
-    st.rz[k] = rz1;
-    st.rn[k] = rn;
-    if (rn > st.tol[0] * st.bn[k]) atomicAdd(&notconv, 1);
+    const double rn = sqrt(wave_sum(part_a, nplanes, t, prange, cell));
+    const double rz1 = wave_sum(part_c, nplanes, t, prange, cell);
+    if (lane == 0) {
+      const double rz = st.rz[k];
+      st.beta[k] = rz1 / (rz == 0.0 ? 1.0 : rz);
+      st.rz[k] = rz1;
+      st.rn[k] = rn;
+      if (rn > st.tol[0] * st.bn[k]) atomicAdd(&notconv, 1);
+    }
   }
   __syncthreads();
-  if (t == 0) {
+  if (threadIdx.x == 0) {
     const int it = st.it[cell] + 1;
     st.it[cell] = it;
     if (notconv == 0 || it >= st.maxit[0]) st.done[cell] = 1;
@@ -206,17 +219,19 @@ __global__ void cg_beta_kernel(CgState st, const double* part_a, const double* p
 __global__ void norms_kernel(const double* part, int nchunks, const int* crange, int m, int M, double eps,
                              double* out, const double* bn, double tol, int* flag, const int* force) {
   __shared__ int notconv;
-  const int cell = blockIdx.x, t = threadIdx.x;
-  if (t == 0) notconv = 0;
+  const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (threadIdx.x == 0) notconv = 0;
   __syncthreads();
   if (t < m) {
     const int k = cell * M + t;
-    const double v = sqrt(sum_range(part, nchunks, t, crange, cell)) + eps;
-    out[k] = v;
-    if (bn != nullptr && v > tol * bn[k]) atomicAdd(&notconv, 1);
+    const double v = sqrt(wave_sum(part, nchunks, t, crange, cell)) + eps;
+    if (lane == 0) {
+      out[k] = v;
+      if (bn != nullptr && v > tol * bn[k]) atomicAdd(&notconv, 1);
+    }
   }
   __syncthreads();
-  if (t == 0 && flag != nullptr) flag[cell] = (notconv == 0 || (force != nullptr && force[cell])) ? 1 : 0;
+  if (threadIdx.x == 0 && flag != nullptr) flag[cell] = (notconv == 0 || (force != nullptr && force[cell])) ? 1 : 0;
 }
 
 // Homogenized stiffness (nlfiber/linear.py:220-229) in the Walsh-mode form: per element and
@@ -400,22 +415,22 @@ hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const
 
 hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
                           const double* pb, int nch, const int* cr, hipStream_t st) {
-  hipLaunchKernelGGL(cg_init_kernel, dim3(ncells), dim3(64), 0, st, cs, pa, pc, np, pr, pb, nch, cr);
+  hipLaunchKernelGGL(cg_init_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, pc, np, pr, pb, nch, cr);
   return hipGetLastError();
 }
 hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int nmv, const int* mr, hipStream_t st) {
-  hipLaunchKernelGGL(cg_alpha_kernel, dim3(ncells), dim3(64), 0, st, cs, pmv, nmv, mr);
+  hipLaunchKernelGGL(cg_alpha_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pmv, nmv, mr);
   return hipGetLastError();
 }
 hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
                           hipStream_t st) {
-  hipLaunchKernelGGL(cg_beta_kernel, dim3(ncells), dim3(64), 0, st, cs, pa, pc, np, pr);
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, pc, np, pr);
   return hipGetLastError();
 }
 
 hipError_t launch_norms(const double* part, int nchunks, const int* crange, int ncells, int m, int M, double eps,
                         double* out, const double* bn, double tol, int* flag, const int* force, hipStream_t st) {
-  hipLaunchKernelGGL(norms_kernel, dim3(ncells), dim3(64), 0, st, part, nchunks, crange, m, M, eps, out, bn, tol, flag,
+  hipLaunchKernelGGL(norms_kernel, dim3(ncells), dim3(64 * m), 0, st, part, nchunks, crange, m, M, eps, out, bn, tol, flag,
                      force);
   return hipGetLastError();
 }

@@ -51,6 +51,15 @@ struct MvGroup {
   int max_ty;
 };
 
+// Preconditioner launch group: the cells of one shape (fast register-FFT path for cubic cells of
+// side 16, 32, 48, 64 or 128, generic path otherwise).
+struct PrecGroup {
+  int shape;
+  int fast;  // side N of the fast path, 0 for generic
+  int plane_begin, plane_count;
+  int line_begin[2], line_count[2];  // [0] float64 line items, [1] float32
+};
+
 template <typename T>
 struct Work {
   T* X = nullptr;
@@ -119,6 +128,10 @@ struct nlfh_plan {
   std::vector<MvGroup> groups;
   std::vector<PlaneItem> planes;
   std::vector<LineItem> lines64, lines32;
+  std::vector<PrecGroup> pgroups;
+  double* d_symsoa64 = nullptr;  // fast shapes: component-planar symbol [6][nx][ny][nzh] / N
+  float* d_symsoa32 = nullptr;
+  std::vector<int64_t> symsoa_off;  // per shape (fast shapes only)
   std::vector<ChunkItem> chunks;
   int rpb64 = 1, rpb32 = 1;
   size_t lds_plane64 = 0, lds_plane32 = 0, lds_line64 = 0, lds_line32 = 0;
@@ -351,22 +364,10 @@ size_t line_len_for(int nx, int64_t npos) {
 }
 
 template <typename T>
-void build_lines(nlfh_plan* p, std::vector<LineItem>& lines, size_t& lds_line) {
-  lines.clear();
-  lds_line = 0;
-  for (int c = 0; c < p->ncells; ++c) {
-    const CellDesc& cd = p->cells[c];
-    const int64_t npos = (int64_t)cd.ny * cd.nzh;
-    const int len = (int)line_len_for<T>(cd.nx, npos);
-    for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
-    lds_line = std::max(lds_line, (size_t)3 * cd.nx * len * sizeof(cplx<T>));
-  }
-}
-
-template <typename T>
 void plane_lds(nlfh_plan* p, int& rpb, size_t& lds) {
   rpb = 32;
   for (int c = 0; c < p->ncells; ++c) {
+    if (p->pgroups[p->cell_shape[c]].fast) continue;
     const CellDesc& cd = p->cells[c];
     const size_t plane = (size_t)cd.ny * cd.nzh * sizeof(cplx<T>);
     const size_t row = (size_t)cd.nz * sizeof(cplx<T>);
@@ -375,6 +376,7 @@ void plane_lds(nlfh_plan* p, int& rpb, size_t& lds) {
   }
   lds = 0;
   for (int c = 0; c < p->ncells; ++c) {
+    if (p->pgroups[p->cell_shape[c]].fast) continue;
     const CellDesc& cd = p->cells[c];
     const int npairs = (cd.ny + 1) / 2;
     lds = std::max(lds, prec_plane_lds<T>(cd.ny, cd.nz, std::min(rpb, npairs)));
@@ -493,10 +495,30 @@ PrecArgs<T> prec_args(nlfh_plan* p, int m) {
 }
 
 template <typename T>
+const T* symsoa_ptr(nlfh_plan* p);
+template <>
+const double* symsoa_ptr<double>(nlfh_plan* p) { return p->d_symsoa64; }
+template <>
+const float* symsoa_ptr<float>(nlfh_plan* p) { return p->d_symsoa32; }
+
+template <typename T>
 int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) {
   const size_t lp = sizeof(T) == 8 ? p->lds_plane64 : p->lds_plane32;
   const size_t ll = sizeof(T) == 8 ? p->lds_line64 : p->lds_line32;
-  HIP_TRY(launch_prec<T>(a, update, dot, lp, ll, st));
+  const int pr = sizeof(T) == 8 ? 0 : 1;
+  for (const PrecGroup& g : p->pgroups) {
+    if (g.plane_count == 0) continue;
+    a.plane_base = g.plane_begin;
+    a.line_base = g.line_begin[pr];
+    if (g.fast) {
+      const Shape& sh = p->shapes[g.shape];
+      const cplx<T>* tw = tw_ptr<T>(p) + p->lens[sh.len[0]].tw_off;
+      const T* symp = symsoa_ptr<T>(p) + p->symsoa_off[g.shape];
+      HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, st));
+    } else {
+      HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update, dot, lp, ll, st));
+    }
+  }
   return NLFH_OK;
 }
 
@@ -789,25 +811,74 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
     shape_len.push_back(sh.len[1]);
     shape_len.push_back(sh.len[2]);
   }
-  // plane, line, chunk items (cell-major)
-  std::vector<int> prange, crange;
+  // plane and line items grouped by shape (a cell's items stay contiguous), chunk items per cell
+  std::vector<int> prange(2 * ncells), crange;
+  for (int sidx = 0; sidx < (int)p->shapes.size(); ++sidx) {
+    const Shape& sh = p->shapes[sidx];
+    PrecGroup g{};
+    g.shape = sidx;
+    g.fast = (sh.nx == sh.ny && sh.ny == sh.nz && fast_fft_size(sh.nx)) ? sh.nx : 0;
+    g.plane_begin = (int)p->planes.size();
+    g.line_begin[0] = (int)p->lines64.size();
+    g.line_begin[1] = (int)p->lines32.size();
+    for (int c = 0; c < ncells; ++c) {
+      if (p->cell_shape[c] != sidx) continue;
+      const CellDesc& cd = p->cells[c];
+      prange[2 * c] = (int)p->planes.size();
+      for (int comp = 0; comp < 3; ++comp)
+        for (int x = 0; x < cd.nx; ++x) p->planes.push_back({c, comp, x, 0});
+      prange[2 * c + 1] = (int)p->planes.size();
+      const int64_t npos = (int64_t)cd.ny * cd.nzh;
+      for (int prec = 0; prec < 2; ++prec) {
+        const int len = g.fast ? fast_line_len(g.fast, prec == 0 ? 8 : 4)
+                               : (int)(prec == 0 ? line_len_for<double>(cd.nx, npos) : line_len_for<float>(cd.nx, npos));
+        std::vector<LineItem>& lines = prec == 0 ? p->lines64 : p->lines32;
+        for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
+      }
+    }
+    g.plane_count = (int)p->planes.size() - g.plane_begin;
+    g.line_count[0] = (int)p->lines64.size() - g.line_begin[0];
+    g.line_count[1] = (int)p->lines32.size() - g.line_begin[1];
+    p->pgroups.push_back(g);
+  }
   for (int c = 0; c < ncells; ++c) {
     const CellDesc& cd = p->cells[c];
-    prange.push_back((int)p->planes.size());
-    for (int comp = 0; comp < 3; ++comp)
-      for (int x = 0; x < cd.nx; ++x) p->planes.push_back({c, comp, x, 0});
-    prange.push_back((int)p->planes.size());
     crange.push_back((int)p->chunks.size());
     for (int64_t b = 0; b < cd.nvox; b += kChunk) p->chunks.push_back({c, 0, b, std::min<int64_t>(b + kChunk, cd.nvox)});
     crange.push_back((int)p->chunks.size());
   }
-  build_lines<double>(p, p->lines64, p->lds_line64);
-  build_lines<float>(p, p->lines32, p->lds_line32);
+  // LDS of the generic passes (over the cells that use them)
+  p->lds_line64 = p->lds_line32 = 0;
+  for (int c = 0; c < ncells; ++c) {
+    const PrecGroup& g = p->pgroups[p->cell_shape[c]];
+    if (g.fast) continue;
+    const CellDesc& cd = p->cells[c];
+    const int64_t npos = (int64_t)cd.ny * cd.nzh;
+    p->lds_line64 = std::max(p->lds_line64, (size_t)3 * cd.nx * line_len_for<double>(cd.nx, npos) * sizeof(cplx<double>));
+    p->lds_line32 = std::max(p->lds_line32, (size_t)3 * cd.nx * line_len_for<float>(cd.nx, npos) * sizeof(cplx<float>));
+  }
   plane_lds<double>(p, p->rpb64, p->lds_plane64);
   plane_lds<float>(p, p->rpb32, p->lds_plane32);
   if (p->lds_plane64 > kLdsMax || p->lds_plane32 > kLdsMax) {
     set_error("a y-z plane of the cell does not fit in LDS (ny * (nz/2+1) too large)");
     return fail(NLFH_EUNSUPPORTED);
+  }
+  // component-planar symbol copies of the fast shapes
+  std::vector<double> soa64;
+  std::vector<float> soa32;
+  p->symsoa_off.assign(p->shapes.size(), -1);
+  for (const PrecGroup& g : p->pgroups) {
+    if (!g.fast) continue;
+    const Shape& sh = p->shapes[g.shape];
+    const int64_t nf = (int64_t)sh.nx * sh.ny * (sh.nz / 2 + 1);
+    p->symsoa_off[g.shape] = (int64_t)soa64.size();
+    const double inv = 1.0 / ((double)sh.nx * sh.ny * sh.nz);
+    for (int e = 0; e < 6; ++e)
+      for (int64_t f = 0; f < nf; ++f) {
+        const double v = sym64[(sh.off_sym + f) * 6 + e] * inv;
+        soa64.push_back(v);
+        soa32.push_back((float)v);
+      }
   }
   int rc = NLFH_OK;
   if ((rc = p->upload(&p->d_cells, p->cells))) return fail(rc);
@@ -819,6 +890,8 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_shape_len, shape_len))) return fail(rc);
   if ((rc = p->upload(&p->d_sym64, symd))) return fail(rc);
   if ((rc = p->upload(&p->d_sym32, symf))) return fail(rc);
+  if ((rc = p->upload(&p->d_symsoa64, soa64))) return fail(rc);
+  if ((rc = p->upload(&p->d_symsoa32, soa32))) return fail(rc);
   if ((rc = p->upload(&p->d_planes, p->planes))) return fail(rc);
   if ((rc = p->upload(&p->d_lines64, p->lines64))) return fail(rc);
   if ((rc = p->upload(&p->d_lines32, p->lines32))) return fail(rc);

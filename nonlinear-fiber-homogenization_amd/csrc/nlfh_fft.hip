@@ -125,7 +125,8 @@ template <typename T, bool UPDATE>
 __global__ void __launch_bounds__(256) prec_pass_a(const PrecArgs<T> a) {
   extern __shared__ double smem_d[];
   __shared__ double red[4];
-  const PlaneItem pi = a.planes[blockIdx.x];
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
   if (a.done != nullptr && a.done[pi.cell]) return;
   const CellDesc cd = a.cells[pi.cell];
   const int rhs = blockIdx.y;
@@ -199,14 +200,14 @@ __global__ void __launch_bounds__(256) prec_pass_a(const PrecArgs<T> a) {
     so[(int64_t)py[p] * nzh + k] = plane[w];
   }
   const double t = block_sum(acc, red);
-  if (threadIdx.x == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + blockIdx.x] = t;
+  if (threadIdx.x == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = t;
 }
 
 // Pass B ---------------------------------------------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(256) prec_pass_b(const PrecArgs<T> a) {
   extern __shared__ double smem_d[];
-  const LineItem li = a.lines[blockIdx.x];
+  const LineItem li = a.lines[a.line_base + blockIdx.x];
   if (a.done != nullptr && a.done[li.cell]) return;
   const CellDesc cd = a.cells[li.cell];
   const int rhs = blockIdx.y;
@@ -257,7 +258,8 @@ template <typename T, bool DOT>
 __global__ void __launch_bounds__(256) prec_pass_c(const PrecArgs<T> a) {
   extern __shared__ double smem_d[];
   __shared__ double red[4];
-  const PlaneItem pi = a.planes[blockIdx.x];
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
   if (a.done != nullptr && a.done[pi.cell]) return;
   const CellDesc cd = a.cells[pi.cell];
   const int rhs = blockIdx.y;
@@ -326,7 +328,7 @@ __global__ void __launch_bounds__(256) prec_pass_c(const PrecArgs<T> a) {
   }
   if (DOT) {
     const double t = block_sum(acc, red);
-    if (threadIdx.x == 0) a.part_c[(int64_t)rhs * a.nplanes + blockIdx.x] = t;
+    if (threadIdx.x == 0) a.part_c[(int64_t)rhs * a.nplanes + item] = t;
   }
 }
 
@@ -337,9 +339,9 @@ size_t prec_plane_lds(int ny, int nz, int rows_per_batch) {
 }
 
 template <typename T>
-hipError_t launch_prec(const PrecArgs<T>& a, int update, int dot, size_t lds_plane, size_t lds_line,
-                       hipStream_t st) {
-  dim3 gp(a.nplanes, a.m, 1), gl(a.nlines, a.m, 1);
+hipError_t launch_prec(const PrecArgs<T>& a, int nplanes, int nlines, int update, int dot, size_t lds_plane,
+                       size_t lds_line, hipStream_t st) {
+  dim3 gp(nplanes, a.m, 1), gl(nlines, a.m, 1);
   if (lds_plane > 65536) {
     (void)hipFuncSetAttribute((const void*)prec_pass_a<T, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
     (void)hipFuncSetAttribute((const void*)prec_pass_a<T, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_plane);
@@ -358,7 +360,7 @@ hipError_t launch_prec(const PrecArgs<T>& a, int update, int dot, size_t lds_pla
 
 template size_t prec_plane_lds<double>(int, int, int);
 template size_t prec_plane_lds<float>(int, int, int);
-template hipError_t launch_prec<double>(const PrecArgs<double>&, int, int, size_t, size_t, hipStream_t);
-template hipError_t launch_prec<float>(const PrecArgs<float>&, int, int, size_t, size_t, hipStream_t);
+template hipError_t launch_prec<double>(const PrecArgs<double>&, int, int, int, int, size_t, size_t, hipStream_t);
+template hipError_t launch_prec<float>(const PrecArgs<float>&, int, int, int, int, size_t, size_t, hipStream_t);
 
 }  // namespace nlfh

@@ -1,0 +1,448 @@
+// Preconditioner passes for cubic cells of side N in {16, 32, 48, 64, 128} with register FFTs
+// (nlfiber/linear.py:167-170). Same three passes and data layout as the generic kernels of
+// nlfh_fft.hip; here every line transform is split over L threads: step 1 Dft<N1> in registers
+// per thread, a padded LDS exchange, step 2 Dft<N2> in registers (N = N1 N2, LineSplit<N>).
+//   A: per (cell, rhs, component, x): z real-to-complex (two rows per complex line), y complex
+//      (the real DC and Nyquist columns packed into one complex line), store the half plane;
+//      optionally fused with r -= alpha Ap, x += alpha p and ||r||^2.
+//   B: per (cell, rhs, run of LPB (ky, kz) positions): x forward for the three components in
+//      registers, z = Kinv r with the real symmetric symbol (component-planar copy), x inverse.
+//   C: per (cell, rhs, component, x): y inverse, z complex-to-real, store z, optional r . z.
+#include "nlfh_fft_codelets.h"
+#include "nlfh_kernels.h"
+
+namespace nlfh {
+
+using fft::Dft;
+using fft::LineSplit;
+
+template <typename T, int N>
+struct FastCfg {
+  static constexpr int N1 = LineSplit<N>::N1, N2 = LineSplit<N>::N2;
+  static constexpr int L = N1 > N2 ? N1 : N2;
+  static constexpr int NH = N / 2 + 1;
+  static constexpr int NP = N / 2;
+  static constexpr int BT = (N == 128 && sizeof(T) == 8) ? 128 : 256;  // threads of passes A, C
+  static constexpr int LPR = BT / L;                                   // lines per round
+  static constexpr int XS = N1 * (N2 + 1) > N2 * (N1 + 1) ? N1 * (N2 + 1) : N2 * (N1 + 1);
+  static constexpr int LPB = (N == 128 ? 8 : 16) * (sizeof(T) == 8 ? 1 : 2);  // pass B positions per block
+  static constexpr int BTB = L * LPB;                                          // threads of pass B
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmulw(cplx<T> v, cplx<T> w, bool conj_w) {
+  const T wi = conj_w ? -w.im : w.im;
+  return {v.re * w.re - v.im * wi, v.re * wi + v.im * w.re};
+}
+
+// Pass A ---------------------------------------------------------------------------------------
+template <typename T, int N, bool UPDATE>
+__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  using C = FastCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, LPR = C::LPR, BT = C::BT;
+  extern __shared__ double smem_d[];
+  __shared__ double red[4];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);  // [N]: e^{-2 pi i m / N}
+  cplx<T>* plane = tw + N;                            // [N][NH]
+  cplx<T>* xch = plane + N * NH;                      // [LPR][XS]
+  cplx<T>* dcn = xch + LPR * C::XS;                   // [N]: packed DC / Nyquist column transform
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
+  if (a.done != nullptr && a.done[pi.cell]) return;
+  const CellDesc cd = a.cells[pi.cell];
+  const int rhs = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  const int64_t nvox = cd.nvox;
+  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * nvox + (int64_t)pi.x * N * N;
+  const T* __restrict__ R = a.r;
+  T* __restrict__ RO = a.r_out;
+  const T* __restrict__ AP = a.Ap;
+  T* __restrict__ X = a.x;
+  const T* __restrict__ P = a.p;
+  T alpha = T(0);
+  if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  const int l = tid / L, t = tid - (tid / L) * L;
+  double acc = 0.0;
+  __syncthreads();
+
+  // z: complex transform of row pairs, unpacked to the two half spectra
+  for (int p0 = 0; p0 < NP; p0 += LPR) {
+    const int p = p0 + l;
+    const bool act = p < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+      const int64_t r0 = fo + (int64_t)(2 * p) * N, r1 = r0 + N;
+      T a0[N1], a1[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        a0[j1] = R[r0 + t + N2 * j1];
+        a1[j1] = R[r1 + t + N2 * j1];
+      }
+      if (UPDATE) {
+#pragma unroll
+        for (int j1 = 0; j1 < N1; ++j1) {
+          const int z = t + N2 * j1;
+          a0[j1] -= alpha * AP[r0 + z];
+          a1[j1] -= alpha * AP[r1 + z];
+          RO[r0 + z] = a0[j1];
+          RO[r1 + z] = a1[j1];
+          X[r0 + z] += alpha * P[r0 + z];
+          X[r1 + z] += alpha * P[r1 + z];
+        }
+      }
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        acc += (double)a0[j1] * (double)a0[j1] + (double)a1[j1] * (double)a1[j1];
+        v[j1] = {a0[j1], a1[j1]};
+      }
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+      cplx<T>* zp = plane + (2 * p) * NH;  // the pair's two rows hold Z[0..N-1] for now
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) zp[t + N1 * k2] = w[k2];
+    }
+    __syncthreads();
+    constexpr int ITEMS = LPR * NH;
+    constexpr int IPT = (ITEMS + BT - 1) / BT;
+    cplx<T> A[IPT], B[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int it = tid + i * BT;
+      const int lp = it / NH, k = it - (it / NH) * NH;
+      if (it < ITEMS && p0 + lp < NP) {
+        const cplx<T>* zp = plane + (2 * (p0 + lp)) * NH;
+        const cplx<T> Z = zp[k], M = zp[k == 0 ? 0 : N - k];
+        A[i] = {T(0.5) * (Z.re + M.re), T(0.5) * (Z.im - M.im)};
+        B[i] = {T(0.5) * (Z.im + M.im), T(-0.5) * (Z.re - M.re)};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int it = tid + i * BT;
+      const int lp = it / NH, k = it - (it / NH) * NH;
+      if (it < ITEMS && p0 + lp < NP) {
+        plane[(2 * (p0 + lp)) * NH + k] = A[i];
+        plane[(2 * (p0 + lp) + 1) * NH + k] = B[i];
+      }
+    }
+  }
+  __syncthreads();
+
+  // y: columns 1..NP-1 complex; columns 0 and NP (real) packed into line 0
+  cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
+  for (int c0 = 0; c0 < NP; c0 += LPR) {
+    const int col = c0 + l;
+    const bool act = col < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        v[j1] = col == 0 ? cplx<T>{plane[y * NH].re, plane[y * NH + NP].re} : plane[y * NH + col];
+      }
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int ky = t + N1 * k2;
+        if (col == 0) dcn[ky] = w[k2];
+        else so[(int64_t)ky * NH + col] = w[k2];
+      }
+    }
+    __syncthreads();
+  }
+  // unpack the packed DC / Nyquist columns
+  for (int ky = tid; ky < N; ky += BT) {
+    const cplx<T> Z = dcn[ky], M = dcn[ky == 0 ? 0 : N - ky];
+    so[(int64_t)ky * NH] = {T(0.5) * (Z.re + M.re), T(0.5) * (Z.im - M.im)};
+    so[(int64_t)ky * NH + NP] = {T(0.5) * (Z.im + M.im), T(-0.5) * (Z.re - M.re)};
+  }
+  const double tsum = block_sum(acc, red);
+  if (tid == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = tsum;
+}
+
+// Pass C ---------------------------------------------------------------------------------------
+template <typename T, int N, bool DOT>
+__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  using C = FastCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, LPR = C::LPR, BT = C::BT;
+  extern __shared__ double smem_d[];
+  __shared__ double red[4];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
+  cplx<T>* plane = tw + N;
+  cplx<T>* xch = plane + N * NH;
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
+  if (a.done != nullptr && a.done[pi.cell]) return;
+  const CellDesc cd = a.cells[pi.cell];
+  const int rhs = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  const cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
+  for (int i = tid; i < N * NH; i += BT) plane[i] = so[i];
+  const int l = tid / L, t = tid - (tid / L) * L;
+  __syncthreads();
+
+  // y inverse; columns 0 and NP packed (their inverse transforms are real)
+  for (int c0 = 0; c0 < NP; c0 += LPR) {
+    const int col = c0 + l;
+    const bool act = col < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        if (col == 0) {
+          const cplx<T> A = plane[y * NH], B = plane[y * NH + NP];
+          v[j1] = {A.re - B.im, A.im + B.re};
+        } else {
+          v[j1] = plane[y * NH + col];
+        }
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int y = t + N1 * k2;
+        if (col == 0) {
+          plane[y * NH] = {w[k2].re, T(0)};
+          plane[y * NH + NP] = {w[k2].im, T(0)};
+        } else {
+          plane[y * NH + col] = w[k2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // z inverse: row pairs, Z = A + i B with the Hermitian extension
+  const int64_t nvox = cd.nvox;
+  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * nvox + (int64_t)pi.x * N * N;
+  double acc = 0.0;
+  for (int p0 = 0; p0 < NP; p0 += LPR) {
+    const int p = p0 + l;
+    const bool act = p < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+      const cplx<T>* ra = plane + (2 * p) * NH;
+      const cplx<T>* rb = ra + NH;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int k = t + N2 * j1;
+        cplx<T> A, B;
+        if (k < NH) {
+          A = ra[k];
+          B = rb[k];
+        } else {
+          A = ra[N - k];
+          A.im = -A.im;
+          B = rb[N - k];
+          B.im = -B.im;
+        }
+        v[j1] = {A.re - B.im, A.im + B.re};
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+      const int64_t r0 = fo + (int64_t)(2 * p) * N, r1 = r0 + N;
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int z = t + N1 * k2;
+        a.z[r0 + z] = w[k2].re;
+        a.z[r1 + z] = w[k2].im;
+        if (DOT) acc += (double)a.r[r0 + z] * (double)w[k2].re + (double)a.r[r1 + z] * (double)w[k2].im;
+      }
+    }
+    __syncthreads();
+  }
+  if (DOT) {
+    const double tsum = block_sum(acc, red);
+    if (tid == 0) a.part_c[(int64_t)rhs * a.nplanes + item] = tsum;
+  }
+}
+
+// Pass B ---------------------------------------------------------------------------------------
+template <typename T, int N>
+__global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg,
+                                                                  const T* __restrict__ symp) {
+  using C = FastCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB, XS = C::XS;
+  extern __shared__ double smem_d[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
+  cplx<T>* xch = tw + N;  // [3][XS][LPB]
+  const LineItem li = a.lines[a.line_base + blockIdx.x];
+  if (a.done != nullptr && a.done[li.cell]) return;
+  const CellDesc cd = a.cells[li.cell];
+  const int rhs = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  const int l = tid % LPB, t = tid / LPB;
+  const bool act = l < li.len;
+  constexpr int64_t npos = (int64_t)N * NH;
+  const int64_t pos = li.pos0 + l;
+  cplx<T>* sp = a.spec + cd.off_spec + (int64_t)rhs * 3 * cd.nspec;
+  __syncthreads();
+  // forward step 1
+  if (act && t < N2) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) v[j1] = sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * j1) * npos + pos];
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xch[((c * XS) + k1 * (N2 + 1) + t) * LPB + l] = cmulw(v[k1], tw[t * k1], false);
+    }
+  }
+  __syncthreads();
+  cplx<T> Xc[3][N2];
+  if (act && t < N1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) Xc[c][j2] = xch[((c * XS) + t * (N2 + 1) + j2) * LPB + l];
+      Dft<T, N2, -1>::run(Xc[c]);
+    }
+    // z = Kinv r at kx = t + N1 k2 (symbol component-planar: [6][nx][ny][nzh], divided by N^3)
+#pragma unroll
+    for (int k2 = 0; k2 < N2; ++k2) {
+      const int64_t o = (int64_t)(t + N1 * k2) * npos + pos;
+      constexpr int64_t ss = (int64_t)N * npos;
+      const T k00 = symp[o], k11 = symp[ss + o], k22 = symp[2 * ss + o];
+      const T k12 = symp[3 * ss + o], k02 = symp[4 * ss + o], k01 = symp[5 * ss + o];
+      const cplx<T> r0 = Xc[0][k2], r1 = Xc[1][k2], r2 = Xc[2][k2];
+      Xc[0][k2] = {k00 * r0.re + k01 * r1.re + k02 * r2.re, k00 * r0.im + k01 * r1.im + k02 * r2.im};
+      Xc[1][k2] = {k01 * r0.re + k11 * r1.re + k12 * r2.re, k01 * r0.im + k11 * r1.im + k12 * r2.im};
+      Xc[2][k2] = {k02 * r0.re + k12 * r1.re + k22 * r2.re, k02 * r0.im + k12 * r1.im + k22 * r2.im};
+    }
+  }
+  __syncthreads();
+  // inverse step 1: in-register length N2 at kx = t + N1 k2; twiddle e^{+2 pi i t k / N}
+  if (act && t < N1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      Dft<T, N2, 1>::run(Xc[c]);
+#pragma unroll
+      for (int k1 = 0; k1 < N2; ++k1) xch[((c * XS) + k1 * (N1 + 1) + t) * LPB + l] = cmulw(Xc[c][k1], tw[t * k1], true);
+    }
+  }
+  __syncthreads();
+  // inverse step 2: thread t < N2 gathers over t' < N1, outputs x = t + N2 k
+  if (act && t < N2) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j = 0; j < N1; ++j) v[j] = xch[((c * XS) + t * (N1 + 1) + j) * LPB + l];
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k = 0; k < N1; ++k) sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * k) * npos + pos] = v[k];
+    }
+  }
+}
+
+// Launchers ------------------------------------------------------------------------------------
+template <typename T, int N>
+size_t fast_lds_plane() {
+  using C = FastCfg<T, N>;
+  return ((size_t)N + (size_t)N * C::NH + (size_t)C::LPR * C::XS + N) * sizeof(cplx<T>);
+}
+template <typename T, int N>
+size_t fast_lds_line() {
+  using C = FastCfg<T, N>;
+  return ((size_t)N + (size_t)3 * C::XS * C::LPB) * sizeof(cplx<T>);
+}
+
+template <typename T, int N>
+static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
+                                int update, int dot, hipStream_t st) {
+  using C = FastCfg<T, N>;
+  const size_t lp = fast_lds_plane<T, N>(), ll = fast_lds_line<T, N>();
+  dim3 gp(nplanes, a.m, 1), gl(nlines, a.m, 1);
+  auto big = [](const void* f, size_t b) {
+    if (b > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
+  };
+  if (update) {
+    big((const void*)prec_a_fast<T, N, true>, lp);
+    hipLaunchKernelGGL((prec_a_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
+  } else {
+    big((const void*)prec_a_fast<T, N, false>, lp);
+    hipLaunchKernelGGL((prec_a_fast<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
+  }
+  big((const void*)prec_b_fast<T, N>, ll);
+  hipLaunchKernelGGL((prec_b_fast<T, N>), gl, dim3(C::BTB), ll, st, a, tw, symp);
+  if (dot) {
+    big((const void*)prec_c_fast<T, N, true>, lp);
+    hipLaunchKernelGGL((prec_c_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
+  } else {
+    big((const void*)prec_c_fast<T, N, false>, lp);
+    hipLaunchKernelGGL((prec_c_fast<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
+  }
+  return hipGetLastError();
+}
+
+bool fast_fft_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 128; }
+
+int fast_line_len(int n, int elem_bytes) {
+  switch (n) {
+    case 128: return elem_bytes == 8 ? FastCfg<double, 128>::LPB : FastCfg<float, 128>::LPB;
+    default: return elem_bytes == 8 ? FastCfg<double, 64>::LPB : FastCfg<float, 64>::LPB;
+  }
+}
+
+template <typename T>
+hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
+                            int update, int dot, hipStream_t st) {
+  switch (n) {
+    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, st);
+    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, st);
+    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, st);
+    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, st);
+    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template hipError_t launch_prec_fast<double>(int, const PrecArgs<double>&, int, int, const cplx<double>*,
+                                             const double*, int, int, hipStream_t);
+template hipError_t launch_prec_fast<float>(int, const PrecArgs<float>&, int, int, const cplx<float>*, const float*,
+                                            int, int, hipStream_t);
+
+}  // namespace nlfh

@@ -65,9 +65,11 @@ struct PrecArgs {
   double* part_c;        // pass C: partial r . z per (rhs, plane item)
   const int* done;       // per-cell early exit (nullable)
   int m;
-  int nplanes;
+  int nplanes;           // total plane items (row stride of part_a / part_c)
   int nlines;
   int rows_per_batch;    // row pairs per LDS batch in passes A and C
+  int plane_base;        // first plane item of this launch
+  int line_base;         // first line item of this launch
 };
 
 struct VecArgs {
@@ -113,7 +115,13 @@ hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, 
 template <typename T>
 size_t prec_plane_lds(int ny, int nz, int rows_per_batch);
 template <typename T>
-hipError_t launch_prec(const PrecArgs<T>& a, int update, int dot, size_t lds_plane, size_t lds_line, hipStream_t st);
+hipError_t launch_prec(const PrecArgs<T>& a, int nplanes, int nlines, int update, int dot, size_t lds_plane,
+                       size_t lds_line, hipStream_t st);
+bool fast_fft_size(int n);
+int fast_line_len(int n, int elem_bytes);
+template <typename T>
+hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
+                            int update, int dot, hipStream_t st);
 hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, double* ds, int64_t n, double emin,
                                   double p, hipStream_t st);
 hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);

@@ -122,9 +122,13 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     }
   };
 
-  T up[3][ZPT], carry[3][ZPT];
+  T up[3][ZPT], carry[3][ZPT], nxt[3][ZPT];
   double dot = 0.0;
-  if (active) load_row(((it.x0 - 1) % nx + nx) % nx, up);
+  if (active) {
+    // planes x0-1 and x0; afterwards each step prefetches the plane of the next step
+    load_row(((it.x0 - 1) % nx + nx) % nx, up);
+    load_row(it.x0 % nx, nxt);
+  }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -136,7 +140,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     T un[3][ZPT];
     T pc[3][ZPT];
     if (active) {
-      load_row(xi1, un);
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) un[c][q] = nxt[c][q];
+      if (step < it.xc) load_row((xi1 + 1) % nx, nxt);
       if (PUPDATE && owns && step < it.xc) {
         const int64_t o = ((int64_t)xi1 * ny + yg) * nz + z0;
 #pragma unroll

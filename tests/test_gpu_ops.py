@@ -13,7 +13,7 @@ from oracle import linear_ref as O
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(4, 4, 4), (3, 4, 5), (6, 5, 4), (8, 8, 1), (12, 10, 6), (16, 16, 16), (32, 24, 48), (20, 18, 64),
-          (8, 6, 128), (5, 9, 3)]
+          (8, 6, 128), (5, 9, 3), (64, 64, 64)]
 
 
 def _rand(shape, seed, m=2):
@@ -65,8 +65,9 @@ def test_load_and_residual(gpu_pkg, shape):
     assert _rel(lh.residual(s, Es, u).cpu().numpy(), ref.residual(s, Es, u)) < 1e-12
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (16, 12, 10), (32, 32, 32), (24, 48, 20),
-                                   (64, 64, 64), (8, 8, 128)])
+# cubic sides 16, 32, 48, 64 and 128 take the register-FFT kernels, the others the generic ones
+@pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (16, 12, 10), (24, 48, 20), (8, 8, 128),
+                                   (16, 16, 16), (32, 32, 32), (48, 48, 48), (64, 64, 64), (128, 128, 128)])
 def test_precond_fp64(gpu_pkg, shape):
     rng = np.random.default_rng(5)
     r = rng.standard_normal((2,) + shape + (3,))
@@ -76,7 +77,7 @@ def test_precond_fp64(gpu_pkg, shape):
     assert _rel(got, ref) < 1e-11
 
 
-@pytest.mark.parametrize("shape", [(16, 12, 10), (64, 64, 64)])
+@pytest.mark.parametrize("shape", [(16, 12, 10), (32, 32, 32), (48, 48, 48), (64, 64, 64), (128, 128, 128)])
 def test_precond_fp32(gpu_pkg, shape):
     import ctypes
     rng = np.random.default_rng(6)
