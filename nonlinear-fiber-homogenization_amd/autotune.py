@@ -1,0 +1,123 @@
+"""Autotuner for the matvec tiling (ty node rows per tile, xc x-planes per chunk, zpt z values per
+lane) of each grid shape of a plan.
+
+Candidates are every valid combination from small sets; each is timed on the device with HIP events
+over repeated launches of the operator K(s) u on m right-hand sides (the same kernel family the
+solver launches every iteration) and the fastest is kept in the plan. Results are cached per
+(device name, shape, precision, m) in a JSON file (NLFH_TUNE_CACHE, default
+~/.cache/nlfh/tiling.json) so later plans skip the sweep.
+"""
+import ctypes
+import json
+import os
+
+import numpy as np
+import torch
+
+from . import _lib
+
+TY_CHOICES = (1, 2, 4, 6, 8, 14, 30, 62)
+XC_CHOICES = (2, 4, 8, 16, 32, 64, 128)
+ZPT_CHOICES = (1, 2, 4)
+
+
+def candidates(shape, elem_bytes=8):
+    nx, ny, nz = shape
+    out = []
+    for zpt in ZPT_CHOICES:
+        if nz % zpt or nz // zpt > 64:
+            continue
+        lz = nz // zpt
+        lzs = 1
+        while lzs < lz:
+            lzs <<= 1
+        for ty in TY_CHOICES:
+            if ty > ny or lzs * (ty + 2) > 512:
+                continue
+            if 4 * (ty + 2) * 6 * nz * elem_bytes > 159 * 1024:
+                continue
+            for xc in XC_CHOICES:
+                if xc > nx:
+                    continue
+                out.append((ty, xc, zpt))
+    return out
+
+
+def _cache_path():
+    return os.environ.get("NLFH_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "nlfh", "tiling.json"))
+
+
+def _load_cache():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except Exception:
+        return {}
+
+
+def _save_cache(c):
+    p = _cache_path()
+    try:
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        with open(p, "w") as f:
+            json.dump(c, f, indent=1)
+    except OSError:
+        pass
+
+
+def time_matvec(plan, s, u, out, m, dtype, reps):
+    lib = _lib.load()
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    args = (plan.handle, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+            m, dtype, sp)
+    for _ in range(2):
+        _lib.check(lib.nlfh_apply_K(*args))
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        _lib.check(lib.nlfh_apply_K(*args))
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e3  # microseconds
+
+
+def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, verbose=False):
+    """Pick the fastest tiling for distinct shape `shape_index` of `plan` (a linear.Plan) and set
+    it. Returns (best (ty, xc, zpt), [(tiling, microseconds), ...])."""
+    # the distinct shapes in order of first appearance
+    distinct = []
+    for s in plan.shapes:
+        if s not in distinct:
+            distinct.append(s)
+    shape = distinct[shape_index]
+    dev = torch.device("cuda", torch.cuda.current_device())
+    key = f"{torch.cuda.get_device_name(dev)}|{shape}|{precision}|{m}"
+    cache = _load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        best = tuple(cache[key]["best"])
+        plan.set_tiling(shape_index, *best)
+        return best, []
+    dtype = 0 if precision == "f64" else 1
+    tdt = torch.float64 if precision == "f64" else torch.float32
+    # a single-cell plan owns the voxel range of this shape at offset 0
+    if len(plan.shapes) != 1:
+        raise ValueError("tune() times single-cell plans; tune each shape with its own plan")
+    s = torch.rand(shape, dtype=torch.float64, device=dev) * 0.99 + 0.01
+    u = torch.randn((m, 3) + tuple(shape), dtype=tdt, device=dev)
+    out = torch.empty_like(u)
+    table = []
+    for cand in candidates(shape, 8):  # the plan's tiling must fit the float64 kernels too
+        plan.set_tiling(shape_index, *cand)
+        t = time_matvec(plan, s, u, out, m, dtype, reps)
+        table.append((cand, t))
+        if verbose:
+            print(f"  tiling ty={cand[0]:3d} xc={cand[1]:3d} zpt={cand[2]}: {t:9.2f} us", flush=True)
+    table.sort(key=lambda x: x[1])
+    best = table[0][0]
+    plan.set_tiling(shape_index, *best)
+    if use_cache:
+        cache[key] = {"best": list(best), "us": table[0][1]}
+        _save_cache(cache)
+    return best, table

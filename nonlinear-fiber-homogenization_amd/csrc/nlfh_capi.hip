@@ -313,6 +313,9 @@ void default_tiling(Shape& sh) {
   int lz = sh.nz / zpt, lzs = 1;
   while (lzs < lz) lzs <<= 1;
   int rows = std::max(3, 512 / lzs);
+  // four LDS buffers of (ty + 2) rows x 6 values x nz (float64) must fit
+  const int lds_rows = (int)(kLdsMax / ((size_t)4 * 6 * sh.nz * sizeof(double)));
+  rows = std::min(rows, lds_rows);
   sh.ty = std::max(1, std::min(rows - 2, sh.ny));
   sh.xc = std::max(1, std::min(8, sh.nx));
 }
@@ -646,8 +649,8 @@ int leave(nlfh_plan* p, void* stream) {
   return NLFH_OK;
 }
 
-int set_strain(nlfh_plan* p, const double* Es_host, int m) {
-  HIP_TRY(hipMemcpyAsync(p->d_strain, Es_host, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, p->stream));
+int set_strain(nlfh_plan* p, const double* Es_host, int m, hipStream_t st) {
+  HIP_TRY(hipMemcpyAsync(p->d_strain, Es_host, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, st));
   return NLFH_OK;
 }
 
@@ -955,7 +958,7 @@ int nlfh_plan_set_tiling(nlfh_plan* p, int shape_index, int ty, int xc, int zpt)
   while (lzs < lz) lzs <<= 1;
   CHECK_ARG(ty >= 1 && lzs * (std::min(ty, sh.ny) + 2) <= 512, "ty too large for a 512-thread block");
   CHECK_ARG(xc >= 1, "xc must be positive");
-  const size_t lds = (size_t)2 * (std::min(ty, sh.ny) + 2) * 6 * sh.nz * sizeof(double);
+  const size_t lds = (size_t)4 * (std::min(ty, sh.ny) + 2) * 6 * sh.nz * sizeof(double);
   CHECK_ARG(lds <= kLdsMax, "tile exceeds LDS");
   sh.ty = std::min(ty, sh.ny);
   sh.xc = std::min(xc, sh.nx);
@@ -982,67 +985,72 @@ int nlfh_plan_get_tiling(const nlfh_plan* p, int shape_index, int* ty, int* xc, 
 int nlfh_stiffness_scale(nlfh_plan* p, const double* rho, double emin, double pw, double* s, double* dsdrho,
                          void* stream) {
   CHECK_ARG(p && rho && (s || dsdrho), "null argument");
-  TRY(enter(p, stream));
-  HIP_TRY(launch_stiffness_scale(rho, s, nullptr, dsdrho, p->tot_vox, emin, pw, p->stream));
-  return leave(p, stream);
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
+  HIP_TRY(launch_stiffness_scale(rho, s, nullptr, dsdrho, p->tot_vox, emin, pw, st));
+  return NLFH_OK;
 }
 
 int nlfh_apply_K(nlfh_plan* p, const double* s, const void* u, void* Ku, int m, int dtype, void* stream) {
   CHECK_ARG(p && s && u && Ku, "null argument");
   TRY(check_m(p, m));
   CHECK_ARG(dtype == NLFH_DTYPE_F64 || dtype == NLFH_DTYPE_F32, "dtype must be 0 (f64) or 1 (f32)");
-  TRY(enter(p, stream));
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
   if (dtype == NLFH_DTYPE_F64) {
     TRY(run_matvec<double>(p, (const double*)u, nullptr, nullptr, (double*)Ku, s, nullptr, nullptr, nullptr, m, 0, 0,
-                           0, 1.0, p->stream));
+                           0, 1.0, st));
   } else {
     TRY(get_ready32(p));
-    HIP_TRY(launch_d2f(s, p->w32.s, p->tot_vox, p->stream));
+    HIP_TRY(launch_d2f(s, p->w32.s, p->tot_vox, st));
     TRY(run_matvec<float>(p, (const float*)u, nullptr, nullptr, (float*)Ku, p->w32.s, nullptr, nullptr, nullptr, m, 0,
-                          0, 0, 1.0f, p->stream));
+                          0, 0, 1.0f, st));
   }
-  return leave(p, stream);
+  return NLFH_OK;
 }
 
 int nlfh_load(nlfh_plan* p, const double* s, const double* Es, int m, double* f, void* stream) {
   CHECK_ARG(p && s && Es && f, "null argument");
   TRY(check_m(p, m));
-  TRY(enter(p, stream));
-  TRY(set_strain(p, Es, m));
-  TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, f, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, p->stream));
-  return leave(p, stream);
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
+  TRY(set_strain(p, Es, m, st));
+  TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, f, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, st));
+  return NLFH_OK;
 }
 
 int nlfh_residual(nlfh_plan* p, const double* s, const double* Es, const double* u, int m, double* r, void* stream) {
   CHECK_ARG(p && s && Es && u && r, "null argument");
   TRY(check_m(p, m));
-  TRY(enter(p, stream));
-  TRY(set_strain(p, Es, m));
-  TRY(run_matvec<double>(p, u, nullptr, nullptr, r, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, p->stream));
-  return leave(p, stream);
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
+  TRY(set_strain(p, Es, m, st));
+  TRY(run_matvec<double>(p, u, nullptr, nullptr, r, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, st));
+  return NLFH_OK;
 }
 
 int nlfh_precond(nlfh_plan* p, const void* r, void* z, int m, int dtype, void* stream) {
   CHECK_ARG(p && r && z, "null argument");
   TRY(check_m(p, m));
   CHECK_ARG(dtype == NLFH_DTYPE_F64 || dtype == NLFH_DTYPE_F32, "dtype must be 0 (f64) or 1 (f32)");
-  TRY(enter(p, stream));
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
   if (dtype == NLFH_DTYPE_F64) {
     TRY(get_ready64(p));
     PrecArgs<double> a = prec_args<double>(p, m);
     a.r = (const double*)r;
     a.spec = p->w64.spec;
     a.z = (double*)z;
-    TRY(run_prec<double>(p, a, 0, 0, p->stream));
+    TRY(run_prec<double>(p, a, 0, 0, st));
   } else {
     TRY(get_ready32(p));
     PrecArgs<float> a = prec_args<float>(p, m);
     a.r = (const float*)r;
     a.spec = p->w32.spec;
     a.z = (float*)z;
-    TRY(run_prec<float>(p, a, 0, 0, p->stream));
+    TRY(run_prec<float>(p, a, 0, 0, st));
   }
-  return leave(p, stream);
+  return NLFH_OK;
 }
 
 }  // extern "C"
@@ -1188,7 +1196,7 @@ int nlfh_solve(nlfh_plan* p, const double* s, const double* Es, int m, const dou
   CHECK_ARG(precision == NLFH_PREC_FP64 || precision == NLFH_PREC_MIXED, "precision must be 0 or 1");
   TRY(enter(p, stream));
   TRY(get_ready64(p));
-  TRY(set_strain(p, Es, m));
+  TRY(set_strain(p, Es, m, p->stream));
   HIP_TRY(launch_copy_d(s, p->w64.s, p->tot_vox, p->stream));
   if (precision == NLFH_PREC_FP64) {
     TRY(solve_fp64(p, m, u0, tol, maxit, it_out, converged_out, relres_out, false));
@@ -1219,8 +1227,9 @@ int nlfh_bilinear(nlfh_plan* p, const double* s, const double* Es, const double*
                   double* dCds_out, void* stream) {
   CHECK_ARG(p && s && Es && u && C_out, "null argument");
   TRY(check_m(p, m));
-  TRY(enter(p, stream));
-  TRY(set_strain(p, Es, m));
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
+  TRY(set_strain(p, Es, m, st));
   EnergyArgs a{};
   a.cells = p->d_cells;
   a.chunks = p->d_chunks;
@@ -1234,11 +1243,11 @@ int nlfh_bilinear(nlfh_plan* p, const double* s, const double* Es, const double*
   a.dcds = dCds_out;
   a.lam16 = p->lam / 16.0;
   a.mu16 = p->mu / 16.0;
-  HIP_TRY(launch_energy(a, p->stream));
+  HIP_TRY(launch_energy(a, st));
   // C = sum / N per cell: rows = m*m; divide by N per cell (sum_partials takes one scale: loop cells)
   HIP_TRY(launch_sum_partials(p->d_part_e, (int)p->chunks.size(), p->d_chunk_range, p->ncells, m * m, 1.0, p->d_cells,
-                              m * m, C_out, p->stream));
-  return leave(p, stream);
+                              m * m, C_out, st));
+  return NLFH_OK;
 }
 
 int nlfh_plan_set_inner_tol(nlfh_plan* p, double inner_tol) {
@@ -1251,9 +1260,10 @@ int nlfh_plan_set_inner_tol(nlfh_plan* p, double inner_tol) {
 int nlfh_project_mean(nlfh_plan* p, double* v, int m, void* stream) {
   CHECK_ARG(p && v, "null argument");
   TRY(check_m(p, m));
-  TRY(enter(p, stream));
-  TRY(project_mean(p, v, m, p->stream));
-  return leave(p, stream);
+  HIP_TRY(hipSetDevice(p->device));
+  hipStream_t st = (hipStream_t)stream;  // standalone operators run on the caller's stream
+  TRY(project_mean(p, v, m, st));
+  return NLFH_OK;
 }
 
 int nlfh_symbol(const nlfh_plan* p, int shape_index, double* kinv_out) {

@@ -62,7 +62,40 @@ __device__ __forceinline__ void modal_op(const T (&h)[3][8], T (&g)[3][8], const
   g[0][0] = T(0); g[1][0] = T(0); g[2][0] = T(0);
 }
 
-template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT>
+// z neighbours: a DPP wave rotation when a row is a whole wavefront (lane i <- i+1 with
+// wave_rol:1, i <- i-1 with wave_ror:1, checked on gfx950), ds_bpermute otherwise.
+template <int CTRL>
+__device__ __forceinline__ double dpp_rot(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_rot(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+constexpr int kDppWaveRol1 = 0x134;  // lane i receives lane i + 1 (mod 64)
+constexpr int kDppWaveRor1 = 0x13C;  // lane i receives lane i - 1 (mod 64)
+
+template <typename T, bool DPP>
+__device__ __forceinline__ T from_next(T v, int src) {
+  if constexpr (DPP) return dpp_rot<kDppWaveRol1>(v);
+  else return __shfl(v, src, 64);
+}
+template <typename T, bool DPP>
+__device__ __forceinline__ T from_prev(T v, int src) {
+  if constexpr (DPP) return dpp_rot<kDppWaveRor1>(v);
+  else return __shfl(v, src, 64);
+}
+
+// One block: y-tile rows [y0, y0 + ty) x all z, x-planes [x0, x0 + xc), one right-hand side.
+// Element layer L_s = x0 - 1 + s (s = 0..xc) lies between node planes L_s and L_s + 1. Step s
+// (one barrier) does: (a) finish layer L_{s-1}: add the row-above half of its y-reverse from LDS,
+// reverse x and store node plane L_{s-1}; (b) layer L_s: y, z butterflies, modal operator, z and y
+// reverse, publish the row-above half in LDS; (c) x butterflies of layer L_{s+1} from the prefetched
+// plane into the other LDS buffer. LDS: two S/D buffers and two y-reverse buffers.
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   extern __shared__ double smem_d[];
   T* lds = reinterpret_cast<T*>(smem_d);
@@ -80,7 +113,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const int lz = tid % LZS;
   const int r = tid / LZS;
   const bool active = r < RB && lz < LZ;
-  const bool erow = r <= ty;              // computes element row yg
+  const bool erow = r <= ty;  // computes element row yg
   const bool owns = active && r >= 1 && r <= ty;
   int yg = it.y0 - 1 + r;
   yg = ((yg % ny) + ny) % ny;
@@ -103,10 +136,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 #pragma unroll
     for (int j = 0; j < 6; ++j) e4[j] = (T)(4.0 * a.strain[rhs * 6 + j]);
   }
+  const T lam16 = a.lam16, mu16 = a.mu16, scale_out = a.scale_out;
 
-  const int plane_sz = 6 * ZPT * LZ;  // values of one row in one LDS buffer
-  T* SD = lds;                          // [RB][6][ZPT][LZ]
-  T* UY = lds + RB * plane_sz;          // [RB][6][ZPT][LZ]
+  const int buf_sz = RB * 6 * ZPT * LZ;  // one LDS buffer
+  T* SDb[2] = {lds, lds + buf_sz};
+  T* UYb[2] = {lds + 2 * buf_sz, lds + 3 * buf_sz};
 #define LIDX(rr, f, q) ((((rr) * 6 + (f)) * ZPT + (q)) * LZ + lz)
 
   auto load_row = [&](int xp, T (&v)[3][ZPT]) {
@@ -121,54 +155,82 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       }
     }
   };
+  const int x0 = it.x0, xc = it.xc;
+  auto wrap = [&](int x) { return ((x % nx) + nx) % nx; };
 
-  T up[3][ZPT], carry[3][ZPT], nxt[3][ZPT];
+  T up[3][ZPT], nxt[3][ZPT], carry[3][ZPT];
+  T pcA[3][ZPT], pcB[3][ZPT];  // node values of the planes awaiting output (layer parity)
+  T Flo[3][2][ZPT];            // own half of the y-reverse of the last element layer
   double dot = 0.0;
-  if (active) {
-    // planes x0-1 and x0; afterwards each step prefetches the plane of the next step
-    load_row(((it.x0 - 1) % nx + nx) % nx, up);
-    load_row(it.x0 % nx, nxt);
-  }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
 #pragma unroll
-    for (int q = 0; q < ZPT; ++q) carry[c][q] = T(0);
+    for (int q = 0; q < ZPT; ++q) {
+      carry[c][q] = T(0);
+      Flo[c][0][q] = T(0);
+      Flo[c][1][q] = T(0);
+      pcA[c][q] = T(0);
+      pcB[c][q] = T(0);
+    }
 
-  for (int step = 0; step <= it.xc; ++step) {
-    const int xi = ((it.x0 - 1 + step) % nx + nx) % nx;  // element layer / finalised node plane
-    const int xi1 = (xi + 1) % nx;
+  // x stage of layer L_t from planes `up` (L_t) and `un` (L_t + 1) into SD buffer t & 1
+  auto x_stage = [&](int t, T (&un)[3][ZPT]) {
+    T* SD = SDb[t & 1];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) {
+        SD[LIDX(r, c, q)] = un[c][q] + up[c][q];
+        SD[LIDX(r, 3 + c, q)] = un[c][q] - up[c][q];
+        if (t & 1) pcB[c][q] = up[c][q];
+        else pcA[c][q] = up[c][q];
+        up[c][q] = un[c][q];
+      }
+  };
+
+  // prologue: layer L_0 from planes x0-1, x0; prefetch x0+1
+  if (active) {
+    load_row(wrap(x0 - 1), up);
     T un[3][ZPT];
-    T pc[3][ZPT];
-    if (active) {
+    load_row(wrap(x0), un);
+    if (xc >= 1) load_row(wrap(x0 + 1), nxt);
+    if (PUPDATE && owns && xc >= 1) {
+      const int64_t o = ((int64_t)wrap(x0) * ny + yg) * nz + z0;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < ZPT; ++q) un[c][q] = nxt[c][q];
-      if (step < it.xc) load_row((xi1 + 1) % nx, nxt);
-      if (PUPDATE && owns && step < it.xc) {
-        const int64_t o = ((int64_t)xi1 * ny + yg) * nz + z0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-#pragma unroll
-          for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
-      }
+        for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
+    }
+    x_stage(0, un);
+  }
+  __syncthreads();
+
+  for (int s = 0; s <= xc + 1; ++s) {
+    // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
+    if (s >= 1 && owns) {
+      const T* UYp = UYb[(s - 1) & 1];
+      const int xo = wrap(x0 - 2 + s);
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int q = 0; q < ZPT; ++q) {
-          const T sx = un[c][q] + up[c][q];
-          const T dx = un[c][q] - up[c][q];
-          SD[LIDX(r, c, q)] = sx;
-          SD[LIDX(r, 3 + c, q)] = dx;
-          pc[c][q] = up[c][q];
-          up[c][q] = un[c][q];
+          const T A = Flo[c][0][q] + UYp[LIDX(r - 1, 2 * c, q)];
+          const T B = Flo[c][1][q] + UYp[LIDX(r - 1, 2 * c + 1, q)];
+          const T lo = A - B, hi = A + B;
+          if (s >= 2) {
+            const T v = (carry[c][q] + lo) * scale_out;
+            O[c * nvox + ((int64_t)xo * ny + yg) * nz + z0 + q] = v;
+            if (DOT) dot += (double)((s - 1) & 1 ? pcB[c][q] : pcA[c][q]) * (double)v;
+          }
+          carry[c][q] = hi;
         }
     }
-    __syncthreads();
-
-    // y stage, z stage, modal operator, reverse z: element row yg (rows 0..ty)
-    T Fz[3][4][ZPT];
+    if (s > xc) break;
+    // (b) element layer L_s
+    const int xi = wrap(x0 - 1 + s);
     if (active && erow) {
+      const T* SD = SDb[s & 1];
+      T* UY = UYb[s & 1];
       T Y[3][4][ZPT];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -181,13 +243,13 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           Y[c][1][q] = d1 + d0;
           Y[c][3][q] = d1 - d0;
         }
-      // z neighbour of the last value: first value of the next lane in the row
       T Yn[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = __shfl(Y[c][mm][0], src_next, 64);
+        for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = from_next<T, DPP>(Y[c][mm][0], src_next);
       const int64_t so = ((int64_t)xi * ny + yg) * nz + z0;
+      T Fz[3][4][ZPT];
       T hi_last[3][4];
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) {
@@ -201,7 +263,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
             h[c][mm + 4] = yn - Y[c][mm][q];
           }
         const T sv = S[so + q];
-        modal_op<T>(h, g, sv * a.lam16, sv * a.mu16, STRAIN ? e4 : nullptr);
+        modal_op<T>(h, g, sv * lam16, sv * mu16, STRAIN ? e4 : nullptr);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -217,8 +279,8 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) Fz[c][mm][0] += __shfl(hi_last[c][mm], src_prev, 64);
-      // reverse y: the upper half goes to row r + 1 through LDS
+        for (int mm = 0; mm < 4; ++mm) Fz[c][mm][0] += from_prev<T, DPP>(hi_last[c][mm], src_prev);
+      // y reverse: the row-above half goes to row r + 1 through LDS, the own half waits in Flo
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -227,34 +289,31 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           const T A1 = Fz[c][1][q], B1 = Fz[c][3][q];
           UY[LIDX(r, 2 * c, q)] = A0 + B0;
           UY[LIDX(r, 2 * c + 1, q)] = A1 + B1;
-          Fz[c][0][q] = A0 - B0;
-          Fz[c][1][q] = A1 - B1;
+          Flo[c][0][q] = A0 - B0;
+          Flo[c][1][q] = A1 - B1;
         }
     }
-    // (all lanes of a row take the same branch, so the shuffles above only read active lanes)
-    __syncthreads();
-
-    if (owns) {
+    // (c) x stage of layer L_{s+1} from planes x0 + s and x0 + s + 1 (prefetched)
+    if (s < xc && active) {
+      T un[3][ZPT];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < ZPT; ++q) {
-          const T A = Fz[c][0][q] + UY[LIDX(r - 1, 2 * c, q)];
-          const T B = Fz[c][1][q] + UY[LIDX(r - 1, 2 * c + 1, q)];
-          const T lo = A - B;
-          const T hi = A + B;
-          if (step >= 1) {
-            const T v = (carry[c][q] + lo) * a.scale_out;
-            O[c * nvox + ((int64_t)xi * ny + yg) * nz + z0 + q] = v;
-            if (DOT) dot += (double)pc[c][q] * (double)v;
-          }
-          carry[c][q] = hi;
-        }
+        for (int q = 0; q < ZPT; ++q) un[c][q] = nxt[c][q];
+      if (s + 1 < xc) load_row(wrap(x0 + s + 2), nxt);
+      if (PUPDATE && owns && s + 1 < xc) {
+        const int64_t o = ((int64_t)wrap(x0 + s + 1) * ny + yg) * nz + z0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
+      }
+      x_stage(s + 1, un);
     }
+    __syncthreads();
   }
 #undef LIDX
   if (DOT) {
-    // rows that straddle wavefronts are fine: every thread of the block joins the reduction
     const double t = block_sum(dot, red);
     if (threadIdx.x == 0) a.dot_part[(int64_t)rhs * a.nitems + a.item_base + blockIdx.x] = t;
   }
@@ -262,14 +321,14 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 
 // Host-side launcher -------------------------------------------------------------------------
 
-template <typename T, int ZPT>
+template <typename T, int ZPT, bool DPP>
 static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, dim3 grid,
                                 int threads, size_t lds, hipStream_t st) {
 #define NLFH_MV(S_, P_, D_)                                                                           \
   if (lds > 65536)                                                                                  \
-    (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_>,                       \
+    (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_, DPP>,                       \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                \
-  hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_>), grid, dim3(threads), lds, st, a);         \
+  hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_, DPP>), grid, dim3(threads), lds, st, a);         \
   return hipGetLastError();
   if (strain) {
     NLFH_MV(true, false, false)
@@ -288,13 +347,17 @@ hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, 
                          hipStream_t st) {
   // max_ty + 2 rows of LZS lanes per block
   const int threads = ((a.LZS * (max_ty + 2) + 63) / 64) * 64;
-  const size_t lds = (size_t)2 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
+  const size_t lds = (size_t)4 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
   dim3 grid(nblk, a.m, 1);
-  switch (zpt) {
-    case 1: return launch_mv_zpt<T, 1>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 2: return launch_mv_zpt<T, 2>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 3: return launch_mv_zpt<T, 3>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 4: return launch_mv_zpt<T, 4>(a, strain, pupdate, dot, grid, threads, lds, st);
+  const bool dpp = a.LZ == 64;  // a row is a whole wavefront
+  switch (zpt * 2 + (dpp ? 1 : 0)) {
+    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, grid, threads, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
