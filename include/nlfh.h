@@ -54,7 +54,8 @@ int nlfh_plan_destroy(nlfh_plan* plan);
 int nlfh_plan_set_tiling(nlfh_plan* plan, int shape_index, int ty, int xc, int zpt);
 int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* xc, int* zpt);
 
-/* Relative tolerance of each FP32 inner solve of NLFH_PREC_MIXED (default 1e-4).
+/* Loosest relative tolerance of an FP32 inner solve of NLFH_PREC_MIXED (default 1e-2); each inner
+ * solve targets max(inner_tol, 0.5 * tol * ||b|| / ||r||), the reduction its refinement step needs.
  * No counterpart in the reference (which has no mixed precision). */
 int nlfh_plan_set_inner_tol(nlfh_plan* plan, double inner_tol);
 

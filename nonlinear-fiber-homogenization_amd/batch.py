@@ -1,0 +1,96 @@
+"""Many cells, of uneven grid sizes, homogenized together: one plan, one launch per kernel and PCG
+iteration for all cells, per-cell convergence flags on the device (a converged cell's blocks exit at
+once), iterations captured in a HIP graph.
+
+The reference homogenizes one cell per `LinearHomogenization` (nlfiber/linear.py:99-236) and
+shards tasks over processes (nlfiber/runner.py:55-75); this class is the batched form of
+`C_full` for the same algorithm, with identical per-cell results.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .linear import Plan, _device, _es_host, _ptr, _stream
+
+
+class BatchedHomogenization:
+    """C_full for a list of cells. `shapes`: list of (nx, ny, nz); `m` right-hand sides per cell
+    (the unit strains 0..m-1). precision: "fp64" or "mixed"."""
+
+    def __init__(self, shapes, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-8, maxit=4000, precision="mixed", m=6,
+                 tune=False):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        self.E, self.nu, self.emin, self.p, self.tol, self.maxit = E, nu, emin, p, tol, maxit
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.m = m
+        self._dev = _device()
+        self._plan = Plan(self.shapes, E, nu, m_layout=m)
+        self._lib = _lib.load()
+        self.nvox = [int(np.prod(s)) for s in self.shapes]
+        self.off = np.concatenate([[0], np.cumsum(self.nvox)]).astype(np.int64)
+        self.tot = int(self.off[-1])
+        self.last_info = {}
+        if tune:
+            self.autotune()
+
+    def autotune(self, precision=None):
+        """Tile every distinct shape with the autotuner (each on a single-cell plan) and apply it."""
+        from . import autotune as at
+        distinct = []
+        for s in self.shapes:
+            if s not in distinct:
+                distinct.append(s)
+        prec = precision or ("f64" if self.precision == "fp64" else "f32")
+        for i, s in enumerate(distinct):
+            tmp = Plan([s], self.E, self.nu)
+            best, _ = at.tune(tmp, 0, m=self.m, precision=prec)
+            self._plan.set_tiling(i, *best)
+
+    def pack(self, rhos):
+        """Concatenate the cells' densities into one device array (cell-major)."""
+        out = torch.empty(self.tot, dtype=torch.float64, device=self._dev)
+        for i, r in enumerate(rhos):
+            t = r if isinstance(r, torch.Tensor) else torch.as_tensor(np.asarray(r, dtype=np.float64))
+            out[self.off[i]:self.off[i + 1]] = t.to(device=self._dev, dtype=torch.float64).reshape(-1)
+        return out
+
+    def field_views(self, buf):
+        """Per-cell (m, nx, ny, nz, 3) views of a batched field buffer."""
+        views = []
+        for i, s in enumerate(self.shapes):
+            a = int(self.off[i]) * 3 * self.m
+            b = int(self.off[i + 1]) * 3 * self.m
+            views.append(buf[a:b].view((self.m, 3) + s).permute(0, 2, 3, 4, 1))
+        return views
+
+    def stiffness_scale(self, rho_packed):
+        s = torch.empty_like(rho_packed)
+        _lib.check(self._lib.nlfh_stiffness_scale(self._plan.handle, _ptr(rho_packed), float(self.emin), float(self.p),
+                                                  _ptr(s), None, _stream()))
+        return s
+
+    def C_full(self, rhos, u0=None):
+        """Returns (C (ncells, m, m) host float64, u (packed device buffer), iterations (ncells,)).
+        `rhos` is a list of arrays or an already packed device array; u0 a packed buffer or None."""
+        rho = rhos if isinstance(rhos, torch.Tensor) and rhos.dim() == 1 else self.pack(rhos)
+        s = self.stiffness_scale(rho)
+        Es = np.eye(6)[: self.m]
+        Es_np, Es_p = _es_host(Es)
+        nc = len(self.shapes)
+        u = torch.empty(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
+        it = (ctypes.c_int * nc)()
+        cv = (ctypes.c_int * nc)()
+        rr = (ctypes.c_double * (nc * self.m))()
+        prec = 0 if self.precision == "fp64" else 1
+        _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(u0), _ptr(u), float(self.tol),
+                                        int(self.maxit), prec, it, cv, rr, _stream()))
+        C = torch.empty((nc, self.m, self.m), dtype=torch.float64, device=self._dev)
+        _lib.check(self._lib.nlfh_bilinear(self._plan.handle, _ptr(s), Es_p, _ptr(u), self.m, _ptr(C), None,
+                                           _stream()))
+        self.last_info = dict(it=list(it), converged=[bool(c) for c in cv],
+                              relres=np.asarray(list(rr)).reshape(nc, self.m))
+        return C.cpu(), u, np.asarray(list(it))

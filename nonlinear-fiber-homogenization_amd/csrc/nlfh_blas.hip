@@ -165,7 +165,7 @@ __global__ void cg_init_kernel(CgState st, const double* part_a, const double* p
       st.rn[k] = rn;
       st.rz[k] = rz;
       st.beta[k] = 0.0;
-      if (rn > st.tol[0] * bn) atomicAdd(&notconv, 1);
+      if (rn > st.tol[cell] * bn) atomicAdd(&notconv, 1);
     }
   }
   __syncthreads();
@@ -203,7 +203,7 @@ __global__ void cg_beta_kernel(CgState st, const double* part_a, const double* p
       st.beta[k] = rz1 / (rz == 0.0 ? 1.0 : rz);
       st.rz[k] = rz1;
       st.rn[k] = rn;
-      if (rn > st.tol[0] * st.bn[k]) atomicAdd(&notconv, 1);
+      if (rn > st.tol[cell] * st.bn[k]) atomicAdd(&notconv, 1);
     }
   }
   __syncthreads();
@@ -215,10 +215,14 @@ __global__ void cg_beta_kernel(CgState st, const double* part_a, const double* p
 }
 
 // Per-(cell, rhs) norms from chunk partials of ||v||^2: out = sqrt(sum) + eps. When `flag` is given
-// (iterative refinement), flag[cell] = 1 when every rhs has out <= tol * bn (or `force`).
+// (iterative refinement), flag[cell] = 1 when every rhs has out <= tol * bn (or `force`), and when
+// `tol_in` is given it receives the relative tolerance of the cell's next inner solve:
+// max(base, safety * min_rhs tol * bn / out), i.e. no tighter than the last refinement step needs.
 __global__ void norms_kernel(const double* part, int nchunks, const int* crange, int m, int M, double eps,
-                             double* out, const double* bn, double tol, int* flag, const int* force) {
+                             double* out, const double* bn, double tol, int* flag, const int* force, double* tol_in,
+                             double tol_in_base, double tol_in_safety) {
   __shared__ int notconv;
+  __shared__ double need[8];
   const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) notconv = 0;
   __syncthreads();
@@ -228,10 +232,19 @@ __global__ void norms_kernel(const double* part, int nchunks, const int* crange,
     if (lane == 0) {
       out[k] = v;
       if (bn != nullptr && v > tol * bn[k]) atomicAdd(&notconv, 1);
+      need[t] = (bn != nullptr && v > 0.0) ? tol * bn[k] / v : 1.0;
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0 && flag != nullptr) flag[cell] = (notconv == 0 || (force != nullptr && force[cell])) ? 1 : 0;
+  if (threadIdx.x == 0) {
+    if (flag != nullptr) flag[cell] = (notconv == 0 || (force != nullptr && force[cell])) ? 1 : 0;
+    if (tol_in != nullptr) {
+      double mn = 1.0;
+      for (int i = 0; i < m; ++i) mn = need[i] < mn ? need[i] : mn;
+      const double want = tol_in_safety * mn;
+      tol_in[cell] = want > tol_in_base ? want : tol_in_base;
+    }
+  }
 }
 
 // Homogenized stiffness (nlfiber/linear.py:220-229) in the Walsh-mode form: per element and
@@ -429,9 +442,10 @@ hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const
 }
 
 hipError_t launch_norms(const double* part, int nchunks, const int* crange, int ncells, int m, int M, double eps,
-                        double* out, const double* bn, double tol, int* flag, const int* force, hipStream_t st) {
+                        double* out, const double* bn, double tol, int* flag, const int* force, double* tol_in,
+                        double tol_in_base, double tol_in_safety, hipStream_t st) {
   hipLaunchKernelGGL(norms_kernel, dim3(ncells), dim3(64 * m), 0, st, part, nchunks, crange, m, M, eps, out, bn, tol, flag,
-                     force);
+                     force, tol_in, tol_in_base, tol_in_safety);
   return hipGetLastError();
 }
 

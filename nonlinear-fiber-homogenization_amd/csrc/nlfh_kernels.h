@@ -88,7 +88,7 @@ struct CgState {
   double* rn;
   int* it;        // [cell]
   int* done;      // [cell]
-  const double* tol;  // [1]
+  const double* tol;  // [cell]: relative tolerance of each cell
   const int* maxit;   // [1]
   const int* skip;    // nullable [cell]: cells that take no iterations (converged outer loop)
   int m;
@@ -143,7 +143,8 @@ hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int
 hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
                           hipStream_t st);
 hipError_t launch_norms(const double* part, int nchunks, const int* crange, int ncells, int m, int M, double eps,
-                        double* out, const double* bn, double tol, int* flag, const int* force, hipStream_t st);
+                        double* out, const double* bn, double tol, int* flag, const int* force, double* tol_in,
+                        double tol_in_base, double tol_in_safety, hipStream_t st);
 hipError_t launch_energy(const EnergyArgs& a, hipStream_t st);
 
 }  // namespace nlfh

@@ -1,0 +1,51 @@
+"""Batched plans: many cells of uneven sizes in one solve give the same C and iteration counts as
+one solve per cell (the reference's per-cell C_full, nlfiber/linear.py:231-236)."""
+import numpy as np
+import pytest
+
+from oracle import linear_ref as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cells(pkg):
+    out = []
+    for n, seed in ((8, 1), (12, 2), (8, 3), (16, 4)):
+        rng = np.random.default_rng(seed)
+        out.append(pkg.cells.exact_symmetrize(pkg.cells.density(rng.standard_normal((n, n, n)), r=1.0, beta=4.0,
+                                                                v=0.3, filter_kernel="hat")))
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_equals_single_cell_solves(gpu_pkg, precision):
+    rhos = _cells(gpu_pkg)
+    bh = gpu_pkg.BatchedHomogenization([r.shape for r in rhos], emin=1e-3, tol=1e-10, precision=precision)
+    C, u, its = bh.C_full(rhos)
+    assert all(bh.last_info["converged"])
+    for i, r in enumerate(rhos):
+        lh = gpu_pkg.LinearHomogenization(r.shape, emin=1e-3, tol=1e-10, precision=precision)
+        Ci, ui, iti = lh.C_full(r)
+        assert np.abs(np.asarray(C[i]) - np.asarray(Ci)).max() < 1e-12 * np.abs(np.asarray(Ci)).max()
+        if precision == "fp64":
+            assert its[i] == iti
+        views = bh.field_views(u)
+        assert np.abs(views[i].cpu().numpy() - ui.cpu().numpy()).max() < 1e-9 * np.abs(ui.cpu().numpy()).max()
+    # and against the oracle for the largest cell
+    Cr, _, _ = O.LinearRef(rhos[3].shape, emin=1e-3, tol=1e-10).C_full(rhos[3])
+    assert np.abs(np.asarray(C[3]) - Cr).max() < 1e-9 * np.abs(Cr).max()
+
+
+def test_mixed_precision_true_residual_and_C(gpu_pkg):
+    rho = gpu_pkg.cells.random_cell(16, 7)[0]
+    lh = gpu_pkg.LinearHomogenization(rho.shape, emin=1e-4, tol=1e-8, precision="mixed")
+    C, u, it = lh.C_full(rho)
+    assert lh.last_info["converged"] and max(lh.last_info["relres"]) <= 1e-8
+    ref = O.LinearRef(rho.shape, emin=1e-4, tol=1e-8)
+    s = ref.stiffness_scale(rho)
+    r = ref.residual(s, np.eye(6), u.cpu().numpy())
+    b = ref.load(s, np.eye(6))
+    true = np.sqrt((r ** 2).sum(axis=(1, 2, 3, 4))) / np.sqrt((b ** 2).sum(axis=(1, 2, 3, 4)))
+    assert true.max() <= 1.5e-8
+    Cr, _, _ = ref.C_full(rho)
+    assert np.abs(np.asarray(C) - Cr).max() <= 1e-6 * np.abs(Cr).max()
