@@ -122,6 +122,50 @@ int nlfh_symbol(const nlfh_plan* plan, int shape_index, double* kinv_out);
 /* Number of bytes of device memory the plan holds. */
 long long nlfh_plan_bytes(const nlfh_plan* plan);
 
+/* ---- Learned warm start: periodic 3D conv network kernels (BF16 MFMA, FP32 accumulation) ----
+ * No counterpart in the reference's linear path (its only network, nlfiber/prior/unet.py:83-139, is a
+ * generative prior with symmetric padding); these kernels train and run the corrector predictor that
+ * gives PCG its initial guess u0 (nlfiber/linear.py:212-216). Activations are channels-last
+ * [batch][x][y][z][C] (z fastest), bf16 stored as uint16; channel counts are multiples of 16.
+ *
+ * out[v][co] = bias[co] + sum_{tap, ci} w[tap][co][ci] in[v + d_tap][ci], periodic, where in is the
+ * concatenation of source a (ca channels, read at half resolution when a_half) and b (cb channels);
+ * w is [27][coutp][ca + cb]; act 1 applies a leaky ReLU (slope 0.1); out_f (f32) and/or out_h
+ * (bf16) receive the first `cout` channels. flip negates the tap offsets. */
+int nlfh_net_conv(const unsigned short* a, int ca, int a_half, const unsigned short* b, int cb,
+                  const unsigned short* w, const float* bias, int n, int batch, int cout, int coutp, int act, int flip,
+                  float* out_f, unsigned short* out_h, void* stream);
+/* dw[tap][co][ci] = sum_v g[v][co] in[v + d_tap][ci] (in as for nlfh_net_conv); part is scratch of
+ * part_len floats for per-chunk partial sums (fixed-order reduction). */
+int nlfh_net_wgrad(const unsigned short* g, int cg, const unsigned short* a, int ca, int a_half,
+                   const unsigned short* b, int cb, int n, int batch, float* part, long long part_len, float* dw,
+                   void* stream);
+/* g = bf16(dy * act'(y)) (act 1: leaky ReLU with y its output, 0: identity) */
+int nlfh_net_act_grad(const float* dy, const unsigned short* y, unsigned short* g, long long len, int act,
+                      void* stream);
+/* db[c] = sum_v dy[v][c] act'(y[v][c]) */
+int nlfh_net_bias_grad(const float* dy, const unsigned short* y, long long nvox, int c, int act, float* db,
+                       void* stream);
+/* 2x2x2 average pooling (bf16) */
+int nlfh_net_pool(const unsigned short* in, unsigned short* out, int n, int batch, int c, void* stream);
+/* coarse (+)= scale * sum of the 8 fine children of channels [c0, c0 + c) of fine (cf channels) */
+int nlfh_net_upsum(const float* fine, int cf, int c0, float* coarse, int n, int batch, int c, float scale, int add,
+                   void* stream);
+/* fine (+)= scale * coarse[v / 2] */
+int nlfh_net_updist(const float* coarse, float* fine, int n, int batch, int c, float scale, int add, void* stream);
+/* dst[v][0..c) (+)= src[v][c0..c0+c) (src has cs channels) */
+int nlfh_net_slice_add(const float* src, int cs, int c0, float* dst, long long nvox, int c, int add, void* stream);
+/* mean squared error: grad = 2 (pred - target) / len, per-block loss partials in loss_part[nparts] */
+int nlfh_net_mse(const float* pred, const float* target, float* grad, long long len, double* loss_part, int nparts,
+                 void* stream);
+/* Adam step (bias-corrected, step >= 1, decoupled weight decay wd) on FP32 weights; refreshes the bf16 copy wh */
+int nlfh_net_adam(float* w, const float* g, float* m1, float* m2, unsigned short* wh, long long len, float lr,
+                  float b1, float b2, float eps, int step, float wd, void* stream);
+/* float32 -> bf16 (round to nearest even) */
+int nlfh_net_f2bf(const float* in, unsigned short* out, long long len, void* stream);
+/* wt[26 - t][ci][co] = w[t][co][ci] (zero for ci >= cin): weights of the data gradient */
+int nlfh_net_wflip(const unsigned short* w, int coutp, int cin, unsigned short* wt, int cinp, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

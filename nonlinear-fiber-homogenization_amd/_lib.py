@@ -35,6 +35,26 @@ SIGNATURES = [
     ("nlfh_project_mean", ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     ("nlfh_symbol", ctypes.c_int, [_vp, ctypes.c_int, _c_double_p]),
     ("nlfh_plan_bytes", ctypes.c_longlong, [_vp]),
+    # learned warm start network kernels
+    ("nlfh_net_conv", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp,
+                                     _vp]),
+    ("nlfh_net_wgrad", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int, _vp, ctypes.c_longlong, _vp, _vp]),
+    ("nlfh_net_act_grad", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int, _vp]),
+    ("nlfh_net_bias_grad", ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("nlfh_net_pool", ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
+    ("nlfh_net_upsum", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_float, ctypes.c_int, _vp]),
+    ("nlfh_net_updist", ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                       ctypes.c_int, _vp]),
+    ("nlfh_net_slice_add", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_longlong, ctypes.c_int,
+                                          ctypes.c_int, _vp]),
+    ("nlfh_net_mse", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong, _vp, ctypes.c_int, _vp]),
+    ("nlfh_net_adam", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_float,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_float, _vp]),
+    ("nlfh_net_f2bf", ctypes.c_int, [_vp, _vp, ctypes.c_longlong, _vp]),
+    ("nlfh_net_wflip", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp]),
 ]
 
 _lib = None

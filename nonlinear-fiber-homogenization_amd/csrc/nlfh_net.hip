@@ -1,0 +1,452 @@
+// Periodic 3D convolution network kernels for the learned warm start (gfx950, BF16 MFMA with FP32
+// accumulation). Activations are channels-last [batch][x][y][z][C] (z fastest), C a multiple of 16.
+//
+//   conv:   out[v][co] = bias[co] + sum_{tap, ci} W[tap][co][ci] in[v + d_tap][ci]     (periodic)
+//           implicit GEMM on v_mfma_f32_16x16x16_bf16: a wavefront owns 16 voxels x all output
+//           channels; K = 27 taps x input channels in steps of 16. Two input sources are
+//           concatenated on the fly (decoder skip connections) and source A may be read at half
+//           resolution (nearest upsampling fused into the load). flip = 1 reads d -> -d, which with
+//           transposed weights is the data gradient of a convolution.
+//   wgrad:  dW[tap][co][ci] = sum_v g[v][co] in[v + d_tap][ci]: per (tap, 16x16 block, voxel chunk)
+//           one wavefront, partial sums per chunk, then a fixed-order reduction.
+//   plus elementwise kernels (activation gradient, pooling, upsample-sum, loss, Adam, bias grads).
+#include <cstdint>
+
+#include "nlfh.h"
+#include "nlfh_common.h"
+
+namespace nlfh {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  // round to nearest even (finite inputs)
+  uint32_t u = __float_as_uint(f);
+  const uint32_t lsb = (u >> 16) & 1u;
+  u += 0x7fffu + lsb;
+  return (uint16_t)(u >> 16);
+}
+
+struct ConvArgs {
+  const uint16_t* a;  // source A [batch][na^3][ca]
+  const uint16_t* b;  // source B [batch][n^3][cb] (nullable)
+  const uint16_t* w;  // [27][coutp][cin], cin = ca + cb
+  const float* bias;  // [cout] nullable
+  float* out_f;       // [batch][n^3][cout] nullable
+  uint16_t* out_h;    // [batch][n^3][cout] nullable (bf16 copy after the activation)
+  int ca, cb, a_half;
+  int n, batch, cout, coutp;
+  int act;   // 0: none, 1: leaky relu (slope 0.1)
+  int flip;  // 1: tap offsets negated
+};
+
+template <int NB>
+__global__ void __launch_bounds__(256) conv3d_kernel(const ConvArgs p) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int n = p.n;
+  const int64_t nv = (int64_t)n * n * n;
+  const int64_t tile = ((int64_t)blockIdx.x * 4 + wave) * 16;  // first voxel (batch-flattened)
+  if (tile >= nv * p.batch) return;
+  const int bidx = (int)(tile / nv);
+  const int64_t vb = tile - (int64_t)bidx * nv;
+  const int row = lane & 15, kg = lane >> 4;
+  // voxel of this lane in the A operand
+  const int64_t va = vb + row;
+  const int zx = (int)(va % n), yx = (int)((va / n) % n), xx = (int)(va / ((int64_t)n * n));
+  const int cin = p.ca + p.cb;
+  const int na = p.a_half ? n / 2 : n;
+  f32x4 acc[NB];
+#pragma unroll
+  for (int j = 0; j < NB; ++j) acc[j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int t = 0; t < 27; ++t) {
+    int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+    if (p.flip) { dx = -dx; dy = -dy; dz = -dz; }
+    const int px = (xx + dx + n) % n, py = (yx + dy + n) % n, pz = (zx + dz + n) % n;
+    const uint16_t* wt = p.w + (int64_t)t * p.coutp * cin;
+    for (int c0 = 0; c0 < cin; c0 += 16) {
+      const int c = c0 + 4 * kg;
+      bf16x4 av;
+      if (c0 < p.ca) {
+        const int ax = p.a_half ? px / 2 : px, ay = p.a_half ? py / 2 : py, az = p.a_half ? pz / 2 : pz;
+        const int64_t ai = (((int64_t)bidx * na + ax) * na + ay) * na + az;
+        av = *reinterpret_cast<const bf16x4*>(p.a + ai * p.ca + c);
+      } else {
+        const int64_t bi = (((int64_t)bidx * n + px) * n + py) * n + pz;
+        av = *reinterpret_cast<const bf16x4*>(p.b + bi * p.cb + (c - p.ca));
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const bf16x4 bv = *reinterpret_cast<const bf16x4*>(wt + (int64_t)(j * 16 + row) * cin + c);
+        acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j], 0, 0, 0);
+      }
+    }
+  }
+  // D: lane holds rows (voxels) 4*kg + r, column (channel) j*16 + row
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int co = j * 16 + row;
+    if (co >= p.cout) continue;
+    const float bias = p.bias ? p.bias[co] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t v = tile + 4 * kg + r;
+      float y = acc[j][r] + bias;
+      if (p.act == 1) y = y > 0.f ? y : 0.1f * y;
+      if (p.out_f) p.out_f[v * p.cout + co] = y;
+      if (p.out_h) p.out_h[v * p.cout + co] = f2bf(y);
+    }
+  }
+}
+
+struct WgradArgs {
+  const uint16_t* g;  // [batch][n^3][cg] output-side gradient (bf16)
+  const uint16_t* a;  // source A [batch][na^3][ca]
+  const uint16_t* b;  // source B [batch][n^3][cb]
+  float* part;        // [nchunk][27][cg][cin]
+  int ca, cb, a_half, cg;
+  int n, batch;
+  int64_t chunk;      // voxels per chunk (multiple of 16)
+  int nchunk;
+};
+
+// one wavefront per (chunk, tap, co block, ci block): D[co][ci] += sum_k g[v_k][co] in[v_k + d][ci]
+__global__ void __launch_bounds__(64) wgrad_kernel(const WgradArgs p) {
+  const int lane = threadIdx.x;
+  const int cin = p.ca + p.cb;
+  const int nci = cin / 16, nco = p.cg / 16;
+  int id = blockIdx.x;
+  const int cib = id % nci; id /= nci;
+  const int cob = id % nco; id /= nco;
+  const int t = id % 27; id /= 27;
+  const int ch = id;
+  if (ch >= p.nchunk) return;
+  const int n = p.n;
+  const int64_t nv = (int64_t)n * n * n;
+  const int64_t total = nv * p.batch;
+  const int64_t v0 = (int64_t)ch * p.chunk;
+  const int64_t v1 = v0 + p.chunk < total ? v0 + p.chunk : total;
+  const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+  const int row = lane & 15, kg = lane >> 4;
+  const int co = cob * 16 + row, ci = cib * 16 + row;
+  const int na = p.a_half ? n / 2 : n;
+  f32x4 acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int64_t vk = v0; vk < v1; vk += 16) {
+    bf16x4 av, bv;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t v = vk + 4 * kg + j;
+      uint16_t ga = 0, sb = 0;
+      if (v < v1) {
+        ga = p.g[v * p.cg + co];
+        const int bidx = (int)(v / nv);
+        const int64_t vv = v - (int64_t)bidx * nv;
+        const int z = (int)(vv % n), y = (int)((vv / n) % n), x = (int)(vv / ((int64_t)n * n));
+        const int px = (x + dx + n) % n, py = (y + dy + n) % n, pz = (z + dz + n) % n;
+        if (ci < p.ca) {
+          const int ax = p.a_half ? px / 2 : px, ay = p.a_half ? py / 2 : py, az = p.a_half ? pz / 2 : pz;
+          sb = p.a[((((int64_t)bidx * na + ax) * na + ay) * na + az) * p.ca + ci];
+        } else {
+          sb = p.b[((((int64_t)bidx * n + px) * n + py) * n + pz) * p.cb + (ci - p.ca)];
+        }
+      }
+      av[j] = (short)ga;
+      bv[j] = (short)sb;
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  // D rows = co (4*kg + r), columns = ci (row)
+  float* out = p.part + (((int64_t)ch * 27 + t) * p.cg) * cin;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) out[(int64_t)(cob * 16 + 4 * kg + r) * cin + ci] = acc[r];
+}
+
+// dW[i] = sum over chunks of part[chunk][i] (fixed order)
+__global__ void reduce_chunks_kernel(const float* part, int nchunk, int64_t len, float* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * len + i];
+    out[i] = s;
+  }
+}
+
+// g = bf16(dy * act'(y)) with leaky relu slope 0.1 (act = 1) or identity (act = 0)
+__global__ void act_grad_kernel(const float* dy, const uint16_t* y, uint16_t* g, int64_t len, int act) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    float d = dy[i];
+    if (act == 1 && bf2f(y[i]) <= 0.f) d *= 0.1f;
+    g[i] = f2bf(d);
+  }
+}
+
+// bias gradient: db[c] = sum_v g[v][c] (g as f32 via dy * act'), one block per channel
+__global__ void bias_grad_kernel(const float* dy, const uint16_t* y, int64_t nvox, int c_total, int act, float* db) {
+  __shared__ double red[4];
+  const int c = blockIdx.x;
+  double s = 0.0;
+  for (int64_t v = threadIdx.x; v < nvox; v += blockDim.x) {
+    float d = dy[v * c_total + c];
+    if (act == 1 && bf2f(y[v * c_total + c]) <= 0.f) d *= 0.1f;
+    s += d;
+  }
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) db[c] = (float)t;
+}
+
+// 2x2x2 average pooling, bf16 -> bf16, [batch][n^3][C] -> [batch][(n/2)^3][C]
+__global__ void pool_kernel(const uint16_t* in, uint16_t* out, int n, int batch, int C) {
+  const int h = n / 2;
+  const int64_t total = (int64_t)batch * h * h * h * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    int64_t v = i / C;
+    const int z = (int)(v % h); v /= h;
+    const int y = (int)(v % h); v /= h;
+    const int x = (int)(v % h);
+    const int b = (int)(v / h);
+    float s = 0.f;
+    for (int a = 0; a < 8; ++a) {
+      const int xi = 2 * x + (a & 1), yi = 2 * y + ((a >> 1) & 1), zi = 2 * z + ((a >> 2) & 1);
+      s += bf2f(in[((((int64_t)b * n + xi) * n + yi) * n + zi) * C + c]);
+    }
+    out[i] = f2bf(0.125f * s);
+  }
+}
+
+// gradient of nearest upsampling / pooling: coarse[v][c] (+)= scale * sum of the 8 fine children
+// (fine is [batch][n^3][cf] and channels [c0, c0 + C) are taken); add = 1 accumulates.
+__global__ void upsum_kernel(const float* fine, int cf, int c0, float* coarse, int n, int batch, int C, float scale,
+                             int add) {
+  const int h = n / 2;
+  const int64_t total = (int64_t)batch * h * h * h * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    int64_t v = i / C;
+    const int z = (int)(v % h); v /= h;
+    const int y = (int)(v % h); v /= h;
+    const int x = (int)(v % h);
+    const int b = (int)(v / h);
+    float s = 0.f;
+    for (int a = 0; a < 8; ++a) {
+      const int xi = 2 * x + (a & 1), yi = 2 * y + ((a >> 1) & 1), zi = 2 * z + ((a >> 2) & 1);
+      s += fine[((((int64_t)b * n + xi) * n + yi) * n + zi) * cf + c0 + c];
+    }
+    coarse[i] = (add ? coarse[i] : 0.f) + scale * s;
+  }
+}
+
+// gradient of average pooling: fine[v][c] (+)= scale * coarse[v / 2][c]
+__global__ void updist_kernel(const float* coarse, float* fine, int n, int batch, int C, float scale, int add) {
+  const int h = n / 2;
+  const int64_t total = (int64_t)batch * n * n * n * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    int64_t v = i / C;
+    const int z = (int)(v % n); v /= n;
+    const int y = (int)(v % n); v /= n;
+    const int x = (int)(v % n);
+    const int b = (int)(v / n);
+    const float s = scale * coarse[((((int64_t)b * h + x / 2) * h + y / 2) * h + z / 2) * C + c];
+    fine[i] = add ? fine[i] + s : s;
+  }
+}
+
+// dst[v][c] (+)= src[v][c0 + c] for c < C (src has cs channels) -- channel slices of gradients
+__global__ void slice_add_kernel(const float* src, int cs, int c0, float* dst, int64_t nvox, int C, int add) {
+  const int64_t total = nvox * C;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(i % C);
+    const int64_t v = i / C;
+    const float s = src[v * cs + c0 + c];
+    dst[i] = add ? dst[i] + s : s;
+  }
+}
+
+// MSE loss on the first `cl` channels: loss partial per block and grad = 2 (pred - target) / count
+__global__ void mse_kernel(const float* pred, const float* target, float* grad, int64_t len, double inv_count,
+                           double* loss_part) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    const float d = pred[i] - target[i];
+    s += (double)d * d;
+    grad[i] = (float)(2.0 * inv_count) * d;
+  }
+  const double t = block_sum(s, red);
+  if (threadIdx.x == 0) loss_part[blockIdx.x] = t * inv_count;
+}
+
+// Adam step on FP32 master weights; refreshes the BF16 copy. Weights laid out [27][coutp][cin];
+// entries with co >= cout stay zero (padding of the output channels).
+__global__ void adam_kernel(float* w, const float* g, float* m1, float* m2, uint16_t* wh, int64_t len, float lr,
+                            float b1, float b2, float eps, float c1, float c2, float wd) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i] + wd * w[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float upd = lr * (a / c1) / (sqrtf(b / c2) + eps);
+    const float wn = w[i] - upd;
+    w[i] = wn;
+    if (wh) wh[i] = f2bf(wn);
+  }
+}
+
+__global__ void f2bf_kernel(const float* in, uint16_t* out, int64_t len) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = f2bf(in[i]);
+}
+
+// transpose-and-flip of weights for the data gradient: wt[26 - t][ci][co] = w[t][co][ci]
+// (output [27][cinp][coutp_src] with zero padding for ci >= cin)
+__global__ void wflip_kernel(const uint16_t* w, int coutp, int cin, uint16_t* wt, int cinp) {
+  const int64_t total = (int64_t)27 * cinp * coutp;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int co = (int)(i % coutp);
+    const int64_t r = i / coutp;
+    const int ci = (int)(r % cinp);
+    const int t = (int)(r / cinp);
+    wt[i] = ci < cin ? w[((int64_t)(26 - t) * coutp + co) * cin + ci] : (uint16_t)0;
+  }
+}
+
+static inline int nb_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (int)(b > 65536 ? 65536 : (b < 1 ? 1 : b));
+}
+
+}  // namespace nlfh
+
+using namespace nlfh;
+
+extern "C" {
+
+int nlfh_net_conv(const uint16_t* a, int ca, int a_half, const uint16_t* b, int cb, const uint16_t* w,
+                  const float* bias, int n, int batch, int cout, int coutp, int act, int flip, float* out_f,
+                  uint16_t* out_h, void* stream) {
+  if (!a || !w || (cb && !b) || ca % 16 || cb % 16 || coutp % 16 || coutp < cout || coutp > 96 || n < 2 ||
+      (a_half && n % 2) || batch < 1 || (n * n * n) % 16) {
+    set_error("nlfh_net_conv: invalid arguments (channels must be multiples of 16, coutp <= 96)");
+    return NLFH_EINVAL;
+  }
+  ConvArgs p{a, b, w, bias, out_f, out_h, ca, cb, a_half, n, batch, cout, coutp, act, flip};
+  const int64_t tiles = ((int64_t)n * n * n * batch + 15) / 16;
+  const int blocks = (int)((tiles + 3) / 4);
+  hipStream_t st = (hipStream_t)stream;
+  switch (coutp / 16) {
+    case 1: hipLaunchKernelGGL(conv3d_kernel<1>, dim3(blocks), dim3(256), 0, st, p); break;
+    case 2: hipLaunchKernelGGL(conv3d_kernel<2>, dim3(blocks), dim3(256), 0, st, p); break;
+    case 3: hipLaunchKernelGGL(conv3d_kernel<3>, dim3(blocks), dim3(256), 0, st, p); break;
+    case 4: hipLaunchKernelGGL(conv3d_kernel<4>, dim3(blocks), dim3(256), 0, st, p); break;
+    case 5: hipLaunchKernelGGL(conv3d_kernel<5>, dim3(blocks), dim3(256), 0, st, p); break;
+    default: hipLaunchKernelGGL(conv3d_kernel<6>, dim3(blocks), dim3(256), 0, st, p); break;
+  }
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_wgrad(const uint16_t* g, int cg, const uint16_t* a, int ca, int a_half, const uint16_t* b, int cb,
+                   int n, int batch, float* part, long long part_len, float* dw, void* stream) {
+  const int cin = ca + cb;
+  if (!g || !a || (cb && !b) || cg % 16 || ca % 16 || cb % 16 || (n * n * n) % 16) {
+    set_error("nlfh_net_wgrad: invalid arguments");
+    return NLFH_EINVAL;
+  }
+  const int64_t total = (int64_t)n * n * n * batch;
+  const int64_t wlen = (int64_t)27 * cg * cin;
+  int64_t chunk = 4096;
+  int64_t nchunk = (total + chunk - 1) / chunk;
+  while (nchunk * wlen > part_len && chunk < total) {
+    chunk *= 2;
+    nchunk = (total + chunk - 1) / chunk;
+  }
+  if (nchunk * wlen > part_len) {
+    set_error("nlfh_net_wgrad: partial buffer too small");
+    return NLFH_EINVAL;
+  }
+  WgradArgs p{g, a, b, part, ca, cb, a_half, cg, n, batch, chunk, (int)nchunk};
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t blocks = nchunk * 27 * (cg / 16) * (cin / 16);
+  hipLaunchKernelGGL(wgrad_kernel, dim3((unsigned)blocks), dim3(64), 0, st, p);
+  hipLaunchKernelGGL(reduce_chunks_kernel, dim3(nb_for(wlen)), dim3(256), 0, st, part, (int)nchunk, wlen, dw);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_act_grad(const float* dy, const uint16_t* y, uint16_t* g, long long len, int act, void* stream) {
+  hipLaunchKernelGGL(act_grad_kernel, dim3(nb_for(len)), dim3(256), 0, (hipStream_t)stream, dy, y, g, (int64_t)len, act);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_bias_grad(const float* dy, const uint16_t* y, long long nvox, int c, int act, float* db, void* stream) {
+  hipLaunchKernelGGL(bias_grad_kernel, dim3(c), dim3(256), 0, (hipStream_t)stream, dy, y, (int64_t)nvox, c, act, db);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_pool(const uint16_t* in, uint16_t* out, int n, int batch, int c, void* stream) {
+  const int64_t total = (int64_t)batch * (n / 2) * (n / 2) * (n / 2) * c;
+  hipLaunchKernelGGL(pool_kernel, dim3(nb_for(total)), dim3(256), 0, (hipStream_t)stream, in, out, n, batch, c);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_upsum(const float* fine, int cf, int c0, float* coarse, int n, int batch, int c, float scale, int add,
+                   void* stream) {
+  const int64_t total = (int64_t)batch * (n / 2) * (n / 2) * (n / 2) * c;
+  hipLaunchKernelGGL(upsum_kernel, dim3(nb_for(total)), dim3(256), 0, (hipStream_t)stream, fine, cf, c0, coarse, n,
+                     batch, c, scale, add);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_updist(const float* coarse, float* fine, int n, int batch, int c, float scale, int add, void* stream) {
+  const int64_t total = (int64_t)batch * n * n * n * c;
+  hipLaunchKernelGGL(updist_kernel, dim3(nb_for(total)), dim3(256), 0, (hipStream_t)stream, coarse, fine, n, batch, c,
+                     scale, add);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_slice_add(const float* src, int cs, int c0, float* dst, long long nvox, int c, int add, void* stream) {
+  hipLaunchKernelGGL(slice_add_kernel, dim3(nb_for(nvox * c)), dim3(256), 0, (hipStream_t)stream, src, cs, c0, dst,
+                     (int64_t)nvox, c, add);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_mse(const float* pred, const float* target, float* grad, long long len, double* loss_part, int nparts,
+                 void* stream) {
+  hipLaunchKernelGGL(mse_kernel, dim3(nparts), dim3(256), 0, (hipStream_t)stream, pred, target, grad, (int64_t)len,
+                     1.0 / (double)len, loss_part);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_adam(float* w, const float* g, float* m1, float* m2, uint16_t* wh, long long len, float lr, float b1,
+                  float b2, float eps, int step, float wd, void* stream) {
+  const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+  hipLaunchKernelGGL(adam_kernel, dim3(nb_for(len)), dim3(256), 0, (hipStream_t)stream, w, g, m1, m2, wh,
+                     (int64_t)len, lr, b1, b2, eps, c1, c2, wd);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_f2bf(const float* in, uint16_t* out, long long len, void* stream) {
+  hipLaunchKernelGGL(f2bf_kernel, dim3(nb_for(len)), dim3(256), 0, (hipStream_t)stream, in, out, (int64_t)len);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_wflip(const uint16_t* w, int coutp, int cin, uint16_t* wt, int cinp, void* stream) {
+  const int64_t total = (int64_t)27 * cinp * coutp;
+  hipLaunchKernelGGL(wflip_kernel, dim3(nb_for(total)), dim3(256), 0, (hipStream_t)stream, w, coutp, cin, wt, cinp);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+}  // extern "C"
