@@ -51,15 +51,17 @@ def oh_average(f):
 
 def random_fourier_phi(n, rng, kmax=2.0):
     """Sum of Fourier modes 0 < |k| <= kmax with N(0,1) cosine and sine coefficients, averaged over
-    the 48 Oh maps."""
+    the 48 Oh maps. a cos(k.x) + b sin(k.x) = Re((a - i b) e^{i k.x}), and e^{i k.x} is the outer
+    product of three 1-D exponentials, so the sum is three small contractions over a 5^3 coefficient
+    grid."""
     x = voxel_centres(n)
     ks = [k for k in itertools.product(range(-2, 3), repeat=3) if 0 < np.dot(k, k) <= kmax * kmax]
-    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
-    phi = np.zeros((n, n, n))
+    coef = np.zeros((5, 5, 5), np.complex128)
     for k in ks:
         a, b = rng.standard_normal(2)
-        arg = k[0] * X + k[1] * Y + k[2] * Z
-        phi += a * np.cos(arg) + b * np.sin(arg)
+        coef[k[0] + 2, k[1] + 2, k[2] + 2] += a - 1j * b
+    e = np.exp(1j * np.outer(x, np.arange(-2, 3)))  # (n, 5)
+    phi = np.einsum("abc,ia,jb,kc->ijk", coef, e, e, e, optimize=True).real
     return oh_average(phi)
 
 
@@ -119,10 +121,13 @@ def solve_eta(u, v, beta):
     hi = u.max() + 40.0 / beta
     for _ in range(200):
         mid = 0.5 * (lo + hi)
+        old = (lo, hi)
         if np.mean(_sigmoid(beta * (u - mid))) > v:
             lo = mid
         else:
             hi = mid
+        if (lo, hi) == old or lo == hi:
+            break  # the bracket has stopped moving: the remaining steps would not change it
     eta = 0.5 * (lo + hi)
     for _ in range(3):
         s = _sigmoid(beta * (u - eta))
