@@ -1,18 +1,28 @@
-"""Benchmark: converged 6-load-case homogenizations per second of 64^3 cells on one MI355X.
+"""Benchmark: converged 6-load-case homogenizations per second on one MI355X.
 
-A step homogenizes a batch of seeded random cubic-symmetric 64^3 cells (Fourier modes |k| <= 2,
-48-fold Oh averaging, Gaussian filter r = 1.5, beta in {2, 4, 8, 16}, volume in [0.2, 0.5],
-emin = 1e-4, p = 3): for every cell the 6 unit-strain correctors by PCG to a relative residual of
-1e-8 and the 6x6 C. Inputs (rho) are resident in HBM before the timed region; the timed region is
-rho -> C on the device (plan and symbol creation excluded, as for the reference's __init__).
+Workloads (`--workload`):
+  cell64     (default) seeded random cubic-symmetric 64^3 cells (Fourier modes |k| <= 2, 48-fold Oh
+             averaging, Gaussian filter r = 1.5, beta in {2, 4, 8, 16}, volume in [0.2, 0.5],
+             emin = 1e-4, p = 3), `--cells` of them homogenized together per step by the batched
+             solver (one launch per kernel and PCG iteration for all cells).
+  batch256   256 such cells of uneven sizes per step: 128 at 32^3, 96 at 48^3, 32 at 64^3.
+  stress128  16 cells at 128^3 with emin = 1e-6, volume 0.15, beta = 16.
+For every cell: the 6 unit-strain correctors by PCG to a true FP64 relative residual of 1e-8 and the
+6x6 C. Inputs (rho) are resident in HBM before the timed region; the timed region is rho -> C on the
+device (plan, symbol and tiling selection excluded, as the reference's __init__ is). With
+`--learned WEIGHTS` the network's prediction (inference included in the timed region) starts the
+solve.
 
-Prints one JSON line. `roofline` is for the matvec kernel (K(s) applied to 6 right-hand sides of
-one 64^3 cell), timed live with HIP events on the stream it runs on; `cpu_baseline` times the
-FP64 oracle on this host's cores over a bounded number of PCG iterations of the same cell and
-scales by the GPU's iteration count.
+Prints one JSON line. `roofline` is for the matvec kernel of the timed solver precision over the
+step's cells, timed live with HIP events on the stream it runs on (algorithmic bytes
+N (2 * 3 * 6 * w + w_s) per application). `cpu_baseline` times the FP64 oracle on this host's
+cores over a bounded number of PCG iterations of cell 0 and scales by the GPU's iteration count.
+`plain_fp64_gpu` is the reference algorithm (plain FP64 PCG, one cell at a time) on the same cells.
 """
 import argparse
+import ctypes
 import json
+import multiprocessing as mp
 import os
 import sys
 import time
@@ -31,12 +41,39 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cells", type=int, default=4, help="64^3 cells per step")
-    ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128"])
+    ap.add_argument("--cells", type=int, default=8, help="64^3 cells per step (cell64)")
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--learned", default=None, help="warm-start weights (npz)")
+    ap.add_argument("--no-tune", action="store_true", help="default tiling instead of the autotuner")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison")
+    ap.add_argument("--check-fp64", action="store_true", help="also solve every cell in FP64 and report C errors")
     ap.add_argument("--cpu-iters", type=int, default=6, help="PCG iterations timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
+
+
+def _cell(job):
+    n, seed, kind = job
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    if kind == "stress":
+        rho, meta = pkg.cells.stress_cell(seed, n)
+    else:
+        rho, meta = pkg.cells.random_cell(n, seed)
+    return rho, meta
+
+
+def workload(args):
+    if args.workload == "cell64":
+        jobs = [(64, s, "random") for s in range(args.cells)]
+        return jobs, 1e-4, "cell64"
+    if args.workload == "batch256":
+        jobs = ([(32, s, "random") for s in range(128)] + [(48, s, "random") for s in range(128, 224)] +
+                [(64, s, "random") for s in range(224, 256)])
+        return jobs, 1e-4, "batch256"
+    jobs = [(128, s, "stress") for s in range(16)]
+    return jobs, 1e-6, "stress128"
 
 
 def main():
@@ -44,86 +81,75 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
-    import torch
-
+    jobs, emin, wname = workload(args)
+    tol = 1e-8
+    # densities on host worker processes before the GPU runtime starts in this process
     import __graft_entry__ as ge
-
     pkg = ge.load_package()
+    with mp.get_context("fork").Pool(min(16, len(jobs))) as pool:
+        made = pool.map(_cell, jobs)
+    rhos = [r for r, _ in made]
+    metas = [m for _, m in made]
+    shapes = [r.shape for r in rhos]
+
+    import torch
     pkg.load_library()
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
-    n = args.n
-    emin, tol = 1e-4, 1e-8
-    seeds = list(range(args.cells))
-    rhos, metas = [], []
-    for sd in seeds:
-        rho, meta = pkg.cells.random_cell(n, sd)
-        rhos.append(torch.tensor(rho, device=dev))
-        metas.append(meta)
-    lh = pkg.LinearHomogenization((n, n, n), emin=emin, tol=tol, maxit=20000, precision=args.precision)
+    net = None
+    if args.learned:
+        from nlfh_amd.learned import WarmStartNet
+        net = WarmStartNet.load(args.learned)
+    bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
+                                   warm_start=net)
+    t_tune = time.perf_counter()
+    if not args.no_tune:
+        bh.autotune()
+    t_tune = time.perf_counter() - t_tune
+    rho_d = bh.pack(rhos)
 
     def step():
-        out = []
-        its = []
-        for r in rhos:
-            C, u, it = lh.C_full(r)
-            if not lh.last_info["converged"]:
-                raise RuntimeError(f"cell did not converge: {lh.last_info}")
-            out.append(C)
-            its.append(it)
-        return out, its
+        C, u, it = bh.C_full(rho_d)
+        if not all(bh.last_info["converged"]):
+            raise RuntimeError(f"cells did not converge: {bh.last_info['converged']}")
+        return C, it, bh.last_info.get("fallback", False)
 
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    its_all = []
+    its_all, fallbacks = [], 0
     for _ in range(args.steps):
-        Cs, its = step()
-        its_all.append(its)
+        C, its, fb = step()
+        its_all.append(its.tolist())
+        fallbacks += int(fb)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    ncell = len(jobs)
     ms_step = 1e3 * dt / args.steps
-    value = args.cells * args.steps / dt
+    value = ncell * args.steps / dt
 
-    # roofline: the matvec kernel, 6 right-hand sides of one cell, FP64, timed with HIP events on
-    # the calling stream (the library waits on it and makes it wait for its own stream)
-    s = lh.stiffness_scale(rhos[0])
-    u = torch.randn((6, n, n, n, 3), dtype=torch.float64, device=dev)
-    up = pkg.to_planar(u, (n, n, n))
-    out = torch.empty_like(up)
-    import ctypes
-    lib = pkg.load_library()
-    stream = torch.cuda.current_stream()
-    sp = ctypes.c_void_p(stream.cuda_stream)
+    roofline = matvec_roofline(pkg, bh, rho_d, args.precision)
 
-    def mv():
-        pkg._lib.check(lib.nlfh_apply_K(lh._plan.handle, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(up.data_ptr()),
-                                        ctypes.c_void_p(out.data_ptr()), 6, 0, sp))
-
-    for _ in range(3):
-        mv()
-    reps = 50
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(reps):
-        mv()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    t_mv = e0.elapsed_time(e1) / reps * 1e-3
-    N = n ** 3
-    bytes_mv = N * (2 * 3 * 6 * 8 + 8)
-    achieved = bytes_mv / t_mv / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "matvec_kernel<double> (K(s) u, 6 RHS, 64^3)", "avg_us": round(t_mv * 1e6, 2)}
+    extra = {}
+    if args.check_fp64 or args.workload == "stress128":
+        extra["fp64_check"] = fp64_check(pkg, shapes, rho_d, emin, tol, np.asarray(C))
+    plain = None
+    if not args.no_plain and args.workload == "cell64":
+        plain = plain_fp64(pkg, rhos, emin, tol, dev)
+        plain["speedup"] = round(value / plain["value"], 3)
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rhos[0].cpu().numpy(), emin, tol, float(np.mean(its_all[-1] if its_all else [0])),
-                           its_all[-1][0], args.cpu_iters)
+        # the oracle runs the plain FP64 algorithm: scale by that algorithm's iteration count
+        it_fp64 = plain["pcg_iterations"][0] if plain else extra.get("fp64_check", {}).get("fp64_iterations",
+                                                                                        its_all[-1])[0]
+        cpu = cpu_baseline(rhos[0], emin, tol, it_fp64, args.cpu_iters)
 
+    dtype = "f64" if args.precision == "fp64" else "f32-inner/f64-refinement"
+    desc = {"cell64": f"{ncell} seeded random cubic-symmetric 64^3 cells per step (seeds 0-{ncell - 1})",
+            "batch256": "256 seeded random cubic-symmetric cells per step: 128 x 32^3, 96 x 48^3, 32 x 64^3",
+            "stress128": "16 seeded high-contrast cells at 128^3 (emin 1e-6, volume 0.15, beta 16)"}[wname]
     res = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -135,22 +161,106 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32-inner/f64-refinement",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
-            "workload": f"{args.cells} seeded random cubic-symmetric {n}^3 cells per step (seeds {seeds[0]}-{seeds[-1]}),"
-                        f" 6 unit strains, PCG to rel. residual {tol:g}, emin {emin:g}, C 6x6",
-            "cells_per_step": args.cells, "grid": n, "rhs": 6, "tol": tol, "precision": args.precision,
+            "workload": f"{desc}, 6 unit strains, PCG to true rel. residual {tol:g}, emin {emin:g}, C 6x6, "
+                        f"batched solver ({args.precision})" + (", learned warm start" if net else ""),
+            "cells_per_step": ncell, "rhs": 6, "tol": tol, "precision": args.precision,
+            "tiling": "autotuned" if not args.no_tune else "default", "tune_seconds": round(t_tune, 2),
+            "learned_start": bool(net), "fallback_steps": fallbacks,
             "pcg_iterations": its_all[-1] if its_all else [],
-            "cells": metas,
+            "cells": metas if ncell <= 16 else f"{ncell} cells (seeds in the workload description)",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "plain_fp64_gpu": plain,
     }
+    res.update(extra)
     print(json.dumps(res), flush=True)
 
 
-def cpu_baseline(rho, emin, tol, mean_it, it0, iters):
+def matvec_roofline(pkg, bh, rho_d, precision):
+    """The matvec over all cells of the step (6 right-hand sides), in the precision of the inner
+    solve, timed with HIP events over repeated launches on the current stream."""
+    import torch
+    lib = pkg.load_library()
+    f32 = precision == "mixed"
+    s = bh.stiffness_scale(rho_d)
+    nf = bh.tot * 3 * 6
+    dt = torch.float32 if f32 else torch.float64
+    u = torch.randn(nf, dtype=dt, device=rho_d.device)
+    out = torch.empty_like(u)
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    args = (bh._plan.handle, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+            ctypes.c_void_p(out.data_ptr()), 6, 1 if f32 else 0, sp)
+
+    def mv():
+        pkg._lib.check(lib.nlfh_apply_K(*args))
+
+    for _ in range(3):
+        mv()
+    reps = max(5, min(50, int(2e8 // max(bh.tot, 1))))
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        mv()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_mv = e0.elapsed_time(e1) / reps * 1e-3
+    w = 4 if f32 else 8
+    # the FP32 operator converts s to FP32 in a separate pass each call; its bytes are not counted
+    bytes_mv = bh.tot * (2 * 3 * 6 * w + w)
+    achieved = bytes_mv / t_mv / 1e9
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "matvec_traffic.json")
+    kname = f"matvec_kernel<{'float' if f32 else 'double'}>"
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        key = f"{kname} voxels={bh.tot}"
+        if key in tr:
+            traffic = tr[key]["hbm_bytes"]
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": f"{kname} (K(s) u, 6 RHS, {bh.tot} voxels = {len(bh.shapes)} cells)",
+            "algorithmic_bytes": bytes_mv, "avg_us": round(t_mv * 1e6, 2)}
+
+
+def plain_fp64(pkg, rhos, emin, tol, dev, steps=1):
+    """The reference algorithm on the GPU: plain FP64 PCG (default tiling), one cell at a time."""
+    import torch
+    shapes = sorted(set(r.shape for r in rhos))
+    lhs = {sh: pkg.LinearHomogenization(sh, emin=emin, tol=tol, maxit=40000, precision="fp64") for sh in shapes}
+    rd = [torch.tensor(r, device=dev) for r in rhos]
+    lhs[rhos[0].shape].C_full(rd[0])  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for _ in range(steps):
+        for r in rd:
+            lh = lhs[tuple(r.shape)]
+            _, _, it = lh.C_full(r)
+            its.append(it)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": round(len(rd) * steps / dt, 3), "unit": "homogenizations/s", "pcg_iterations": its[:len(rd)]}
+
+
+def fp64_check(pkg, shapes, rho_d, emin, tol, C):
+    """Every cell solved again by the batched plain FP64 PCG; max over cells and entries of
+    |dC_ij| / max|C| (the reference's normalisation)."""
+    bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision="fp64")
+    C64, _, it = bh.C_full(rho_d)
+    C64 = np.asarray(C64)
+    err = np.abs(C - C64) / np.abs(C64).max(axis=(1, 2), keepdims=True)
+    return {"max_rel_err_C": float(err.max()), "fp64_iterations": it.tolist(),
+            "fp64_converged": all(bh.last_info["converged"])}
+
+
+def cpu_baseline(rho, emin, tol, it0, iters):
     """Oracle (numpy FP64, FFTs on all host cores) on cell 0: setup + `iters` PCG iterations + C,
     scaled to the GPU's iteration count for that cell."""
     from oracle import linear_ref as O
@@ -169,8 +279,9 @@ def cpu_baseline(rho, emin, tol, mean_it, it0, iters):
     per_it = (t2 - t1) / max(iters, 1)
     t_cell = (t1 - t0) + per_it * it0 + (t3 - t2)
     return {"value": round(1.0 / t_cell, 6), "unit": "homogenizations/s", "cores": cores, "kind": "port",
-            "sample": f"numpy FP64 oracle on cell 0: setup + {iters} timed PCG iterations "
-                      f"({per_it:.3f} s/iteration) + C assembly, scaled to the {it0} iterations the cell needs",
+            "sample": f"numpy FP64 oracle on cell 0 ({rho.shape[0]}^3): setup + {iters} timed PCG iterations "
+                      f"({per_it:.3f} s/iteration) + C assembly, scaled to the {it0} iterations the GPU "
+                      f"solve of that cell took",
             "seconds_per_homogenization": round(t_cell, 2)}
 
 

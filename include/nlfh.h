@@ -54,6 +54,15 @@ int nlfh_plan_destroy(nlfh_plan* plan);
 int nlfh_plan_set_tiling(nlfh_plan* plan, int shape_index, int ty, int xc, int zpt);
 int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* xc, int* zpt);
 
+/* Energy-optimal (Ritz-Galerkin) start from predicted correctors: for every cell and right-hand side
+ * I, u_out_I = sum_J a_IJ u_in_J with a_I minimising the PCG energy 1/2 u.Ku + f_I.u over the span
+ * of the cell's m predicted fields (G a_I = -(f_I . u_in_J)_J, G_JK = u_in_J . K u_in_K, f = load).
+ * A cell whose Gram matrix is singular or non-finite gets the zero start. u_out may equal u_in;
+ * coef_out (host, ncells x m x m, nullable) receives a. No counterpart in the reference: it feeds
+ * the u0 argument of the solve (nlfiber/linear.py:212-216) from a learned prediction. */
+int nlfh_ritz_start(nlfh_plan* plan, const double* s, const double* Es_host, int m, const double* u_in,
+                    double* u_out, double* coef_out_host, void* stream);
+
 /* Loosest relative tolerance of an FP32 inner solve of NLFH_PREC_MIXED (default 1e-2); each inner
  * solve targets max(inner_tol, 0.5 * tol * ||b|| / ||r||), the reduction its refinement step needs.
  * No counterpart in the reference (which has no mixed precision). */

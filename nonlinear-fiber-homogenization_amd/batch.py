@@ -20,7 +20,7 @@ class BatchedHomogenization:
     (the unit strains 0..m-1). precision: "fp64" or "mixed"."""
 
     def __init__(self, shapes, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-8, maxit=4000, precision="mixed", m=6,
-                 tune=False):
+                 tune=False, warm_start=None):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.E, self.nu, self.emin, self.p, self.tol, self.maxit = E, nu, emin, p, tol, maxit
         if precision not in ("fp64", "mixed"):
@@ -34,6 +34,10 @@ class BatchedHomogenization:
         self.off = np.concatenate([[0], np.cumsum(self.nvox)]).astype(np.int64)
         self.tot = int(self.off[-1])
         self.last_info = {}
+        if isinstance(warm_start, str):
+            from .learned import WarmStartNet
+            warm_start = WarmStartNet.load(warm_start)
+        self.warm_start = warm_start
         if tune:
             self.autotune()
 
@@ -73,13 +77,33 @@ class BatchedHomogenization:
                                                   _ptr(s), None, _stream()))
         return s
 
-    def C_full(self, rhos, u0=None):
-        """Returns (C (ncells, m, m) host float64, u (packed device buffer), iterations (ncells,)).
-        `rhos` is a list of arrays or an already packed device array; u0 a packed buffer or None."""
-        rho = rhos if isinstance(rhos, torch.Tensor) and rhos.dim() == 1 else self.pack(rhos)
-        s = self.stiffness_scale(rho)
-        Es = np.eye(6)[: self.m]
-        Es_np, Es_p = _es_host(Es)
+    def warm_start_fields(self, rho_packed, s, max_batch=32):
+        """Learned start for every cell: the network's prediction per group of equal cubic shapes
+        (cells it cannot take start from zero), then the per-cell energy-optimal combination
+        (nlfh_ritz_start). Returns a packed field buffer."""
+        if self.m != 6:
+            raise ValueError("the learned start predicts all six unit-strain correctors (m = 6)")
+        P = torch.zeros(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
+        groups = {}
+        for i, sh in enumerate(self.shapes):
+            if sh[0] == sh[1] == sh[2] and sh[0] % 4 == 0 and sh[0] ** 3 % 16 == 0:
+                groups.setdefault(sh, []).append(i)
+        for sh, idx in groups.items():
+            n = sh[0]
+            for k in range(0, len(idx), max_batch):
+                part = idx[k:k + max_batch]
+                r = torch.stack([rho_packed[self.off[i]:self.off[i + 1]].view(sh) for i in part])
+                pred = self.warm_start.predict(r)  # (B, 6, n, n, n, 3)
+                planar = pred.permute(0, 1, 5, 2, 3, 4).reshape(len(part), -1)
+                for j, i in enumerate(part):
+                    a = int(self.off[i]) * 3 * self.m
+                    P[a:a + planar.shape[1]] = planar[j]
+        Es_np, Es_p = _es_host(np.eye(6))
+        _lib.check(self._lib.nlfh_ritz_start(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(P), _ptr(P), None,
+                                             _stream()))
+        return P
+
+    def _solve(self, s, Es_p, u0):
         nc = len(self.shapes)
         u = torch.empty(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
         it = (ctypes.c_int * nc)()
@@ -88,9 +112,30 @@ class BatchedHomogenization:
         prec = 0 if self.precision == "fp64" else 1
         _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(u0), _ptr(u), float(self.tol),
                                         int(self.maxit), prec, it, cv, rr, _stream()))
+        info = dict(it=list(it), converged=[bool(c) for c in cv], relres=np.asarray(list(rr)).reshape(nc, self.m))
+        return u, info
+
+    def C_full(self, rhos, u0=None):
+        """Returns (C (ncells, m, m) host float64, u (packed device buffer), iterations (ncells,)).
+        `rhos` is a list of arrays or an already packed device array; u0 a packed buffer or None.
+        With a warm_start network and no u0, the learned start is used; if any cell then fails to
+        converge, the batch is solved again from zero (last_info["fallback"])."""
+        rho = rhos if isinstance(rhos, torch.Tensor) and rhos.dim() == 1 else self.pack(rhos)
+        s = self.stiffness_scale(rho)
+        Es = np.eye(6)[: self.m]
+        Es_np, Es_p = _es_host(Es)
+        nc = len(self.shapes)
+        warm = u0 is None and self.warm_start is not None
+        if warm:
+            u0 = self.warm_start_fields(rho, s)
+        u, info = self._solve(s, Es_p, u0)
+        info["warm_start"], info["fallback"] = warm, False
+        if warm and (not all(info["converged"]) or not np.all(np.isfinite(info["relres"]))):
+            it_warm = info["it"]
+            u, info = self._solve(s, Es_p, None)
+            info.update(warm_start=True, fallback=True, it_warm=it_warm)
         C = torch.empty((nc, self.m, self.m), dtype=torch.float64, device=self._dev)
         _lib.check(self._lib.nlfh_bilinear(self._plan.handle, _ptr(s), Es_p, _ptr(u), self.m, _ptr(C), None,
                                            _stream()))
-        self.last_info = dict(it=list(it), converged=[bool(c) for c in cv],
-                              relres=np.asarray(list(rr)).reshape(nc, self.m))
-        return C.cpu(), u, np.asarray(list(it))
+        self.last_info = info
+        return C.cpu(), u, np.asarray(info["it"])

@@ -70,6 +70,66 @@ __global__ void __launch_bounds__(256) reduce_kernel(const VecArgs va, const T* 
   }
 }
 
+// Gram partials of two field sets: part[I*m + J][chunk] = sum over the chunk's voxels and the 3
+// components of a_I * b_J (a, b component-planar with m fields per cell).
+__global__ void __launch_bounds__(256) gram_kernel(const VecArgs va, const double* __restrict__ a,
+                                                   const double* __restrict__ b, double* __restrict__ part) {
+  __shared__ double red[4];
+  const ChunkItem ch = va.chunks[blockIdx.x];
+  const CellDesc cd = va.cells[ch.cell];
+  const int m = va.m;
+  double acc[36];
+#pragma unroll
+  for (int k = 0; k < 36; ++k) acc[k] = 0.0;
+  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+    for (int c = 0; c < 3; ++c) {
+      double av[6], bv[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) {
+        const int64_t off = cd.off_field + ((int64_t)r * 3 + c) * cd.nvox + i;
+        av[r] = r < m ? a[off] : 0.0;
+        bv[r] = r < m ? b[off] : 0.0;
+      }
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[r * 6 + q] += av[r] * bv[q];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      if (r < m && q < m) {  // uniform over the block
+        const double t = block_sum(acc[r * 6 + q], red);
+        if (threadIdx.x == 0) part[(int64_t)(r * m + q) * va.nchunks + blockIdx.x] = t;
+      }
+    }
+}
+
+// out_I = sum_J coef[cell][I][J] x_J per voxel and component (out may alias x: each thread reads all
+// m inputs of its voxel before writing).
+__global__ void __launch_bounds__(256) combine_kernel(const VecArgs va, const double* x, const double* coef,
+                                                      double* out) {
+  const ChunkItem ch = va.chunks[blockIdx.x];
+  const CellDesc cd = va.cells[ch.cell];
+  const int m = va.m;
+  const double* A = coef + (int64_t)ch.cell * m * m;
+  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+    for (int c = 0; c < 3; ++c) {
+      double xv[6];
+#pragma unroll
+      for (int r = 0; r < 6; ++r) xv[r] = r < m ? x[cd.off_field + ((int64_t)r * 3 + c) * cd.nvox + i] : 0.0;
+      for (int r = 0; r < m; ++r) {
+        double y = 0.0;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) y += (q < m ? A[r * m + q] : 0.0) * xv[q];
+        out[cd.off_field + ((int64_t)r * 3 + c) * cd.nvox + i] = y;
+      }
+    }
+  }
+}
+
 // Per-cell, per-row sums of partials in a fixed order: out[cell*out_stride + row] = sum over the
 // cell's item range of part[row][item], times `scale`, or times 1/nvox of the cell when `cells` is
 // given (means). One thread per (cell, row).
@@ -399,6 +459,16 @@ hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part
 }
 template hipError_t launch_reduce<double>(const VecArgs&, const double*, const double*, double*, int, hipStream_t);
 template hipError_t launch_reduce<float>(const VecArgs&, const float*, const float*, double*, int, hipStream_t);
+
+hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st) {
+  hipLaunchKernelGGL(gram_kernel, dim3(va.nchunks), dim3(256), 0, st, va, a, b, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(combine_kernel, dim3(va.nchunks), dim3(256), 0, st, va, x, coef, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_sum_partials(const double* part, int nitems, const int* range, int ncells, int rows,
                                double scale, const CellDesc* cells, int out_stride, double* out, hipStream_t st) {

@@ -165,6 +165,7 @@ struct nlfh_plan {
   double* d_part_c = nullptr;
   double* d_part_b = nullptr;
   double* d_part_e = nullptr;
+  double* d_ritz = nullptr;    // Ritz start: G, H (ncells*36 each) and coefficients (ncells*36)
   double* d_strain = nullptr;
   double* d_params = nullptr;  // per cell: [0, ncells) FP64 tolerance, [ncells, 2 ncells) inner FP32 tolerance
   int* d_iparams = nullptr;    // maxit64, maxit32
@@ -1246,6 +1247,87 @@ int nlfh_bilinear(nlfh_plan* p, const double* s, const double* Es, const double*
   HIP_TRY(launch_sum_partials(p->d_part_e, (int)p->chunks.size(), p->d_chunk_range, p->ncells, m * m, 1.0, p->d_cells,
                               m * m, C_out, st));
   return NLFH_OK;
+}
+
+int nlfh_ritz_start(nlfh_plan* p, const double* s, const double* Es, int m, const double* u_in, double* u_out,
+                    double* coef_out, void* stream) {
+  CHECK_ARG(p && s && Es && u_in && u_out, "null argument");
+  TRY(check_m(p, m));
+  TRY(enter(p, stream));
+  TRY(get_ready64(p));
+  const int nc = p->ncells;
+  if (!p->d_ritz) TRY(p->alloc(&p->d_ritz, (size_t)3 * 36 * nc));
+  hipStream_t st = p->stream;
+  Work<double>& w = p->w64;
+  TRY(set_strain(p, Es, m, st));
+  HIP_TRY(launch_copy_d(u_in, w.X, p->field_len_m(m), st));
+  // K P and the load f = K_uE E; the correctors solve K u = -f (nlfiber/linear.py:212-216)
+  TRY(run_matvec<double>(p, w.X, nullptr, nullptr, w.Ap, s, nullptr, nullptr, nullptr, m, 0, 0, 0, 1.0, st));
+  TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, w.B, s, nullptr, nullptr, nullptr, m, 1, 0, 0, 1.0, st));
+  VecArgs va = vec_args(p, m, nullptr);
+  const int nch = (int)p->chunks.size();
+  double* G = p->d_ritz;
+  double* H = p->d_ritz + (size_t)36 * nc;
+  double* A = p->d_ritz + (size_t)72 * nc;
+  HIP_TRY(launch_gram(va, w.X, w.Ap, p->d_part_e, st));
+  HIP_TRY(launch_sum_partials(p->d_part_e, nch, p->d_chunk_range, nc, m * m, 1.0, nullptr, m * m, G, st));
+  HIP_TRY(launch_gram(va, w.B, w.X, p->d_part_e, st));
+  HIP_TRY(launch_sum_partials(p->d_part_e, nch, p->d_chunk_range, nc, m * m, 1.0, nullptr, m * m, H, st));
+  std::vector<double> hg((size_t)2 * m * m * nc);
+  HIP_TRY(hipMemcpyAsync(hg.data(), G, (size_t)m * m * nc * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(hg.data() + (size_t)m * m * nc, H, (size_t)m * m * nc * sizeof(double),
+                         hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  // per cell and right-hand side I: G a_I = -H[I][:], G_JK = P_J . K P_K, H_IJ = f_I . P_J
+  std::vector<double> coef((size_t)m * m * nc, 0.0);
+  for (int c = 0; c < nc; ++c) {
+    const double* g = hg.data() + (size_t)c * m * m;
+    const double* h = hg.data() + (size_t)m * m * nc + (size_t)c * m * m;
+    // Cholesky of G with a relative diagonal floor; a field with no energy (P_J = 0) gets weight 0
+    long double L[6][6] = {};
+    double dmax = 0.0;
+    for (int j = 0; j < m; ++j) dmax = std::max(dmax, g[j * m + j]);
+    bool ok = std::isfinite(dmax) && dmax > 0.0;
+    std::vector<int> act;
+    for (int j = 0; ok && j < m; ++j)
+      if (g[j * m + j] > 1e-14 * dmax) act.push_back(j);
+    const int na = (int)act.size();
+    for (int i = 0; ok && i < na; ++i) {
+      for (int j = 0; j <= i; ++j) {
+        long double v = g[act[i] * m + act[j]];
+        for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+        if (i == j) {
+          if (!(v > 1e-14L * g[act[i] * m + act[i]])) { ok = false; break; }
+          L[i][i] = std::sqrt(v);
+        } else {
+          L[i][j] = v / L[j][j];
+        }
+      }
+    }
+    if (!ok) continue;  // zero coefficients: the zero start
+    for (int I = 0; I < m; ++I) {
+      long double y[6], x[6];
+      for (int i = 0; i < na; ++i) {
+        long double v = -h[I * m + act[i]];
+        for (int k = 0; k < i; ++k) v -= L[i][k] * y[k];
+        y[i] = v / L[i][i];
+      }
+      for (int i = na - 1; i >= 0; --i) {
+        long double v = y[i];
+        for (int k = i + 1; k < na; ++k) v -= L[k][i] * x[k];
+        x[i] = v / L[i][i];
+      }
+      bool finite = true;
+      for (int i = 0; i < na; ++i) finite = finite && std::isfinite((double)x[i]);
+      if (!finite) continue;
+      for (int i = 0; i < na; ++i) coef[(size_t)c * m * m + I * m + act[i]] = (double)x[i];
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(A, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_TRY(launch_combine(va, w.X, A, u_out, st));
+  HIP_TRY(hipStreamSynchronize(st));  // coef (host) must outlive the copy
+  if (coef_out) std::memcpy(coef_out, coef.data(), coef.size() * sizeof(double));
+  return leave(p, stream);
 }
 
 int nlfh_plan_set_inner_tol(nlfh_plan* p, double inner_tol) {

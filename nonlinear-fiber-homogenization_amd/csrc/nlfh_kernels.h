@@ -130,6 +130,8 @@ template <typename T>
 hipError_t launch_fill(T* out, int64_t n, T v, hipStream_t st);
 template <typename T>
 hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part, int mode, hipStream_t st);
+hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
+hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);
 hipError_t launch_sum_partials(const double* part, int nitems, const int* range, int ncells, int rows, double scale,
                                const CellDesc* cells, int out_stride, double* out, hipStream_t st);
 template <typename T>

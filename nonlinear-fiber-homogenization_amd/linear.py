@@ -233,11 +233,14 @@ class _CorrectorsFn(torch.autograd.Function):
 class LinearHomogenization:
     """Linear homogenization on a fixed grid. `shape` is (nx, ny, nz) or (n, n) for plane strain.
 
-    Extra keyword (not in the reference): precision = "fp64" (the reference's PCG, default) or
-    "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule)."""
+    Extra keywords (not in the reference): precision = "fp64" (the reference's PCG, default) or
+    "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule); warm_start = a
+    `learned.WarmStartNet` (or the path of its weights) whose Ritz-projected prediction starts
+    `C_full` when no u0 is given, with a plain zero-start solve as the fallback if that solve does
+    not converge."""
 
     def __init__(self, shape, E=1.0, nu=0.3, emin=1e-3, p=3.0, dtype=torch.float64, tol=1e-9, maxit=4000,
-                 precision="fp64"):
+                 precision="fp64", warm_start=None):
         shape = tuple(int(n) for n in shape)
         self.plane = len(shape) == 2
         self.shape = shape + (1,) if self.plane else shape
@@ -256,6 +259,10 @@ class LinearHomogenization:
         self._lib = _lib.load()
         self.last_info = {}
         self.correctors = lambda s, Es, u0: _CorrectorsFn.apply(_as_f64(s, self._dev), self, Es, u0)
+        if isinstance(warm_start, str):
+            from .learned import WarmStartNet
+            warm_start = WarmStartNet.load(warm_start)
+        self.warm_start = warm_start
 
     # --- internals ------------------------------------------------------------------------
     @property
@@ -393,11 +400,36 @@ class LinearHomogenization:
         up = to_planar(u.detach() if isinstance(u, torch.Tensor) else u, self.shape)
         return _BilinearFn.apply(s.contiguous(), self, Es, up)
 
+    def ritz_start(self, s, Es, P):
+        """Energy-optimal combination of the predicted fields P (m, nx, ny, nz, 3) for each right-hand
+        side (nlfh_ritz_start); returns the start in the reference layout and the (m, m) weights."""
+        Es_np, Es_p = _es_host(Es)
+        m = Es_np.shape[0]
+        Pp = to_planar(P, self.shape)
+        out = torch.empty_like(Pp)
+        coef = np.zeros(m * m)
+        _lib.check(self._lib.nlfh_ritz_start(self._plan.handle, _ptr(self._s(s).detach()), Es_p, m, _ptr(Pp),
+                                             _ptr(out), coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             _stream()))
+        return from_planar(out), coef.reshape(m, m)
+
     def C_full(self, rho, u0=None):
         """6x6 C^H (all six unit strains). Returns (C, u, cg iterations)."""
         s = self.stiffness_scale(rho)
         Es = np.eye(6)
-        u, it = self.solve(s.detach(), Es, u0)
+        if u0 is None and self.warm_start is not None and not self.plane:
+            r = rho.detach() if isinstance(rho, torch.Tensor) else torch.as_tensor(np.asarray(rho))
+            P = self.warm_start.predict(r.to(self._dev, torch.float64))[0]
+            u_w, _ = self.ritz_start(s.detach(), Es, P)
+            u, it = self.solve(s.detach(), Es, u_w)
+            info = dict(self.last_info, warm_start=True, fallback=False)
+            if not info["converged"] or not np.all(np.isfinite(info["relres"])):
+                u, it2 = self.solve(s.detach(), Es, None)
+                info = dict(self.last_info, warm_start=True, fallback=True, it_warm=it)
+                it = it2
+            self.last_info = info
+        else:
+            u, it = self.solve(s.detach(), Es, u0)
         return self.bilinear(s, Es, u), u, it
 
     def _shear(self):
