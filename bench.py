@@ -46,10 +46,17 @@ def parse():
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--learned", default=None, help="warm-start weights (npz)")
     ap.add_argument("--no-tune", action="store_true", help="default tiling instead of the autotuner")
+    ap.add_argument("--tiling", default=None, help="TY,XC,ZPT matvec tiling for every shape (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison")
     ap.add_argument("--check-fp64", action="store_true", help="also solve every cell in FP64 and report C errors")
     ap.add_argument("--cpu-iters", type=int, default=6, help="PCG iterations timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workers", type=int, default=16,
+                    help="host processes generating the cells (0: in this process, e.g. under a profiler)")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="skip the timed solve: only the matvec roofline launches (for counter collection)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (HBM-counter calibration)")
     return ap.parse_args()
 
 
@@ -86,8 +93,11 @@ def main():
     # densities on host worker processes before the GPU runtime starts in this process
     import __graft_entry__ as ge
     pkg = ge.load_package()
-    with mp.get_context("fork").Pool(min(16, len(jobs))) as pool:
-        made = pool.map(_cell, jobs)
+    if args.workers > 0:
+        with mp.get_context("fork").Pool(min(args.workers, len(jobs))) as pool:
+            made = pool.map(_cell, jobs)
+    else:
+        made = [_cell(j) for j in jobs]
     rhos = [r for r, _ in made]
     metas = [m for _, m in made]
     shapes = [r.shape for r in rhos]
@@ -103,10 +113,22 @@ def main():
     bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
                                    warm_start=net)
     t_tune = time.perf_counter()
-    if not args.no_tune:
+    if args.tiling:
+        ty, xc, zpt = (int(v) for v in args.tiling.split(","))
+        for i in range(len(set(shapes))):
+            bh._plan.set_tiling(i, ty, xc, zpt)
+    elif not args.no_tune:
         bh.autotune()
     t_tune = time.perf_counter() - t_tune
+    tiling = {f"{sh[0]}x{sh[1]}x{sh[2]}": bh._plan.get_tiling(i)
+              for i, sh in enumerate(dict.fromkeys(shapes))}
     rho_d = bh.pack(rhos)
+
+    if args.kernels_only:
+        for prec in ("mixed", "fp64"):
+            print(json.dumps({"roofline_" + prec: matvec_roofline(pkg, bh, rho_d, prec)}), flush=True)
+        copy_probe(args.copy_probe_mb)
+        return
 
     def step():
         C, u, it = bh.C_full(rho_d)
@@ -167,7 +189,8 @@ def main():
             "workload": f"{desc}, 6 unit strains, PCG to true rel. residual {tol:g}, emin {emin:g}, C 6x6, "
                         f"batched solver ({args.precision})" + (", learned warm start" if net else ""),
             "cells_per_step": ncell, "rhs": 6, "tol": tol, "precision": args.precision,
-            "tiling": "autotuned" if not args.no_tune else "default", "tune_seconds": round(t_tune, 2),
+            "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
+                       "ty_xc_zpt": tiling}, "tune_seconds": round(t_tune, 2),
             "learned_start": bool(net), "fallback_steps": fallbacks,
             "pcg_iterations": its_all[-1] if its_all else [],
             "cells": metas if ncell <= 16 else f"{ncell} cells (seeds in the workload description)",
@@ -227,6 +250,20 @@ def matvec_roofline(pkg, bh, rho_d, precision):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": f"{kname} (K(s) u, 6 RHS, {bh.tot} voxels = {len(bh.shapes)} cells)",
             "algorithmic_bytes": bytes_mv, "avg_us": round(t_mv * 1e6, 2)}
+
+
+def copy_probe(mib):
+    """One device-to-device copy of `mib` MiB: a kernel with known HBM traffic (read + write of that
+    size) whose FETCH_SIZE / WRITE_SIZE counters calibrate the counters of the other kernels."""
+    if mib <= 0:
+        return
+    import torch
+    src = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": src.numel() * 4}), flush=True)
 
 
 def plain_fp64(pkg, rhos, emin, tol, dev, steps=1):

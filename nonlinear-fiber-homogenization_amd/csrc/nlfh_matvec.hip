@@ -18,6 +18,8 @@
 // loads TY + 2 rows and computes TY + 1 element rows (one halo row each side). When nz / ZPT is
 // not a power of two the row is padded to the next power of two lanes (idle lanes), e.g. 48
 // lanes in a slot of 64.
+#include <type_traits>
+
 #include "nlfh_kernels.h"
 
 namespace nlfh {
@@ -94,11 +96,15 @@ __device__ __forceinline__ T from_prev(T v, int src) {
 // (one barrier) does: (a) finish layer L_{s-1}: add the row-above half of its y-reverse from LDS,
 // reverse x and store node plane L_{s-1}; (b) layer L_s: y, z butterflies, modal operator, z and y
 // reverse, publish the row-above half in LDS; (c) x butterflies of layer L_{s+1} from the prefetched
-// plane into the other LDS buffer. LDS: two S/D buffers and two y-reverse buffers.
+// plane into the other LDS buffer. LDS: two S/D buffers and two y-reverse buffers, selected by the
+// parity of s, which is a compile-time constant (the step loop is unrolled by two).
+// Addressing: every global access is a wave-uniform base (cell, right-hand side, component, x-plane)
+// plus a 32-bit per-lane offset inside the plane; with a row per wavefront (DPP) the row index is
+// wave-uniform too.
 template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   extern __shared__ double smem_d[];
-  T* lds = reinterpret_cast<T*>(smem_d);
+  T* const lds = reinterpret_cast<T*>(smem_d);
   __shared__ double red[8];
 
   const MvItem it = a.items[blockIdx.x];
@@ -106,28 +112,39 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const CellDesc cd = a.cells[it.cell];
   const int rhs = blockIdx.y;
   const int nx = cd.nx, ny = cd.ny, nz = cd.nz;
-  const int LZ = a.LZ, LZS = a.LZS;
+  const int LZ = DPP ? 64 : a.LZ;
+  const int LZS = DPP ? 64 : a.LZS;
   const int ty = it.ty;
   const int RB = ty + 2;
   const int tid = threadIdx.x;
-  const int lz = tid % LZS;
-  const int r = tid / LZS;
+  const int lz = DPP ? (tid & 63) : tid % LZS;
+  const int r = DPP ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / LZS;
   const bool active = r < RB && lz < LZ;
   const bool erow = r <= ty;  // computes element row yg
   const bool owns = active && r >= 1 && r <= ty;
   int yg = it.y0 - 1 + r;
-  yg = ((yg % ny) + ny) % ny;
+  yg = yg < 0 ? yg + ny : yg;
+  yg = yg >= ny ? yg - ny : yg;
   const int z0 = lz * ZPT;
   const int lane = tid & 63;
   const int src_next = (lane - lz) + (lz + 1 == LZ ? 0 : lz + 1);
   const int src_prev = (lane - lz) + (lz == 0 ? LZ - 1 : lz - 1);
 
   const int64_t nvox = cd.nvox;
+  const int plane = ny * nz;
+  const unsigned roff = (unsigned)(yg * nz + z0);  // offset of this lane's nodes inside a plane
   const int64_t fbase = cd.off_field + (int64_t)rhs * 3 * nvox;
-  const T* __restrict__ U = a.u ? a.u + fbase : nullptr;
-  const T* __restrict__ P = PUPDATE ? a.p_old + fbase : nullptr;
-  T* __restrict__ PN = PUPDATE ? a.p_new + fbase : nullptr;
-  T* __restrict__ O = a.out + fbase;
+  const T* Uc[3];
+  const T* Pc[3];
+  T* PNc[3];
+  T* Oc[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    Uc[c] = a.u ? a.u + fbase + c * nvox : nullptr;
+    Pc[c] = PUPDATE ? a.p_old + fbase + c * nvox : nullptr;
+    PNc[c] = PUPDATE ? a.p_new + fbase + c * nvox : nullptr;
+    Oc[c] = a.out + fbase + c * nvox;
+  }
   const T* __restrict__ S = a.s + cd.off_vox;
   T beta = T(0);
   if (PUPDATE) beta = (T)a.beta[it.cell * a.m + rhs];
@@ -139,28 +156,36 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const T lam16 = a.lam16, mu16 = a.mu16, scale_out = a.scale_out;
 
   const int buf_sz = RB * 6 * ZPT * LZ;  // one LDS buffer
-  T* SDb[2] = {lds, lds + buf_sz};
-  T* UYb[2] = {lds + 2 * buf_sz, lds + 3 * buf_sz};
-#define LIDX(rr, f, q) ((((rr) * 6 + (f)) * ZPT + (q)) * LZ + lz)
+  const int lbase = r * 6 * ZPT * LZ + lz;
+  const int rowstep = 6 * ZPT * LZ;
+#define LOFF(f, q) (((f) * ZPT + (q)) * LZ)
 
-  auto load_row = [&](int xp, T (&v)[3][ZPT]) {
-    const int64_t o = ((int64_t)xp * ny + yg) * nz + z0;
+  // x-plane index -> offset of the plane (x in [x0 - 2, x0 + xc + 1], wrapped periodically)
+  auto plane_off = [&](int x) -> unsigned {
+    x = x < 0 ? x + nx : x;
+    x = x < 0 ? x + nx : x;
+    x = x >= nx ? x - nx : x;
+    x = x >= nx ? x - nx : x;
+    return (unsigned)(x * plane);
+  };
+  const bool has_u = a.u != nullptr;
+  auto load_row = [&](unsigned po, T (&v)[3][ZPT]) {
+    const unsigned o = po + roff;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) {
-        T val = U ? U[c * nvox + o + q] : T(0);
-        if (PUPDATE) val += beta * P[c * nvox + o + q];
+        T val = has_u ? Uc[c][o + q] : T(0);
+        if (PUPDATE) val += beta * Pc[c][o + q];
         v[c][q] = val;
       }
     }
   };
   const int x0 = it.x0, xc = it.xc;
-  auto wrap = [&](int x) { return ((x % nx) + nx) % nx; };
 
   T up[3][ZPT], nxt[3][ZPT], carry[3][ZPT];
-  T pcA[3][ZPT], pcB[3][ZPT];  // node values of the planes awaiting output (layer parity)
-  T Flo[3][2][ZPT];            // own half of the y-reverse of the last element layer
+  T pc[2][3][ZPT];  // node values of the planes awaiting output (by layer parity)
+  T Flo[3][2][ZPT];  // own half of the y-reverse of the last element layer
   double dot = 0.0;
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -169,75 +194,77 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       carry[c][q] = T(0);
       Flo[c][0][q] = T(0);
       Flo[c][1][q] = T(0);
-      pcA[c][q] = T(0);
-      pcB[c][q] = T(0);
+      pc[0][c][q] = T(0);
+      pc[1][c][q] = T(0);
     }
 
-  // x stage of layer L_t from planes `up` (L_t) and `un` (L_t + 1) into SD buffer t & 1
-  auto x_stage = [&](int t, T (&un)[3][ZPT]) {
-    T* SD = SDb[t & 1];
+  // x stage of layer L_t (parity PT = t & 1) from planes `up` (L_t) and `un` (L_t + 1)
+  auto x_stage = [&](auto ptag, T (&un)[3][ZPT]) {
+    constexpr int PT = decltype(ptag)::value;
+    T* SD = lds + PT * buf_sz + lbase;
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) {
-        SD[LIDX(r, c, q)] = un[c][q] + up[c][q];
-        SD[LIDX(r, 3 + c, q)] = un[c][q] - up[c][q];
-        if (t & 1) pcB[c][q] = up[c][q];
-        else pcA[c][q] = up[c][q];
+        SD[LOFF(c, q)] = un[c][q] + up[c][q];
+        SD[LOFF(3 + c, q)] = un[c][q] - up[c][q];
+        pc[PT][c][q] = up[c][q];
         up[c][q] = un[c][q];
       }
   };
 
   // prologue: layer L_0 from planes x0-1, x0; prefetch x0+1
   if (active) {
-    load_row(wrap(x0 - 1), up);
+    load_row(plane_off(x0 - 1), up);
     T un[3][ZPT];
-    load_row(wrap(x0), un);
-    if (xc >= 1) load_row(wrap(x0 + 1), nxt);
+    const unsigned p0 = plane_off(x0);
+    load_row(p0, un);
+    if (xc >= 1) load_row(plane_off(x0 + 1), nxt);
     if (PUPDATE && owns && xc >= 1) {
-      const int64_t o = ((int64_t)wrap(x0) * ny + yg) * nz + z0;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
+        for (int q = 0; q < ZPT; ++q) PNc[c][p0 + roff + q] = un[c][q];
     }
-    x_stage(0, un);
+    x_stage(std::integral_constant<int, 0>{}, un);
   }
   __syncthreads();
 
-  for (int s = 0; s <= xc + 1; ++s) {
+  // one step; returns true when the march is over (s > xc after part (a))
+  auto step = [&](int s, auto ptag) -> bool {
+    constexpr int PS = decltype(ptag)::value;  // s & 1
+    constexpr int PQ = 1 - PS;                 // (s - 1) & 1 == (s + 1) & 1
     // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
     if (s >= 1 && owns) {
-      const T* UYp = UYb[(s - 1) & 1];
-      const int xo = wrap(x0 - 2 + s);
+      const T* UYp = lds + (2 + PQ) * buf_sz + lbase - rowstep;
+      const unsigned oo = plane_off(x0 - 2 + s) + roff;
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int q = 0; q < ZPT; ++q) {
-          const T A = Flo[c][0][q] + UYp[LIDX(r - 1, 2 * c, q)];
-          const T B = Flo[c][1][q] + UYp[LIDX(r - 1, 2 * c + 1, q)];
+          const T A = Flo[c][0][q] + UYp[LOFF(2 * c, q)];
+          const T B = Flo[c][1][q] + UYp[LOFF(2 * c + 1, q)];
           const T lo = A - B, hi = A + B;
           if (s >= 2) {
             const T v = (carry[c][q] + lo) * scale_out;
-            O[c * nvox + ((int64_t)xo * ny + yg) * nz + z0 + q] = v;
-            if (DOT) dot += (double)((s - 1) & 1 ? pcB[c][q] : pcA[c][q]) * (double)v;
+            Oc[c][oo + q] = v;
+            if (DOT) dot += (double)pc[PQ][c][q] * (double)v;
           }
           carry[c][q] = hi;
         }
     }
-    if (s > xc) break;
+    if (s > xc) return true;
     // (b) element layer L_s
-    const int xi = wrap(x0 - 1 + s);
     if (active && erow) {
-      const T* SD = SDb[s & 1];
-      T* UY = UYb[s & 1];
+      const T* SD = lds + PS * buf_sz + lbase;
+      T* UY = lds + (2 + PS) * buf_sz + lbase;
       T Y[3][4][ZPT];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int q = 0; q < ZPT; ++q) {
-          const T s0 = SD[LIDX(r, c, q)], d0 = SD[LIDX(r, 3 + c, q)];
-          const T s1 = SD[LIDX(r + 1, c, q)], d1 = SD[LIDX(r + 1, 3 + c, q)];
+          const T s0 = SD[LOFF(c, q)], d0 = SD[LOFF(3 + c, q)];
+          const T s1 = SD[rowstep + LOFF(c, q)], d1 = SD[rowstep + LOFF(3 + c, q)];
           Y[c][0][q] = s1 + s0;
           Y[c][2][q] = s1 - s0;
           Y[c][1][q] = d1 + d0;
@@ -248,7 +275,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = from_next<T, DPP>(Y[c][mm][0], src_next);
-      const int64_t so = ((int64_t)xi * ny + yg) * nz + z0;
+      const unsigned so = plane_off(x0 - 1 + s) + roff;
       T Fz[3][4][ZPT];
       T hi_last[3][4];
 #pragma unroll
@@ -287,8 +314,8 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
         for (int q = 0; q < ZPT; ++q) {
           const T A0 = Fz[c][0][q], B0 = Fz[c][2][q];
           const T A1 = Fz[c][1][q], B1 = Fz[c][3][q];
-          UY[LIDX(r, 2 * c, q)] = A0 + B0;
-          UY[LIDX(r, 2 * c + 1, q)] = A1 + B1;
+          UY[LOFF(2 * c, q)] = A0 + B0;
+          UY[LOFF(2 * c + 1, q)] = A1 + B1;
           Flo[c][0][q] = A0 - B0;
           Flo[c][1][q] = A1 - B1;
         }
@@ -300,19 +327,25 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int q = 0; q < ZPT; ++q) un[c][q] = nxt[c][q];
-      if (s + 1 < xc) load_row(wrap(x0 + s + 2), nxt);
+      if (s + 1 < xc) load_row(plane_off(x0 + s + 2), nxt);
       if (PUPDATE && owns && s + 1 < xc) {
-        const int64_t o = ((int64_t)wrap(x0 + s + 1) * ny + yg) * nz + z0;
+        const unsigned o = plane_off(x0 + s + 1) + roff;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int q = 0; q < ZPT; ++q) PN[c * nvox + o + q] = un[c][q];
+          for (int q = 0; q < ZPT; ++q) PNc[c][o + q] = un[c][q];
       }
-      x_stage(s + 1, un);
+      x_stage(std::integral_constant<int, PQ>{}, un);
     }
     __syncthreads();
+    return false;
+  };
+
+  for (int s = 0; s <= xc + 1; s += 2) {
+    if (step(s, std::integral_constant<int, 0>{})) break;
+    if (step(s + 1, std::integral_constant<int, 1>{})) break;
   }
-#undef LIDX
+#undef LOFF
   if (DOT) {
     const double t = block_sum(dot, red);
     if (threadIdx.x == 0) a.dot_part[(int64_t)rhs * a.nitems + a.item_base + blockIdx.x] = t;
