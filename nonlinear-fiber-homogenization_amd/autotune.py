@@ -5,7 +5,8 @@ Candidates are every valid combination from small sets; each is timed on the dev
 over repeated launches of the operator K(s) u on m right-hand sides (the same kernel family the
 solver launches every iteration) and the fastest is kept in the plan. Results are cached per
 (device name, shape, precision, m) in a JSON file (NLFH_TUNE_CACHE, default
-~/.cache/nlfh/tiling.json) so later plans skip the sweep.
+~/.cache/nlfh/tiling.json) so later plans skip the sweep. On gfx950 a table measured with
+tune_table.py and shipped in data/tiling_gfx950.json is consulted before sweeping.
 """
 import ctypes
 import json
@@ -41,6 +42,26 @@ def candidates(shape, elem_bytes=8):
                     continue
                 out.append((ty, xc, zpt))
     return out
+
+
+_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tiling_gfx950.json")
+
+
+def _packaged(shape, precision, m, dev):
+    """Tiling measured on MI355X (gfx950) and shipped with the package (tune_table.py writes it)."""
+    try:
+        arch = torch.cuda.get_device_properties(dev).gcnArchName
+    except Exception:
+        return None
+    if not arch.startswith("gfx950"):
+        return None
+    try:
+        with open(_TABLE) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    e = table.get(f"{tuple(shape)}|{precision}|{m}")
+    return tuple(e["best"]) if e else None
 
 
 def _cache_path():
@@ -99,6 +120,11 @@ def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, ver
         best = tuple(cache[key]["best"])
         plan.set_tiling(shape_index, *best)
         return best, []
+    if use_cache:
+        best = _packaged(shape, precision, m, dev)
+        if best is not None:
+            plan.set_tiling(shape_index, *best)
+            return best, []
     dtype = 0 if precision == "f64" else 1
     tdt = torch.float64 if precision == "f64" else torch.float32
     # a single-cell plan owns the voxel range of this shape at offset 0

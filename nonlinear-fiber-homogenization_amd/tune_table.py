@@ -1,0 +1,49 @@
+"""Measure the matvec tiling table shipped with the package (data/tiling_gfx950.json): for every
+shape and precision, the autotuner's sweep (no cache) repeated `--repeat` times; the tiling with the
+lowest median time wins.
+
+    python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 32 48 64 128
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--shapes", type=int, nargs="+", default=[16, 32, 48, 64, 128])
+    ap.add_argument("--precisions", nargs="+", default=["f32", "f64"])
+    ap.add_argument("--m", type=int, default=6)
+    ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                                   "tiling_gfx950.json"))
+    args = ap.parse_args()
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    from nlfh_amd import autotune as at
+    from nlfh_amd.linear import Plan
+    table = {}
+    for n in args.shapes:
+        shape = (n, n, n)
+        for prec in args.precisions:
+            runs = {}
+            for _ in range(args.repeat):
+                plan = Plan([shape], 1.0, 0.3)
+                _, tab = at.tune(plan, 0, m=args.m, precision=prec, reps=10, use_cache=False)
+                for cand, t in tab:
+                    runs.setdefault(cand, []).append(t)
+            med = sorted((float(np.median(v)), c) for c, v in runs.items())
+            best_t, best = med[0]
+            table[f"{shape}|{prec}|{args.m}"] = {"best": list(best), "us": round(best_t, 2),
+                                                 "runner_up": [list(c) for _, c in med[1:4]]}
+            print(shape, prec, best, f"{best_t:.1f} us", flush=True)
+    with open(args.out, "w") as f:
+        json.dump(table, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()
