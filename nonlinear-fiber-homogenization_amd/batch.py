@@ -134,8 +134,28 @@ class BatchedHomogenization:
             it_warm = info["it"]
             u, info = self._solve(s, Es_p, None)
             info.update(warm_start=True, fallback=True, it_warm=it_warm)
+        info["fallback_fp64"] = []
+        if self.precision == "mixed" and not all(info["converged"]):
+            self._fp64_fallback(rho, u, info)
         C = torch.empty((nc, self.m, self.m), dtype=torch.float64, device=self._dev)
         _lib.check(self._lib.nlfh_bilinear(self._plan.handle, _ptr(s), Es_p, _ptr(u), self.m, _ptr(C), None,
                                            _stream()))
         self.last_info = info
         return C.cpu(), u, np.asarray(info["it"])
+
+    def _fp64_fallback(self, rho, u, info):
+        """Cells the mixed-precision solve left unconverged (iteration cap or a refinement stall) are
+        solved again by plain FP64 PCG from zero in a plan of their own; their correctors replace
+        the mixed ones in `u` and their iteration counts add to the cell's."""
+        bad = [i for i, c in enumerate(info["converged"]) if not c]
+        sub = BatchedHomogenization([self.shapes[i] for i in bad], E=self.E, nu=self.nu, emin=self.emin, p=self.p,
+                                    tol=self.tol, maxit=self.maxit, precision="fp64", m=self.m)
+        rho_sub = torch.cat([rho[self.off[i]:self.off[i + 1]] for i in bad])
+        _, u_sub, it_sub = sub.C_full(rho_sub)
+        f = 3 * self.m
+        for j, i in enumerate(bad):
+            u[int(self.off[i]) * f:int(self.off[i + 1]) * f] = u_sub[int(sub.off[j]) * f:int(sub.off[j + 1]) * f]
+            info["it"][i] += int(it_sub[j])
+            info["converged"][i] = sub.last_info["converged"][j]
+            info["relres"][i] = sub.last_info["relres"][j]
+        info["fallback_fp64"] = bad

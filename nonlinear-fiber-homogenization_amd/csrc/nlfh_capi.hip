@@ -1138,7 +1138,8 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
   }
   HIP_TRY(launch_fill<int>(p->d_maxed, nc, 0, st));
   v.cs.skip = p->d_outer_done;
-  std::vector<int> total(nc, 0), its(nc, 0);
+  std::vector<int> total(nc, 0), its(nc, 0), stall(nc, 0);
+  std::vector<double> rn_prev((size_t)nk, 0.0);
   int* h = p->h_pinned;
   for (int outer = 0; outer < 200; ++outer) {
     // true residual r = b - K x = -(K x + load), its norms and the outer convergence flags
@@ -1148,8 +1149,30 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
     HIP_TRY(launch_norms(p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, nc, m, m, 0.0, rn_true, bn, tol,
                          p->d_outer_done, p->d_maxed, p->d_params + nc, p->inner_tol, 0.5, st));
     HIP_TRY(hipMemcpyAsync(h, p->d_outer_done, nc * sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(p->h_pinned_d, rn_true, nk * sizeof(double), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<int> odone(h, h + nc);  // h is reused by run_cg
+    // stall: a refinement step that did not reduce some right-hand side's true residual by 10%
+    // (FP32 inner solves that stopped helping) ends the cell unconverged
+    bool stall_any = false;
+    for (int c = 0; c < nc; ++c) {
+      bool stalled = false;
+      for (int t = 0; t < m; ++t) {
+        const double rn = p->h_pinned_d[c * m + t];
+        if (outer > 0 && !odone[c] && !(rn < 0.9 * rn_prev[c * m + t])) stalled = true;
+        rn_prev[c * m + t] = rn;
+      }
+      if (stalled && !odone[c]) {
+        stall[c] = 1;
+        odone[c] = 1;
+        stall_any = true;
+      }
+    }
+    if (stall_any) {
+      for (int c = 0; c < nc; ++c) h[c] = odone[c];
+      HIP_TRY(hipMemcpyAsync(p->d_outer_done, h, nc * sizeof(int), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
     bool all = true;
     for (int c = 0; c < nc; ++c) all = all && odone[c];
     if (all) break;

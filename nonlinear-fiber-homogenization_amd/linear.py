@@ -388,7 +388,17 @@ class LinearHomogenization:
         prec = 0 if self.precision == "fp64" else 1
         _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, m, _ptr(u0p), _ptr(u), float(self.tol),
                                         int(self.maxit), prec, it, cv, rr, _stream()))
-        self.last_info = dict(it=it[0], converged=bool(cv[0]), relres=list(rr), precision=self.precision)
+        self.last_info = dict(it=it[0], converged=bool(cv[0]), relres=list(rr), precision=self.precision,
+                              fallback_fp64=False)
+        if prec == 1 and not cv[0]:
+            # mixed precision stopped unconverged (iteration cap or refinement stall): plain FP64 PCG
+            # from zero, the reference's method
+            it_mixed = it[0]
+            _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, m, None, _ptr(u), float(self.tol),
+                                            int(self.maxit), 0, it, cv, rr, _stream()))
+            self.last_info = dict(it=it_mixed + it[0], converged=bool(cv[0]), relres=list(rr),
+                                  precision=self.precision, fallback_fp64=True)
+            return from_planar(u), it_mixed + it[0]
         return from_planar(u), it[0]
 
     # --- homogenized stiffness (nlfiber/linear.py:220-264) ------------------------------------
