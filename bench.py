@@ -90,6 +90,8 @@ def main():
         sys.exit(2)
     jobs, emin, wname = workload(args)
     tol = 1e-8
+    if wname == "stress128":
+        os.environ.setdefault("NLFH_PROGRESS", "1")  # solver progress on stderr during minutes-long steps
     # densities on host worker processes before the GPU runtime starts in this process
     import __graft_entry__ as ge
     pkg = ge.load_package()
@@ -130,10 +132,13 @@ def main():
         copy_probe(args.copy_probe_mb)
         return
 
+    fallback_fp64 = []
+
     def step():
         C, u, it = bh.C_full(rho_d)
         if not all(bh.last_info["converged"]):
             raise RuntimeError(f"cells did not converge: {bh.last_info['converged']}")
+        fallback_fp64.append(list(bh.last_info.get("fallback_fp64", [])))
         return C, it, bh.last_info.get("fallback", False)
 
     for _ in range(args.warmup):
@@ -141,10 +146,11 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     its_all, fallbacks = [], 0
-    for _ in range(args.steps):
+    for k in range(args.steps):
         C, its, fb = step()
         its_all.append(its.tolist())
         fallbacks += int(fb)
+        print(f"step {k + 1}/{args.steps} done, {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ncell = len(jobs)
@@ -191,7 +197,8 @@ def main():
             "cells_per_step": ncell, "rhs": 6, "tol": tol, "precision": args.precision,
             "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
                        "ty_xc_zpt": tiling}, "tune_seconds": round(t_tune, 2),
-            "learned_start": bool(net), "fallback_steps": fallbacks,
+            "learned_start": bool(net), "learned_start_fallback_steps": fallbacks,
+            "fp64_fallback_cells": fallback_fp64[-1] if fallback_fp64 else [],
             "pcg_iterations": its_all[-1] if its_all else [],
             "cells": metas if ncell <= 16 else f"{ncell} cells (seeds in the workload description)",
         },

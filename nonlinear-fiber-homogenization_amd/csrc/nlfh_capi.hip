@@ -1,7 +1,10 @@
 // C-ABI of the library (include/nlfh.h): plans, launch tables, the preconditioner symbol, and the
 // PCG / iterative-refinement drivers. Host code; every device step is a kernel of this library.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <complex>
 #include <cstring>
 #include <map>
@@ -17,6 +20,30 @@ void set_error(const std::string& m) { g_err = m; }
 }  // namespace nlfh
 
 using namespace nlfh;
+
+namespace {
+// Progress lines on stderr from the solver loops, at most one per 20 s, when NLFH_PROGRESS is set
+// (long solves, e.g. many 128^3 cells at high contrast, would otherwise be silent for minutes).
+bool progress_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("NLFH_PROGRESS");
+    return e != nullptr && e[0] != '\0' && e[0] != '0';
+  }();
+  return on;
+}
+template <typename... A>
+void progress(const char* fmt, A... args) {
+  if (!progress_on()) return;
+  static auto last = std::chrono::steady_clock::now();
+  const auto now = std::chrono::steady_clock::now();
+  if (std::chrono::duration<double>(now - last).count() < 20.0) return;
+  last = now;
+  std::fprintf(stderr, "[nlfh] ");
+  std::fprintf(stderr, fmt, args...);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+}
+}  // namespace
 
 #define CHECK_ARG(cond, msg)        \
   do {                              \
@@ -613,8 +640,14 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
     HIP_TRY(hipMemcpyAsync(h_done, w.cs.done, nc * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     bool all = true;
-    for (int c = 0; c < nc; ++c) all = all && h_done[c];
+    int ndone = 0;
+    for (int c = 0; c < nc; ++c) {
+      all = all && h_done[c];
+      ndone += h_done[c] ? 1 : 0;
+    }
     if (all) break;
+    progress("%s PCG: %ld iterations launched, %d of %d cells converged", sizeof(T) == 8 ? "FP64" : "FP32",
+             launched, ndone, nc);
     if (use_graph) {
       hipGraphExec_t ge;
       TRY(get_graph<T>(p, w, m, &ge));
@@ -1174,8 +1207,13 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
       HIP_TRY(hipStreamSynchronize(st));
     }
     bool all = true;
-    for (int c = 0; c < nc; ++c) all = all && odone[c];
+    int ndone = 0;
+    for (int c = 0; c < nc; ++c) {
+      all = all && odone[c];
+      ndone += odone[c] ? 1 : 0;
+    }
     if (all) break;
+    progress("refinement step %d: %d of %d cells done", outer, ndone, nc);
     // inner FP32 solve of K d = r / ||r|| from d = 0
     HIP_TRY(launch_ir_down(vec_args(p, m, p->d_outer_done), w.R, v.R, v.X, rn_true, st));
     TRY(run_cg<float>(p, v, m, v.R, its.data(), 1, maxit));
