@@ -56,22 +56,22 @@ def test_mixed_unconverged_cells_fall_back_to_fp64(gpu_pkg):
     """Cells the mixed-precision solve leaves unconverged are solved again by plain FP64 PCG from zero
     (iteration cap forced from the host): the fallback cell's correctors are exactly those of an FP64
     plan of its own with the same cap, the converged cell keeps its mixed-precision correctors."""
-    rho_small = gpu_pkg.cells.random_cell(8, 1)[0]
+    rho_small = gpu_pkg.cells.random_cell(4, 1)[0]
     rho_big = gpu_pkg.cells.random_cell(16, 2)[0]
-    maxit = 70  # the 8^3 cell needs ~46 FP64 iterations, the 16^3 cell ~78 (oracle counts)
-    bh = gpu_pkg.BatchedHomogenization([(8, 8, 8), (16, 16, 16)], emin=1e-2, tol=1e-8, maxit=maxit, precision="mixed")
+    maxit = 80  # the 4^3 cell needs 15 FP64 iterations, the 16^3 cell 203 (oracle counts)
+    bh = gpu_pkg.BatchedHomogenization([(4, 4, 4), (16, 16, 16)], emin=1e-3, tol=1e-8, maxit=maxit, precision="mixed")
     C, u, it = bh.C_full([rho_small, rho_big])
     info = bh.last_info
     assert info["fallback_fp64"] == [1]
     assert info["converged"][0] and not info["converged"][1]
-    one64 = gpu_pkg.BatchedHomogenization([(16, 16, 16)], emin=1e-2, tol=1e-8, maxit=maxit, precision="fp64")
+    one64 = gpu_pkg.BatchedHomogenization([(16, 16, 16)], emin=1e-3, tol=1e-8, maxit=maxit, precision="fp64")
     _, u64, _ = one64.C_full([rho_big])
     f = 3 * 6
     assert torch.equal(u[int(bh.off[1]) * f:int(bh.off[2]) * f], u64)
-    alone = gpu_pkg.BatchedHomogenization([(8, 8, 8)], emin=1e-2, tol=1e-8, maxit=maxit, precision="mixed")
+    alone = gpu_pkg.BatchedHomogenization([(4, 4, 4)], emin=1e-3, tol=1e-8, maxit=maxit, precision="mixed")
     _, u8, _ = alone.C_full([rho_small])
     assert torch.equal(u[:int(bh.off[1]) * f], u8)
     # the single-cell class reports the same fallback
-    lh = gpu_pkg.LinearHomogenization((16, 16, 16), emin=1e-2, tol=1e-8, maxit=maxit, precision="mixed")
+    lh = gpu_pkg.LinearHomogenization((16, 16, 16), emin=1e-3, tol=1e-8, maxit=maxit, precision="mixed")
     lh.C_full(torch.tensor(rho_big))
     assert lh.last_info["fallback_fp64"] and not lh.last_info["converged"]
