@@ -460,6 +460,19 @@ hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part
 template hipError_t launch_reduce<double>(const VecArgs&, const double*, const double*, double*, int, hipStream_t);
 template hipError_t launch_reduce<float>(const VecArgs&, const float*, const float*, double*, int, hipStream_t);
 
+__global__ void set_small_kernel(const SmallParams v, int n, double* dd, int* di) {
+  const int t = threadIdx.x;
+  if (t < n) {
+    if (dd != nullptr) dd[t] = v.d[t];
+    if (di != nullptr) di[t] = v.i[t];
+  }
+}
+
+hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st) {
+  hipLaunchKernelGGL(set_small_kernel, dim3(1), dim3(64), 0, st, v, n, dd, di);
+  return hipGetLastError();
+}
+
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st) {
   hipLaunchKernelGGL(gram_kernel, dim3(va.nchunks), dim3(256), 0, st, va, a, b, part);
   return hipGetLastError();

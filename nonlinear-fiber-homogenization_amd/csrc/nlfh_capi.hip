@@ -683,8 +683,20 @@ int leave(nlfh_plan* p, void* stream) {
   return NLFH_OK;
 }
 
+// Small parameters reach the device as kernel arguments (copied at launch), never as asynchronous
+// copies from pageable host memory, which may be read after the caller's buffer is gone.
 int set_strain(nlfh_plan* p, const double* Es_host, int m, hipStream_t st) {
-  HIP_TRY(hipMemcpyAsync(p->d_strain, Es_host, (size_t)m * 6 * sizeof(double), hipMemcpyHostToDevice, st));
+  SmallParams v{};
+  for (int i = 0; i < m * 6; ++i) v.d[i] = Es_host[i];
+  HIP_TRY(launch_set_small(v, m * 6, p->d_strain, nullptr, st));
+  return NLFH_OK;
+}
+
+int set_maxit(nlfh_plan* p, int maxit, hipStream_t st) {
+  SmallParams v{};
+  v.i[0] = maxit;
+  v.i[1] = maxit;
+  HIP_TRY(launch_set_small(v, 2, nullptr, p->d_iparams, st));
   return NLFH_OK;
 }
 
@@ -1114,9 +1126,8 @@ int solve_fp64(nlfh_plan* p, int m, const double* u0, double tol, int maxit, int
                double* relres_out, bool rhs_given) {
   hipStream_t st = p->stream;
   Work<double>& w = p->w64;
-  const int iparams[2] = {maxit, maxit};
   HIP_TRY(launch_fill<double>(p->d_params, p->ncells, tol, st));
-  HIP_TRY(hipMemcpyAsync(p->d_iparams, iparams, sizeof(iparams), hipMemcpyHostToDevice, st));
+  TRY(set_maxit(p, maxit, st));
   const int64_t F = p->field_len();
   if (!rhs_given) {
     // b = -load(s, Es)  (nlfiber/linear.py:205-206)
@@ -1154,9 +1165,8 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
   const int nk = nc * m;  // per-(cell, rhs) arrays use stride m
   double* rn_true = p->d_ir;
   double* bn = p->d_ir + nk;
-  const int iparams[2] = {maxit, maxit};
   HIP_TRY(launch_fill<double>(p->d_params, 2 * (int64_t)nc, tol, st));
-  HIP_TRY(hipMemcpyAsync(p->d_iparams, iparams, sizeof(iparams), hipMemcpyHostToDevice, st));
+  TRY(set_maxit(p, maxit, st));
   HIP_TRY(launch_d2f(w.s, v.s, p->tot_vox, st));
   TRY(run_matvec<double>(p, nullptr, nullptr, nullptr, w.B, w.s, nullptr, nullptr, nullptr, m, 1, 0, 0, -1.0, st));
   VecArgs va = vec_args(p, m, nullptr);
@@ -1218,14 +1228,17 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
     HIP_TRY(launch_ir_down(vec_args(p, m, p->d_outer_done), w.R, v.R, v.X, rn_true, st));
     TRY(run_cg<float>(p, v, m, v.R, its.data(), 1, maxit));
     HIP_TRY(launch_ir_acc(vec_args(p, m, p->d_outer_done), w.X, v.X, rn_true, st));
-    std::vector<int> maxed(nc, 0);
+    int* maxed = p->h_pinned + nc;  // pinned: the copy below may run after this loop iteration
     bool any_max = false;
     for (int c = 0; c < nc; ++c) {
       if (!odone[c]) total[c] += its[c];
       maxed[c] = total[c] >= maxit ? 1 : 0;
       any_max = any_max || maxed[c];
     }
-    if (any_max) HIP_TRY(hipMemcpyAsync(p->d_maxed, maxed.data(), nc * sizeof(int), hipMemcpyHostToDevice, st));
+    if (any_max) {
+      HIP_TRY(hipMemcpyAsync(p->d_maxed, maxed, nc * sizeof(int), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+    }
   }
   // outputs: true relative residual per rhs
   HIP_TRY(hipMemcpyAsync(p->h_pinned_d, rn_true, nk * sizeof(double), hipMemcpyDeviceToHost, st));

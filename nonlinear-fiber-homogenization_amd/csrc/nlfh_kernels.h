@@ -130,6 +130,12 @@ template <typename T>
 hipError_t launch_fill(T* out, int64_t n, T v, hipStream_t st);
 template <typename T>
 hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part, int mode, hipStream_t st);
+// Parameters small enough to travel as kernel arguments (strains m x 6, iteration caps).
+struct SmallParams {
+  double d[36];
+  int i[4];
+};
+hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st);
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);
 hipError_t launch_sum_partials(const double* part, int nitems, const int* range, int ncells, int rows, double scale,

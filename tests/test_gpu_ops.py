@@ -132,3 +132,32 @@ def test_bilinear_and_element_energy(gpu_pkg, shape):
     _, dcds = lh._bilinear_raw(st, Es, gpu_pkg.to_planar(u, shape), True)
     b = ref.element_energy(s, Es, u) / np.prod(shape)
     assert _rel(dcds.cpu().numpy(), b) < 1e-12
+
+
+def test_host_parameters_are_captured_at_call_time(gpu_pkg):
+    """The strain matrix is read from host memory during the call: overwriting the caller's buffer
+    right after the (stream-ordered, unsynchronised) calls must not change what the kernels use."""
+    import ctypes
+    n = 8
+    lib = gpu_pkg.load_library()
+    lh = gpu_pkg.LinearHomogenization((n, n, n), emin=1e-3)
+    rng = np.random.default_rng(3)
+    s = torch.tensor(rng.uniform(0.1, 1.0, (n, n, n)), device="cuda")
+    u = torch.tensor(rng.standard_normal((6, 3, n, n, n)), device="cuda")
+    C_ref, _ = lh._bilinear_raw(s, np.eye(6), u, False)
+    f_ref = gpu_pkg.to_planar(lh.load(s, np.eye(6)), (n, n, n))
+    torch.cuda.synchronize()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    dp = ctypes.POINTER(ctypes.c_double)
+    for _ in range(20):
+        Es = np.eye(6)
+        C = torch.empty((6, 6), dtype=torch.float64, device="cuda")
+        f = torch.empty_like(u)
+        gpu_pkg._lib.check(lib.nlfh_bilinear(lh._plan.handle, ctypes.c_void_p(s.data_ptr()), Es.ctypes.data_as(dp),
+                                             ctypes.c_void_p(u.data_ptr()), 6, ctypes.c_void_p(C.data_ptr()), None, st))
+        gpu_pkg._lib.check(lib.nlfh_load(lh._plan.handle, ctypes.c_void_p(s.data_ptr()), Es.ctypes.data_as(dp), 6,
+                                         ctypes.c_void_p(f.data_ptr()), st))
+        Es[:] = 1e30
+        torch.cuda.synchronize()
+        assert torch.equal(C.cpu(), C_ref)
+        assert torch.equal(f, f_ref)
