@@ -85,6 +85,7 @@ struct PrecGroup {
   int fast;  // side N of the fast path, 0 for generic
   int plane_begin, plane_count;
   int line_begin[2], line_count[2];  // [0] float64 line items, [1] float32
+  int vec;  // every cell's field and spectrum offsets allow 16-byte vector access (staged passes)
 };
 
 template <typename T>
@@ -545,7 +546,7 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       const Shape& sh = p->shapes[g.shape];
       const cplx<T>* tw = tw_ptr<T>(p) + p->lens[sh.len[0]].tw_off;
       const T* symp = symsoa_ptr<T>(p) + p->symsoa_off[g.shape];
-      HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, st));
+      HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
       HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update, dot, lp, ll, st));
     }
@@ -885,6 +886,9 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
         for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
       }
     }
+    g.vec = 1;
+    for (int c = 0; c < ncells; ++c)
+      if (p->cell_shape[c] == sidx && (p->cells[c].off_field % 4 != 0 || p->cells[c].off_spec % 2 != 0)) g.vec = 0;
     g.plane_count = (int)p->planes.size() - g.plane_begin;
     g.line_count[0] = (int)p->lines64.size() - g.line_begin[0];
     g.line_count[1] = (int)p->lines32.size() - g.line_begin[1];

@@ -298,6 +298,350 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
   }
 }
 
+// Passes A and C with the x-plane staged through LDS (sizes whose lines all fit one round,
+// LPR >= N / 2): every global access is a 16-byte vector over consecutive z, so a wavefront reads or
+// writes 1 KiB contiguous per instruction. The real plane [N][RS] (RS = N + pad, keeps the row-pair
+// reads of the z transform free of bank conflicts) aliases the exchange buffer: it is read completely
+// before the exchange is written (pass A) and written only after the exchange is read (pass C).
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  using type = float4;
+  static constexpr int n = 4;
+  __device__ static float get(const float4& v, int k) { return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w; }
+};
+template <>
+struct Vec16<double> {
+  using type = double2;
+  static constexpr int n = 2;
+  __device__ static double get(const double2& v, int k) { return k == 0 ? v.x : v.y; }
+};
+
+template <typename T, int N>
+struct StageCfg {
+  static constexpr int RS = N + (sizeof(T) == 4 ? 4 : 2);
+  static constexpr bool fits = FastCfg<T, N>::LPR >= N / 2 &&
+                               (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
+};
+
+template <typename T, int N, bool UPDATE>
+__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  using C = FastCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
+  constexpr int RS = StageCfg<T, N>::RS;
+  static_assert(StageCfg<T, N>::fits, "staged pass A needs one round of lines and room for the plane");
+  using VT = Vec16<T>;
+  using V = typename VT::type;
+  constexpr int VN = VT::n;
+  extern __shared__ double smem_d[];
+  __shared__ double red[4];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
+  cplx<T>* plane = tw + N;
+  cplx<T>* xch = plane + N * NH;
+  cplx<T>* dcn = xch + C::LPR * C::XS;
+  T* rp = reinterpret_cast<T*>(xch);
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
+  if (a.done != nullptr && a.done[pi.cell]) return;
+  const CellDesc cd = a.cells[pi.cell];
+  const int rhs = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
+  T alpha = T(0);
+  if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  double acc = 0.0;
+  // load (and update) the plane with 16-byte vectors into rp
+  {
+    const V* Rv = reinterpret_cast<const V*>(a.r + fo);
+    V* ROv = reinterpret_cast<V*>(a.r_out + fo);
+    const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
+    V* Xv = reinterpret_cast<V*>(a.x + fo);
+    const V* Pv = reinterpret_cast<const V*>(a.p + fo);
+    constexpr int NV = N * N / VN;
+    for (int i = tid; i < NV; i += BT) {
+      V rv = Rv[i];
+      if (UPDATE) {
+        const V ap = APv[i], pv = Pv[i];
+        V xv = Xv[i];
+        if constexpr (VN == 4) {
+          rv.x -= alpha * ap.x; rv.y -= alpha * ap.y; rv.z -= alpha * ap.z; rv.w -= alpha * ap.w;
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
+        } else {
+          rv.x -= alpha * ap.x; rv.y -= alpha * ap.y;
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+        }
+        ROv[i] = rv;
+        Xv[i] = xv;
+      }
+      const int e = i * VN;
+      const int y = e / N, z = e - (e / N) * N;
+      T* dst = rp + y * RS + z;
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const T v = VT::get(rv, k);
+        acc += (double)v * (double)v;
+        dst[k] = v;
+      }
+    }
+  }
+  const int l = tid / L, t = tid - (tid / L) * L;
+  __syncthreads();
+
+  // z: complex transform of row pairs (one round: line l = pair p)
+  {
+    const int p = l;
+    const bool act = p < NP && t < N2;
+    cplx<T> v[N1];
+    if (act) {
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) v[j1] = {rp[(2 * p) * RS + t + N2 * j1], rp[(2 * p + 1) * RS + t + N2 * j1]};
+    }
+    __syncthreads();  // rp is read completely before the exchange (same memory) is written
+    cplx<T>* xl = xch + l * C::XS;
+    if (act) {
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (p < NP && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+      cplx<T>* zp = plane + (2 * p) * NH;  // the pair's two rows hold Z[0..N-1] for now
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) zp[t + N1 * k2] = w[k2];
+    }
+    __syncthreads();
+    constexpr int ITEMS = NP * NH;
+    constexpr int IPT = (ITEMS + BT - 1) / BT;
+    cplx<T> A[IPT], B[IPT];
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int it = tid + i * BT;
+      const int lp = it / NH, k = it - (it / NH) * NH;
+      if (it < ITEMS) {
+        const cplx<T>* zq = plane + (2 * lp) * NH;
+        const cplx<T> Z = zq[k], M = zq[k == 0 ? 0 : N - k];
+        A[i] = {T(0.5) * (Z.re + M.re), T(0.5) * (Z.im - M.im)};
+        B[i] = {T(0.5) * (Z.im + M.im), T(-0.5) * (Z.re - M.re)};
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int it = tid + i * BT;
+      const int lp = it / NH, k = it - (it / NH) * NH;
+      if (it < ITEMS) {
+        plane[(2 * lp) * NH + k] = A[i];
+        plane[(2 * lp + 1) * NH + k] = B[i];
+      }
+    }
+  }
+  __syncthreads();
+
+  // y: columns 1..NP-1 complex, columns 0 and NP (real) packed into line 0; results in place
+  {
+    const int col = l;
+    const bool act = col < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        v[j1] = col == 0 ? cplx<T>{plane[y * NH].re, plane[y * NH + NP].re} : plane[y * NH + col];
+      }
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int ky = t + N1 * k2;
+        if (col == 0) dcn[ky] = w[k2];
+        else plane[ky * NH + col] = w[k2];
+      }
+    }
+    __syncthreads();
+  }
+  // unpack the packed DC / Nyquist columns
+  for (int ky = tid; ky < N; ky += BT) {
+    const cplx<T> Z = dcn[ky], M = dcn[ky == 0 ? 0 : N - ky];
+    plane[ky * NH] = {T(0.5) * (Z.re + M.re), T(0.5) * (Z.im - M.im)};
+    plane[ky * NH + NP] = {T(0.5) * (Z.im + M.im), T(-0.5) * (Z.re - M.re)};
+  }
+  __syncthreads();
+  // the half plane [ky][kz] is contiguous in the spectrum: 16-byte vector copy
+  {
+    cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
+    constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
+    constexpr int NVS = N * NH / CPV;
+    const V* src = reinterpret_cast<const V*>(plane);
+    V* dst = reinterpret_cast<V*>(so);
+    for (int i = tid; i < NVS; i += BT) dst[i] = src[i];
+  }
+  const double tsum = block_sum(acc, red);
+  if (tid == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = tsum;
+}
+
+template <typename T, int N, bool DOT>
+__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  using C = FastCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
+  constexpr int RS = StageCfg<T, N>::RS;
+  static_assert(StageCfg<T, N>::fits, "staged pass C needs one round of lines and room for the plane");
+  using VT = Vec16<T>;
+  using V = typename VT::type;
+  constexpr int VN = VT::n;
+  extern __shared__ double smem_d[];
+  __shared__ double red[4];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
+  cplx<T>* plane = tw + N;
+  cplx<T>* xch = plane + N * NH;
+  T* rp = reinterpret_cast<T*>(xch);
+  const int item = a.plane_base + blockIdx.x;
+  const PlaneItem pi = a.planes[item];
+  if (a.done != nullptr && a.done[pi.cell]) return;
+  const CellDesc cd = a.cells[pi.cell];
+  const int rhs = blockIdx.y;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  {
+    const cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
+    constexpr int CPV = 16 / sizeof(cplx<T>);
+    constexpr int NVS = N * NH / CPV;
+    const V* src = reinterpret_cast<const V*>(so);
+    V* dst = reinterpret_cast<V*>(plane);
+    for (int i = tid; i < NVS; i += BT) dst[i] = src[i];
+  }
+  const int l = tid / L, t = tid - (tid / L) * L;
+  __syncthreads();
+
+  // y inverse (one round); columns 0 and NP packed (their inverse transforms are real); in place
+  {
+    const int col = l;
+    const bool act = col < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        if (col == 0) {
+          const cplx<T> A = plane[y * NH], B = plane[y * NH + NP];
+          v[j1] = {A.re - B.im, A.im + B.re};
+        } else {
+          v[j1] = plane[y * NH + col];
+        }
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int y = t + N1 * k2;
+        if (col == 0) {
+          plane[y * NH] = {w[k2].re, T(0)};
+          plane[y * NH + NP] = {w[k2].im, T(0)};
+        } else {
+          plane[y * NH + col] = w[k2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // z inverse: row pairs, Z = A + i B with the Hermitian extension; outputs into rp
+  {
+    const int p = l;
+    const bool act = p < NP;
+    cplx<T>* xl = xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+      const cplx<T>* ra = plane + (2 * p) * NH;
+      const cplx<T>* rb = ra + NH;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int k = t + N2 * j1;
+        cplx<T> A, B;
+        if (k < NH) {
+          A = ra[k];
+          B = rb[k];
+        } else {
+          A = ra[N - k];
+          A.im = -A.im;
+          B = rb[N - k];
+          B.im = -B.im;
+        }
+        v[j1] = {A.re - B.im, A.im + B.re};
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    cplx<T> w[N2];
+    if (act && t < N1) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+    }
+    __syncthreads();  // the exchange is read completely before rp (same memory) is written
+    if (act && t < N1) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int z = t + N1 * k2;
+        rp[(2 * p) * RS + z] = w[k2].re;
+        rp[(2 * p + 1) * RS + z] = w[k2].im;
+      }
+    }
+    __syncthreads();
+  }
+  // store z (and r . z) with 16-byte vectors
+  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
+  double acc = 0.0;
+  {
+    V* Zv = reinterpret_cast<V*>(a.z + fo);
+    const V* Rv = reinterpret_cast<const V*>(a.r + fo);
+    constexpr int NV = N * N / VN;
+    for (int i = tid; i < NV; i += BT) {
+      const int e = i * VN;
+      const int y = e / N, z = e - (e / N) * N;
+      const T* srcp = rp + y * RS + z;
+      V zv;
+      if constexpr (VN == 4) zv = {srcp[0], srcp[1], srcp[2], srcp[3]};
+      else zv = {srcp[0], srcp[1]};
+      Zv[i] = zv;
+      if (DOT) {
+        const V rv = Rv[i];
+#pragma unroll
+        for (int k = 0; k < VN; ++k) acc += (double)VT::get(rv, k) * (double)VT::get(zv, k);
+      }
+    }
+  }
+  if (DOT) {
+    const double tsum = block_sum(acc, red);
+    if (tid == 0) a.part_c[(int64_t)rhs * a.nplanes + item] = tsum;
+  }
+}
+
 // Pass B ---------------------------------------------------------------------------------------
 template <typename T, int N>
 __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg,
@@ -392,13 +736,34 @@ size_t fast_lds_line() {
 
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                                int update, int dot, hipStream_t st) {
+                                int update, int dot, int vec, hipStream_t st) {
   using C = FastCfg<T, N>;
   const size_t lp = fast_lds_plane<T, N>(), ll = fast_lds_line<T, N>();
   dim3 gp(nplanes, a.m, 1), gl(nlines, a.m, 1);
   auto big = [](const void* f, size_t b) {
     if (b > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
   };
+  if constexpr (StageCfg<T, N>::fits) {
+    if (vec) {
+      if (update) {
+        big((const void*)prec_a_staged<T, N, true>, lp);
+        hipLaunchKernelGGL((prec_a_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
+      } else {
+        big((const void*)prec_a_staged<T, N, false>, lp);
+        hipLaunchKernelGGL((prec_a_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
+      }
+      big((const void*)prec_b_fast<T, N>, ll);
+      hipLaunchKernelGGL((prec_b_fast<T, N>), gl, dim3(C::BTB), ll, st, a, tw, symp);
+      if (dot) {
+        big((const void*)prec_c_staged<T, N, true>, lp);
+        hipLaunchKernelGGL((prec_c_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
+      } else {
+        big((const void*)prec_c_staged<T, N, false>, lp);
+        hipLaunchKernelGGL((prec_c_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
+      }
+      return hipGetLastError();
+    }
+  }
   if (update) {
     big((const void*)prec_a_fast<T, N, true>, lp);
     hipLaunchKernelGGL((prec_a_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
@@ -429,20 +794,20 @@ int fast_line_len(int n, int elem_bytes) {
 
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, hipStream_t st) {
+                            int update, int dot, int vec, hipStream_t st) {
   switch (n) {
-    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, st);
-    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, st);
-    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, st);
-    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, st);
-    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, st);
+    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
+    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
+    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
+    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
+    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template hipError_t launch_prec_fast<double>(int, const PrecArgs<double>&, int, int, const cplx<double>*,
-                                             const double*, int, int, hipStream_t);
+                                             const double*, int, int, int, hipStream_t);
 template hipError_t launch_prec_fast<float>(int, const PrecArgs<float>&, int, int, const cplx<float>*, const float*,
-                                            int, int, hipStream_t);
+                                            int, int, int, hipStream_t);
 
 }  // namespace nlfh

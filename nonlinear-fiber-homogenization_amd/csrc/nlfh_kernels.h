@@ -121,7 +121,7 @@ bool fast_fft_size(int n);
 int fast_line_len(int n, int elem_bytes);
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, hipStream_t st);
+                            int update, int dot, int vec, hipStream_t st);
 hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, double* ds, int64_t n, double emin,
                                   double p, hipStream_t st);
 hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);
