@@ -209,8 +209,8 @@ __device__ __forceinline__ double wave_sum(const double* part, int nitems, int r
 
 // After the first preconditioner application: bn = ||b|| + 1e-300 (from the chunk partials of
 // b), rn (from the ||r||^2 partials of pass A), rz; decide `done` (nlfiber/linear.py:176-188).
-__global__ void cg_init_kernel(CgState st, const double* part_a, const double* part_c, int nplanes,
-                               const int* prange, const double* part_b, int nchunks, const int* crange) {
+__global__ void cg_init_kernel(CgState st, const double* part_a, int nplanes, const int* prange, const double* part_z,
+                               int nz_items, const int* zrange, const double* part_b, int nchunks, const int* crange) {
   __shared__ int notconv;
   const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) notconv = 0;
@@ -219,7 +219,7 @@ __global__ void cg_init_kernel(CgState st, const double* part_a, const double* p
     const int k = cell * st.m + t;
     const double bn = sqrt(wave_sum(part_b, nchunks, t, crange, cell)) + 1e-300;
     const double rn = sqrt(wave_sum(part_a, nplanes, t, prange, cell));
-    const double rz = wave_sum(part_c, nplanes, t, prange, cell);
+    const double rz = wave_sum(part_z, nz_items, t, zrange, cell);
     if (lane == 0) {
       st.bn[k] = bn;
       st.rn[k] = rn;
@@ -247,8 +247,8 @@ __global__ void cg_alpha_kernel(CgState st, const double* part_mv, int nmv, cons
 
 // beta = rz1 / rz (guard), rz = rz1, rn = ||r||, it += 1, done when every rhs has rn <= tol*bn or
 // it reaches maxit.
-__global__ void cg_beta_kernel(CgState st, const double* part_a, const double* part_c, int nplanes,
-                               const int* prange) {
+__global__ void cg_beta_kernel(CgState st, const double* part_a, int nplanes, const int* prange, const double* part_z,
+                               int nz_items, const int* zrange) {
   __shared__ int notconv;
   const int cell = blockIdx.x, t = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (st.done[cell]) return;
@@ -257,7 +257,7 @@ __global__ void cg_beta_kernel(CgState st, const double* part_a, const double* p
   if (t < st.m) {
     const int k = cell * st.m + t;
     const double rn = sqrt(wave_sum(part_a, nplanes, t, prange, cell));
-    const double rz1 = wave_sum(part_c, nplanes, t, prange, cell);
+    const double rz1 = wave_sum(part_z, nz_items, t, zrange, cell);
     if (lane == 0) {
       const double rz = st.rz[k];
       st.beta[k] = rz1 / (rz == 0.0 ? 1.0 : rz);
@@ -509,18 +509,18 @@ hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const
   return hipGetLastError();
 }
 
-hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
-                          const double* pb, int nch, const int* cr, hipStream_t st) {
-  hipLaunchKernelGGL(cg_init_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, pc, np, pr, pb, nch, cr);
+hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, int np, const int* pr, const double* pz,
+                          int nz, const int* zr, const double* pb, int nch, const int* cr, hipStream_t st) {
+  hipLaunchKernelGGL(cg_init_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, np, pr, pz, nz, zr, pb, nch, cr);
   return hipGetLastError();
 }
 hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int nmv, const int* mr, hipStream_t st) {
   hipLaunchKernelGGL(cg_alpha_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pmv, nmv, mr);
   return hipGetLastError();
 }
-hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
-                          hipStream_t st) {
-  hipLaunchKernelGGL(cg_beta_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, pc, np, pr);
+hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, int np, const int* pr, const double* pz,
+                          int nz, const int* zr, hipStream_t st) {
+  hipLaunchKernelGGL(cg_beta_kernel, dim3(ncells), dim3(64 * cs.m), 0, st, cs, pa, np, pr, pz, nz, zr);
   return hipGetLastError();
 }
 

@@ -182,6 +182,9 @@ struct nlfh_plan {
   ChunkItem* d_chunks = nullptr;
   int* d_mv_range = nullptr;
   int* d_plane_range = nullptr;
+  int* d_line_range32 = nullptr;  // per cell [begin, end) of its float32 line items
+  double* d_part_bdot = nullptr;  // pass B spectral r . z partials (float32 lines)
+  bool lean32 = false;            // every shape group on the register-FFT path: lean FP32 iteration
   int* d_chunk_range = nullptr;
 
   // workspaces
@@ -523,6 +526,7 @@ PrecArgs<T> prec_args(nlfh_plan* p, int m) {
   a.nplanes = (int)p->planes.size();
   a.nlines = (int)(sizeof(T) == 8 ? p->lines64.size() : p->lines32.size());
   a.rows_per_batch = sizeof(T) == 8 ? p->rpb64 : p->rpb32;
+  a.part_bdot = p->d_part_bdot;
   return a;
 }
 
@@ -548,7 +552,9 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       const T* symp = symsoa_ptr<T>(p) + p->symsoa_off[g.shape];
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
-      HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update, dot, lp, ll, st));
+      // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
+      if (dot == 2) return NLFH_EUNSUPPORTED;
+      HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update != 0 ? 1 : 0, dot, lp, ll, st));
     }
   }
   return NLFH_OK;
@@ -566,10 +572,19 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 
 // One PCG iteration (nlfiber/linear.py:190-200) with the buffers of `w`; `cur` selects the old
 // search direction.
+// Lean iteration (FP32 inner solves when every shape group is on the register-FFT path): r . z comes
+// from the spectra in pass B (Parseval), so pass C writes z without reading r. Same PCG recurrences;
+// the dot differs from the direct sum only by rounding.
+template <typename T>
+bool lean(const nlfh_plan* p) {
+  return sizeof(T) == 4 && p->lean32;
+}
+
 template <typename T>
 int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   T* p_old = w.P[cur];
   T* p_new = w.P[1 - cur];
+  const bool ln = lean<T>(p);
   TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
   HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
   PrecArgs<T> a = prec_args<T>(p, m);
@@ -584,8 +599,14 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   a.part_a = p->d_part_a;
   a.part_c = p->d_part_c;
   a.done = w.cs.done;
-  TRY(run_prec<T>(p, a, 1, 1, st));
-  HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, p->d_part_c, (int)p->planes.size(), p->d_plane_range, st));
+  TRY(run_prec<T>(p, a, 2, ln ? 2 : 1, st));
+  if (ln) {
+    HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
+                           (int)p->lines32.size(), p->d_line_range32, st));
+  } else {
+    HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_c,
+                           (int)p->planes.size(), p->d_plane_range, st));
+  }
   return NLFH_OK;
 }
 
@@ -629,9 +650,17 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
   a.part_a = p->d_part_a;
   a.part_c = p->d_part_c;
   a.done = w.cs.skip;
-  TRY(run_prec<T>(p, a, 0, 1, st));
-  HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, p->d_part_c, (int)p->planes.size(), p->d_plane_range,
-                         p->d_part_b, (int)p->chunks.size(), p->d_chunk_range, st));
+  const bool ln = lean<T>(p);
+  TRY(run_prec<T>(p, a, 0, ln ? 2 : 1, st));
+  if (ln) {
+    HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
+                           (int)p->lines32.size(), p->d_line_range32, p->d_part_b, (int)p->chunks.size(),
+                           p->d_chunk_range, st));
+  } else {
+    HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_c,
+                           (int)p->planes.size(), p->d_plane_range, p->d_part_b, (int)p->chunks.size(),
+                           p->d_chunk_range, st));
+  }
   HIP_TRY(launch_fill<T>(w.P[0], p->field_len(), T(0), st));
   int* h_done = p->h_pinned;
   const int nc = p->ncells;
@@ -862,7 +891,7 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
     shape_len.push_back(sh.len[2]);
   }
   // plane and line items grouped by shape (a cell's items stay contiguous), chunk items per cell
-  std::vector<int> prange(2 * ncells), crange;
+  std::vector<int> prange(2 * ncells), crange, lrange32(2 * ncells);
   for (int sidx = 0; sidx < (int)p->shapes.size(); ++sidx) {
     const Shape& sh = p->shapes[sidx];
     PrecGroup g{};
@@ -883,7 +912,9 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
         const int len = g.fast ? fast_line_len(g.fast, prec == 0 ? 8 : 4)
                                : (int)(prec == 0 ? line_len_for<double>(cd.nx, npos) : line_len_for<float>(cd.nx, npos));
         std::vector<LineItem>& lines = prec == 0 ? p->lines64 : p->lines32;
+        if (prec == 1) lrange32[2 * c] = (int)lines.size();
         for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
+        if (prec == 1) lrange32[2 * c + 1] = (int)lines.size();
       }
     }
     g.vec = 1;
@@ -950,6 +981,9 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_lines32, p->lines32))) return fail(rc);
   if ((rc = p->upload(&p->d_chunks, p->chunks))) return fail(rc);
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
+  if ((rc = p->upload(&p->d_line_range32, lrange32))) return fail(rc);
+  p->lean32 = true;
+  for (const PrecGroup& g : p->pgroups) p->lean32 = p->lean32 && g.fast != 0;
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);
   // partials: the matvec item count may change with the tiling, so size for the worst case
@@ -959,6 +993,7 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->alloc(&p->d_part_mv, (size_t)M * max_mv))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_a, (size_t)M * p->planes.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_c, (size_t)M * p->planes.size()))) return fail(rc);
+  if ((rc = p->alloc(&p->d_part_bdot, (size_t)M * std::max<size_t>(1, p->lines32.size())))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_b, (size_t)3 * M * p->chunks.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_e, (size_t)36 * p->chunks.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_strain, (size_t)6 * kMaxRhs))) return fail(rc);

@@ -363,16 +363,20 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
     for (int i = tid; i < NV; i += BT) {
       V rv = Rv[i];
       if (UPDATE) {
-        const V ap = APv[i], pv = Pv[i];
-        V xv = Xv[i];
+        const V ap = APv[i];
         if constexpr (VN == 4) {
           rv.x -= alpha * ap.x; rv.y -= alpha * ap.y; rv.z -= alpha * ap.z; rv.w -= alpha * ap.w;
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
         } else {
           rv.x -= alpha * ap.x; rv.y -= alpha * ap.y;
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
         }
         ROv[i] = rv;
+        const V pv = Pv[i];
+        V xv = Xv[i];
+        if constexpr (VN == 4) {
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
+        } else {
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+        }
         Xv[i] = xv;
       }
       const int e = i * VN;
@@ -643,9 +647,14 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
 }
 
 // Pass B ---------------------------------------------------------------------------------------
-template <typename T, int N>
+// DOTB: also the partial r . z of the line item from the spectra (Parseval over the half spectrum:
+// weight 1 at kz = 0 and kz = N/2, 2 elsewhere; z-hat = Kinv r-hat / N is what the inverse transform
+// turns into z), so pass C need not read r.
+template <typename T, int N, bool DOTB>
 __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg,
                                                                   const T* __restrict__ symp) {
+  __shared__ double red[4];
+  double dacc = 0.0;
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB, XS = C::XS;
   extern __shared__ double smem_d[];
@@ -695,7 +704,18 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
       Xc[0][k2] = {k00 * r0.re + k01 * r1.re + k02 * r2.re, k00 * r0.im + k01 * r1.im + k02 * r2.im};
       Xc[1][k2] = {k01 * r0.re + k11 * r1.re + k12 * r2.re, k01 * r0.im + k11 * r1.im + k12 * r2.im};
       Xc[2][k2] = {k02 * r0.re + k12 * r1.re + k22 * r2.re, k02 * r0.im + k12 * r1.im + k22 * r2.im};
+      if (DOTB) {
+        const double d = (double)r0.re * (double)Xc[0][k2].re + (double)r0.im * (double)Xc[0][k2].im +
+                         (double)r1.re * (double)Xc[1][k2].re + (double)r1.im * (double)Xc[1][k2].im +
+                         (double)r2.re * (double)Xc[2][k2].re + (double)r2.im * (double)Xc[2][k2].im;
+        const int kz = (int)(pos % NH);
+        dacc += (kz == 0 || 2 * kz == N) ? d : 2.0 * d;
+      }
     }
+  }
+  if (DOTB) {
+    const double tsum = block_sum(dacc, red);
+    if (tid == 0) a.part_bdot[(int64_t)rhs * a.nlines + a.line_base + blockIdx.x] = tsum;
   }
   __syncthreads();
   // inverse step 1: in-register length N2 at kx = t + N1 k2; twiddle e^{+2 pi i t k / N}
@@ -734,6 +754,8 @@ size_t fast_lds_line() {
   return ((size_t)N + (size_t)3 * C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
+// update: 0 none, else r -= alpha Ap and x += alpha p in pass A; dot: 0 none, 1 r . z in pass C,
+// 2 r . z in pass B (spectral)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
                                 int update, int dot, int vec, hipStream_t st) {
@@ -743,18 +765,19 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   auto big = [](const void* f, size_t b) {
     if (b > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
   };
+#define NLFH_A(KERN, U)                                                                 \
+  big((const void*)KERN<T, N, U>, lp);                                                \
+  hipLaunchKernelGGL((KERN<T, N, U>), gp, dim3(C::BT), lp, st, a, tw);
+#define NLFH_B(D)                                                                       \
+  big((const void*)prec_b_fast<T, N, D>, ll);                                         \
+  hipLaunchKernelGGL((prec_b_fast<T, N, D>), gl, dim3(C::BTB), ll, st, a, tw, symp);
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update) {
-        big((const void*)prec_a_staged<T, N, true>, lp);
-        hipLaunchKernelGGL((prec_a_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
-      } else {
-        big((const void*)prec_a_staged<T, N, false>, lp);
-        hipLaunchKernelGGL((prec_a_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
-      }
-      big((const void*)prec_b_fast<T, N>, ll);
-      hipLaunchKernelGGL((prec_b_fast<T, N>), gl, dim3(C::BTB), ll, st, a, tw, symp);
-      if (dot) {
+      if (update) { NLFH_A(prec_a_staged, true) }
+      else { NLFH_A(prec_a_staged, false) }
+      if (dot == 2) { NLFH_B(true) }
+      else { NLFH_B(false) }
+      if (dot == 1) {
         big((const void*)prec_c_staged<T, N, true>, lp);
         hipLaunchKernelGGL((prec_c_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
       } else {
@@ -764,16 +787,13 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       return hipGetLastError();
     }
   }
-  if (update) {
-    big((const void*)prec_a_fast<T, N, true>, lp);
-    hipLaunchKernelGGL((prec_a_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
-  } else {
-    big((const void*)prec_a_fast<T, N, false>, lp);
-    hipLaunchKernelGGL((prec_a_fast<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
-  }
-  big((const void*)prec_b_fast<T, N>, ll);
-  hipLaunchKernelGGL((prec_b_fast<T, N>), gl, dim3(C::BTB), ll, st, a, tw, symp);
-  if (dot) {
+  if (update) { NLFH_A(prec_a_fast, true) }
+  else { NLFH_A(prec_a_fast, false) }
+  if (dot == 2) { NLFH_B(true) }
+  else { NLFH_B(false) }
+#undef NLFH_A
+#undef NLFH_B
+  if (dot == 1) {
     big((const void*)prec_c_fast<T, N, true>, lp);
     hipLaunchKernelGGL((prec_c_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
   } else {

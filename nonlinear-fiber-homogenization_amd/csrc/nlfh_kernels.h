@@ -63,6 +63,7 @@ struct PrecArgs {
   T* z;                  // pass C output
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
   double* part_c;        // pass C: partial r . z per (rhs, plane item)
+  double* part_bdot;     // pass B (spectral dot): partial r . z per (rhs, line item)
   const int* done;       // per-cell early exit (nullable)
   int m;
   int nplanes;           // total plane items (row stride of part_a / part_c)
@@ -145,11 +146,11 @@ hipError_t launch_sub_mean(const VecArgs& va, T* v, const double* mean, hipStrea
 hipError_t launch_ir_down(const VecArgs& va, const double* r64, float* r32, float* d32, const double* scale,
                           hipStream_t st);
 hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, hipStream_t st);
-hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
-                          const double* pb, int nch, const int* cr, hipStream_t st);
+hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, int np, const int* pr, const double* pz,
+                          int nz, const int* zr, const double* pb, int nch, const int* cr, hipStream_t st);
 hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int nmv, const int* mr, hipStream_t st);
-hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, const double* pc, int np, const int* pr,
-                          hipStream_t st);
+hipError_t launch_cg_beta(const CgState& cs, int ncells, const double* pa, int np, const int* pr, const double* pz,
+                          int nz, const int* zr, hipStream_t st);
 hipError_t launch_norms(const double* part, int nchunks, const int* crange, int ncells, int m, int M, double eps,
                         double* out, const double* bn, double tol, int* flag, const int* force, double* tol_in,
                         double tol_in_base, double tol_in_safety, hipStream_t st);
