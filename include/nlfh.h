@@ -175,6 +175,24 @@ int nlfh_net_f2bf(const float* in, unsigned short* out, long long len, void* str
 /* wt[26 - t][ci][co] = w[t][co][ci] (zero for ci >= cin): weights of the data gradient */
 int nlfh_net_wflip(const unsigned short* w, int coutp, int cin, unsigned short* wt, int cinp, void* stream);
 
+/* ---- Density map (nlfiber/density.py:76-91, nlfiber/symmetry.py:26-102) ----------------------------
+ * phi: symmetric octant (n_o, n_o, n_o) float64; rho, u: full cell (2 n_o)^3 float64, device memory.
+ * Replaces DensityMap.__init__ (nlfiber/density.py:79-84): the periodic hat filter of radius r voxels
+ * becomes a direct stencil (its support is |offset| <= r). */
+typedef struct nlfh_dmap nlfh_dmap;
+int nlfh_dmap_create(int n_o, double r, double v, nlfh_dmap** out);
+int nlfh_dmap_destroy(nlfh_dmap* dm);
+/* u = exact_symmetrize(F mirror(sym_perm(phi)) / std). Replaces DensityMap.smooth (nlfiber/density.py:80-84). */
+int nlfh_dmap_smooth(nlfh_dmap* dm, const double* phi, double* u_out, void* stream);
+/* rho = sigmoid(beta (u - eta)) with mean(rho) = v; eta_out (host, nullable; synchronises the stream).
+ * Replaces DensityMap.rho / eta and threshold / solve_eta (nlfiber/density.py:43-72, 86-91). */
+int nlfh_dmap_rho(nlfh_dmap* dm, const double* phi, double beta, double* rho_out, double* eta_out_host, void* stream);
+/* phibar_r = (d rho / d phi)^T rbar_r for k in 1..3 rows (rbar: k full cells, phibar: k octants), with the
+ * implicit eta derivative of nlfiber/density.py:66-72: the backward pass JAX derives for DensityMap.rho,
+ * used by Fiber.value_jac (nlfiber/fiber.py:35-38). Stateless: recomputes the forward pass. */
+int nlfh_dmap_rho_vjp(nlfh_dmap* dm, const double* phi, double beta, const double* rbar, int k, double* phibar_out,
+                      void* stream);
+
 #ifdef __cplusplus
 }
 #endif

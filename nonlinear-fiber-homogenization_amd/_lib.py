@@ -14,6 +14,11 @@ _vp = ctypes.c_void_p
 
 # (name, restype, argtypes): every symbol of include/nlfh.h
 SIGNATURES = [
+    ("nlfh_dmap_create", ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
+    ("nlfh_dmap_destroy", ctypes.c_int, [_vp]),
+    ("nlfh_dmap_smooth", ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    ("nlfh_dmap_rho", ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, _c_double_p, _vp]),
+    ("nlfh_dmap_rho_vjp", ctypes.c_int, [_vp, _vp, ctypes.c_double, _vp, ctypes.c_int, _vp, _vp]),
     ("nlfh_last_error", ctypes.c_char_p, []),
     ("nlfh_version", ctypes.c_int, []),
     ("nlfh_plan_create", ctypes.c_int, [_c_int_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,

@@ -179,3 +179,60 @@ def test_cell_generator_volume_and_symmetry(pkg, n):
     assert np.array_equal(rho, rho[::-1]) and np.array_equal(rho, rho[:, ::-1]) and np.array_equal(rho, rho[:, :, ::-1])
     r0 = pkg.cells.schwarz_cell(n)
     assert abs(r0.mean() - 0.3) < 1e-12
+
+
+# --- density map (nlfiber/density.py, nlfiber/symmetry.py) -----------------------------------------
+from oracle import density_ref as DR  # noqa: E402
+
+
+def test_density_symmetry_helpers():
+    """mirror/fold round trip, sym_perm an orthogonal projection, the mirrored cell has every symmetry
+    (the reference's tests/test_core.py:20-41), and exact_symmetrize equals the 48-map group average."""
+    rng = np.random.default_rng(0)
+    x = DR.sym_perm(rng.standard_normal((5, 5, 5)))
+    full = DR.mirror(x)
+    np.testing.assert_allclose(DR.fold(full), x, atol=1e-14)
+    y = rng.standard_normal((5, 5, 5))
+    assert abs(np.vdot(DR.sym_perm(y), x) - np.vdot(y, x)) < 1e-10
+    f = DR.mirror(DR.sym_perm(rng.standard_normal((4, 4, 4))))
+    np.testing.assert_allclose(f, f[::-1], atol=0)
+    np.testing.assert_allclose(f, np.transpose(f, (2, 0, 1)), atol=0)
+    g = rng.standard_normal((8, 8, 8))
+    e = DR.exact_symmetrize(g)
+    np.testing.assert_allclose(e, DR.group_average(g), atol=1e-14)
+    assert np.array_equal(e, e[::-1]) and np.array_equal(e, np.transpose(e, (1, 2, 0)))
+    # the group average is self-adjoint
+    h = rng.standard_normal((8, 8, 8))
+    assert abs(np.vdot(DR.group_average(g), h) - np.vdot(g, DR.group_average(h))) < 1e-10
+
+
+def test_density_filter_is_the_periodic_hat_stencil():
+    """The rfftn filter equals the brute-force periodic convolution with the hat kernel."""
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((6, 6, 6))
+    for r in (1.5, 2.5):
+        k = DR.hat_kernel(x.shape, r)
+        want = np.zeros_like(x)
+        for i, j, l in zip(*np.nonzero(k)):
+            want += k[i, j, l] * np.roll(x, (i, j, l), axis=(0, 1, 2))
+        np.testing.assert_allclose(DR.filt(x, r), want, atol=1e-13)
+
+
+def test_density_exact_volume_and_vjp():
+    """mean rho = v for every beta and the adjoint against central differences of sum(rho^3)
+    (the reference's tests/test_core.py:43-53)."""
+    rng = np.random.default_rng(2)
+    phi = DR.sym_perm(rng.standard_normal((4, 4, 4)))
+    for beta in (2.0, 8.0, 32.0):
+        assert abs(DR.rho(phi, beta, 1.5, 0.27).mean() - 0.27) < 1e-12
+    beta = 4.0
+    r = DR.rho(phi, beta, 1.5, 0.27)
+    g = DR.rho_vjp(phi, beta, 1.5, 0.27, 3 * r ** 2)[0]
+    d = DR.sym_perm(rng.standard_normal((4, 4, 4)))
+    h = 1e-6
+    fd = (np.sum(DR.rho(phi + h * d, beta, 1.5, 0.27) ** 3) - np.sum(DR.rho(phi - h * d, beta, 1.5, 0.27) ** 3)) / (2 * h)
+    assert abs(np.vdot(g, d) - fd) < 1e-7 * abs(fd) + 1e-10
+    # a non-symmetric direction too (the map symmetrises its input)
+    d2 = rng.standard_normal((4, 4, 4))
+    fd2 = (np.sum(DR.rho(phi + h * d2, beta, 1.5, 0.27) ** 3) - np.sum(DR.rho(phi - h * d2, beta, 1.5, 0.27) ** 3)) / (2 * h)
+    assert abs(np.vdot(g, d2) - fd2) < 1e-7 * abs(fd2) + 1e-10
