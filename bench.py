@@ -41,7 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128"])
+    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber"])
+    ap.add_argument("--fiber-n-o", type=int, default=16, help="octant side of the fiber workload (cell 2 n_o)")
     ap.add_argument("--cells", type=int, default=8, help="64^3 cells per step (cell64)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--learned", default=None, help="warm-start weights (npz)")
@@ -88,6 +89,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "fiber":
+        return fiber_bench(args)
     jobs, emin, wname = workload(args)
     tol = 1e-8
     if wname == "stress128":
@@ -257,6 +260,86 @@ def matvec_roofline(pkg, bh, rho_d, precision):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": f"{kname} (K(s) u, 6 RHS, {bh.tot} voxels = {len(bh.shapes)} cells)",
             "algorithmic_bytes": bytes_mv, "avg_us": round(t_mv * 1e6, 2)}
+
+
+def fiber_bench(args):
+    """Fiber.value_jac (density map + warm-started 2-RHS PCG + invariant gradient + density-map VJP) along a
+    path of pre-densities phi_k = phi_0 + 0.02 k d, as a projection loop evaluates them."""
+    import torch
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    pkg.load_library()
+    torch.cuda.set_device(0)
+    from nlfh_amd.fiber import Fiber, random_start
+    n_o, beta, v, r = args.fiber_n_o, 8.0, 0.3, 1.5
+    phi0 = random_start(n_o, seed=0)
+    d = random_start(n_o, seed=1)
+    fib = Fiber(n_o, v=v, r=r)  # the reference's defaults: emin 1e-3, tol 1e-9
+    phis = [torch.tensor(phi0 + 0.02 * k * d, device="cuda") for k in range(args.warmup + args.steps)]
+    for k in range(args.warmup):
+        fib.value_jac(phis[k], beta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = []
+    for k in range(args.warmup, args.warmup + args.steps):
+        c, J = fib.value_jac(phis[k], beta)
+        its.append(fib.lh.last_info["it"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    value = args.steps / dt
+    # roofline: the FP64 matvec on the fiber's 2 right-hand sides
+    lib = pkg.load_library()
+    s = fib.lh.stiffness_scale(fib.rho(phis[-1], beta))
+    n = 2 * n_o
+    u = torch.randn((2, 3, n, n, n), dtype=torch.float64, device="cuda")
+    out = torch.empty_like(u)
+    st = torch.cuda.current_stream()
+    a = (fib.lh._plan.handle, ctypes.c_void_p(s.data_ptr()), ctypes.c_void_p(u.data_ptr()),
+         ctypes.c_void_p(out.data_ptr()), 2, 0, ctypes.c_void_p(st.cuda_stream))
+    for _ in range(3):
+        pkg._lib.check(lib.nlfh_apply_K(*a))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(50):
+        pkg._lib.check(lib.nlfh_apply_K(*a))
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_mv = e0.elapsed_time(e1) / 50 * 1e-3
+    bytes_mv = n ** 3 * (2 * 3 * 2 * 8 + 8)
+    roof = {"bound": "hbm", "achieved": round(bytes_mv / t_mv / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(bytes_mv / t_mv / 1e9 / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"matvec_kernel<double> (K(s) u, 2 RHS, {n}^3)", "algorithmic_bytes": bytes_mv,
+            "avg_us": round(t_mv * 1e6, 2)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = fiber_cpu(phis[args.warmup].cpu().numpy(), beta, v, r)
+    print(json.dumps({
+        "metric": "Fiber.value_jac evaluations/sec (density map + 2-RHS PCG + Jacobian rows)",
+        "value": round(value, 3), "unit": "value_jac/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Fiber(n_o={n_o}) on a {n}^3 cell, beta {beta}, v {v}, r {r}, emin 1e-3, tol 1e-9, "
+                               f"warm-started along phi_k = phi_0 + 0.02 k d", "pcg_iterations": its},
+        "roofline": roof, "cpu_baseline": cpu}), flush=True)
+
+
+def fiber_cpu(phi, beta, v, r):
+    """The oracle chain for one cold value_jac on the host cores."""
+    from oracle import density_ref as DR
+    from oracle import linear_ref as O
+    cores = os.cpu_count() or 1
+    O.set_fft_workers(cores)
+    t0 = time.perf_counter()
+    rho = DR.rho(phi, beta, r, v)
+    ref = O.LinearRef(rho.shape, emin=1e-3, tol=1e-9)
+    c, u2, it = ref.invariants(rho)
+    Jrho = ref.invariants_grad(rho, u2)
+    DR.rho_vjp(phi, beta, r, v, Jrho)
+    t = time.perf_counter() - t0
+    return {"value": round(1.0 / t, 5), "unit": "value_jac/s", "cores": cores, "kind": "port",
+            "sample": f"numpy oracle: one cold value_jac ({it} PCG iterations, 2 RHS) on the host cores",
+            "seconds": round(t, 2)}
 
 
 def copy_probe(mib):
