@@ -348,8 +348,12 @@ void default_tiling(Shape& sh) {
   // four LDS buffers of (ty + 2) rows x 6 values x nz (float64) must fit
   const int lds_rows = (int)(kLdsMax / ((size_t)4 * 6 * sh.nz * sizeof(double)));
   rows = std::min(rows, lds_rows);
-  sh.ty = std::max(1, std::min(rows - 2, sh.ny));
-  sh.xc = std::max(1, std::min(8, sh.nx));
+  // small tiles keep every CU busy: the autotuner's measurements on gfx950 pick ty = 2-4 and xc = 2 (nx <= 32),
+  // 8 (nx <= 64) or 16 (larger) for cubic cells from 16^3 to 128^3
+  const int ty_pref = sh.ny <= 16 ? 2 : 4;
+  sh.ty = std::max(1, std::min(std::min(rows - 2, ty_pref), sh.ny));
+  const int xc_pref = sh.nx <= 32 ? 2 : (sh.nx <= 64 ? 8 : 16);
+  sh.xc = std::max(1, std::min(xc_pref, sh.nx));
 }
 
 int build_mv_items(nlfh_plan* p) {

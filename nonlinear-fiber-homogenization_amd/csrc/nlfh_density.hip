@@ -5,8 +5,8 @@
 //                                          radius r voxels, so |offset| <= R = max nonzero offset)
 //   u   = exact_symmetrize(f / std(f))     every voxel takes its orbit's value, computed once from the
 //                                          sorted octant representative (bitwise equal over the orbit)
-//   rho = sigmoid(beta (u - eta)),  mean(rho) = v   (eta: bisection until the bracket stops moving,
-//                                                    then 3 Newton steps, in one block)
+//   rho = sigmoid(beta (u - eta)),  mean(rho) = v   (eta: a grid-wide bracket search, 64 candidates
+//                                                    per pass, then 3 Newton steps)
 //
 // The adjoint: threshold with the implicit d eta, the orbit average (self-adjoint), the std scaling,
 // the same stencil (the hat kernel is even), and the octant collapse (sum of the 8 mirror images,
@@ -160,9 +160,18 @@ __global__ void __launch_bounds__(1024) dm_moments_kernel(const double* __restri
   }
 }
 
-// eta with mean(sigmoid(beta (u - eta))) = v (nlfiber/density.py:43-63), one block; stats[2] = eta
-__global__ void __launch_bounds__(1024) dm_eta_kernel(const double* __restrict__ u, int64_t N, double v, double beta,
-                                                      double* stats) {
+// eta with mean(sigmoid(beta (u - eta))) = v (nlfiber/density.py:43-63). The reference bisects 200
+// times and takes 3 Newton steps; here each pass evaluates the volume at kEtaCand interior points of the
+// bracket over the whole grid at once and keeps the sub-interval that contains the root, so 10 passes
+// shrink the bracket below double resolution; then the same 3 Newton steps. stats: [2] eta, [3] lo,
+// [4] hi.
+constexpr int kEtaCand = 64;  // candidates per pass (blockDim of the partial kernel is kEtaCand * kEtaLanes)
+constexpr int kEtaLanes = 4;
+constexpr int kEtaVox = 2048;  // voxels per block of the partial kernel
+
+// stats[3], stats[4] = min(u) - 40 / beta, max(u) + 40 / beta (one block)
+__global__ void __launch_bounds__(1024) dm_bracket_kernel(const double* __restrict__ u, int64_t N, double beta,
+                                                          double* stats) {
   __shared__ double red[16];
   double mn = INFINITY, mx = -INFINITY;
   for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
@@ -171,31 +180,87 @@ __global__ void __launch_bounds__(1024) dm_eta_kernel(const double* __restrict__
   }
   const double umin = block_min_all(mn, red);
   const double umax = -block_min_all(-mx, red);
-  double lo = umin - 40.0 / beta, hi = umax + 40.0 / beta;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) s += sig(beta * (u[i] - mid));
-    const double vol = block_sum_all(s, red) / (double)N;
-    const double nlo = vol > v ? mid : lo, nhi = vol > v ? hi : mid;
-    const bool same = nlo == lo && nhi == hi;
-    lo = nlo;
-    hi = nhi;
-    if (same || lo == hi) break;  // the bracket stopped moving: later steps would not change it
+  if (threadIdx.x == 0) {
+    stats[3] = umin - 40.0 / beta;
+    stats[4] = umax + 40.0 / beta;
   }
-  double eta = 0.5 * (lo + hi);
-  for (int it = 0; it < 3; ++it) {
-    double s = 0.0, ds = 0.0;
-    for (int64_t i = threadIdx.x; i < N; i += blockDim.x) {
-      const double g = sig(beta * (u[i] - eta));
-      s += g;
-      ds += beta * g * (1.0 - g);
-    }
-    const double ms = block_sum_all(s, red) / (double)N;
-    const double dv = -block_sum_all(ds, red) / (double)N;
-    eta = eta - (ms - v) / (dv != 0.0 ? dv : -1.0);
+}
+
+__device__ __forceinline__ double eta_candidate(double lo, double hi, int j) {
+  return lo + (hi - lo) * (double)(j + 1) / (double)(kEtaCand + 1);
+}
+
+// part[block][j] = sum over the block's voxels of sigmoid(beta (u - eta_j))
+__global__ void __launch_bounds__(kEtaCand* kEtaLanes) dm_eta_partial_kernel(const double* __restrict__ u, int64_t N,
+                                                                             double beta, const double* stats,
+                                                                             double* part) {
+  __shared__ double acc[kEtaLanes][kEtaCand];
+  const int j = threadIdx.x % kEtaCand, w = threadIdx.x / kEtaCand;
+  const double eta = eta_candidate(stats[3], stats[4], j);
+  const int64_t b0 = (int64_t)blockIdx.x * kEtaVox;
+  const int64_t b1 = b0 + kEtaVox < N ? b0 + kEtaVox : N;
+  double s = 0.0;
+  for (int64_t i = b0 + w; i < b1; i += kEtaLanes) s += sig(beta * (u[i] - eta));
+  acc[w][j] = s;
+  __syncthreads();
+  if (w == 0) {
+    double t = 0.0;
+    for (int q = 0; q < kEtaLanes; ++q) t += acc[q][j];
+    part[(int64_t)blockIdx.x * kEtaCand + j] = t;
   }
-  if (threadIdx.x == 0) stats[2] = eta;
+}
+
+// volumes of the candidates (fixed-order sums over the blocks); the new bracket is the sub-interval
+// where the volume crosses v (the volume decreases with eta). One block of kEtaCand threads.
+__global__ void dm_eta_select_kernel(const double* part, int nblocks, int64_t N, double v, double* stats) {
+  __shared__ int above[kEtaCand];
+  const int j = threadIdx.x;
+  const double lo = stats[3], hi = stats[4];
+  double t = 0.0;
+  for (int b = 0; b < nblocks; ++b) t += part[(int64_t)b * kEtaCand + j];
+  above[j] = t / (double)N > v ? 1 : 0;
+  __syncthreads();
+  if (j == 0) {
+    int k = 0;  // number of candidates with volume above v: the root lies after candidate k - 1
+    while (k < kEtaCand && above[k]) ++k;
+    const double nlo = k == 0 ? lo : eta_candidate(lo, hi, k - 1);
+    const double nhi = k == kEtaCand ? hi : eta_candidate(lo, hi, k);
+    stats[3] = nlo;
+    stats[4] = nhi;
+    stats[2] = 0.5 * (nlo + nhi);
+  }
+}
+
+// part[block][0..1] = sums of sigmoid and beta sigmoid (1 - sigmoid) at stats[2]
+__global__ void __launch_bounds__(256) dm_newton_partial_kernel(const double* __restrict__ u, int64_t N, double beta,
+                                                                const double* stats, double* part) {
+  __shared__ double red[4];
+  const double eta = stats[2];
+  const int64_t b0 = (int64_t)blockIdx.x * kEtaVox;
+  const int64_t b1 = b0 + kEtaVox < N ? b0 + kEtaVox : N;
+  double s = 0.0, ds = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const double g = sig(beta * (u[i] - eta));
+    s += g;
+    ds += beta * g * (1.0 - g);
+  }
+  const double ts = block_sum(s, red);
+  const double tds = block_sum(ds, red);
+  if (threadIdx.x == 0) {
+    part[2 * (int64_t)blockIdx.x] = ts;
+    part[2 * (int64_t)blockIdx.x + 1] = tds;
+  }
+}
+
+// one Newton step eta -= (mean s - v) / (-mean ds) (guard as the reference), one thread
+__global__ void dm_newton_step_kernel(const double* part, int nblocks, int64_t N, double v, double* stats) {
+  double s = 0.0, ds = 0.0;
+  for (int b = 0; b < nblocks; ++b) {
+    s += part[2 * (int64_t)b];
+    ds += part[2 * (int64_t)b + 1];
+  }
+  const double dv = -ds / (double)N;
+  stats[2] = stats[2] - (s / (double)N - v) / (dv != 0.0 ? dv : -1.0);
 }
 
 // rho = sigmoid(beta (u - eta)), d = rho (1 - rho)
@@ -289,10 +354,12 @@ struct nlfh_dmap {
   double* d_f = nullptr;      // filtered field
   double* d_u = nullptr;      // symmetrised, scaled field
   double* d_d = nullptr;      // rho (1 - rho)
-  double* d_stats = nullptr;  // mean, std, eta
+  double* d_stats = nullptr;  // mean, std, eta, bracket lo, hi
   double* d_sums = nullptr;   // [0] sum d, [1..3] rbar . d, [4..6] usbar . f
   double* d_a = nullptr;      // 3 N work rows
   double* d_b = nullptr;      // 3 N work rows
+  double* d_part = nullptr;   // eta search partials (blocks x candidates)
+  int eta_blocks = 0;
   std::vector<void*> allocs;
 };
 
@@ -315,7 +382,19 @@ int dm_forward(nlfh_dmap* dm, const double* phi, double beta, double* rho, hipSt
   hipLaunchKernelGGL(dm_symmetrize_kernel, dim3(grid_for(N)), dim3(256), 0, st, dm->d_f, dm->n, dm->d_stats, dm->d_u,
                      1);
   if (beta > 0.0) {
-    hipLaunchKernelGGL(dm_eta_kernel, dim3(1), dim3(1024), 0, st, dm->d_u, N, dm->v, beta, dm->d_stats);
+    const int nb = dm->eta_blocks;
+    hipLaunchKernelGGL(dm_bracket_kernel, dim3(1), dim3(1024), 0, st, dm->d_u, N, beta, dm->d_stats);
+    for (int pass = 0; pass < 10; ++pass) {
+      hipLaunchKernelGGL(dm_eta_partial_kernel, dim3(nb), dim3(kEtaCand * kEtaLanes), 0, st, dm->d_u, N, beta,
+                         dm->d_stats, dm->d_part);
+      hipLaunchKernelGGL(dm_eta_select_kernel, dim3(1), dim3(kEtaCand), 0, st, dm->d_part, nb, N, dm->v,
+                         dm->d_stats);
+    }
+    for (int it = 0; it < 3; ++it) {
+      hipLaunchKernelGGL(dm_newton_partial_kernel, dim3(nb), dim3(256), 0, st, dm->d_u, N, beta, dm->d_stats,
+                         dm->d_part);
+      hipLaunchKernelGGL(dm_newton_step_kernel, dim3(1), dim3(1), 0, st, dm->d_part, nb, N, dm->v, dm->d_stats);
+    }
     hipLaunchKernelGGL(dm_sigmoid_kernel, dim3(grid_for(N)), dim3(256), 0, st, dm->d_u, N, beta, dm->d_stats, rho,
                        dm->d_d);
   }
@@ -369,11 +448,13 @@ int nlfh_dmap_create(int n_o, double r, double v, nlfh_dmap** out) {
   dm->r = r;
   dm->v = v;
   dm->N = (int64_t)n * n * n;
+  dm->eta_blocks = (int)((dm->N + kEtaVox - 1) / kEtaVox);
   int rc = NLFH_OK;
   if ((rc = dm_alloc(dm, &dm->d_w, w.size())) || (rc = dm_alloc(dm, &dm->d_f, dm->N)) ||
       (rc = dm_alloc(dm, &dm->d_u, dm->N)) || (rc = dm_alloc(dm, &dm->d_d, dm->N)) ||
-      (rc = dm_alloc(dm, &dm->d_stats, 4)) || (rc = dm_alloc(dm, &dm->d_sums, 8)) ||
-      (rc = dm_alloc(dm, &dm->d_a, 3 * dm->N)) || (rc = dm_alloc(dm, &dm->d_b, 3 * dm->N))) {
+      (rc = dm_alloc(dm, &dm->d_stats, 8)) || (rc = dm_alloc(dm, &dm->d_sums, 8)) ||
+      (rc = dm_alloc(dm, &dm->d_a, 3 * dm->N)) || (rc = dm_alloc(dm, &dm->d_b, 3 * dm->N)) ||
+      (rc = dm_alloc(dm, &dm->d_part, (size_t)dm->eta_blocks * kEtaCand))) {
     nlfh_dmap_destroy(dm);
     return rc;
   }
