@@ -25,7 +25,7 @@ struct FastCfg {
   static constexpr int BT = (N == 128 && sizeof(T) == 8) ? 128 : 256;  // threads of passes A, C
   static constexpr int LPR = BT / L;                                   // lines per round
   static constexpr int XS = N1 * (N2 + 1) > N2 * (N1 + 1) ? N1 * (N2 + 1) : N2 * (N1 + 1);
-  static constexpr int LPB = (N == 128 ? 8 : 16) * (sizeof(T) == 8 ? 1 : 2);  // pass B positions per block
+  static constexpr int LPB = N == 128 ? 8 : 16;  // pass B positions per block
   static constexpr int BTB = L * LPB;                                          // threads of pass B
 };
 
