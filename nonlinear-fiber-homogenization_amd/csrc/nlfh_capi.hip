@@ -185,6 +185,7 @@ struct nlfh_plan {
   int* d_line_range32 = nullptr;  // per cell [begin, end) of its float32 line items
   double* d_part_bdot = nullptr;  // pass B spectral r . z partials (float32 lines)
   bool lean32 = false;            // every shape group on the register-FFT path: lean FP32 iteration
+  bool pair32 = false;            // ... and on the staged pass A: x is updated every second iteration
   int* d_chunk_range = nullptr;
 
   // workspaces
@@ -435,13 +436,14 @@ int ensure_work(nlfh_plan* p, Work<T>& w) {
   TRY(p->alloc(&w.s, p->tot_vox));
   TRY(p->alloc(&w.spec, p->spec_len()));
   const int nk = p->ncells * p->M;
-  TRY(p->alloc(&w.st, 5 * nk));
+  TRY(p->alloc(&w.st, 6 * nk));
   TRY(p->alloc(&w.ist, 2 * p->ncells));
   w.cs.rz = w.st;
   w.cs.alpha = w.st + nk;
   w.cs.beta = w.st + 2 * nk;
   w.cs.bn = w.st + 3 * nk;
   w.cs.rn = w.st + 4 * nk;
+  w.cs.alpha_prev = nullptr;
   w.cs.it = w.ist;
   w.cs.done = w.ist + p->ncells;
   w.cs.skip = nullptr;
@@ -463,6 +465,7 @@ int get_ready32(nlfh_plan* p) {
     TRY(ensure_work(p, p->w32));
     p->w32.cs.tol = p->d_params + p->ncells;
     p->w32.cs.maxit = p->d_iparams + 1;
+    if (p->pair32) p->w32.cs.alpha_prev = p->w32.st + 5 * p->ncells * p->M;
     p->w32_ready = true;
   }
   return NLFH_OK;
@@ -557,7 +560,7 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
-      if (dot == 2) return NLFH_EUNSUPPORTED;
+      if (dot == 2 || update == 1 || update == 3) return NLFH_EUNSUPPORTED;
       HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update != 0 ? 1 : 0, dot, lp, ll, st));
     }
   }
@@ -577,8 +580,11 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // One PCG iteration (nlfiber/linear.py:190-200) with the buffers of `w`; `cur` selects the old
 // search direction.
 // Lean iteration (FP32 inner solves when every shape group is on the register-FFT path): r . z comes
-// from the spectra in pass B (Parseval), so pass C writes z without reading r. Same PCG recurrences;
-// the dot differs from the direct sum only by rounding.
+// from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged pass A
+// (pair32) x is updated every second iteration, x += alpha_{k-1} p_{k-1} + alpha_k p_k at odd k (both
+// directions are still in the ping-pong buffers then), and launch_x_flush_odd adds the step a cell
+// owes when it stops after an even iteration. Same PCG recurrences; the dot differs from the direct sum
+// only by rounding.
 template <typename T>
 bool lean(const nlfh_plan* p) {
   return sizeof(T) == 4 && p->lean32;
@@ -603,7 +609,13 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   a.part_a = p->d_part_a;
   a.part_c = p->d_part_c;
   a.done = w.cs.done;
-  TRY(run_prec<T>(p, a, 2, ln ? 2 : 1, st));
+  int upd = 2;
+  if (ln && p->pair32) {
+    upd = cur == 1 ? 3 : 1;  // cur == k & 1 (every graph holds an even number of iterations)
+    a.alpha_prev = w.cs.alpha_prev;
+    a.p_prev = p_old;
+  }
+  TRY(run_prec<T>(p, a, upd, ln ? 2 : 1, st));
   if (ln) {
     HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
                            (int)p->lines32.size(), p->d_line_range32, st));
@@ -698,6 +710,10 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
       set_error("internal: the solver did not flag completion within maxit iterations");
       return NLFH_EHIP;
     }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (ln && p->pair32)
+      HIP_TRY(launch_x_flush_odd<float>(vec_args(p, m, nullptr), w.X, w.P[0], w.P[1], w.cs.alpha, w.cs.it, st));
   }
   HIP_TRY(hipMemcpyAsync(it_host, w.cs.it, nc * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -987,7 +1003,11 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
   if ((rc = p->upload(&p->d_line_range32, lrange32))) return fail(rc);
   p->lean32 = true;
-  for (const PrecGroup& g : p->pgroups) p->lean32 = p->lean32 && g.fast != 0;
+  p->pair32 = true;
+  for (const PrecGroup& g : p->pgroups) {
+    p->lean32 = p->lean32 && g.fast != 0;
+    p->pair32 = p->pair32 && g.fast != 0 && g.fast <= 64 && g.vec;
+  }
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);
   // partials: the matvec item count may change with the tiling, so size for the worst case

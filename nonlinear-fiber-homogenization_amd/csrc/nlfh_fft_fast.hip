@@ -325,8 +325,11 @@ struct StageCfg {
                                (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
 };
 
-template <typename T, int N, bool UPDATE>
+// UPD: 0 none; 1 r -= alpha Ap; 2 also x += alpha p; 3 also x += alpha_prev p_prev + alpha p (the
+// pair update of odd iterations, see the FP32 iteration in nlfh_capi.hip)
+template <typename T, int N, int UPD>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  constexpr bool UPDATE = UPD >= 1;
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
   constexpr int RS = StageCfg<T, N>::RS;
@@ -349,11 +352,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += BT) tw[i] = twg[i];
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
-  T alpha = T(0);
+  T alpha = T(0), alpha_p = T(0);
   if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  if (UPD == 3) alpha_p = (T)a.alpha_prev[pi.cell * a.m + rhs];
   double acc = 0.0;
   // load (and update) the plane with 16-byte vectors into rp
   {
+    const V* PPv = reinterpret_cast<const V*>(UPD == 3 ? a.p_prev + fo : nullptr);
     const V* Rv = reinterpret_cast<const V*>(a.r + fo);
     V* ROv = reinterpret_cast<V*>(a.r_out + fo);
     const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
@@ -370,14 +375,24 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
           rv.x -= alpha * ap.x; rv.y -= alpha * ap.y;
         }
         ROv[i] = rv;
-        const V pv = Pv[i];
-        V xv = Xv[i];
-        if constexpr (VN == 4) {
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
-        } else {
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+        if (UPD >= 2) {
+          const V pv = Pv[i];
+          V xv = Xv[i];
+          if (UPD == 3) {
+            const V qv = PPv[i];
+            if constexpr (VN == 4) {
+              xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y; xv.z += alpha_p * qv.z; xv.w += alpha_p * qv.w;
+            } else {
+              xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y;
+            }
+          }
+          if constexpr (VN == 4) {
+            xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
+          } else {
+            xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+          }
+          Xv[i] = xv;
         }
-        Xv[i] = xv;
       }
       const int e = i * VN;
       const int y = e / N, z = e - (e / N) * N;
@@ -754,8 +769,8 @@ size_t fast_lds_line() {
   return ((size_t)N + (size_t)3 * C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
-// update: 0 none, else r -= alpha Ap and x += alpha p in pass A; dot: 0 none, 1 r . z in pass C,
-// 2 r . z in pass B (spectral)
+// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 3 (r and the pair update)
+// on the staged pass only; dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
                                 int update, int dot, int vec, hipStream_t st) {
@@ -773,8 +788,10 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   hipLaunchKernelGGL((prec_b_fast<T, N, D>), gl, dim3(C::BTB), ll, st, a, tw, symp);
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update) { NLFH_A(prec_a_staged, true) }
-      else { NLFH_A(prec_a_staged, false) }
+      if (update == 3) { NLFH_A(prec_a_staged, 3) }
+      else if (update == 1) { NLFH_A(prec_a_staged, 1) }
+      else if (update) { NLFH_A(prec_a_staged, 2) }
+      else { NLFH_A(prec_a_staged, 0) }
       if (dot == 2) { NLFH_B(true) }
       else { NLFH_B(false) }
       if (dot == 1) {
@@ -787,6 +804,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       return hipGetLastError();
     }
   }
+  if (update == 1 || update == 3) return hipErrorInvalidValue;  // the pair schedule needs the staged pass
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }

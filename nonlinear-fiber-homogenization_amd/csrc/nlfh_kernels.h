@@ -59,6 +59,8 @@ struct PrecArgs {
   T* x;                  // pass A fused update: x += alpha p
   const T* p;            // pass A fused update
   const double* alpha;   // per (cell, rhs)
+  const double* alpha_prev;  // pass A pair update: alpha of the previous iteration
+  const T* p_prev;           // pass A pair update: direction of the previous iteration
   cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
   T* z;                  // pass C output
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
@@ -84,6 +86,7 @@ struct VecArgs {
 struct CgState {
   double* rz;     // [cell*m + rhs]
   double* alpha;
+  double* alpha_prev;  // nullable: receives the previous alpha before cg_alpha overwrites it
   double* beta;
   double* bn;
   double* rn;
@@ -136,6 +139,9 @@ struct SmallParams {
   double d[36];
   int i[4];
 };
+template <typename T>
+hipError_t launch_x_flush_odd(const VecArgs& va, T* x, const T* p0, const T* p1, const double* alpha, const int* it,
+                              hipStream_t st);
 hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st);
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);
