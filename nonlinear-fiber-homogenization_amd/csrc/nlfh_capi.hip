@@ -488,6 +488,10 @@ const cplx<float>* tw_ptr<float>(nlfh_plan* p) { return p->d_tw32; }
 template <typename T>
 int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const T* s, const double* beta,
                double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
+  if (!strain && scale != T(1)) {
+    set_error("internal: the matvec scales its output only together with the strain term");
+    return NLFH_EINVAL;
+  }
   for (const MvGroup& g : p->groups) {
     const Shape& sh = p->shapes[g.shape];
     MatvecArgs<T> a{};

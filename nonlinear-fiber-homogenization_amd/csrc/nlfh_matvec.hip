@@ -238,6 +238,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     if (s >= 1 && owns) {
       const T* UYp = lds + (2 + PQ) * buf_sz + lbase - rowstep;
       const unsigned oo = plane_off(x0 - 2 + s) + roff;
+      T dsum = T(0);  // float32: the step's p . Ap terms summed in float, added to the double total once
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -246,12 +247,17 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           const T B = Flo[c][1][q] + UYp[LOFF(2 * c + 1, q)];
           const T lo = A - B, hi = A + B;
           if (s >= 2) {
-            const T v = (carry[c][q] + lo) * scale_out;
+            T v = carry[c][q] + lo;
+            if (STRAIN) v *= scale_out;  // only the load launches scale (by -1); the others pass 1
             Oc[c][oo + q] = v;
-            if (DOT) dot += (double)pc[PQ][c][q] * (double)v;
+            if (DOT) {
+              if constexpr (sizeof(T) == 8) dot += pc[PQ][c][q] * v;
+              else dsum += pc[PQ][c][q] * v;
+            }
           }
           carry[c][q] = hi;
         }
+      if (DOT && sizeof(T) == 4 && s >= 2) dot += (double)dsum;
     }
     if (s > xc) return true;
     // (b) element layer L_s

@@ -31,7 +31,11 @@ def test_batched_equals_single_cell_solves(gpu_pkg, precision):
         if precision == "fp64":
             assert its[i] == iti
         views = bh.field_views(u)
-        assert np.abs(views[i].cpu().numpy() - ui.cpu().numpy()).max() < 1e-9 * np.abs(ui.cpu().numpy()).max()
+        # mixed: a plan whose cells all take the register-FFT path runs the lean FP32 schedule (spectral r.z,
+        # x updated every second iteration), so the batch (with a 12^3 cell) and the lone 16^3 plan round
+        # differently in FP32; both converge to tol 1e-10 and C agrees to 1e-12 above
+        utol = 1e-9 if precision == "fp64" else 1e-8
+        assert np.abs(views[i].cpu().numpy() - ui.cpu().numpy()).max() < utol * np.abs(ui.cpu().numpy()).max()
     # and against the oracle for the largest cell
     Cr, _, _ = O.LinearRef(rhos[3].shape, emin=1e-3, tol=1e-10).C_full(rhos[3])
     assert np.abs(np.asarray(C[3]) - Cr).max() < 1e-9 * np.abs(Cr).max()
