@@ -80,3 +80,32 @@ def test_fiber_value_and_jacobian(gpu_pkg):
     assert fib.u is not None and fib.solves == 1
     c2 = fib.value(phi, beta)
     assert fib.lh.last_info["it"] <= 2 and np.abs(c2 - c).max() < 1e-10 * np.abs(c).max()
+
+
+def _schwarz_octant(n_o):
+    x = 2 * np.pi * (np.arange(n_o) + 0.5) / (2 * n_o)
+    G = np.meshgrid(x, x, x, indexing="ij")
+    return -sum(np.cos(g) for g in G)
+
+
+def test_fiber_invariant_gradient_reference_case(gpu_pkg):
+    """The reference's tests/test_core.py:100-108 on the GPU Fiber (default tolerance 1e-9): J against
+    central differences of cold solves, and the rank check of the normalised Jacobian."""
+    from nlfh_amd.fiber import Fiber, random_start, sigma_min_normalized
+    fib = Fiber(4, v=0.3, r=1.5)
+    phi = DR.sym_perm(random_start(4, seed=4, kmax=1) + _schwarz_octant(4))
+    c, J = fib.value_jac(phi, 4.0)
+    d = DR.sym_perm(np.random.default_rng(5).standard_normal((4, 4, 4)))
+    h = 1e-5
+    fd = (fib.value(phi + h * d, 4.0, warm=False) - fib.value(phi - h * d, 4.0, warm=False)) / (2 * h)
+    np.testing.assert_allclose(np.tensordot(J.cpu().numpy(), d, 3), fd, rtol=1e-5, atol=1e-10)
+    assert sigma_min_normalized(J) > 1e-3
+
+
+def test_identity_filter_radius(gpu_pkg):
+    """r = 1.0 (the reference's test_pipeline fibers): the hat kernel is the identity."""
+    from nlfh_amd.density import DensityMap
+    phi = _phi(4, 11)
+    dm = DensityMap(4, r=1.0, v=0.3)
+    np.testing.assert_allclose(dm.smooth(phi).cpu().numpy(), DR.smooth(phi, 1.0), rtol=0, atol=1e-12)
+    assert abs(dm.rho(phi, 4.0).mean().item() - 0.3) < 1e-12
