@@ -1,6 +1,7 @@
 // C-ABI of the library (include/nlfh.h): plans, launch tables, the preconditioner symbol, and the
 // PCG / iterative-refinement drivers. Host code; every device step is a kernel of this library.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -34,10 +35,13 @@ bool progress_on() {
 template <typename... A>
 void progress(const char* fmt, A... args) {
   if (!progress_on()) return;
-  static auto last = std::chrono::steady_clock::now();
-  const auto now = std::chrono::steady_clock::now();
-  if (std::chrono::duration<double>(now - last).count() < 20.0) return;
-  last = now;
+  // at most one line per 20 s over all threads (plans may solve concurrently on their own threads)
+  using clk = std::chrono::steady_clock;
+  static std::atomic<int64_t> last{clk::now().time_since_epoch().count()};
+  const int64_t now = clk::now().time_since_epoch().count();
+  int64_t prev = last.load();
+  const int64_t gap = std::chrono::duration_cast<clk::duration>(std::chrono::seconds(20)).count();
+  if (now - prev < gap || !last.compare_exchange_strong(prev, now)) return;
   std::fprintf(stderr, "[nlfh] ");
   std::fprintf(stderr, fmt, args...);
   std::fprintf(stderr, "\n");
@@ -490,6 +494,10 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
                double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
   if (!strain && scale != T(1)) {
     set_error("internal: the matvec scales its output only together with the strain term");
+    return NLFH_EINVAL;
+  }
+  if (!strain && u == nullptr) {
+    set_error("internal: a matvec without the strain term needs an input field");
     return NLFH_EINVAL;
   }
   for (const MvGroup& g : p->groups) {

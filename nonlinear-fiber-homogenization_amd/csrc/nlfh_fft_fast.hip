@@ -25,7 +25,8 @@ struct FastCfg {
   static constexpr int BT = (N == 128 && sizeof(T) == 8) ? 128 : 256;  // threads of passes A, C
   static constexpr int LPR = BT / L;                                   // lines per round
   static constexpr int XS = N1 * (N2 + 1) > N2 * (N1 + 1) ? N1 * (N2 + 1) : N2 * (N1 + 1);
-  static constexpr int LPB = N == 128 ? 8 : 16;  // pass B positions per block
+  // pass B positions per block: 256 threads (128 for FP64 at 128), one component's exchange buffer in LDS
+  static constexpr int LPB = N == 128 ? (sizeof(T) == 8 ? 8 : 16) : 256 / L;
   static constexpr int BTB = L * LPB;                                          // threads of pass B
 };
 
@@ -671,10 +672,10 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
   __shared__ double red[4];
   double dacc = 0.0;
   using C = FastCfg<T, N>;
-  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB, XS = C::XS;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB;
   extern __shared__ double smem_d[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
-  cplx<T>* xch = tw + N;  // [3][XS][LPB]
+  cplx<T>* xch = tw + N;  // [XS][LPB], one component at a time
   const LineItem li = a.lines[a.line_base + blockIdx.x];
   if (a.done != nullptr && a.done[li.cell]) return;
   const CellDesc cd = a.cells[li.cell];
@@ -686,28 +687,34 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
   constexpr int64_t npos = (int64_t)N * NH;
   const int64_t pos = li.pos0 + l;
   cplx<T>* sp = a.spec + cd.off_spec + (int64_t)rhs * 3 * cd.nspec;
-  __syncthreads();
-  // forward step 1
+  // the three components' columns are loaded before the first exchange (all loads in flight together)
+  cplx<T> v[3][N1];
   if (act && t < N2) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cplx<T> v[N1];
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j1 = 0; j1 < N1; ++j1) v[j1] = sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * j1) * npos + pos];
-      Dft<T, N1, -1>::run(v);
-#pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) xch[((c * XS) + k1 * (N2 + 1) + t) * LPB + l] = cmulw(v[k1], tw[t * k1], false);
-    }
+      for (int j1 = 0; j1 < N1; ++j1) v[c][j1] = sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * j1) * npos + pos];
   }
   __syncthreads();
   cplx<T> Xc[3][N2];
-  if (act && t < N1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < 3; ++c) {
+    // forward step 1: length N1 over x = t + N2 j1, twiddled into the exchange buffer
+    if (act && t < N2) {
+      Dft<T, N1, -1>::run(v[c]);
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) Xc[c][j2] = xch[((c * XS) + t * (N2 + 1) + j2) * LPB + l];
+      for (int k1 = 0; k1 < N1; ++k1) xch[(k1 * (N2 + 1) + t) * LPB + l] = cmulw(v[c][k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    // forward step 2: length N2, outputs kx = t + N1 k2
+    if (act && t < N1) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) Xc[c][j2] = xch[(t * (N2 + 1) + j2) * LPB + l];
       Dft<T, N2, -1>::run(Xc[c]);
     }
+    __syncthreads();
+  }
+  if (act && t < N1) {
     // z = Kinv r at kx = t + N1 k2 (symbol component-planar: [6][nx][ny][nzh], divided by N^3)
 #pragma unroll
     for (int k2 = 0; k2 < N2; ++k2) {
@@ -732,28 +739,25 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
     const double tsum = block_sum(dacc, red);
     if (tid == 0) a.part_bdot[(int64_t)rhs * a.nlines + a.line_base + blockIdx.x] = tsum;
   }
-  __syncthreads();
-  // inverse step 1: in-register length N2 at kx = t + N1 k2; twiddle e^{+2 pi i t k / N}
-  if (act && t < N1) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
+  for (int c = 0; c < 3; ++c) {
+    // inverse step 1: in-register length N2 at kx = t + N1 k2; twiddle e^{+2 pi i t k / N}
+    if (act && t < N1) {
       Dft<T, N2, 1>::run(Xc[c]);
 #pragma unroll
-      for (int k1 = 0; k1 < N2; ++k1) xch[((c * XS) + k1 * (N1 + 1) + t) * LPB + l] = cmulw(Xc[c][k1], tw[t * k1], true);
+      for (int k1 = 0; k1 < N2; ++k1) xch[(k1 * (N1 + 1) + t) * LPB + l] = cmulw(Xc[c][k1], tw[t * k1], true);
     }
-  }
-  __syncthreads();
-  // inverse step 2: thread t < N2 gathers over t' < N1, outputs x = t + N2 k
-  if (act && t < N2) {
+    __syncthreads();
+    // inverse step 2: thread t < N2 gathers over t' < N1, outputs x = t + N2 k
+    if (act && t < N2) {
+      cplx<T> w[N1];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      cplx<T> v[N1];
+      for (int j = 0; j < N1; ++j) w[j] = xch[(t * (N1 + 1) + j) * LPB + l];
+      Dft<T, N1, 1>::run(w);
 #pragma unroll
-      for (int j = 0; j < N1; ++j) v[j] = xch[((c * XS) + t * (N1 + 1) + j) * LPB + l];
-      Dft<T, N1, 1>::run(v);
-#pragma unroll
-      for (int k = 0; k < N1; ++k) sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * k) * npos + pos] = v[k];
+      for (int k = 0; k < N1; ++k) sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * k) * npos + pos] = w[k];
     }
+    if (c < 2) __syncthreads();
   }
 }
 
@@ -766,7 +770,7 @@ size_t fast_lds_plane() {
 template <typename T, int N>
 size_t fast_lds_line() {
   using C = FastCfg<T, N>;
-  return ((size_t)N + (size_t)3 * C::XS * C::LPB) * sizeof(cplx<T>);
+  return ((size_t)N + (size_t)C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
 // update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 3 (r and the pair update)
@@ -825,6 +829,9 @@ bool fast_fft_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n
 
 int fast_line_len(int n, int elem_bytes) {
   switch (n) {
+    case 16: return elem_bytes == 8 ? FastCfg<double, 16>::LPB : FastCfg<float, 16>::LPB;
+    case 32: return elem_bytes == 8 ? FastCfg<double, 32>::LPB : FastCfg<float, 32>::LPB;
+    case 48: return elem_bytes == 8 ? FastCfg<double, 48>::LPB : FastCfg<float, 48>::LPB;
     case 128: return elem_bytes == 8 ? FastCfg<double, 128>::LPB : FastCfg<float, 128>::LPB;
     default: return elem_bytes == 8 ? FastCfg<double, 64>::LPB : FastCfg<float, 64>::LPB;
   }

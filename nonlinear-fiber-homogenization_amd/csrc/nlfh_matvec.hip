@@ -66,17 +66,32 @@ __device__ __forceinline__ void modal_op(const T (&h)[3][8], T (&g)[3][8], const
 
 // z neighbours: a DPP wave rotation when a row is a whole wavefront (lane i <- i+1 with
 // wave_rol:1, i <- i-1 with wave_ror:1, checked on gfx950), ds_bpermute otherwise.
+// A wave rotation reads a valid source lane for every lane, so the instruction's `old` operand is
+// never used: mov_dpp leaves it undefined, which spares a register copy per rotation and lets the
+// compiler fold the rotation into the adds and subtracts that consume it.
 template <int CTRL>
 __device__ __forceinline__ double dpp_rot(double v) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffll), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
   return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 template <int CTRL>
 __device__ __forceinline__ float dpp_rot(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
+// Marks the registers of a prefetch as read here, on every path, with no instruction: the wait for the
+// load lands at this point (one step after the load was issued) and the registers are settled
+// afterwards, so the compiler need not wait early on the paths where the consumer is skipped and the
+// register is reused.
+template <typename T, int K>
+__device__ __forceinline__ void settle(const T (&v)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) asm volatile("" ::"v"(v[k]));
+}
+
+// s_waitcnt immediate (gfx9 encoding): vmcnt = 0, expcnt and lgkmcnt at their maxima (no wait)
+constexpr int kWaitVmcnt0 = 0x0F70;
 constexpr int kDppWaveRol1 = 0x134;  // lane i receives lane i + 1 (mod 64)
 constexpr int kDppWaveRor1 = 0x13C;  // lane i receives lane i - 1 (mod 64)
 
@@ -89,6 +104,19 @@ template <typename T, bool DPP>
 __device__ __forceinline__ T from_prev(T v, int src) {
   if constexpr (DPP) return dpp_rot<kDppWaveRor1>(v);
   else return __shfl(v, src, 64);
+}
+
+// Element i of one component field of one cell, addressed as a 32-bit byte offset from the field's
+// wave-uniform base, so that loads and stores use the scalar-base + 32-bit vector-offset form with no
+// 64-bit address arithmetic per access. The largest accepted cell (512 x 512 x 256 voxels) has
+// 512 MiB per FP64 component field, so the byte offset fits in 32 bits.
+template <typename T>
+__device__ __forceinline__ const T& at(const T* base, unsigned i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ T& at(T* base, unsigned i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (unsigned)sizeof(T));
 }
 
 // One block: y-tile rows [y0, y0 + ty) x all z, x-planes [x0, x0 + xc), one right-hand side.
@@ -168,22 +196,33 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     x = x >= nx ? x - nx : x;
     return (unsigned)(x * plane);
   };
-  const bool has_u = a.u != nullptr;
-  auto load_row = [&](unsigned po, T (&v)[3][ZPT]) {
-    const unsigned o = po + roff;
+  const unsigned roff_ld = lz < LZ ? roff : 0u;  // idle lanes of a padded row read the row start
+  const bool has_u = !STRAIN || a.u != nullptr;  // u may be null only in load launches (host-checked)
+  // loads only: the values are combined (u + beta p) where they are consumed, so that a plane
+  // prefetched one step ahead stays in flight across the step's barrier
+  auto load_row = [&](unsigned po, T (&vu)[3][ZPT], T (&vp)[3][ZPT]) {
+    const unsigned o = po + roff_ld;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) {
-        T val = has_u ? Uc[c][o + q] : T(0);
-        if (PUPDATE) val += beta * Pc[c][o + q];
-        v[c][q] = val;
+        vu[c][q] = has_u ? at(Uc[c], o + q) : T(0);
+        if (PUPDATE) vp[c][q] = at(Pc[c], o + q);
       }
     }
   };
+  auto combine = [&](const T (&vu)[3][ZPT], const T (&vp)[3][ZPT], T (&v)[3][ZPT]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) v[c][q] = PUPDATE ? vu[c][q] + beta * vp[c][q] : vu[c][q];
+  };
   const int x0 = it.x0, xc = it.xc;
 
-  T up[3][ZPT], nxt[3][ZPT], carry[3][ZPT];
+  T up[3][ZPT], carry[3][ZPT];
+  // one step ahead, by the parity of the step that loads them (fixed registers, no copies of values
+  // in flight): the next x-plane (u and p) and the stiffness of the next element layer
+  T nxt_u[2][3][ZPT], nxt_p[2][3][ZPT], sv[2][ZPT];
   T pc[2][3][ZPT];  // node values of the planes awaiting output (by layer parity)
   T Flo[3][2][ZPT];  // own half of the y-reverse of the last element layer
   double dot = 0.0;
@@ -215,25 +254,45 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 
   // prologue: layer L_0 from planes x0-1, x0; prefetch x0+1
   if (active) {
-    load_row(plane_off(x0 - 1), up);
-    T un[3][ZPT];
+    T un[3][ZPT], lu[3][ZPT], lp[3][ZPT];
+    load_row(plane_off(x0 - 1), lu, lp);
+    combine(lu, lp, up);
     const unsigned p0 = plane_off(x0);
-    load_row(p0, un);
-    if (xc >= 1) load_row(plane_off(x0 + 1), nxt);
+    load_row(p0, lu, lp);
+    combine(lu, lp, un);
+    if (xc >= 1) load_row(plane_off(x0 + 1), nxt_u[1], nxt_p[1]);
+    {
+      const unsigned so = plane_off(x0 - 1) + roff_ld;
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) sv[1][q] = at(S, so + q);
+    }
     if (PUPDATE && owns && xc >= 1) {
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int q = 0; q < ZPT; ++q) PNc[c][p0 + roff + q] = un[c][q];
+        for (int q = 0; q < ZPT; ++q) at(PNc[c], p0 + roff + q) = un[c][q];
     }
     x_stage(std::integral_constant<int, 0>{}, un);
   }
+  // Drain the prologue's loads here (they were issued just after the ones x_stage waited for): the
+  // step loop's entry then carries no outstanding load, and its steady-state waits cover only the
+  // plane prefetched one step earlier instead of the whole queue at the top of every step.
+  __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
   __syncthreads();
 
   // one step; returns true when the march is over (s > xc after part (a))
   auto step = [&](int s, auto ptag) -> bool {
     constexpr int PS = decltype(ptag)::value;  // s & 1
     constexpr int PQ = 1 - PS;                 // (s - 1) & 1 == (s + 1) & 1
+    // prefetch for the next step, issued on every path (the last steps repeat the chunk's last plane,
+    // a cache hit): x-plane x0 + s + 2 (consumed by part (c) of step s + 1) and the stiffness of
+    // element layer L_{s+1} (part (b) of step s + 1)
+    {
+      load_row(plane_off(x0 + (s + 2 < xc ? s + 2 : xc)), nxt_u[PS], nxt_p[PS]);
+      const unsigned so = plane_off(x0 + (s < xc - 1 ? s : xc - 1)) + roff_ld;
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) sv[PS][q] = at(S, so + q);
+    }
     // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
     if (s >= 1 && owns) {
       const T* UYp = lds + (2 + PQ) * buf_sz + lbase - rowstep;
@@ -249,7 +308,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           if (s >= 2) {
             T v = carry[c][q] + lo;
             if (STRAIN) v *= scale_out;  // only the load launches scale (by -1); the others pass 1
-            Oc[c][oo + q] = v;
+            at(Oc[c], oo + q) = v;
             if (DOT) {
               if constexpr (sizeof(T) == 8) dot += pc[PQ][c][q] * v;
               else dsum += pc[PQ][c][q] * v;
@@ -261,6 +320,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     }
     if (s > xc) return true;
     // (b) element layer L_s
+    settle(sv[PQ]);
     if (active && erow) {
       const T* SD = lds + PS * buf_sz + lbase;
       T* UY = lds + (2 + PS) * buf_sz + lbase;
@@ -281,7 +341,9 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       for (int c = 0; c < 3; ++c)
 #pragma unroll
         for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = from_next<T, DPP>(Y[c][mm][0], src_next);
-      const unsigned so = plane_off(x0 - 1 + s) + roff;
+      T sv_cur[ZPT];
+#pragma unroll
+      for (int q = 0; q < ZPT; ++q) sv_cur[q] = sv[PQ][q];
       T Fz[3][4][ZPT];
       T hi_last[3][4];
 #pragma unroll
@@ -295,7 +357,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
             h[c][mm] = yn + Y[c][mm][q];
             h[c][mm + 4] = yn - Y[c][mm][q];
           }
-        const T sv = S[so + q];
+        const T sv = sv_cur[q];
         modal_op<T>(h, g, sv * lam16, sv * mu16, STRAIN ? e4 : nullptr);
 #pragma unroll
         for (int c = 0; c < 3; ++c)
@@ -327,19 +389,20 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
         }
     }
     // (c) x stage of layer L_{s+1} from planes x0 + s and x0 + s + 1 (prefetched)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      settle(nxt_u[PQ][c]);
+      if (PUPDATE) settle(nxt_p[PQ][c]);
+    }
     if (s < xc && active) {
       T un[3][ZPT];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-#pragma unroll
-        for (int q = 0; q < ZPT; ++q) un[c][q] = nxt[c][q];
-      if (s + 1 < xc) load_row(plane_off(x0 + s + 2), nxt);
+      combine(nxt_u[PQ], nxt_p[PQ], un);
       if (PUPDATE && owns && s + 1 < xc) {
         const unsigned o = plane_off(x0 + s + 1) + roff;
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
-          for (int q = 0; q < ZPT; ++q) PNc[c][o + q] = un[c][q];
+          for (int q = 0; q < ZPT; ++q) at(PNc[c], o + q) = un[c][q];
       }
       x_stage(std::integral_constant<int, PQ>{}, un);
     }
