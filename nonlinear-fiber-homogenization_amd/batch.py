@@ -2,11 +2,18 @@
 iteration for all cells, per-cell convergence flags on the device (a converged cell's blocks exit at
 once), iterations captured in a HIP graph.
 
+The solve itself runs the cells in `streams` groups (default 2), each a plan of its own on its own
+stream and host thread. The matvec is bound by instruction issue and the FFT passes by HBM, so two
+groups whose iterations drift out of phase share the GPU better than one launch sequence (8 cells of
+64^3: 337 ms against 363 ms per solve on MI355X). Every cell's PCG is independent, so the grouping
+does not change any cell's iterates.
+
 The reference homogenizes one cell per `LinearHomogenization` (nlfiber/linear.py:99-236) and
 shards tasks over processes (nlfiber/runner.py:55-75); this class is the batched form of
 `C_full` for the same algorithm, with identical per-cell results.
 """
 import ctypes
+import threading
 
 import numpy as np
 import torch
@@ -20,7 +27,7 @@ class BatchedHomogenization:
     (the unit strains 0..m-1). precision: "fp64" or "mixed"."""
 
     def __init__(self, shapes, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-8, maxit=4000, precision="mixed", m=6,
-                 tune=False, warm_start=None):
+                 tune=False, warm_start=None, streams=2):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.E, self.nu, self.emin, self.p, self.tol, self.maxit = E, nu, emin, p, tol, maxit
         if precision not in ("fp64", "mixed"):
@@ -38,8 +45,39 @@ class BatchedHomogenization:
             from .learned import WarmStartNet
             warm_start = WarmStartNet.load(warm_start)
         self.warm_start = warm_start
+        self.streams = max(1, int(streams))
+        self._groups = self._partition(min(self.streams, len(self.shapes)))
+        self._subs = None  # (plan, stream) per group, made at the first solve
         if tune:
             self.autotune()
+
+    def _partition(self, G):
+        """Cells in G groups of about equal voxel count (largest first onto the lightest group), each
+        group in cell order; None for a single group."""
+        if G <= 1:
+            return None
+        load, groups = [0] * G, [[] for _ in range(G)]
+        for i in sorted(range(len(self.shapes)), key=lambda i: -self.nvox[i]):
+            g = int(np.argmin(load))
+            groups[g].append(i)
+            load[g] += self.nvox[i]
+        return [sorted(g) for g in groups]
+
+    def _sub_plans(self):
+        """One plan and stream per group, tiled like this batch's plan for the same shapes."""
+        if self._subs is None:
+            self._subs = []
+            for g in self._groups:
+                shapes = [self.shapes[i] for i in g]
+                off = np.concatenate([[0], np.cumsum([self.nvox[i] for i in g])]).astype(np.int64)
+                self._subs.append((Plan(shapes, self.E, self.nu, m_layout=self.m), torch.cuda.Stream(), g, off))
+        distinct = list(dict.fromkeys(self.shapes))
+        for plan, _, g, _ in self._subs:
+            for j, sh in enumerate(dict.fromkeys(self.shapes[i] for i in g)):
+                want = self._plan.get_tiling(distinct.index(sh))
+                if plan.get_tiling(j) != want:
+                    plan.set_tiling(j, *want)
+        return self._subs
 
     def autotune(self, precision=None):
         """Tile every distinct shape with the autotuner (each on a single-cell plan) and apply it."""
@@ -103,16 +141,57 @@ class BatchedHomogenization:
                                              _stream()))
         return P
 
-    def _solve(self, s, Es_p, u0):
-        nc = len(self.shapes)
-        u = torch.empty(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
+    def _solve_plan(self, plan, s, Es_p, u0, u, nc, stream):
         it = (ctypes.c_int * nc)()
         cv = (ctypes.c_int * nc)()
         rr = (ctypes.c_double * (nc * self.m))()
         prec = 0 if self.precision == "fp64" else 1
-        _lib.check(self._lib.nlfh_solve(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(u0), _ptr(u), float(self.tol),
-                                        int(self.maxit), prec, it, cv, rr, _stream()))
-        info = dict(it=list(it), converged=[bool(c) for c in cv], relres=np.asarray(list(rr)).reshape(nc, self.m))
+        _lib.check(self._lib.nlfh_solve(plan.handle, _ptr(s), Es_p, self.m, _ptr(u0), _ptr(u), float(self.tol),
+                                        int(self.maxit), prec, it, cv, rr, stream))
+        return dict(it=list(it), converged=[bool(c) for c in cv], relres=np.asarray(list(rr)).reshape(nc, self.m))
+
+    def _solve(self, s, Es_p, u0):
+        nc = len(self.shapes)
+        u = torch.empty(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
+        if self._groups is None:
+            return u, self._solve_plan(self._plan, s, Es_p, u0, u, nc, _stream())
+        # inputs and outputs of every group are allocated on the caller's stream, which each group's
+        # stream waits for before and which waits for every group after
+        f = 3 * self.m
+        cur = torch.cuda.current_stream()
+        jobs = []
+        for plan, st, g, off in self._sub_plans():
+            s_g = torch.cat([s[self.off[i]:self.off[i + 1]] for i in g])
+            u0_g = None if u0 is None else torch.cat([u0[self.off[i] * f:self.off[i + 1] * f] for i in g])
+            u_g = torch.empty(int(off[-1]) * f, dtype=torch.float64, device=self._dev)
+            st.wait_stream(cur)
+            jobs.append([plan, st, g, off, s_g, u0_g, u_g, None, None])
+
+        def work(job):
+            plan, st, g, _, s_g, u0_g, u_g = job[:7]
+            try:
+                job[7] = self._solve_plan(plan, s_g, Es_p, u0_g, u_g, len(g), ctypes.c_void_p(st.cuda_stream))
+            except BaseException as e:  # re-raised on the caller's thread
+                job[8] = e
+
+        threads = [threading.Thread(target=work, args=(j,)) for j in jobs[1:]]
+        for t in threads:
+            t.start()
+        work(jobs[0])
+        for t in threads:
+            t.join()
+        for j in jobs:
+            cur.wait_stream(j[1])
+        for j in jobs:
+            if j[8] is not None:
+                raise j[8]
+        info = dict(it=[0] * nc, converged=[False] * nc, relres=np.zeros((nc, self.m)))
+        for plan, st, g, off, s_g, u0_g, u_g, inf, _ in jobs:
+            for k, i in enumerate(g):
+                u[self.off[i] * f:self.off[i + 1] * f] = u_g[off[k] * f:off[k + 1] * f]
+                info["it"][i] = inf["it"][k]
+                info["converged"][i] = inf["converged"][k]
+                info["relres"][i] = inf["relres"][k]
         return u, info
 
     def C_full(self, rhos, u0=None):

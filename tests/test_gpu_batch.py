@@ -79,3 +79,24 @@ def test_mixed_unconverged_cells_fall_back_to_fp64(gpu_pkg):
     lh = gpu_pkg.LinearHomogenization((16, 16, 16), emin=1e-3, tol=1e-8, maxit=maxit, precision="mixed")
     lh.C_full(torch.tensor(rho_big))
     assert lh.last_info["fallback_fp64"] and not lh.last_info["converged"]
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_stream_groups_do_not_change_results(gpu_pkg, precision):
+    """The solve split over 2 or 3 groups (a plan, a stream and a host thread each) gives every cell the
+    iterates of the single-plan solve: C, correctors and iteration counts agree exactly (every cell is of
+    a register-FFT size, so all groups run the same FP32 schedule)."""
+    rhos = [gpu_pkg.cells.random_cell(n, seed)[0] for n, seed in ((16, 11), (32, 12), (16, 13), (16, 14), (32, 15))]
+    res = {}
+    for streams in (1, 2, 3):
+        bh = gpu_pkg.BatchedHomogenization([r.shape for r in rhos], emin=1e-4, tol=1e-8, precision=precision,
+                                           streams=streams)
+        C, u, its = bh.C_full(rhos)
+        assert all(bh.last_info["converged"])
+        res[streams] = (np.asarray(C), u.cpu().numpy(), np.asarray(its), bh._groups)
+    assert res[1][3] is None and len(res[2][3]) == 2 and len(res[3][3]) == 3
+    for streams in (2, 3):
+        assert sorted(i for g in res[streams][3] for i in g) == list(range(len(rhos)))
+        np.testing.assert_array_equal(res[streams][2], res[1][2])
+        np.testing.assert_array_equal(res[streams][0], res[1][0])
+        np.testing.assert_array_equal(res[streams][1], res[1][1])
