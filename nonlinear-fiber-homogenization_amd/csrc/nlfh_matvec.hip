@@ -95,6 +95,25 @@ constexpr int kWaitVmcnt0 = 0x0F70;
 constexpr int kDppWaveRol1 = 0x134;  // lane i receives lane i + 1 (mod 64)
 constexpr int kDppWaveRor1 = 0x13C;  // lane i receives lane i - 1 (mod 64)
 
+// z butterflies of the last z value of a lane with the first of the next lane, one component, FP32, full
+// wavefront rows: h[mm] = next(a[mm]) + b[mm] (mm = 1..3; the sum of mode 0 is never used by the element
+// operator) and h[mm + 4] = next(a[mm]) - b[mm] (mm = 0..3), with the rotation (lane i <- lane i + 1)
+// applied to the first operand of each add/sub instead of a separate move. The leading s_nop covers the
+// two wait states that a DPP read of a register written by the previous VALU instruction needs.
+__device__ __forceinline__ void z_next_butterflies(const float (&a)[4], const float (&b)[4], float (&h)[8]) {
+  asm("s_nop 1\n\t"
+      "v_sub_f32_dpp %3, %7, %11 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_add_f32_dpp %0, %8, %12 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_sub_f32_dpp %4, %8, %12 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_add_f32_dpp %1, %9, %13 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_sub_f32_dpp %5, %9, %13 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_add_f32_dpp %2, %10, %14 wave_rol:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_sub_f32_dpp %6, %10, %14 wave_rol:1 row_mask:0xf bank_mask:0xf"
+      : "=&v"(h[1]), "=&v"(h[2]), "=&v"(h[3]), "=&v"(h[4]), "=&v"(h[5]), "=&v"(h[6]), "=&v"(h[7])
+      : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(b[0]), "v"(b[1]), "v"(b[2]), "v"(b[3]));
+  h[0] = 0.0f;  // unused
+}
+
 template <typename T, bool DPP>
 __device__ __forceinline__ T from_next(T v, int src) {
   if constexpr (DPP) return dpp_rot<kDppWaveRol1>(v);
@@ -336,11 +355,13 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           Y[c][1][q] = d1 + d0;
           Y[c][3][q] = d1 - d0;
         }
+      // FP32 full-wavefront rows fold the rotation into the butterflies (z_next_butterflies)
+      constexpr bool kFoldNext = DPP && std::is_same<T, float>::value;
       T Yn[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
 #pragma unroll
-        for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = from_next<T, DPP>(Y[c][mm][0], src_next);
+        for (int mm = 0; mm < 4; ++mm) Yn[c][mm] = kFoldNext ? T(0) : from_next<T, DPP>(Y[c][mm][0], src_next);
       T sv_cur[ZPT];
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) sv_cur[q] = sv[PQ][q];
@@ -349,10 +370,21 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) {
         T h[3][8], g[3][8];
+        if constexpr (kFoldNext) {
+          if (q + 1 == ZPT) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const float a[4] = {Y[c][0][0], Y[c][1][0], Y[c][2][0], Y[c][3][0]};
+              const float b[4] = {Y[c][0][q], Y[c][1][q], Y[c][2][q], Y[c][3][q]};
+              z_next_butterflies(a, b, h[c]);
+            }
+          }
+        }
 #pragma unroll
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int mm = 0; mm < 4; ++mm) {
+            if (kFoldNext && q + 1 == ZPT) continue;
             const T yn = (q + 1 < ZPT) ? Y[c][mm][(q + 1) % ZPT] : Yn[c][mm];
             h[c][mm] = yn + Y[c][mm][q];
             h[c][mm + 4] = yn - Y[c][mm][q];
@@ -363,8 +395,9 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
         for (int c = 0; c < 3; ++c)
 #pragma unroll
           for (int mm = 0; mm < 4; ++mm) {
-            const T lo = g[c][mm] - g[c][mm + 4];
-            const T hi = g[c][mm] + g[c][mm + 4];
+            // mode 0 of the element forces is zero (a constant field has no energy)
+            const T lo = mm == 0 ? -g[c][4] : g[c][mm] - g[c][mm + 4];
+            const T hi = mm == 0 ? g[c][4] : g[c][mm] + g[c][mm + 4];
             if (q == 0) Fz[c][mm][0] = lo;
             else Fz[c][mm][q] += lo;
             if (q + 1 < ZPT) Fz[c][mm][(q + 1) % ZPT] = hi;
