@@ -189,7 +189,7 @@ struct nlfh_plan {
   int* d_line_range32 = nullptr;  // per cell [begin, end) of its float32 line items
   double* d_part_bdot = nullptr;  // pass B spectral r . z partials (float32 lines)
   bool lean32 = false;            // every shape group on the register-FFT path: lean FP32 iteration
-  bool pair32 = false;            // ... and on the staged pass A: x is updated every second iteration
+  bool mvx32 = false;             // ... and on the staged pass A: the matvec updates x, pass A only r
   int* d_chunk_range = nullptr;
 
   // workspaces
@@ -447,7 +447,6 @@ int ensure_work(nlfh_plan* p, Work<T>& w) {
   w.cs.beta = w.st + 2 * nk;
   w.cs.bn = w.st + 3 * nk;
   w.cs.rn = w.st + 4 * nk;
-  w.cs.alpha_prev = nullptr;
   w.cs.it = w.ist;
   w.cs.done = w.ist + p->ncells;
   w.cs.skip = nullptr;
@@ -469,7 +468,6 @@ int get_ready32(nlfh_plan* p) {
     TRY(ensure_work(p, p->w32));
     p->w32.cs.tol = p->d_params + p->ncells;
     p->w32.cs.maxit = p->d_iparams + 1;
-    if (p->pair32) p->w32.cs.alpha_prev = p->w32.st + 5 * p->ncells * p->M;
     p->w32_ready = true;
   }
   return NLFH_OK;
@@ -491,7 +489,8 @@ const cplx<float>* tw_ptr<float>(nlfh_plan* p) { return p->d_tw32; }
 // K u (+ load) for every shape group. u may be null (load only).
 template <typename T>
 int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const T* s, const double* beta,
-               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
+               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st,
+               T* x = nullptr, const double* alpha = nullptr) {
   if (!strain && scale != T(1)) {
     set_error("internal: the matvec scales its output only together with the strain term");
     return NLFH_EINVAL;
@@ -513,6 +512,8 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.strain = p->d_strain;
     a.beta = beta;
     a.dot_part = dot_part;
+    a.x = x;
+    a.alpha = alpha;
     a.done = done;
     a.m = m;
     a.nitems = (int)p->mv.size();
@@ -524,7 +525,7 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.lam16 = (T)(p->lam / 16.0);
     a.mu16 = (T)(p->mu / 16.0);
     a.scale_out = scale;
-    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, st));
+    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, x != nullptr ? 1 : 0, st));
   }
   return NLFH_OK;
 }
@@ -572,7 +573,7 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
-      if (dot == 2 || update == 1 || update == 3) return NLFH_EUNSUPPORTED;
+      if (dot == 2 || update == 1) return NLFH_EUNSUPPORTED;
       HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update != 0 ? 1 : 0, dot, lp, ll, st));
     }
   }
@@ -593,10 +594,11 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // search direction.
 // Lean iteration (FP32 inner solves when every shape group is on the register-FFT path): r . z comes
 // from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged pass A
-// (pair32) x is updated every second iteration, x += alpha_{k-1} p_{k-1} + alpha_k p_k at odd k (both
-// directions are still in the ping-pong buffers then), and launch_x_flush_odd adds the step a cell
-// owes when it stops after an even iteration. Same PCG recurrences; the dot differs from the direct sum
-// only by rounding.
+// (mvx32) the matvec of iteration k adds the previous step to the iterate, x += alpha_{k-1} p_{k-1},
+// from the p_{k-1} it reads for the direction update (the matvec is bound by instruction issue, pass A
+// by HBM, so the x traffic moves to the kernel with bandwidth to spare); pass A updates r only, and
+// launch_x_flush_last adds the last step alpha_K p_K once the cell is done. Same PCG recurrences; the
+// dot differs from the direct sum only by rounding.
 template <typename T>
 bool lean(const nlfh_plan* p) {
   return sizeof(T) == 4 && p->lean32;
@@ -607,7 +609,9 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   T* p_old = w.P[cur];
   T* p_new = w.P[1 - cur];
   const bool ln = lean<T>(p);
-  TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
+  const bool mvx = ln && p->mvx32;
+  TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st,
+                    mvx ? w.X : nullptr, mvx ? w.cs.alpha : nullptr));
   HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
   PrecArgs<T> a = prec_args<T>(p, m);
   a.r = w.R;
@@ -621,12 +625,7 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   a.part_a = p->d_part_a;
   a.part_c = p->d_part_c;
   a.done = w.cs.done;
-  int upd = 2;
-  if (ln && p->pair32) {
-    upd = cur == 1 ? 3 : 1;  // cur == k & 1 (every graph holds an even number of iterations)
-    a.alpha_prev = w.cs.alpha_prev;
-    a.p_prev = p_old;
-  }
+  const int upd = mvx ? 1 : 2;
   TRY(run_prec<T>(p, a, upd, ln ? 2 : 1, st));
   if (ln) {
     HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
@@ -724,8 +723,8 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
     }
   }
   if constexpr (sizeof(T) == 4) {
-    if (ln && p->pair32)
-      HIP_TRY(launch_x_flush_odd<float>(vec_args(p, m, nullptr), w.X, w.P[0], w.P[1], w.cs.alpha, w.cs.it, st));
+    if (ln && p->mvx32)
+      HIP_TRY(launch_x_flush_last<float>(vec_args(p, m, nullptr), w.X, w.P[0], w.P[1], w.cs.alpha, w.cs.it, st));
   }
   HIP_TRY(hipMemcpyAsync(it_host, w.cs.it, nc * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1015,10 +1014,10 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
   if ((rc = p->upload(&p->d_line_range32, lrange32))) return fail(rc);
   p->lean32 = true;
-  p->pair32 = true;
+  p->mvx32 = true;
   for (const PrecGroup& g : p->pgroups) {
     p->lean32 = p->lean32 && g.fast != 0;
-    p->pair32 = p->pair32 && g.fast != 0 && g.fast <= 64 && g.vec;
+    p->mvx32 = p->mvx32 && g.fast != 0 && g.fast <= 64 && g.vec;
   }
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);

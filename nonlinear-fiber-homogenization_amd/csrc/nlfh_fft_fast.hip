@@ -326,8 +326,7 @@ struct StageCfg {
                                (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
 };
 
-// UPD: 0 none; 1 r -= alpha Ap; 2 also x += alpha p; 3 also x += alpha_prev p_prev + alpha p (the
-// pair update of odd iterations, see the FP32 iteration in nlfh_capi.hip)
+// UPD: 0 none; 1 r -= alpha Ap (the FP32 iteration whose matvec updates x); 2 also x += alpha p
 template <typename T, int N, int UPD>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   constexpr bool UPDATE = UPD >= 1;
@@ -353,13 +352,11 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += BT) tw[i] = twg[i];
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
-  T alpha = T(0), alpha_p = T(0);
+  T alpha = T(0);
   if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
-  if (UPD == 3) alpha_p = (T)a.alpha_prev[pi.cell * a.m + rhs];
   double acc = 0.0;
   // load (and update) the plane with 16-byte vectors into rp
   {
-    const V* PPv = reinterpret_cast<const V*>(UPD == 3 ? a.p_prev + fo : nullptr);
     const V* Rv = reinterpret_cast<const V*>(a.r + fo);
     V* ROv = reinterpret_cast<V*>(a.r_out + fo);
     const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
@@ -379,14 +376,6 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
         if (UPD >= 2) {
           const V pv = Pv[i];
           V xv = Xv[i];
-          if (UPD == 3) {
-            const V qv = PPv[i];
-            if constexpr (VN == 4) {
-              xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y; xv.z += alpha_p * qv.z; xv.w += alpha_p * qv.w;
-            } else {
-              xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y;
-            }
-          }
           if constexpr (VN == 4) {
             xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
           } else {
@@ -773,8 +762,8 @@ size_t fast_lds_line() {
   return ((size_t)N + (size_t)C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
-// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 3 (r and the pair update)
-// on the staged pass only; dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral)
+// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only, the matvec updates x) on the
+// staged pass only; dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
                                 int update, int dot, int vec, hipStream_t st) {
@@ -792,8 +781,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   hipLaunchKernelGGL((prec_b_fast<T, N, D>), gl, dim3(C::BTB), ll, st, a, tw, symp);
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update == 3) { NLFH_A(prec_a_staged, 3) }
-      else if (update == 1) { NLFH_A(prec_a_staged, 1) }
+      if (update == 1) { NLFH_A(prec_a_staged, 1) }
       else if (update) { NLFH_A(prec_a_staged, 2) }
       else { NLFH_A(prec_a_staged, 0) }
       if (dot == 2) { NLFH_B(true) }
@@ -808,7 +796,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       return hipGetLastError();
     }
   }
-  if (update == 1 || update == 3) return hipErrorInvalidValue;  // the pair schedule needs the staged pass
+  if (update == 1) return hipErrorInvalidValue;  // the r-only update belongs to the staged pass
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }

@@ -18,6 +18,8 @@ struct MatvecArgs {
   const double* strain;  // STRAIN: macroscopic strains (m, 6) shared by all cells
   const double* beta;    // PUPDATE: beta per (cell, rhs)
   double* dot_part;      // DOT: partial sums of p . out per (rhs, item_base + item)
+  T* x;                  // XUPD: iterate updated x += alpha p_old on owned nodes
+  const double* alpha;   // XUPD: alpha of the previous iteration per (cell, rhs)
   const int* done;       // optional per-cell early-exit flags
   int m;
   int nitems;     // total matvec items of the plan (row stride of dot_part)
@@ -59,8 +61,6 @@ struct PrecArgs {
   T* x;                  // pass A fused update: x += alpha p
   const T* p;            // pass A fused update
   const double* alpha;   // per (cell, rhs)
-  const double* alpha_prev;  // pass A pair update: alpha of the previous iteration
-  const T* p_prev;           // pass A pair update: direction of the previous iteration
   cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
   T* z;                  // pass C output
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
@@ -86,7 +86,6 @@ struct VecArgs {
 struct CgState {
   double* rz;     // [cell*m + rhs]
   double* alpha;
-  double* alpha_prev;  // nullable: receives the previous alpha before cg_alpha overwrites it
   double* beta;
   double* bn;
   double* rn;
@@ -115,7 +114,7 @@ struct EnergyArgs {
 
 template <typename T>
 hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
-                         hipStream_t st);
+                         int xupd, hipStream_t st);
 template <typename T>
 size_t prec_plane_lds(int ny, int nz, int rows_per_batch);
 template <typename T>
@@ -140,8 +139,8 @@ struct SmallParams {
   int i[4];
 };
 template <typename T>
-hipError_t launch_x_flush_odd(const VecArgs& va, T* x, const T* p0, const T* p1, const double* alpha, const int* it,
-                              hipStream_t st);
+hipError_t launch_x_flush_last(const VecArgs& va, T* x, const T* p0, const T* p1, const double* alpha, const int* it,
+                               hipStream_t st);
 hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st);
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);

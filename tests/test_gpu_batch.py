@@ -32,7 +32,7 @@ def test_batched_equals_single_cell_solves(gpu_pkg, precision):
             assert its[i] == iti
         views = bh.field_views(u)
         # mixed: a plan whose cells all take the register-FFT path runs the lean FP32 schedule (spectral r.z,
-        # x updated every second iteration), so the batch (with a 12^3 cell) and the lone 16^3 plan round
+        # x updated by the matvec), so the batch (with a 12^3 cell) and the lone 16^3 plan round
         # differently in FP32; both converge to tol 1e-10 and C agrees to 1e-12 above
         utol = 1e-9 if precision == "fp64" else 1e-8
         assert np.abs(views[i].cpu().numpy() - ui.cpu().numpy()).max() < utol * np.abs(ui.cpu().numpy()).max()
