@@ -7,6 +7,10 @@ solver launches every iteration) and the fastest is kept in the plan. Results ar
 (device name, shape, precision, m) in a JSON file (NLFH_TUNE_CACHE, default
 ~/.cache/nlfh/tiling.json) so later plans skip the sweep. On gfx950 a table measured with
 tune_table.py and shipped in data/tiling_gfx950.json is consulted before sweeping.
+
+A launch over many cells of one shape fills the GPU with larger tiles than a single cell can (fewer
+halo rows and planes recomputed, enough blocks either way), so batched plans are tuned on a plan of
+`batch_cells(shape)` cells and keyed separately ("|batch").
 """
 import ctypes
 import json
@@ -47,7 +51,17 @@ def candidates(shape, elem_bytes=8):
 _TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "tiling_gfx950.json")
 
 
-def _packaged(shape, precision, m, dev):
+def batch_cells(shape):
+    """Cells of `shape` in the representative batched launch used for tuning: about 2^21 voxels."""
+    nvox = int(np.prod(shape))
+    return max(1, min(64, (1 << 21) // nvox))
+
+
+def _key(shape, precision, m, batched):
+    return f"{tuple(shape)}|{precision}|{m}" + ("|batch" if batched else "")
+
+
+def _packaged(shape, precision, m, dev, batched=False):
     """Tiling measured on MI355X (gfx950) and shipped with the package (tune_table.py writes it)."""
     try:
         arch = torch.cuda.get_device_properties(dev).gcnArchName
@@ -60,7 +74,7 @@ def _packaged(shape, precision, m, dev):
             table = json.load(f)
     except (OSError, ValueError):
         return None
-    e = table.get(f"{tuple(shape)}|{precision}|{m}")
+    e = table.get(_key(shape, precision, m, batched)) or table.get(_key(shape, precision, m, False))
     return tuple(e["best"]) if e else None
 
 
@@ -104,39 +118,48 @@ def time_matvec(plan, s, u, out, m, dtype, reps):
     return e0.elapsed_time(e1) / reps * 1e3  # microseconds
 
 
-def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, verbose=False):
+def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, verbose=False, batched=False):
     """Pick the fastest tiling for distinct shape `shape_index` of `plan` (a linear.Plan) and set
-    it. Returns (best (ty, xc, zpt), [(tiling, microseconds), ...])."""
+    it. With `batched`, candidates are timed on a plan of batch_cells(shape) cells of that shape (the
+    caller's plan is only tiled). Returns (best (ty, xc, zpt), [(tiling, microseconds), ...])."""
     # the distinct shapes in order of first appearance
     distinct = []
     for s in plan.shapes:
         if s not in distinct:
             distinct.append(s)
     shape = distinct[shape_index]
+    batched = batched and batch_cells(shape) > 1
     dev = torch.device("cuda", torch.cuda.current_device())
-    key = f"{torch.cuda.get_device_name(dev)}|{shape}|{precision}|{m}"
+    key = f"{torch.cuda.get_device_name(dev)}|" + _key(shape, precision, m, batched)
     cache = _load_cache() if use_cache else {}
     if use_cache and key in cache:
         best = tuple(cache[key]["best"])
         plan.set_tiling(shape_index, *best)
         return best, []
     if use_cache:
-        best = _packaged(shape, precision, m, dev)
+        best = _packaged(shape, precision, m, dev, batched)
         if best is not None:
             plan.set_tiling(shape_index, *best)
             return best, []
     dtype = 0 if precision == "f64" else 1
     tdt = torch.float64 if precision == "f64" else torch.float32
-    # a single-cell plan owns the voxel range of this shape at offset 0
-    if len(plan.shapes) != 1:
-        raise ValueError("tune() times single-cell plans; tune each shape with its own plan")
-    s = torch.rand(shape, dtype=torch.float64, device=dev) * 0.99 + 0.01
-    u = torch.randn((m, 3) + tuple(shape), dtype=tdt, device=dev)
+    if batched:
+        from .linear import Plan
+        nb = batch_cells(shape)
+        tplan, ti = Plan([shape] * nb, 1.0, 0.3, m_layout=m), 0
+    else:
+        # a single-cell plan owns the voxel range of this shape at offset 0
+        if len(plan.shapes) != 1:
+            raise ValueError("tune() times single-cell plans; tune each shape with its own plan")
+        nb, tplan, ti = 1, plan, shape_index
+    nvox = int(np.prod(shape))
+    s = torch.rand(nb * nvox, dtype=torch.float64, device=dev) * 0.99 + 0.01
+    u = torch.randn(nb * m * 3 * nvox, dtype=tdt, device=dev)
     out = torch.empty_like(u)
     table = []
     for cand in candidates(shape, 8):  # the plan's tiling must fit the float64 kernels too
-        plan.set_tiling(shape_index, *cand)
-        t = time_matvec(plan, s, u, out, m, dtype, reps)
+        tplan.set_tiling(ti, *cand)
+        t = time_matvec(tplan, s, u, out, m, dtype, reps)
         table.append((cand, t))
         if verbose:
             print(f"  tiling ty={cand[0]:3d} xc={cand[1]:3d} zpt={cand[2]}: {t:9.2f} us", flush=True)

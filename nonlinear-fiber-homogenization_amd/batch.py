@@ -80,16 +80,19 @@ class BatchedHomogenization:
         return self._subs
 
     def autotune(self, precision=None):
-        """Tile every distinct shape with the autotuner (each on a single-cell plan) and apply it."""
+        """Tile every distinct shape with the autotuner and apply it: shapes that several cells of one
+        stream group share take the batched tiling, the others the single-cell one."""
         from . import autotune as at
         distinct = []
         for s in self.shapes:
             if s not in distinct:
                 distinct.append(s)
         prec = precision or ("f64" if self.precision == "fp64" else "f32")
+        groups = self._groups or [list(range(len(self.shapes)))]
         for i, s in enumerate(distinct):
+            per_group = max(sum(1 for c in g if self.shapes[c] == s) for g in groups)
             tmp = Plan([s], self.E, self.nu)
-            best, _ = at.tune(tmp, 0, m=self.m, precision=prec)
+            best, _ = at.tune(tmp, 0, m=self.m, precision=prec, batched=per_group >= 2)
             self._plan.set_tiling(i, *best)
 
     def pack(self, rhos):

@@ -1,8 +1,10 @@
 """Measure the matvec tiling table shipped with the package (data/tiling_gfx950.json): for every
 shape and precision, the autotuner's sweep (no cache) repeated `--repeat` times; the tiling with the
-lowest median time wins.
+lowest median time wins. `--batched` measures the entries of batched plans (a plan of
+autotune.batch_cells(shape) cells per sweep). Entries are merged into the existing table.
 
     python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 32 48 64 128
+    python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 32 48 64 --precisions f32 --batched
 """
 import argparse
 import json
@@ -18,6 +20,7 @@ def main():
     ap.add_argument("--precisions", nargs="+", default=["f32", "f64"])
     ap.add_argument("--m", type=int, default=6)
     ap.add_argument("--repeat", type=int, default=3)
+    ap.add_argument("--batched", action="store_true")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                    "tiling_gfx950.json"))
     args = ap.parse_args()
@@ -26,21 +29,27 @@ def main():
     pkg = ge.load_package()
     from nlfh_amd import autotune as at
     from nlfh_amd.linear import Plan
-    table = {}
+    try:
+        with open(args.out) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        table = {}
     for n in args.shapes:
         shape = (n, n, n)
         for prec in args.precisions:
             runs = {}
             for _ in range(args.repeat):
                 plan = Plan([shape], 1.0, 0.3)
-                _, tab = at.tune(plan, 0, m=args.m, precision=prec, reps=10, use_cache=False)
+                _, tab = at.tune(plan, 0, m=args.m, precision=prec, reps=10, use_cache=False, batched=args.batched)
                 for cand, t in tab:
                     runs.setdefault(cand, []).append(t)
             med = sorted((float(np.median(v)), c) for c, v in runs.items())
             best_t, best = med[0]
-            table[f"{shape}|{prec}|{args.m}"] = {"best": list(best), "us": round(best_t, 2),
-                                                 "runner_up": [list(c) for _, c in med[1:4]]}
-            print(shape, prec, best, f"{best_t:.1f} us", flush=True)
+            key = at._key(shape, prec, args.m, args.batched)
+            table[key] = {"best": list(best), "us": round(best_t, 2), "runner_up": [list(c) for _, c in med[1:4]]}
+            if args.batched:
+                table[key]["cells"] = at.batch_cells(shape)
+            print(key, best, f"{best_t:.1f} us", flush=True)
     with open(args.out, "w") as f:
         json.dump(table, f, indent=1)
 
