@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="default tiling instead of the autotuner")
     ap.add_argument("--tiling", default=None, help="TY,XC,ZPT matvec tiling for every shape (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison")
+    ap.add_argument("--streams", type=int, default=2, help="stream groups of the batched solver")
     ap.add_argument("--check-fp64", action="store_true", help="also solve every cell in FP64 and report C errors")
     ap.add_argument("--cpu-iters", type=int, default=6, help="PCG iterations timed on the CPU oracle")
     ap.add_argument("--no-cpu", action="store_true")
@@ -116,7 +117,7 @@ def main():
         from nlfh_amd.learned import WarmStartNet
         net = WarmStartNet.load(args.learned)
     bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
-                                   warm_start=net)
+                                   warm_start=net, streams=args.streams)
     t_tune = time.perf_counter()
     if args.tiling:
         ty, xc, zpt = (int(v) for v in args.tiling.split(","))
@@ -198,6 +199,7 @@ def main():
             "workload": f"{desc}, 6 unit strains, PCG to true rel. residual {tol:g}, emin {emin:g}, C 6x6, "
                         f"batched solver ({args.precision})" + (", learned warm start" if net else ""),
             "cells_per_step": ncell, "rhs": 6, "tol": tol, "precision": args.precision,
+            "streams": args.streams,
             "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
                        "ty_xc_zpt": tiling}, "tune_seconds": round(t_tune, 2),
             "learned_start": bool(net), "learned_start_fallback_steps": fallbacks,
@@ -253,7 +255,8 @@ def matvec_roofline(pkg, bh, rho_d, precision):
     if os.path.exists(prof):
         with open(prof) as f:
             tr = json.load(f)
-        key = f"{kname} voxels={bh.tot}"
+        tiling = ",".join(str(v) for v in bh._plan.get_tiling(0))
+        key = f"{kname} voxels={bh.tot} tiling={tiling}"  # counters of the same launch shape only
         if key in tr:
             traffic = tr[key]["hbm_bytes"]
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
