@@ -194,15 +194,6 @@ __global__ void ir_accumulate_kernel(const VecArgs va, double* __restrict__ x64,
 
 // PCG scalar recurrences (nlfiber/linear.py:173-203), one block per cell, thread t = rhs.
 
-// Sum of part[row][range(cell)] by one wavefront: lane-strided partial sums, then a fixed
-// butterfly, so the result is deterministic; every lane returns the total.
-__device__ __forceinline__ double wave_sum(const double* part, int nitems, int row, const int* range, int cell) {
-  const int lane = threadIdx.x & 63;
-  double acc = 0.0;
-  for (int w = range[2 * cell] + lane; w < range[2 * cell + 1]; w += 64) acc += part[(int64_t)row * nitems + w];
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-  return acc;
-}
 
 // The scalar kernels below run one block per cell and one wavefront per right-hand side
 // (blockDim = 64 m).
@@ -221,7 +212,6 @@ __global__ void cg_init_kernel(CgState st, const double* part_a, int nplanes, co
     const double rn = sqrt(wave_sum(part_a, nplanes, t, prange, cell));
     const double rz = wave_sum(part_z, nz_items, t, zrange, cell);
     if (lane == 0) {
-      st.alpha[k] = 0.0;  // the first step of an iteration that adds alpha_{k-1} p_{k-1} adds nothing
       st.bn[k] = bn;
       st.rn[k] = rn;
       st.rz[k] = rz;
@@ -462,35 +452,6 @@ hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part
 }
 template hipError_t launch_reduce<double>(const VecArgs&, const double*, const double*, double*, int, hipStream_t);
 template hipError_t launch_reduce<float>(const VecArgs&, const float*, const float*, double*, int, hipStream_t);
-
-// Last step of the FP32 iteration whose matvec updates x (x += alpha_{k-1} p_{k-1} at iteration k): a
-// cell that stopped after iteration K (it = K + 1 >= 1) still owes alpha_K p_K, and p_K is P[it & 1]
-// (the ping-pong starts at P[0] and iteration k writes P[(k + 1) & 1]); alpha still holds alpha_K.
-template <typename T>
-__global__ void __launch_bounds__(256) x_flush_last_kernel(const VecArgs va, T* x, const T* p0, const T* p1,
-                                                           const double* alpha, const int* it) {
-  const ChunkItem ch = va.chunks[blockIdx.x];
-  const int k = it[ch.cell];
-  if (k < 1) return;
-  const CellDesc cd = va.cells[ch.cell];
-  const int rhs = blockIdx.y;
-  const T a = (T)alpha[ch.cell * va.m + rhs];
-  const T* pl = (k & 1) ? p1 : p0;
-  const int64_t base = cd.off_field + (int64_t)rhs * 3 * cd.nvox;
-  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) x[base + c * cd.nvox + i] += a * pl[base + c * cd.nvox + i];
-  }
-}
-
-template <typename T>
-hipError_t launch_x_flush_last(const VecArgs& va, T* x, const T* p0, const T* p1, const double* alpha, const int* it,
-                               hipStream_t st) {
-  hipLaunchKernelGGL((x_flush_last_kernel<T>), dim3(va.nchunks, va.m), dim3(256), 0, st, va, x, p0, p1, alpha, it);
-  return hipGetLastError();
-}
-template hipError_t launch_x_flush_last<float>(const VecArgs&, float*, const float*, const float*, const double*,
-                                               const int*, hipStream_t);
 
 __global__ void set_small_kernel(const SmallParams v, int n, double* dd, int* di) {
   const int t = threadIdx.x;

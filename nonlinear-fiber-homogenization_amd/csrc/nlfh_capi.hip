@@ -189,7 +189,7 @@ struct nlfh_plan {
   int* d_line_range32 = nullptr;  // per cell [begin, end) of its float32 line items
   double* d_part_bdot = nullptr;  // pass B spectral r . z partials (float32 lines)
   bool lean32 = false;            // every shape group on the register-FFT path: lean FP32 iteration
-  bool mvx32 = false;             // ... and on the staged pass A: the matvec updates x, pass A only r
+  bool dir32 = false;             // ... and on the staged passes: pass C forms p and steps x
   int* d_chunk_range = nullptr;
 
   // workspaces
@@ -489,8 +489,7 @@ const cplx<float>* tw_ptr<float>(nlfh_plan* p) { return p->d_tw32; }
 // K u (+ load) for every shape group. u may be null (load only).
 template <typename T>
 int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const T* s, const double* beta,
-               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st,
-               T* x = nullptr, const double* alpha = nullptr) {
+               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
   if (!strain && scale != T(1)) {
     set_error("internal: the matvec scales its output only together with the strain term");
     return NLFH_EINVAL;
@@ -512,8 +511,6 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.strain = p->d_strain;
     a.beta = beta;
     a.dot_part = dot_part;
-    a.x = x;
-    a.alpha = alpha;
     a.done = done;
     a.m = m;
     a.nitems = (int)p->mv.size();
@@ -525,7 +522,7 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.lam16 = (T)(p->lam / 16.0);
     a.mu16 = (T)(p->mu / 16.0);
     a.scale_out = scale;
-    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, x != nullptr ? 1 : 0, st));
+    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, st));
   }
   return NLFH_OK;
 }
@@ -573,7 +570,7 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
-      if (dot == 2 || update == 1) return NLFH_EUNSUPPORTED;
+      if (dot >= 2 || update == 1) return NLFH_EUNSUPPORTED;
       HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update != 0 ? 1 : 0, dot, lp, ll, st));
     }
   }
@@ -593,12 +590,13 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // One PCG iteration (nlfiber/linear.py:190-200) with the buffers of `w`; `cur` selects the old
 // search direction.
 // Lean iteration (FP32 inner solves when every shape group is on the register-FFT path): r . z comes
-// from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged pass A
-// (mvx32) the matvec of iteration k adds the previous step to the iterate, x += alpha_{k-1} p_{k-1},
-// from the p_{k-1} it reads for the direction update (the matvec is bound by instruction issue, pass A
-// by HBM, so the x traffic moves to the kernel with bandwidth to spare); pass A updates r only, and
-// launch_x_flush_last adds the last step alpha_K p_K once the cell is done. Same PCG recurrences; the
-// dot differs from the direct sum only by rounding.
+// from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged passes
+// (dir32) the search direction is formed in pass C instead of the matvec: iteration k runs
+//   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  alpha_k  ->  pass A r -= alpha_k Ap_k  ->  pass B
+//   ->  pass C p_{k+1} = z_{k+1} + beta_k p_k, x += alpha_k p_k  ->  beta / convergence,
+// so z is never stored (one field pass less than forming p in the matvec), the matvec reads p alone,
+// and x holds every step when a cell stops. Same PCG recurrences; the dot differs from the direct sum
+// only by rounding.
 template <typename T>
 bool lean(const nlfh_plan* p) {
   return sizeof(T) == 4 && p->lean32;
@@ -609,9 +607,11 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   T* p_old = w.P[cur];
   T* p_new = w.P[1 - cur];
   const bool ln = lean<T>(p);
-  const bool mvx = ln && p->mvx32;
-  TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st,
-                    mvx ? w.X : nullptr, mvx ? w.cs.alpha : nullptr));
+  const bool dir = ln && p->dir32;
+  if (dir)  // p_old holds p_k
+    TRY(run_matvec<T>(p, p_old, nullptr, nullptr, w.Ap, w.s, nullptr, p->d_part_mv, w.cs.done, m, 0, 0, 1, T(1), st));
+  else
+    TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
   HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
   PrecArgs<T> a = prec_args<T>(p, m);
   a.r = w.R;
@@ -625,8 +625,13 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   a.part_a = p->d_part_a;
   a.part_c = p->d_part_c;
   a.done = w.cs.done;
-  const int upd = mvx ? 1 : 2;
-  TRY(run_prec<T>(p, a, upd, ln ? 2 : 1, st));
+  if (dir) {
+    a.p = p_old;  // p_k: the iterate's step and the old direction of p_{k+1}
+    a.z = p_new;  // receives p_{k+1}
+    a.line_range = p->d_line_range32;
+    a.rz = w.cs.rz;
+  }
+  TRY(run_prec<T>(p, a, dir ? 1 : 2, dir ? 3 : (ln ? 2 : 1), st));
   if (ln) {
     HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
                            (int)p->lines32.size(), p->d_line_range32, st));
@@ -678,6 +683,8 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
   a.part_c = p->d_part_c;
   a.done = w.cs.skip;
   const bool ln = lean<T>(p);
+  const bool dir = ln && p->dir32;
+  if (dir) a.z = w.P[0];  // p_0 = z_0
   TRY(run_prec<T>(p, a, 0, ln ? 2 : 1, st));
   if (ln) {
     HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
@@ -688,7 +695,7 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
                            (int)p->planes.size(), p->d_plane_range, p->d_part_b, (int)p->chunks.size(),
                            p->d_chunk_range, st));
   }
-  HIP_TRY(launch_fill<T>(w.P[0], p->field_len(), T(0), st));
+  if (!dir) HIP_TRY(launch_fill<T>(w.P[0], p->field_len(), T(0), st));  // p_{-1} = 0
   int* h_done = p->h_pinned;
   const int nc = p->ncells;
   int cur = 0;
@@ -721,10 +728,6 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
       set_error("internal: the solver did not flag completion within maxit iterations");
       return NLFH_EHIP;
     }
-  }
-  if constexpr (sizeof(T) == 4) {
-    if (ln && p->mvx32)
-      HIP_TRY(launch_x_flush_last<float>(vec_args(p, m, nullptr), w.X, w.P[0], w.P[1], w.cs.alpha, w.cs.it, st));
   }
   HIP_TRY(hipMemcpyAsync(it_host, w.cs.it, nc * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1014,10 +1017,10 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
   if ((rc = p->upload(&p->d_line_range32, lrange32))) return fail(rc);
   p->lean32 = true;
-  p->mvx32 = true;
+  p->dir32 = true;
   for (const PrecGroup& g : p->pgroups) {
     p->lean32 = p->lean32 && g.fast != 0;
-    p->mvx32 = p->mvx32 && g.fast != 0 && g.fast <= 64 && g.vec;
+    p->dir32 = p->dir32 && g.fast != 0 && g.fast <= 64 && g.vec;
   }
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);

@@ -64,6 +64,16 @@ __device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) { return {a.re + b
 template <typename T>
 __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.re - b.re, a.im - b.im}; }
 
+// Sum of part[row][range(cell)] by one wavefront: lane-strided partial sums, then a fixed
+// butterfly, so the result is deterministic; every lane returns the total.
+__device__ __forceinline__ double wave_sum(const double* part, int nitems, int row, const int* range, int cell) {
+  const int lane = threadIdx.x & 63;
+  double acc = 0.0;
+  for (int w = range[2 * cell] + lane; w < range[2 * cell + 1]; w += 64) acc += part[(int64_t)row * nitems + w];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  return acc;
+}
+
 // Block-wide sum of one double per thread (blockDim.x a multiple of 64). Result valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double* red) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);

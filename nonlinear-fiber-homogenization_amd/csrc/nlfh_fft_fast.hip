@@ -503,8 +503,14 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   if (tid == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = tsum;
 }
 
-template <typename T, int N, bool DOT>
+// DIR (FP32 inner iteration): instead of z, the new search direction p_{k+1} = z + beta_k p_k goes to
+// a.z and the iterate takes its step, x += alpha_k p_k (p_k = a.p, the direction of this iteration's
+// matvec). beta_k = (r . z)_{k+1} / (r . z)_k is formed by every block from pass B's partials with the
+// same fixed-order sum (wave_sum) and guard as cg_beta_kernel, so it is bitwise the value cg_beta stores.
+// The matvec then reads p alone: z is neither written nor read again.
+template <typename T, int N, bool DOT, bool DIR = false>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+  static_assert(!(DOT && DIR), "the direction pass takes r . z from pass B");
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
   constexpr int RS = StageCfg<T, N>::RS;
@@ -525,6 +531,12 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   const int rhs = blockIdx.y;
   const int tid = threadIdx.x;
   for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  __shared__ double beta_sh;
+  if (DIR && tid < 64) {
+    const double rz1 = wave_sum(a.part_bdot, a.nlines, rhs, a.line_range, pi.cell);
+    const double rz = a.rz[pi.cell * a.m + rhs];
+    if (tid == 0) beta_sh = rz1 / (rz == 0.0 ? 1.0 : rz);
+  }
   {
     const cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
     constexpr int CPV = 16 / sizeof(cplx<T>);
@@ -623,12 +635,19 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     }
     __syncthreads();
   }
-  // store z (and r . z) with 16-byte vectors
+  // store z (and r . z) with 16-byte vectors; DIR: p_new = z + beta p and x += alpha p instead
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   double acc = 0.0;
   {
     V* Zv = reinterpret_cast<V*>(a.z + fo);
     const V* Rv = reinterpret_cast<const V*>(a.r + fo);
+    const V* Pv = reinterpret_cast<const V*>(DIR ? a.p + fo : nullptr);
+    V* Xv = reinterpret_cast<V*>(DIR ? a.x + fo : nullptr);
+    T beta = T(0), alpha = T(0);
+    if (DIR) {
+      beta = (T)beta_sh;  // written before the barriers above
+      alpha = (T)a.alpha[pi.cell * a.m + rhs];
+    }
     constexpr int NV = N * N / VN;
     for (int i = tid; i < NV; i += BT) {
       const int e = i * VN;
@@ -637,6 +656,18 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
       V zv;
       if constexpr (VN == 4) zv = {srcp[0], srcp[1], srcp[2], srcp[3]};
       else zv = {srcp[0], srcp[1]};
+      if (DIR) {
+        const V pv = Pv[i];
+        V xv = Xv[i];
+        if constexpr (VN == 4) {
+          zv.x += beta * pv.x; zv.y += beta * pv.y; zv.z += beta * pv.z; zv.w += beta * pv.w;
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
+        } else {
+          zv.x += beta * pv.x; zv.y += beta * pv.y;
+          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+        }
+        Xv[i] = xv;
+      }
       Zv[i] = zv;
       if (DOT) {
         const V rv = Rv[i];
@@ -762,8 +793,9 @@ size_t fast_lds_line() {
   return ((size_t)N + (size_t)C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
-// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only, the matvec updates x) on the
-// staged pass only; dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral)
+// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) on the staged pass only;
+// dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral), 3 as 2 and pass C forms the new search
+// direction and the iterate step (staged pass only, see prec_c_staged)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
                                 int update, int dot, int vec, hipStream_t st) {
@@ -784,11 +816,14 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       if (update == 1) { NLFH_A(prec_a_staged, 1) }
       else if (update) { NLFH_A(prec_a_staged, 2) }
       else { NLFH_A(prec_a_staged, 0) }
-      if (dot == 2) { NLFH_B(true) }
+      if (dot >= 2) { NLFH_B(true) }
       else { NLFH_B(false) }
       if (dot == 1) {
         big((const void*)prec_c_staged<T, N, true>, lp);
         hipLaunchKernelGGL((prec_c_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
+      } else if (dot == 3) {
+        big((const void*)prec_c_staged<T, N, false, true>, lp);
+        hipLaunchKernelGGL((prec_c_staged<T, N, false, true>), gp, dim3(C::BT), lp, st, a, tw);
       } else {
         big((const void*)prec_c_staged<T, N, false>, lp);
         hipLaunchKernelGGL((prec_c_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
@@ -796,7 +831,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       return hipGetLastError();
     }
   }
-  if (update == 1) return hipErrorInvalidValue;  // the r-only update belongs to the staged pass
+  if (update == 1 || dot == 3) return hipErrorInvalidValue;  // r-only update, direction pass: staged only
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }

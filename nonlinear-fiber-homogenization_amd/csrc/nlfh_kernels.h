@@ -18,8 +18,6 @@ struct MatvecArgs {
   const double* strain;  // STRAIN: macroscopic strains (m, 6) shared by all cells
   const double* beta;    // PUPDATE: beta per (cell, rhs)
   double* dot_part;      // DOT: partial sums of p . out per (rhs, item_base + item)
-  T* x;                  // XUPD: iterate updated x += alpha p_old on owned nodes
-  const double* alpha;   // XUPD: alpha of the previous iteration per (cell, rhs)
   const int* done;       // optional per-cell early-exit flags
   int m;
   int nitems;     // total matvec items of the plan (row stride of dot_part)
@@ -66,6 +64,8 @@ struct PrecArgs {
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
   double* part_c;        // pass C: partial r . z per (rhs, plane item)
   double* part_bdot;     // pass B (spectral dot): partial r . z per (rhs, line item)
+  const int* line_range; // direction pass C: per cell [begin, end) of its line items (part_bdot columns)
+  const double* rz;      // direction pass C: r . z of the previous iteration per (cell, rhs)
   const int* done;       // per-cell early exit (nullable)
   int m;
   int nplanes;           // total plane items (row stride of part_a / part_c)
@@ -114,7 +114,7 @@ struct EnergyArgs {
 
 template <typename T>
 hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
-                         int xupd, hipStream_t st);
+                         hipStream_t st);
 template <typename T>
 size_t prec_plane_lds(int ny, int nz, int rows_per_batch);
 template <typename T>
@@ -138,9 +138,6 @@ struct SmallParams {
   double d[36];
   int i[4];
 };
-template <typename T>
-hipError_t launch_x_flush_last(const VecArgs& va, T* x, const T* p0, const T* p1, const double* alpha, const int* it,
-                               hipStream_t st);
 hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st);
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);

@@ -148,12 +148,8 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
 // Addressing: every global access is a wave-uniform base (cell, right-hand side, component, x-plane)
 // plus a 32-bit per-lane offset inside the plane; with a row per wavefront (DPP) the row index is
 // wave-uniform too.
-// XUPD (with PUPDATE): the PCG iterate takes the previous step here, x += alpha_{k-1} p_{k-1} on the
-// owned nodes, from the p_{k-1} values this kernel reads anyway; alpha_{k-1} is still in `alpha` because
-// the current alpha is computed after the matvec. The caller adds the last step after the iterations.
-template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP, bool XUPD>
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
-  static_assert(!XUPD || PUPDATE, "the x update uses the old search direction of the p update");
   extern __shared__ double smem_d[];
   T* const lds = reinterpret_cast<T*>(smem_d);
   __shared__ double red[8];
@@ -189,19 +185,16 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const T* Pc[3];
   T* PNc[3];
   T* Oc[3];
-  T* Xc[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     Uc[c] = a.u ? a.u + fbase + c * nvox : nullptr;
     Pc[c] = PUPDATE ? a.p_old + fbase + c * nvox : nullptr;
     PNc[c] = PUPDATE ? a.p_new + fbase + c * nvox : nullptr;
-    Xc[c] = XUPD ? a.x + fbase + c * nvox : nullptr;
     Oc[c] = a.out + fbase + c * nvox;
   }
   const T* __restrict__ S = a.s + cd.off_vox;
-  T beta = T(0), alpha_prev = T(0);
+  T beta = T(0);
   if (PUPDATE) beta = (T)a.beta[it.cell * a.m + rhs];
-  if (XUPD) alpha_prev = (T)a.alpha[it.cell * a.m + rhs];
   T e4[6];
   if (STRAIN) {
 #pragma unroll
@@ -226,7 +219,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const bool has_u = !STRAIN || a.u != nullptr;  // u may be null only in load launches (host-checked)
   // loads only: the values are combined (u + beta p) where they are consumed, so that a plane
   // prefetched one step ahead stays in flight across the step's barrier
-  auto load_row = [&](unsigned po, T (&vu)[3][ZPT], T (&vp)[3][ZPT], T (&vx)[3][ZPT]) {
+  auto load_row = [&](unsigned po, T (&vu)[3][ZPT], T (&vp)[3][ZPT]) {
     const unsigned o = po + roff_ld;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -234,19 +227,15 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       for (int q = 0; q < ZPT; ++q) {
         vu[c][q] = has_u ? at(Uc[c], o + q) : T(0);
         if (PUPDATE) vp[c][q] = at(Pc[c], o + q);
-        if (XUPD) vx[c][q] = owns ? at(Xc[c], o + q) : T(0);
       }
     }
   };
-  // p_new and (XUPD) x of an owned node plane
-  auto store_dir = [&](unsigned o, const T (&un)[3][ZPT], const T (&vp)[3][ZPT], const T (&vx)[3][ZPT]) {
+  // p_new of an owned node plane
+  auto store_dir = [&](unsigned o, const T (&un)[3][ZPT]) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < ZPT; ++q) {
-        at(PNc[c], o + q) = un[c][q];
-        if (XUPD) at(Xc[c], o + q) = vx[c][q] + alpha_prev * vp[c][q];
-      }
+      for (int q = 0; q < ZPT; ++q) at(PNc[c], o + q) = un[c][q];
   };
   auto combine = [&](const T (&vu)[3][ZPT], const T (&vp)[3][ZPT], T (&v)[3][ZPT]) {
 #pragma unroll
@@ -259,7 +248,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   T up[3][ZPT], carry[3][ZPT];
   // one step ahead, by the parity of the step that loads them (fixed registers, no copies of values
   // in flight): the next x-plane (u and p) and the stiffness of the next element layer
-  T nxt_u[2][3][ZPT], nxt_p[2][3][ZPT], nxt_x[2][3][ZPT], sv[2][ZPT];
+  T nxt_u[2][3][ZPT], nxt_p[2][3][ZPT], sv[2][ZPT];
   T pc[2][3][ZPT];  // node values of the planes awaiting output (by layer parity)
   T Flo[3][2][ZPT];  // own half of the y-reverse of the last element layer
   double dot = 0.0;
@@ -291,19 +280,19 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 
   // prologue: layer L_0 from planes x0-1, x0; prefetch x0+1
   if (active) {
-    T un[3][ZPT], lu[3][ZPT], lp[3][ZPT], lx[3][ZPT];
-    load_row(plane_off(x0 - 1), lu, lp, lx);
+    T un[3][ZPT], lu[3][ZPT], lp[3][ZPT];
+    load_row(plane_off(x0 - 1), lu, lp);
     combine(lu, lp, up);
     const unsigned p0 = plane_off(x0);
-    load_row(p0, lu, lp, lx);
+    load_row(p0, lu, lp);
     combine(lu, lp, un);
-    if (xc >= 1) load_row(plane_off(x0 + 1), nxt_u[1], nxt_p[1], nxt_x[1]);
+    if (xc >= 1) load_row(plane_off(x0 + 1), nxt_u[1], nxt_p[1]);
     {
       const unsigned so = plane_off(x0 - 1) + roff_ld;
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) sv[1][q] = at(S, so + q);
     }
-    if (PUPDATE && owns && xc >= 1) store_dir(p0 + roff, un, lp, lx);
+    if (PUPDATE && owns && xc >= 1) store_dir(p0 + roff, un);
     x_stage(std::integral_constant<int, 0>{}, un);
   }
   // Drain the prologue's loads here (they were issued just after the ones x_stage waited for): the
@@ -320,7 +309,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     // a cache hit): x-plane x0 + s + 2 (consumed by part (c) of step s + 1) and the stiffness of
     // element layer L_{s+1} (part (b) of step s + 1)
     {
-      load_row(plane_off(x0 + (s + 2 < xc ? s + 2 : xc)), nxt_u[PS], nxt_p[PS], nxt_x[PS]);
+      load_row(plane_off(x0 + (s + 2 < xc ? s + 2 : xc)), nxt_u[PS], nxt_p[PS]);
       const unsigned so = plane_off(x0 + (s < xc - 1 ? s : xc - 1)) + roff_ld;
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) sv[PS][q] = at(S, so + q);
@@ -439,12 +428,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     for (int c = 0; c < 3; ++c) {
       settle(nxt_u[PQ][c]);
       if (PUPDATE) settle(nxt_p[PQ][c]);
-      if (XUPD) settle(nxt_x[PQ][c]);
     }
     if (s < xc && active) {
       T un[3][ZPT];
       combine(nxt_u[PQ], nxt_p[PQ], un);
-      if (PUPDATE && owns && s + 1 < xc) store_dir(plane_off(x0 + s + 1) + roff, un, nxt_p[PQ], nxt_x[PQ]);
+      if (PUPDATE && owns && s + 1 < xc) store_dir(plane_off(x0 + s + 1) + roff, un);
       x_stage(std::integral_constant<int, PQ>{}, un);
     }
     __syncthreads();
@@ -465,54 +453,47 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 // Host-side launcher -------------------------------------------------------------------------
 
 template <typename T, int ZPT, bool DPP>
-static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, int xupd, dim3 grid,
-                                int threads, size_t lds, hipStream_t st) {
-#define NLFH_MV(S_, P_, D_, X_)                                                                       \
+static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, dim3 grid, int threads,
+                                size_t lds, hipStream_t st) {
+#define NLFH_MV(S_, P_, D_)                                                                           \
   if (lds > 65536)                                                                                  \
-    (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_, DPP, X_>,                \
+    (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_, DPP>,                     \
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                \
-  hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_, DPP, X_>), grid, dim3(threads), lds, st, a);  \
+  hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_, DPP>), grid, dim3(threads), lds, st, a);       \
   return hipGetLastError();
-  if (xupd) {
-    // the fused x update serves the FP32 inner iteration (p update and p . Ap) only
-    if constexpr (sizeof(T) == 4) {
-      if (pupdate && dot && !strain) { NLFH_MV(false, true, true, true) }
-    }
-    return hipErrorInvalidValue;
-  }
   if (strain) {
-    NLFH_MV(true, false, false, false)
+    NLFH_MV(true, false, false)
   }
   if (pupdate) {
-    if (dot) { NLFH_MV(false, true, true, false) }
-    NLFH_MV(false, true, false, false)
+    if (dot) { NLFH_MV(false, true, true) }
+    NLFH_MV(false, true, false)
   }
-  if (dot) { NLFH_MV(false, false, true, false) }
-  NLFH_MV(false, false, false, false)
+  if (dot) { NLFH_MV(false, false, true) }
+  NLFH_MV(false, false, false)
 #undef NLFH_MV
 }
 
 template <typename T>
 hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
-                         int xupd, hipStream_t st) {
+                         hipStream_t st) {
   // max_ty + 2 rows of LZS lanes per block
   const int threads = ((a.LZS * (max_ty + 2) + 63) / 64) * 64;
   const size_t lds = (size_t)4 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
   dim3 grid(nblk, a.m, 1);
   const bool dpp = a.LZ == 64;  // a row is a whole wavefront
   switch (zpt * 2 + (dpp ? 1 : 0)) {
-    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
-    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, xupd, grid, threads, lds, st);
+    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, grid, threads, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-template hipError_t launch_matvec<double>(const MatvecArgs<double>&, int, int, int, int, int, int, int, hipStream_t);
-template hipError_t launch_matvec<float>(const MatvecArgs<float>&, int, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_matvec<double>(const MatvecArgs<double>&, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_matvec<float>(const MatvecArgs<float>&, int, int, int, int, int, int, hipStream_t);
 
 }  // namespace nlfh
