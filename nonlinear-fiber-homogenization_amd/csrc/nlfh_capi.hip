@@ -102,6 +102,7 @@ struct Work {
   T* Z = nullptr;
   T* s = nullptr;
   cplx<T>* spec = nullptr;
+  cplx<T>* rspec = nullptr;  // FP32 direction iteration: the residual's (z, y) spectrum
   double* st = nullptr;  // rz, alpha, beta, bn, rn (5 * ncells * M)
   int* ist = nullptr;    // it, done (2 * ncells)
   CgState cs{};
@@ -466,6 +467,7 @@ int get_ready64(nlfh_plan* p) {
 int get_ready32(nlfh_plan* p) {
   if (!p->w32_ready) {
     TRY(ensure_work(p, p->w32));
+    if (p->dir32) TRY(p->alloc(&p->w32.rspec, p->spec_len()));
     p->w32.cs.tol = p->d_params + p->ncells;
     p->w32.cs.maxit = p->d_iparams + 1;
     p->w32_ready = true;
@@ -570,7 +572,7 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
-      if (dot >= 2 || update == 1) return NLFH_EUNSUPPORTED;
+      if (dot >= 2 || update == 1 || update == 4 || a.rspec != nullptr) return NLFH_EUNSUPPORTED;
       HIP_TRY(launch_prec<T>(a, g.plane_count, g.line_count[pr], update != 0 ? 1 : 0, dot, lp, ll, st));
     }
   }
@@ -591,12 +593,14 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // search direction.
 // Lean iteration (FP32 inner solves when every shape group is on the register-FFT path): r . z comes
 // from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged passes
-// (dir32) the search direction is formed in pass C instead of the matvec: iteration k runs
-//   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  alpha_k  ->  pass A r -= alpha_k Ap_k  ->  pass B
-//   ->  pass C p_{k+1} = z_{k+1} + beta_k p_k, x += alpha_k p_k  ->  beta / convergence,
-// so z is never stored (one field pass less than forming p in the matvec), the matvec reads p alone,
-// and x holds every step when a cell stops. Same PCG recurrences; the dot differs from the direct sum
-// only by rounding.
+// (dir32) the residual is kept as its (z, y) spectrum r-hat and the search direction is formed in
+// pass C instead of the matvec: iteration k runs
+//   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  alpha_k  ->  pass A r-hat -= alpha_k FFT_zy(Ap_k), ||r||
+//   ->  pass B (r-hat in, z-hat out)  ->  pass C p_{k+1} = z_{k+1} + beta_k p_k, x += alpha_k p_k
+//   ->  beta / convergence,
+// so neither r nor z is stored in real space (two field passes less than the plain order), the matvec
+// reads p alone, and x holds every step when a cell stops. Same PCG recurrences (the transform is
+// linear); the norms and dots differ from the direct sums only by rounding.
 template <typename T>
 bool lean(const nlfh_plan* p) {
   return sizeof(T) == 4 && p->lean32;
@@ -628,10 +632,11 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
   if (dir) {
     a.p = p_old;  // p_k: the iterate's step and the old direction of p_{k+1}
     a.z = p_new;  // receives p_{k+1}
+    a.rspec = w.rspec;
     a.line_range = p->d_line_range32;
     a.rz = w.cs.rz;
   }
-  TRY(run_prec<T>(p, a, dir ? 1 : 2, dir ? 3 : (ln ? 2 : 1), st));
+  TRY(run_prec<T>(p, a, dir ? 4 : 2, dir ? 3 : (ln ? 2 : 1), st));
   if (ln) {
     HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
                            (int)p->lines32.size(), p->d_line_range32, st));
@@ -684,7 +689,10 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
   a.done = w.cs.skip;
   const bool ln = lean<T>(p);
   const bool dir = ln && p->dir32;
-  if (dir) a.z = w.P[0];  // p_0 = z_0
+  if (dir) {
+    a.z = w.P[0];  // p_0 = z_0
+    a.rspec = w.rspec;  // r-hat_0
+  }
   TRY(run_prec<T>(p, a, 0, ln ? 2 : 1, st));
   if (ln) {
     HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,

@@ -326,10 +326,15 @@ struct StageCfg {
                                (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
 };
 
-// UPD: 0 none; 1 r -= alpha Ap (the FP32 iteration whose matvec updates x); 2 also x += alpha p
+// UPD: 0 none; 1 r -= alpha Ap; 2 also x += alpha p; 4 spectral residual (FP32 inner iteration): the
+// plane of Ap is transformed and the residual's (z, y) spectrum takes the update in place,
+// r-hat -= alpha Ap-hat, with ||r||^2 from that spectrum by Parseval (weight 1 at kz = 0 and N/2, 2
+// elsewhere, over N^2), so r itself is never read or written. Without UPD 4, the plane's spectrum goes
+// to a.rspec when that is set (the start of such an iteration), else to a.spec.
 template <typename T, int N, int UPD>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
-  constexpr bool UPDATE = UPD >= 1;
+  constexpr bool UPDATE = UPD == 1 || UPD == 2;
+  constexpr bool SPEC = UPD == 4;
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
   constexpr int RS = StageCfg<T, N>::RS;
@@ -353,11 +358,26 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   for (int i = tid; i < N; i += BT) tw[i] = twg[i];
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   T alpha = T(0);
-  if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  if (UPDATE || SPEC) alpha = (T)a.alpha[pi.cell * a.m + rhs];
   double acc = 0.0;
+  // SPEC: this plane's residual spectrum is loaded now, into registers, so that its latency passes
+  // during the transforms (it is consumed by the final update)
+  constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
+  constexpr int NVS = N * NH / CPV;
+  constexpr int RPT = SPEC ? (NVS + BT - 1) / BT : 1;
+  V* const rsp = reinterpret_cast<V*>((a.rspec != nullptr ? a.rspec : a.spec) + cd.off_spec +
+                                      ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH);
+  V rpre[RPT];
+  if constexpr (SPEC) {
+#pragma unroll
+    for (int j = 0; j < RPT; ++j) {
+      const int i = tid + j * BT;
+      if (i < NVS) rpre[j] = rsp[i];
+    }
+  }
   // load (and update) the plane with 16-byte vectors into rp
   {
-    const V* Rv = reinterpret_cast<const V*>(a.r + fo);
+    const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + fo);
     V* ROv = reinterpret_cast<V*>(a.r_out + fo);
     const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
     V* Xv = reinterpret_cast<V*>(a.x + fo);
@@ -390,7 +410,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
 #pragma unroll
       for (int k = 0; k < VN; ++k) {
         const T v = VT::get(rv, k);
-        acc += (double)v * (double)v;
+        if (!SPEC) acc += (double)v * (double)v;
         dst[k] = v;
       }
     }
@@ -492,12 +512,33 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   __syncthreads();
   // the half plane [ky][kz] is contiguous in the spectrum: 16-byte vector copy
   {
-    cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
-    constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
-    constexpr int NVS = N * NH / CPV;
     const V* src = reinterpret_cast<const V*>(plane);
-    V* dst = reinterpret_cast<V*>(so);
-    for (int i = tid; i < NVS; i += BT) dst[i] = src[i];
+    V* dst = rsp;
+    if constexpr (SPEC) {
+      double sacc = 0.0;
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int i = tid + j * BT;
+        if (i >= NVS) break;
+        V rh = rpre[j];
+        const V ap = src[i];
+        if constexpr (VN == 4) {
+          rh.x -= alpha * ap.x; rh.y -= alpha * ap.y; rh.z -= alpha * ap.z; rh.w -= alpha * ap.w;
+        } else {
+          rh.x -= alpha * ap.x; rh.y -= alpha * ap.y;
+        }
+        dst[i] = rh;
+#pragma unroll
+        for (int c = 0; c < CPV; ++c) {
+          const int kz = (i * CPV + c) % NH;
+          const double re = (double)VT::get(rh, 2 * c), im = (double)VT::get(rh, 2 * c + 1);
+          sacc += (kz == 0 || 2 * kz == N ? 1.0 : 2.0) * (re * re + im * im);
+        }
+      }
+      acc = sacc * (1.0 / ((double)N * (double)N));
+    } else {
+      for (int i = tid; i < NVS; i += BT) dst[i] = src[i];
+    }
   }
   const double tsum = block_sum(acc, red);
   if (tid == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = tsum;
@@ -536,6 +577,24 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     const double rz1 = wave_sum(a.part_bdot, a.nlines, rhs, a.line_range, pi.cell);
     const double rz = a.rz[pi.cell * a.m + rhs];
     if (tid == 0) beta_sh = rz1 / (rz == 0.0 ? 1.0 : rz);
+  }
+  // DIR: the plane's p and x are loaded now, into registers, so that their latency passes during the
+  // transforms (they are consumed by the final store)
+  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
+  constexpr int NV = N * N / VN;
+  constexpr int PPT = DIR ? (NV + BT - 1) / BT : 1;
+  V ppre[PPT], xpre[PPT];
+  if constexpr (DIR) {
+    const V* Pv = reinterpret_cast<const V*>(a.p + fo);
+    const V* Xv = reinterpret_cast<const V*>(a.x + fo);
+#pragma unroll
+    for (int j = 0; j < PPT; ++j) {
+      const int i = tid + j * BT;
+      if (i < NV) {
+        ppre[j] = Pv[i];
+        xpre[j] = Xv[i];
+      }
+    }
   }
   {
     const cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
@@ -636,29 +695,27 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     __syncthreads();
   }
   // store z (and r . z) with 16-byte vectors; DIR: p_new = z + beta p and x += alpha p instead
-  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   double acc = 0.0;
   {
     V* Zv = reinterpret_cast<V*>(a.z + fo);
     const V* Rv = reinterpret_cast<const V*>(a.r + fo);
-    const V* Pv = reinterpret_cast<const V*>(DIR ? a.p + fo : nullptr);
     V* Xv = reinterpret_cast<V*>(DIR ? a.x + fo : nullptr);
     T beta = T(0), alpha = T(0);
     if (DIR) {
       beta = (T)beta_sh;  // written before the barriers above
       alpha = (T)a.alpha[pi.cell * a.m + rhs];
     }
-    constexpr int NV = N * N / VN;
-    for (int i = tid; i < NV; i += BT) {
+    // vector i of the plane; j: its slot among the prefetched values (DIR)
+    auto store = [&](int i, int j) {
       const int e = i * VN;
       const int y = e / N, z = e - (e / N) * N;
       const T* srcp = rp + y * RS + z;
       V zv;
       if constexpr (VN == 4) zv = {srcp[0], srcp[1], srcp[2], srcp[3]};
       else zv = {srcp[0], srcp[1]};
-      if (DIR) {
-        const V pv = Pv[i];
-        V xv = Xv[i];
+      if constexpr (DIR) {
+        const V pv = ppre[j];
+        V xv = xpre[j];
         if constexpr (VN == 4) {
           zv.x += beta * pv.x; zv.y += beta * pv.y; zv.z += beta * pv.z; zv.w += beta * pv.w;
           xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
@@ -674,6 +731,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
 #pragma unroll
         for (int k = 0; k < VN; ++k) acc += (double)VT::get(rv, k) * (double)VT::get(zv, k);
       }
+    };
+    if constexpr (DIR) {
+#pragma unroll
+      for (int j = 0; j < PPT; ++j)
+        if (tid + j * BT < NV) store(tid + j * BT, j);
+    } else {
+      for (int i = tid; i < NV; i += BT) store(i, 0);
     }
   }
   if (DOT) {
@@ -707,13 +771,15 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
   constexpr int64_t npos = (int64_t)N * NH;
   const int64_t pos = li.pos0 + l;
   cplx<T>* sp = a.spec + cd.off_spec + (int64_t)rhs * 3 * cd.nspec;
+  // input: the residual's spectrum when the plan keeps one (out of place), else spec itself
+  const cplx<T>* spi = (a.rspec != nullptr ? a.rspec : a.spec) + cd.off_spec + (int64_t)rhs * 3 * cd.nspec;
   // the three components' columns are loaded before the first exchange (all loads in flight together)
   cplx<T> v[3][N1];
   if (act && t < N2) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int j1 = 0; j1 < N1; ++j1) v[c][j1] = sp[(int64_t)c * cd.nspec + (int64_t)(t + N2 * j1) * npos + pos];
+      for (int j1 = 0; j1 < N1; ++j1) v[c][j1] = spi[(int64_t)c * cd.nspec + (int64_t)(t + N2 * j1) * npos + pos];
   }
   __syncthreads();
   cplx<T> Xc[3][N2];
@@ -793,7 +859,8 @@ size_t fast_lds_line() {
   return ((size_t)N + (size_t)C::XS * C::LPB) * sizeof(cplx<T>);
 }
 
-// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) on the staged pass only;
+// update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 4 (spectral residual) on the
+// staged pass only;
 // dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral), 3 as 2 and pass C forms the new search
 // direction and the iterate step (staged pass only, see prec_c_staged)
 template <typename T, int N>
@@ -813,7 +880,8 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   hipLaunchKernelGGL((prec_b_fast<T, N, D>), gl, dim3(C::BTB), ll, st, a, tw, symp);
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update == 1) { NLFH_A(prec_a_staged, 1) }
+      if (update == 4) { NLFH_A(prec_a_staged, 4) }
+      else if (update == 1) { NLFH_A(prec_a_staged, 1) }
       else if (update) { NLFH_A(prec_a_staged, 2) }
       else { NLFH_A(prec_a_staged, 0) }
       if (dot >= 2) { NLFH_B(true) }
@@ -831,7 +899,8 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
       return hipGetLastError();
     }
   }
-  if (update == 1 || dot == 3) return hipErrorInvalidValue;  // r-only update, direction pass: staged only
+  // r-only and spectral updates, the direction pass and a kept residual spectrum: staged passes only
+  if (update == 1 || update == 4 || dot == 3 || a.rspec != nullptr) return hipErrorInvalidValue;
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }

@@ -60,6 +60,8 @@ struct PrecArgs {
   const T* p;            // pass A fused update
   const double* alpha;   // per (cell, rhs)
   cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
+  cplx<T>* rspec;        // nullable: the residual's (z, y) spectrum, kept across iterations; when set, pass A
+                         // writes it (not spec) and pass B reads it and writes spec
   T* z;                  // pass C output
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
   double* part_c;        // pass C: partial r . z per (rhs, plane item)
