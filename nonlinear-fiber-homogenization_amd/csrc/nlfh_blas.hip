@@ -234,6 +234,7 @@ __global__ void cg_alpha_kernel(CgState st, const double* part_mv, int nmv, cons
   const int k = cell * st.m + t;
   const double pAp = wave_sum(part_mv, nmv, t, mrange, cell);
   if (lane == 0) {
+    if (st.alpha_prev != nullptr) st.alpha_prev[k] = st.alpha[k];
     st.alpha[k] = st.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
   }
 }
@@ -452,6 +453,33 @@ hipError_t launch_reduce(const VecArgs& va, const T* a, const T* b, double* part
 }
 template hipError_t launch_reduce<double>(const VecArgs&, const double*, const double*, double*, int, hipStream_t);
 template hipError_t launch_reduce<float>(const VecArgs&, const float*, const float*, double*, int, hipStream_t);
+
+// Paired iterate steps of the FP32 direction iteration (x += alpha_{k-1} p_{k-1} + alpha_k p_k in pass C of
+// odd k): a cell whose last iteration K was even (it = K + 1 odd) still owes alpha_K p_K, and p_K is P[0]
+// (iteration k reads its direction from P[k & 1]); alpha still holds alpha_K.
+template <typename T>
+__global__ void __launch_bounds__(256) x_flush_even_kernel(const VecArgs va, T* x, const T* p0, const double* alpha,
+                                                           const int* it) {
+  const ChunkItem ch = va.chunks[blockIdx.x];
+  if ((it[ch.cell] & 1) == 0) return;
+  const CellDesc cd = va.cells[ch.cell];
+  const int rhs = blockIdx.y;
+  const T a = (T)alpha[ch.cell * va.m + rhs];
+  const int64_t base = cd.off_field + (int64_t)rhs * 3 * cd.nvox;
+  for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[base + c * cd.nvox + i] += a * p0[base + c * cd.nvox + i];
+  }
+}
+
+template <typename T>
+hipError_t launch_x_flush_even(const VecArgs& va, T* x, const T* p0, const double* alpha, const int* it,
+                               hipStream_t st) {
+  hipLaunchKernelGGL((x_flush_even_kernel<T>), dim3(va.nchunks, va.m), dim3(256), 0, st, va, x, p0, alpha, it);
+  return hipGetLastError();
+}
+template hipError_t launch_x_flush_even<float>(const VecArgs&, float*, const float*, const double*, const int*,
+                                               hipStream_t);
 
 __global__ void set_small_kernel(const SmallParams v, int n, double* dd, int* di) {
   const int t = threadIdx.x;

@@ -467,7 +467,10 @@ int get_ready64(nlfh_plan* p) {
 int get_ready32(nlfh_plan* p) {
   if (!p->w32_ready) {
     TRY(ensure_work(p, p->w32));
-    if (p->dir32) TRY(p->alloc(&p->w32.rspec, p->spec_len()));
+    if (p->dir32) {
+      TRY(p->alloc(&p->w32.rspec, p->spec_len()));
+      p->w32.cs.alpha_prev = p->w32.st + 5 * p->ncells * p->M;
+    }
     p->w32.cs.tol = p->d_params + p->ncells;
     p->w32.cs.maxit = p->d_iparams + 1;
     p->w32_ready = true;
@@ -596,11 +599,12 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // (dir32) the residual is kept as its (z, y) spectrum r-hat and the search direction is formed in
 // pass C instead of the matvec: iteration k runs
 //   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  alpha_k  ->  pass A r-hat -= alpha_k FFT_zy(Ap_k), ||r||
-//   ->  pass B (r-hat in, z-hat out)  ->  pass C p_{k+1} = z_{k+1} + beta_k p_k, x += alpha_k p_k
-//   ->  beta / convergence,
+//   ->  pass B (r-hat in, z-hat out)  ->  pass C p_{k+1} = z_{k+1} + beta_k p_k
+//   (odd k also x += alpha_{k-1} p_{k-1} + alpha_k p_k)  ->  beta / convergence,
 // so neither r nor z is stored in real space (two field passes less than the plain order), the matvec
-// reads p alone, and x holds every step when a cell stops. Same PCG recurrences (the transform is
-// linear); the norms and dots differ from the direct sums only by rounding.
+// reads p alone, and x is read and written every second iteration; launch_x_flush_even adds the step a
+// cell owes when it stops after an even iteration. Same PCG recurrences (the transform is linear); the
+// norms and dots differ from the direct sums only by rounding.
 template <typename T>
 bool lean(const nlfh_plan* p) {
   return sizeof(T) == 4 && p->lean32;
@@ -635,8 +639,10 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
     a.rspec = w.rspec;
     a.line_range = p->d_line_range32;
     a.rz = w.cs.rz;
+    a.alpha_prev = w.cs.alpha_prev;
   }
-  TRY(run_prec<T>(p, a, dir ? 4 : 2, dir ? 3 : (ln ? 2 : 1), st));
+  // cur == k & 1: every graph holds an even number of iterations
+  TRY(run_prec<T>(p, a, dir ? 4 : 2, dir ? (cur ? 4 : 3) : (ln ? 2 : 1), st));
   if (ln) {
     HIP_TRY(launch_cg_beta(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
                            (int)p->lines32.size(), p->d_line_range32, st));
@@ -736,6 +742,9 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
       set_error("internal: the solver did not flag completion within maxit iterations");
       return NLFH_EHIP;
     }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (dir) HIP_TRY(launch_x_flush_even<float>(vec_args(p, m, nullptr), w.X, w.P[0], w.cs.alpha, w.cs.it, st));
   }
   HIP_TRY(hipMemcpyAsync(it_host, w.cs.it, nc * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));

@@ -545,11 +545,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
 }
 
 // DIR (FP32 inner iteration): instead of z, the new search direction p_{k+1} = z + beta_k p_k goes to
-// a.z and the iterate takes its step, x += alpha_k p_k (p_k = a.p, the direction of this iteration's
-// matvec). beta_k = (r . z)_{k+1} / (r . z)_k is formed by every block from pass B's partials with the
-// same fixed-order sum (wave_sum) and guard as cg_beta_kernel, so it is bitwise the value cg_beta stores.
-// The matvec then reads p alone: z is neither written nor read again.
-template <typename T, int N, bool DOT, bool DIR = false>
+// a.z (p_k = a.p, the direction of this iteration's matvec). beta_k = (r . z)_{k+1} / (r . z)_k is formed
+// by every block from pass B's partials with the same fixed-order sum (wave_sum) and guard as
+// cg_beta_kernel, so it is bitwise the value cg_beta stores. The matvec then reads p alone: z is neither
+// written nor read again. DIR 1 (even k) leaves the iterate alone; DIR 2 (odd k) gives it the steps of
+// both iterations, x += alpha_{k-1} p_{k-1} + alpha_k p_k, where p_{k-1} is the old content of a.z, read
+// by the same thread before it writes p_{k+1} there (x is read and written every second iteration).
+template <typename T, int N, bool DOT, int DIR = 0>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   static_assert(!(DOT && DIR), "the direction pass takes r . z from pass B");
   using C = FastCfg<T, N>;
@@ -578,21 +580,26 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     const double rz = a.rz[pi.cell * a.m + rhs];
     if (tid == 0) beta_sh = rz1 / (rz == 0.0 ? 1.0 : rz);
   }
-  // DIR: the plane's p and x are loaded now, into registers, so that their latency passes during the
-  // transforms (they are consumed by the final store)
+  // DIR: the plane's p (and with DIR 2 x and p_{k-1}) are loaded now, into registers, so that their
+  // latency passes during the transforms (they are consumed by the final store)
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   constexpr int NV = N * N / VN;
   constexpr int PPT = DIR ? (NV + BT - 1) / BT : 1;
-  V ppre[PPT], xpre[PPT];
-  if constexpr (DIR) {
+  constexpr int XPT = DIR == 2 ? PPT : 1;
+  V ppre[PPT], xpre[XPT], qpre[XPT];
+  if constexpr (DIR != 0) {
     const V* Pv = reinterpret_cast<const V*>(a.p + fo);
     const V* Xv = reinterpret_cast<const V*>(a.x + fo);
+    const V* Qv = reinterpret_cast<const V*>(a.z + fo);
 #pragma unroll
     for (int j = 0; j < PPT; ++j) {
       const int i = tid + j * BT;
       if (i < NV) {
         ppre[j] = Pv[i];
-        xpre[j] = Xv[i];
+        if constexpr (DIR == 2) {
+          xpre[j] = Xv[i];
+          qpre[j] = Qv[i];
+        }
       }
     }
   }
@@ -700,10 +707,11 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     V* Zv = reinterpret_cast<V*>(a.z + fo);
     const V* Rv = reinterpret_cast<const V*>(a.r + fo);
     V* Xv = reinterpret_cast<V*>(DIR ? a.x + fo : nullptr);
-    T beta = T(0), alpha = T(0);
-    if (DIR) {
-      beta = (T)beta_sh;  // written before the barriers above
+    T beta = T(0), alpha = T(0), alpha_p = T(0);
+    if (DIR) beta = (T)beta_sh;  // written before the barriers above
+    if (DIR == 2) {
       alpha = (T)a.alpha[pi.cell * a.m + rhs];
+      alpha_p = (T)a.alpha_prev[pi.cell * a.m + rhs];
     }
     // vector i of the plane; j: its slot among the prefetched values (DIR)
     auto store = [&](int i, int j) {
@@ -713,17 +721,25 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
       V zv;
       if constexpr (VN == 4) zv = {srcp[0], srcp[1], srcp[2], srcp[3]};
       else zv = {srcp[0], srcp[1]};
-      if constexpr (DIR) {
+      if constexpr (DIR != 0) {
         const V pv = ppre[j];
-        V xv = xpre[j];
         if constexpr (VN == 4) {
           zv.x += beta * pv.x; zv.y += beta * pv.y; zv.z += beta * pv.z; zv.w += beta * pv.w;
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
         } else {
           zv.x += beta * pv.x; zv.y += beta * pv.y;
-          xv.x += alpha * pv.x; xv.y += alpha * pv.y;
         }
-        Xv[i] = xv;
+        if constexpr (DIR == 2) {
+          const V qv = qpre[j];
+          V xv = xpre[j];
+          if constexpr (VN == 4) {
+            xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y; xv.z += alpha_p * qv.z; xv.w += alpha_p * qv.w;
+            xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
+          } else {
+            xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y;
+            xv.x += alpha * pv.x; xv.y += alpha * pv.y;
+          }
+          Xv[i] = xv;
+        }
       }
       Zv[i] = zv;
       if (DOT) {
@@ -732,7 +748,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
         for (int k = 0; k < VN; ++k) acc += (double)VT::get(rv, k) * (double)VT::get(zv, k);
       }
     };
-    if constexpr (DIR) {
+    if constexpr (DIR != 0) {
 #pragma unroll
       for (int j = 0; j < PPT; ++j)
         if (tid + j * BT < NV) store(tid + j * BT, j);
@@ -862,7 +878,7 @@ size_t fast_lds_line() {
 // update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 4 (spectral residual) on the
 // staged pass only;
 // dot: 0 none, 1 r . z in pass C, 2 r . z in pass B (spectral), 3 as 2 and pass C forms the new search
-// direction and the iterate step (staged pass only, see prec_c_staged)
+// direction, 4 as 3 with the paired iterate step of odd iterations (staged pass only, see prec_c_staged)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
                                 int update, int dot, int vec, hipStream_t st) {
@@ -890,8 +906,11 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
         big((const void*)prec_c_staged<T, N, true>, lp);
         hipLaunchKernelGGL((prec_c_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
       } else if (dot == 3) {
-        big((const void*)prec_c_staged<T, N, false, true>, lp);
-        hipLaunchKernelGGL((prec_c_staged<T, N, false, true>), gp, dim3(C::BT), lp, st, a, tw);
+        big((const void*)prec_c_staged<T, N, false, 1>, lp);
+        hipLaunchKernelGGL((prec_c_staged<T, N, false, 1>), gp, dim3(C::BT), lp, st, a, tw);
+      } else if (dot == 4) {
+        big((const void*)prec_c_staged<T, N, false, 2>, lp);
+        hipLaunchKernelGGL((prec_c_staged<T, N, false, 2>), gp, dim3(C::BT), lp, st, a, tw);
       } else {
         big((const void*)prec_c_staged<T, N, false>, lp);
         hipLaunchKernelGGL((prec_c_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
@@ -900,7 +919,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
     }
   }
   // r-only and spectral updates, the direction pass and a kept residual spectrum: staged passes only
-  if (update == 1 || update == 4 || dot == 3 || a.rspec != nullptr) return hipErrorInvalidValue;
+  if (update == 1 || update == 4 || dot >= 3 || a.rspec != nullptr) return hipErrorInvalidValue;
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }

@@ -59,6 +59,7 @@ struct PrecArgs {
   T* x;                  // pass A fused update: x += alpha p
   const T* p;            // pass A fused update
   const double* alpha;   // per (cell, rhs)
+  const double* alpha_prev;  // direction pass C, odd iterations: alpha of the previous iteration
   cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
   cplx<T>* rspec;        // nullable: the residual's (z, y) spectrum, kept across iterations; when set, pass A
                          // writes it (not spec) and pass B reads it and writes spec
@@ -88,6 +89,7 @@ struct VecArgs {
 struct CgState {
   double* rz;     // [cell*m + rhs]
   double* alpha;
+  double* alpha_prev;  // nullable: receives the previous alpha before cg_alpha overwrites it
   double* beta;
   double* bn;
   double* rn;
@@ -140,6 +142,9 @@ struct SmallParams {
   double d[36];
   int i[4];
 };
+template <typename T>
+hipError_t launch_x_flush_even(const VecArgs& va, T* x, const T* p0, const double* alpha, const int* it,
+                               hipStream_t st);
 hipError_t launch_set_small(const SmallParams& v, int n, double* dd, int* di, hipStream_t st);
 hipError_t launch_gram(const VecArgs& va, const double* a, const double* b, double* part, hipStream_t st);
 hipError_t launch_combine(const VecArgs& va, const double* x, const double* coef, double* out, hipStream_t st);
