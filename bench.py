@@ -162,6 +162,18 @@ def main():
     value = ncell * args.steps / dt
 
     roofline = matvec_roofline(pkg, bh, rho_d, args.precision)
+    # solver level: the reference algorithm's bytes per PCG iteration (13 field passes of 6 right-hand
+    # sides x 3 components, s once, the 3x3 symmetric symbol once) times the iterations every cell took,
+    # over the step's wall time; fusion below 13 passes shows as time saved, not as fewer bytes
+    w_it = 8 if args.precision == "fp64" else 4
+    b_it = [int(np.prod(sh)) * (13 * 3 * 6 * w_it + w_it) + sh[0] * sh[1] * (sh[2] // 2 + 1) * 6 * w_it
+            for sh in shapes]
+    iters = its_all[-1] if its_all else []
+    achieved_solve = sum(b * it for b, it in zip(b_it, iters)) / (ms_step * 1e-3) / 1e9
+    solver_bw = {"achieved": round(achieved_solve, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(achieved_solve / HBM_PEAK_GBS, 4), "bytes_per_iteration_all_cells": sum(b_it),
+                 "iterations_summed": int(sum(iters)),
+                 "basis": "13-pass PCG iteration bytes of the inner precision x PCG iterations per cell / wall time"}
 
     extra = {}
     if args.check_fp64 or args.workload == "stress128":
@@ -208,6 +220,7 @@ def main():
             "cells": metas if ncell <= 16 else f"{ncell} cells (seeds in the workload description)",
         },
         "roofline": roofline,
+        "solver_bandwidth": solver_bw,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
     }

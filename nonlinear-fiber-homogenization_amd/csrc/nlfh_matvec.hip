@@ -215,7 +215,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     x = x >= nx ? x - nx : x;
     return (unsigned)(x * plane);
   };
-  const unsigned roff_ld = lz < LZ ? roff : 0u;  // idle lanes of a padded row read the row start
+  // Inactive lanes (idle lanes of a padded row, and rows r >= RB of a block that has more threads than
+  // the tile's rows: its thread count is rounded up to whole wavefronts and sized for the largest tile)
+  // take part in the step prefetch, which is issued on every path: they read their plane's first row.
+  // Their own row index is not a valid row (y0 - 1 + r may exceed 2 ny, beyond the single wrap above).
+  const unsigned roff_ld = active ? roff : 0u;
   const bool has_u = !STRAIN || a.u != nullptr;  // u may be null only in load launches (host-checked)
   // loads only: the values are combined (u + beta p) where they are consumed, so that a plane
   // prefetched one step ahead stays in flight across the step's barrier
