@@ -259,8 +259,9 @@ def matvec_roofline(pkg, bh, rho_d, precision):
     torch.cuda.synchronize()
     t_mv = e0.elapsed_time(e1) / reps * 1e-3
     w = 4 if f32 else 8
-    # the FP32 operator converts s to FP32 in a separate pass each call; its bytes are not counted
-    bytes_mv = bh.tot * (2 * 3 * 6 * w + w)
+    # the field values in the operator's precision; s is read in FP64 by both (the FP32 operator rounds it
+    # as it loads it)
+    bytes_mv = bh.tot * (2 * 3 * 6 * w + 8)
     achieved = bytes_mv / t_mv / 1e9
     traffic = None
     prof = os.path.join(ROOT, "profiles", "matvec_traffic.json")

@@ -494,7 +494,8 @@ const cplx<float>* tw_ptr<float>(nlfh_plan* p) { return p->d_tw32; }
 // K u (+ load) for every shape group. u may be null (load only).
 template <typename T>
 int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const T* s, const double* beta,
-               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st) {
+               double* dot_part, const int* done, int m, int strain, int pupdate, int dot, T scale, hipStream_t st,
+               const double* s64 = nullptr) {
   if (!strain && scale != T(1)) {
     set_error("internal: the matvec scales its output only together with the strain term");
     return NLFH_EINVAL;
@@ -513,6 +514,7 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.p_new = p_new;
     a.out = out;
     a.s = s;
+    a.s64 = s64;
     a.strain = p->d_strain;
     a.beta = beta;
     a.dot_part = dot_part;
@@ -1141,10 +1143,9 @@ int nlfh_apply_K(nlfh_plan* p, const double* s, const void* u, void* Ku, int m, 
     TRY(run_matvec<double>(p, (const double*)u, nullptr, nullptr, (double*)Ku, s, nullptr, nullptr, nullptr, m, 0, 0,
                            0, 1.0, st));
   } else {
-    TRY(get_ready32(p));
-    HIP_TRY(launch_d2f(s, p->w32.s, p->tot_vox, st));
-    TRY(run_matvec<float>(p, (const float*)u, nullptr, nullptr, (float*)Ku, p->w32.s, nullptr, nullptr, nullptr, m, 0,
-                          0, 0, 1.0f, st));
+    // the FP32 operator rounds s as it loads it (no conversion pass)
+    TRY(run_matvec<float>(p, (const float*)u, nullptr, nullptr, (float*)Ku, nullptr, nullptr, nullptr, nullptr, m, 0,
+                          0, 0, 1.0f, st, s));
   }
   return NLFH_OK;
 }

@@ -15,6 +15,7 @@ struct MatvecArgs {
   T* p_new;              // PUPDATE: new search direction p = z + beta p_old (owned nodes)
   T* out;                // output field
   const T* s;            // stiffness scale per voxel
+  const double* s64;     // nullable: FP64 stiffness read (and rounded to T) instead of s
   const double* strain;  // STRAIN: macroscopic strains (m, 6) shared by all cells
   const double* beta;    // PUPDATE: beta per (cell, rhs)
   double* dot_part;      // DOT: partial sums of p . out per (rhs, item_base + item)

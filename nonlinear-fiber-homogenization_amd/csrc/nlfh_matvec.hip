@@ -148,7 +148,9 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
 // Addressing: every global access is a wave-uniform base (cell, right-hand side, component, x-plane)
 // plus a 32-bit per-lane offset inside the plane; with a row per wavefront (DPP) the row index is
 // wave-uniform too.
-template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP>
+// S64: the stiffness is read in FP64 (a.s64) and rounded to T as it is loaded (the standalone FP32
+// operator, whose caller holds s in FP64; the solver converts s once per solve and reads a.s).
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP, bool S64 = false>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   extern __shared__ double smem_d[];
   T* const lds = reinterpret_cast<T*>(smem_d);
@@ -192,7 +194,12 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     PNc[c] = PUPDATE ? a.p_new + fbase + c * nvox : nullptr;
     Oc[c] = a.out + fbase + c * nvox;
   }
-  const T* __restrict__ S = a.s + cd.off_vox;
+  const T* __restrict__ S = S64 ? nullptr : a.s + cd.off_vox;
+  const double* __restrict__ S8 = S64 ? a.s64 + cd.off_vox : nullptr;
+  auto load_s = [&](unsigned i) -> T {
+    if constexpr (S64) return (T)at(S8, i);
+    else return at(S, i);
+  };
   T beta = T(0);
   if (PUPDATE) beta = (T)a.beta[it.cell * a.m + rhs];
   T e4[6];
@@ -294,7 +301,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     {
       const unsigned so = plane_off(x0 - 1) + roff_ld;
 #pragma unroll
-      for (int q = 0; q < ZPT; ++q) sv[1][q] = at(S, so + q);
+      for (int q = 0; q < ZPT; ++q) sv[1][q] = load_s(so + q);
     }
     if (PUPDATE && owns && xc >= 1) store_dir(p0 + roff, un);
     x_stage(std::integral_constant<int, 0>{}, un);
@@ -316,7 +323,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       load_row(plane_off(x0 + (s + 2 < xc ? s + 2 : xc)), nxt_u[PS], nxt_p[PS]);
       const unsigned so = plane_off(x0 + (s < xc - 1 ? s : xc - 1)) + roff_ld;
 #pragma unroll
-      for (int q = 0; q < ZPT; ++q) sv[PS][q] = at(S, so + q);
+      for (int q = 0; q < ZPT; ++q) sv[PS][q] = load_s(so + q);
     }
     // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
     if (s >= 1 && owns) {
@@ -465,6 +472,19 @@ static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                \
   hipLaunchKernelGGL((matvec_kernel<T, ZPT, S_, P_, D_, DPP>), grid, dim3(threads), lds, st, a);       \
   return hipGetLastError();
+  if (a.s64 != nullptr) {
+    // the FP64-stiffness read serves the standalone FP32 operator only (no strain, p update or dot)
+    if constexpr (sizeof(T) == 4) {
+      if (!strain && !pupdate && !dot) {
+        if (lds > 65536)
+          (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, false, false, false, DPP, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL((matvec_kernel<T, ZPT, false, false, false, DPP, true>), grid, dim3(threads), lds, st, a);
+        return hipGetLastError();
+      }
+    }
+    return hipErrorInvalidValue;
+  }
   if (strain) {
     NLFH_MV(true, false, false)
   }
