@@ -1039,7 +1039,7 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   p->dir32 = true;
   for (const PrecGroup& g : p->pgroups) {
     p->lean32 = p->lean32 && g.fast != 0;
-    p->dir32 = p->dir32 && g.fast != 0 && g.fast <= 64 && g.vec;
+    p->dir32 = p->dir32 && g.fast != 0 && fast_staged(g.fast, 4) && g.vec;
   }
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);

@@ -22,7 +22,9 @@ struct FastCfg {
   static constexpr int L = N1 > N2 ? N1 : N2;
   static constexpr int NH = N / 2 + 1;
   static constexpr int NP = N / 2;
-  static constexpr int BT = (N == 128 && sizeof(T) == 8) ? 128 : 256;  // threads of passes A, C
+  // threads of passes A, C; FP32 at 128 takes 1024 so that one round of lines covers every row pair of a
+  // plane (the staged passes need that; 139 KiB of LDS, one block per CU)
+  static constexpr int BT = N == 128 ? (sizeof(T) == 8 ? 128 : 1024) : 256;
   static constexpr int LPR = BT / L;                                   // lines per round
   static constexpr int XS = N1 * (N2 + 1) > N2 * (N1 + 1) ? N1 * (N2 + 1) : N2 * (N1 + 1);
   // pass B positions per block: 256 threads (128 for FP64 at 128), one component's exchange buffer in LDS
@@ -42,7 +44,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, LPR = C::LPR, BT = C::BT;
   extern __shared__ double smem_d[];
-  __shared__ double red[4];
+  __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);  // [N]: e^{-2 pi i m / N}
   cplx<T>* plane = tw + N;                            // [N][NH]
   cplx<T>* xch = plane + N * NH;                      // [LPR][XS]
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, LPR = C::LPR, BT = C::BT;
   extern __shared__ double smem_d[];
-  __shared__ double red[4];
+  __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* plane = tw + N;
   cplx<T>* xch = plane + N * NH;
@@ -343,7 +345,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   using V = typename VT::type;
   constexpr int VN = VT::n;
   extern __shared__ double smem_d[];
-  __shared__ double red[4];
+  __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* plane = tw + N;
   cplx<T>* xch = plane + N * NH;
@@ -562,7 +564,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   using V = typename VT::type;
   constexpr int VN = VT::n;
   extern __shared__ double smem_d[];
-  __shared__ double red[4];
+  __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* plane = tw + N;
   cplx<T>* xch = plane + N * NH;
@@ -769,7 +771,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
 template <typename T, int N, bool DOTB>
 __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg,
                                                                   const T* __restrict__ symp) {
-  __shared__ double red[4];
+  __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   double dacc = 0.0;
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB;
@@ -937,6 +939,17 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
 }
 
 bool fast_fft_size(int n) { return n == 16 || n == 32 || n == 48 || n == 64 || n == 128; }
+
+bool fast_staged(int n, int elem_bytes) {
+  switch (n) {
+    case 16: return elem_bytes == 8 ? StageCfg<double, 16>::fits : StageCfg<float, 16>::fits;
+    case 32: return elem_bytes == 8 ? StageCfg<double, 32>::fits : StageCfg<float, 32>::fits;
+    case 48: return elem_bytes == 8 ? StageCfg<double, 48>::fits : StageCfg<float, 48>::fits;
+    case 64: return elem_bytes == 8 ? StageCfg<double, 64>::fits : StageCfg<float, 64>::fits;
+    case 128: return elem_bytes == 8 ? StageCfg<double, 128>::fits : StageCfg<float, 128>::fits;
+    default: return false;
+  }
+}
 
 int fast_line_len(int n, int elem_bytes) {
   switch (n) {

@@ -126,6 +126,7 @@ template <typename T>
 hipError_t launch_prec(const PrecArgs<T>& a, int nplanes, int nlines, int update, int dot, size_t lds_plane,
                        size_t lds_line, hipStream_t st);
 bool fast_fft_size(int n);
+bool fast_staged(int n, int elem_bytes);  // the staged passes A and C (one round of lines) fit this length
 int fast_line_len(int n, int elem_bytes);
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,

@@ -102,15 +102,17 @@ def test_stream_groups_do_not_change_results(gpu_pkg, precision):
         np.testing.assert_array_equal(res[streams][1], res[1][1])
 
 
-def test_fp32_direction_iteration_matches_plain_fp32_schedule(gpu_pkg):
+@pytest.mark.parametrize("cells", [((16, 21), (16, 22), (32, 23), (32, 24), (16, 25)), ((128, 41), (32, 42))])
+def test_fp32_direction_iteration_matches_plain_fp32_schedule(gpu_pkg, cells):
     """The FP32 inner iteration of plans whose cells all take the staged register-FFT passes (search
     direction formed in pass C, residual kept as its (z, y) spectrum, iterate stepped every second
     iteration with the owed step of an even last iteration flushed) runs the same PCG recurrences as the
-    plain FP32 schedule, which a plan takes as soon as one cell (here 12^3) has no register FFT. Per cell,
+    plain FP32 schedule, which a plan takes as soon as one cell (here 12^3) has no register FFT (128^3 cells
+    run the staged passes with 1024-thread blocks). Per cell,
     the mixed solve needs at most 3 % more inner iterations than in the plain schedule (an inner solve that
     lost a step would need extra refinement steps; measured on MI355X: equal or up to 4 % fewer) and C
     agrees to 1e-12 of max|C|."""
-    rhos = [gpu_pkg.cells.random_cell(n, seed)[0] for n, seed in ((16, 21), (16, 22), (32, 23), (32, 24), (16, 25))]
+    rhos = [gpu_pkg.cells.random_cell(n, seed)[0] for n, seed in cells]
     extra = gpu_pkg.cells.random_cell(12, 99)[0]
     shapes = [r.shape for r in rhos]
     lean = gpu_pkg.BatchedHomogenization(shapes, emin=1e-4, tol=1e-8, precision="mixed", streams=1)
