@@ -124,3 +124,21 @@ def test_fp32_direction_iteration_matches_plain_fp32_schedule(gpu_pkg, cells):
         assert itl[i] <= itp[i] + max(8, 0.03 * itp[i]), (i, itl[i], itp[i])
         Ci, Pi = np.asarray(Cl[i]), np.asarray(Cp[i])
         assert np.abs(Ci - Pi).max() < 1e-12 * np.abs(Pi).max()
+
+
+def test_misaligned_128_cell_runs_the_unstaged_passes(gpu_pkg):
+    """A 128^3 cell placed after a 3^3 cell sits at a field offset that is not 16-byte aligned, so its FP32
+    passes run the unstaged register-FFT kernels (1024-thread blocks, like the staged ones) in the plain FP32
+    schedule (the 3^3 cell has no register FFT). Its C and iteration count match the same cell solved alone,
+    aligned, in the staged direction schedule (C to 1e-10 of max|C|, iterations within 3 %)."""
+    small = gpu_pkg.cells.random_cell(3, 51)[0]
+    big = gpu_pkg.cells.random_cell(128, 52)[0]
+    both = gpu_pkg.BatchedHomogenization([small.shape, big.shape], emin=1e-3, tol=1e-8, precision="mixed", streams=1)
+    alone = gpu_pkg.BatchedHomogenization([big.shape], emin=1e-3, tol=1e-8, precision="mixed", streams=1)
+    assert int(both.off[1]) * 3 * 6 % 4 != 0  # the 128^3 cell's fields start off a 16-byte boundary
+    Cb, _, itb = both.C_full([small, big])
+    Ca, _, ita = alone.C_full([big])
+    assert all(both.last_info["converged"]) and all(alone.last_info["converged"])
+    assert abs(int(itb[1]) - int(ita[0])) <= max(8, 0.03 * int(ita[0]))
+    c_b, c_a = np.asarray(Cb[1]), np.asarray(Ca[0])
+    assert np.abs(c_b - c_a).max() < 1e-10 * np.abs(c_a).max()
