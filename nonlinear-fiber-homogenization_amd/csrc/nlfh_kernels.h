@@ -56,15 +56,15 @@ struct PrecArgs {
   const T* sym;          // per shape: [kx][ky][kz][6] real symmetric Kinv / (nx ny nz)
   const T* r;            // pass A input (and pass C for the r . z dot)
   T* r_out;              // pass A fused update: r - alpha Ap written here (may equal r)
-  const T* Ap;           // pass A fused update
-  T* x;                  // pass A fused update: x += alpha p
-  const T* p;            // pass A fused update
+  const T* Ap;           // pass A fused update (spectral update: the plane transformed instead of r)
+  T* x;                  // pass A fused update x += alpha p; direction pass C: the iterate's paired step
+  const T* p;            // pass A fused update; direction pass C: p_k, the old direction
   const double* alpha;   // per (cell, rhs)
   const double* alpha_prev;  // direction pass C, odd iterations: alpha of the previous iteration
   cplx<T>* spec;         // half spectra [cell][rhs][comp][x][ky][kz]
   cplx<T>* rspec;        // nullable: the residual's (z, y) spectrum, kept across iterations; when set, pass A
                          // writes it (not spec) and pass B reads it and writes spec
-  T* z;                  // pass C output
+  T* z;                  // pass C output (direction pass C: p_{k+1}; odd k first reads p_{k-1} there)
   double* part_a;        // pass A: partial ||r||^2 per (rhs, plane item)
   double* part_c;        // pass C: partial r . z per (rhs, plane item)
   double* part_bdot;     // pass B (spectral dot): partial r . z per (rhs, line item)
