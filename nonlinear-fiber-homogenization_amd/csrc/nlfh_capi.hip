@@ -600,7 +600,7 @@ VecArgs vec_args(nlfh_plan* p, int m, const int* done) {
 // from the spectra in pass B (Parseval), so pass C writes z without reading r. With the staged passes
 // (dir32) the residual is kept as its (z, y) spectrum r-hat and the search direction is formed in
 // pass C instead of the matvec: iteration k runs
-//   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  alpha_k  ->  pass A r-hat -= alpha_k FFT_zy(Ap_k), ||r||
+//   matvec Ap_k = K p_k (+ p_k . Ap_k)  ->  pass A alpha_k, r-hat -= alpha_k FFT_zy(Ap_k), ||r||
 //   ->  pass B (r-hat in, z-hat out)  ->  pass C p_{k+1} = z_{k+1} + beta_k p_k
 //   (odd k also x += alpha_{k-1} p_{k-1} + alpha_k p_k)  ->  beta / convergence,
 // so neither r nor z is stored in real space (two field passes less than the plain order), the matvec
@@ -622,7 +622,7 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
     TRY(run_matvec<T>(p, p_old, nullptr, nullptr, w.Ap, w.s, nullptr, p->d_part_mv, w.cs.done, m, 0, 0, 1, T(1), st));
   else
     TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
-  HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
+  if (!dir) HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
   PrecArgs<T> a = prec_args<T>(p, m);
   a.r = w.R;
   a.r_out = w.R;
@@ -642,6 +642,11 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
     a.line_range = p->d_line_range32;
     a.rz = w.cs.rz;
     a.alpha_prev = w.cs.alpha_prev;
+    a.part_mv = p->d_part_mv;  // pass A forms and stores alpha (no alpha kernel)
+    a.mv_range = p->d_mv_range;
+    a.nmv = (int)p->mv.size();
+    a.alpha_out = w.cs.alpha;
+    a.alpha_prev_out = w.cs.alpha_prev;
   }
   // cur == k & 1: every graph holds an even number of iterations
   TRY(run_prec<T>(p, a, dir ? 4 : 2, dir ? (cur ? 4 : 3) : (ln ? 2 : 1), st));

@@ -331,8 +331,10 @@ struct StageCfg {
 // UPD: 0 none; 1 r -= alpha Ap; 2 also x += alpha p; 4 spectral residual (FP32 inner iteration): the
 // plane of Ap is transformed and the residual's (z, y) spectrum takes the update in place,
 // r-hat -= alpha Ap-hat, with ||r||^2 from that spectrum by Parseval (weight 1 at kz = 0 and N/2, 2
-// elsewhere, over N^2), so r itself is never read or written. Without UPD 4, the plane's spectrum goes
-// to a.rspec when that is set (the start of such an iteration), else to a.spec.
+// elsewhere, over N^2), so r itself is never read or written; alpha = (r . z) / (p . Ap) is formed by every
+// block from the matvec's partials with the fixed-order sum and guard of cg_alpha_kernel (bitwise its
+// value), and the block of plane 0, component 0 stores it for pass C. Without UPD 4, the plane's spectrum
+// goes to a.rspec when that is set (the start of such an iteration), else to a.spec.
 template <typename T, int N, int UPD>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   constexpr bool UPDATE = UPD == 1 || UPD == 2;
@@ -360,7 +362,20 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   for (int i = tid; i < N; i += BT) tw[i] = twg[i];
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   T alpha = T(0);
-  if (UPDATE || SPEC) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  __shared__ double alpha_sh;
+  if (SPEC && tid < 64) {
+    const int k = pi.cell * a.m + rhs;
+    const double pAp = wave_sum(a.part_mv, a.nmv, rhs, a.mv_range, pi.cell);
+    const double al = a.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
+    if (tid == 0) {
+      alpha_sh = al;
+      if (pi.x == 0 && pi.comp == 0) {  // no other block of this pass reads the stored values
+        a.alpha_prev_out[k] = a.alpha_out[k];
+        a.alpha_out[k] = al;
+      }
+    }
+  }
   double acc = 0.0;
   // SPEC: this plane's residual spectrum is loaded now, into registers, so that its latency passes
   // during the transforms (it is consumed by the final update)
@@ -517,6 +532,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
     const V* src = reinterpret_cast<const V*>(plane);
     V* dst = rsp;
     if constexpr (SPEC) {
+      alpha = (T)alpha_sh;  // written before the barriers above
       double sacc = 0.0;
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {

@@ -70,6 +70,13 @@ struct PrecArgs {
   double* part_bdot;     // pass B (spectral dot): partial r . z per (rhs, line item)
   const int* line_range; // direction pass C: per cell [begin, end) of its line items (part_bdot columns)
   const double* rz;      // direction pass C: r . z of the previous iteration per (cell, rhs)
+  // spectral pass A forms alpha itself from the matvec's p . Ap partials and stores it (and the previous
+  // alpha) from one block per (cell, rhs)
+  const double* part_mv; // p . Ap partials per (rhs, matvec item)
+  const int* mv_range;   // per cell [begin, end) of its matvec items
+  double* alpha_out;     // receives alpha per (cell, rhs)
+  double* alpha_prev_out;  // receives the alpha it replaces
+  int nmv;               // matvec items (row stride of part_mv)
   const int* done;       // per-cell early exit (nullable)
   int m;
   int nplanes;           // total plane items (row stride of part_a / part_c)
