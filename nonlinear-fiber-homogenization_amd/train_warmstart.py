@@ -8,9 +8,11 @@ Cells are the seeded random cubic-symmetric cells of `cells.random_cell` (traini
 `--seed0`, held-out seeds from `--heldout-seed0`; the ranges must not overlap). Their densities
 are generated by a pool of
 CPU workers while the GPU solves; training targets are the six unit-strain correctors of the batched
-mixed-precision solver at `--data-tol`. Evaluation solves every held-out cell twice at `--tol`, from
-zero and from the network's prediction (inference time included in the warm timing), and reports
-iterations, wall-clock solves/s and the largest C difference between the two.
+mixed-precision solver at `--data-tol`. Evaluation solves every held-out cell at `--tol` (or at each of `--tols`) from zero, from the
+network's prediction combined by energy-optimal Ritz weights (inference time included in the warm timing)
+and from the raw prediction, and reports iterations, wall-clock solves/s and the largest C difference; for
+the first `--start-diagnostics` cells it also reports how good a start the prediction is (relative
+residual, which the stopping rule measures, and relative energy-norm error, per right-hand side).
 
 The reference has no learned warm start: it restarts each solve from the previous corrector
 (nlfiber/fiber.py:41-56) through the u0 argument (nlfiber/linear.py:212-216) that this uses.
@@ -65,6 +67,79 @@ def to_target(u_packed, B, n):
     return (u / n).float()
 
 
+def evaluate(net, held, n, bs, emin, tol):
+    """Zero start against the learned start (prediction + Ritz weights) and the raw prediction as u0, on
+    the held-out cells in batches of `bs`, mixed precision to relative tolerance `tol`."""
+    import torch
+    from nlfh_amd import BatchedHomogenization
+
+    def plans(B):
+        kw = dict(emin=emin, tol=tol, precision="mixed")
+        return (BatchedHomogenization([(n, n, n)] * B, **kw),
+                BatchedHomogenization([(n, n, n)] * B, warm_start=net, **kw))
+
+    cold_it, warm_it, raw_it, cold_t, warm_t, dC, fallbacks = [], [], [], 0.0, 0.0, [], 0
+    bh, bw = plans(bs)
+    for k in range(0, len(held), bs):
+        buf = [rho for _, rho in held[k:k + bs]]
+        B = len(buf)
+        if B != bs:
+            bh, bw = plans(B)
+        rho_d = bh.pack(buf)
+        torch.cuda.synchronize()
+        t = time.time()
+        C0, _, it0 = bh.C_full(rho_d)
+        torch.cuda.synchronize()
+        cold_t += time.time() - t
+        t = time.time()
+        C1, _, it1 = bw.C_full(rho_d)  # prediction + Ritz weights + solve
+        torch.cuda.synchronize()
+        warm_t += time.time() - t
+        fallbacks += int(bw.last_info["fallback"]) * B
+        # the unprojected prediction as u0, for comparison
+        _, _, it2 = bh.C_full(rho_d, u0=pack_prediction(net.predict(rho_d.view(B, n, n, n))))
+        cold_it.extend(it0.tolist())
+        warm_it.extend(it1.tolist())
+        raw_it.extend(it2.tolist())
+        c0, c1 = np.asarray(C0), np.asarray(C1)
+        dC.append(float(np.max(np.abs(c1 - c0) / np.abs(c0).max(axis=(1, 2), keepdims=True))))
+        _log(f"tol {tol:g} eval {k + B}/{len(held)}: mean it cold {np.mean(cold_it):.1f} learned+ritz "
+             f"{np.mean(warm_it):.1f} learned raw {np.mean(raw_it):.1f}")
+    return dict(cold_mean_it=float(np.mean(cold_it)), warm_mean_it=float(np.mean(warm_it)),
+                raw_mean_it=float(np.mean(raw_it)), cold_solves_per_s=len(held) / cold_t,
+                warm_solves_per_s=len(held) / warm_t, it_ratio=float(np.mean(warm_it) / np.mean(cold_it)),
+                max_rel_dC=float(max(dC)), fallback_cells=fallbacks, cold_seconds=cold_t, warm_seconds=warm_t,
+                warm_better_frac=float(np.mean(np.asarray(warm_it) < np.asarray(cold_it))))
+
+
+def start_quality(net, rho, emin):
+    """How good a start the prediction is, per right-hand side: the relative residual ||b - K x0|| / ||b|| (a
+    zero start has 1, and the stopping rule measures this norm) and the relative energy-norm error
+    sqrt(e.K e / u.K u), e = x0 - u with u the converged corrector (a zero start has 1), for the Ritz-combined
+    start and for the raw prediction."""
+    import torch
+    from nlfh_amd import BatchedHomogenization, LinearHomogenization
+    n = rho.shape[0]
+    lh = LinearHomogenization((n, n, n), emin=emin, tol=1e-10, precision="fp64")
+    bw = BatchedHomogenization([(n, n, n)], emin=emin, tol=1e-10, precision="fp64", warm_start=net)
+    packed = bw.pack([rho])
+    ritz = bw.field_views(bw.warm_start_fields(packed, bw.stiffness_scale(packed)))[0]
+    raw = net.predict(rho.unsqueeze(0))[0]
+    s = lh.stiffness_scale(rho)
+    eye = np.eye(6)
+    bnorm = lh.load(s, eye).reshape(6, -1).norm(dim=1)
+    _, u, _ = lh.C_full(rho)
+    uKu = (u.reshape(6, -1) * lh.apply_K(s, u).reshape(6, -1)).sum(1)
+    out = {}
+    for name, x in (("ritz", ritz), ("raw", raw)):
+        res = lh.residual(s, eye, x).reshape(6, -1).norm(dim=1) / bnorm
+        e = x - u
+        eKe = (e.reshape(6, -1) * lh.apply_K(s, e).reshape(6, -1)).sum(1)
+        out[name] = dict(residual=[round(float(v), 4) for v in res],
+                         energy_error=[round(float(v), 4) for v in torch.sqrt(eKe / uKu)])
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=64)
@@ -85,6 +160,10 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="write the trained weights here (npz)")
     ap.add_argument("--report", default=None, help="write the evaluation here (json)")
     ap.add_argument("--skip-train", action="store_true")
+    ap.add_argument("--tols", type=float, nargs="+", default=None,
+                    help="evaluate the held-out cells at each of these tolerances (default: --tol)")
+    ap.add_argument("--start-diagnostics", type=int, default=4,
+                    help="held-out cells whose starting residual and energy error are reported")
     args = ap.parse_args(argv)
 
     # the package (no GPU call yet), then the worker pool: the workers only run numpy and must not
@@ -162,51 +241,16 @@ def main(argv=None):
     # ---- evaluation on held-out cells: zero start vs learned start --------------------------------
     if args.heldout > 0:
         seeds = np.arange(args.heldout_seed0, args.heldout_seed0 + args.heldout)
-
-        def plans(B):
-            kw = dict(emin=args.emin, tol=args.tol, precision="mixed")
-            return (BatchedHomogenization([(n, n, n)] * B, **kw),
-                    BatchedHomogenization([(n, n, n)] * B, warm_start=net, **kw))
-
-        bh, bw = plans(bs)
-        cold_it, warm_it, raw_it, cold_t, warm_t, dC, fallbacks = [], [], [], 0.0, 0.0, [], 0
-        buf, done = [], 0
-        for seed, rho, _ in cell_stream(pool, n, seeds):
-            buf.append(rho)
-            if len(buf) < bs and done + len(buf) < args.heldout:
-                continue
-            B = len(buf)
-            if B != bs:
-                bh, bw = plans(B)
-            rho_d = bh.pack(buf)
-            torch.cuda.synchronize()
-            t = time.time()
-            C0, _, it0 = bh.C_full(rho_d)
-            torch.cuda.synchronize()
-            cold_t += time.time() - t
-            t = time.time()
-            C1, _, it1 = bw.C_full(rho_d)  # prediction + Ritz weights + solve
-            torch.cuda.synchronize()
-            warm_t += time.time() - t
-            fallbacks += int(bw.last_info["fallback"]) * B
-            # the unprojected prediction as u0, for comparison
-            _, _, it2 = bh.C_full(rho_d, u0=pack_prediction(net.predict(rho_d.view(B, n, n, n))))
-            cold_it.extend(it0.tolist())
-            warm_it.extend(it1.tolist())
-            raw_it.extend(it2.tolist())
-            c0, c1 = np.asarray(C0), np.asarray(C1)
-            dC.append(float(np.max(np.abs(c1 - c0) / np.abs(c0).max(axis=(1, 2), keepdims=True))))
-            done += B
-            buf = []
-            _log(f"eval {done}/{args.heldout}: mean it cold {np.mean(cold_it):.1f} learned+ritz "
-                 f"{np.mean(warm_it):.1f} learned raw {np.mean(raw_it):.1f}")
-        report.update(
-            cold_mean_it=float(np.mean(cold_it)), warm_mean_it=float(np.mean(warm_it)),
-            raw_mean_it=float(np.mean(raw_it)),
-            cold_solves_per_s=args.heldout / cold_t, warm_solves_per_s=args.heldout / warm_t,
-            it_ratio=float(np.mean(warm_it) / np.mean(cold_it)), max_rel_dC=float(max(dC)),
-            fallback_cells=fallbacks, cold_seconds=cold_t, warm_seconds=warm_t,
-            warm_better_frac=float(np.mean(np.asarray(warm_it) < np.asarray(cold_it))))
+        held = [(seed, rho) for seed, rho, _ in cell_stream(pool, n, seeds)]
+        report["by_tol"] = {}
+        for tol in (args.tols or [args.tol]):
+            r = evaluate(net, held, n, bs, args.emin, tol)
+            report["by_tol"][f"{tol:g}"] = r
+            if tol == args.tol:
+                report.update(r)
+            _log(f"tol {tol:g}: " + json.dumps(r))
+        report["start_quality"] = [start_quality(net, torch.tensor(rho, device="cuda"), args.emin)
+                                   for _, rho in held[:args.start_diagnostics]]
         _log(json.dumps(report))
     if args.report:
         with open(args.report, "w") as f:
