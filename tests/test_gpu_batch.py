@@ -142,3 +142,28 @@ def test_misaligned_128_cell_runs_the_unstaged_passes(gpu_pkg):
     assert abs(int(itb[1]) - int(ita[0])) <= max(8, 0.03 * int(ita[0]))
     c_b, c_a = np.asarray(Cb[1]), np.asarray(Ca[0])
     assert np.abs(c_b - c_a).max() < 1e-10 * np.abs(c_a).max()
+
+
+def test_mixed_precision_with_two_and_three_right_hand_sides(gpu_pkg):
+    """The FP32 direction iteration with fewer right-hand sides than the plan's layout (per-(cell, rhs)
+    scalars and partials at stride m): the cubic invariants' two cases (m = 2, as Fiber solves them) and a
+    three-strain solve (m = 3) on a 32^3 cell agree with the FP64 solve and converge in at most 25 % more
+    inner iterations. Both stop at residual 1e-9 (true and recursive residual), so the correctors may differ
+    by the conditioning times that (bound: 1e-4 of their largest entry; a stride mistake gives O(1)); the
+    invariants' error is quadratic in the correctors' (bound: 1e-8)."""
+    rho = gpu_pkg.cells.random_cell(32, 61)[0]
+    mixed = gpu_pkg.LinearHomogenization(rho.shape, emin=1e-4, tol=1e-9, precision="mixed")
+    fp64 = gpu_pkg.LinearHomogenization(rho.shape, emin=1e-4, tol=1e-9)
+    cm, um, itm = mixed.invariants(rho)
+    assert mixed.last_info["converged"] and not mixed.last_info["fallback_fp64"]
+    cd, ud, itd = fp64.invariants(rho)
+    np.testing.assert_allclose(cm.detach().cpu().numpy(), cd.detach().cpu().numpy(), rtol=1e-8)
+    assert (um - ud).abs().max() < 1e-4 * ud.abs().max()
+    assert itm <= 1.25 * itd + 10
+    Es = np.stack([gpu_pkg.unit_strain(0), gpu_pkg.unit_strain(2), gpu_pkg.unit_strain(4)])
+    s = fp64.stiffness_scale(rho)
+    u3m, it3m = mixed.solve(s, Es)
+    assert mixed.last_info["converged"]
+    u3d, it3d = fp64.solve(s, Es)
+    assert (u3m - u3d).abs().max() < 1e-4 * u3d.abs().max()
+    assert it3m <= 1.25 * it3d + 10
