@@ -166,8 +166,10 @@ __global__ void __launch_bounds__(1024) dm_moments_kernel(const double* __restri
 // shrink the bracket below double resolution; then the same 3 Newton steps. stats: [2] eta, [3] lo,
 // [4] hi.
 constexpr int kEtaCand = 64;  // candidates per pass (blockDim of the partial kernel is kEtaCand * kEtaLanes)
-constexpr int kEtaLanes = 4;
-constexpr int kEtaVox = 2048;  // voxels per block of the partial kernel
+// 16 lanes per candidate and 128 voxels per block: 8 sigmoids per thread, 256 blocks for a 32^3 cell (the
+// search is latency-bound: FP64 tanh per voxel and candidate, so it needs the whole GPU's threads)
+constexpr int kEtaLanes = 16;
+constexpr int kEtaVox = 128;  // voxels per block of the partial kernel
 
 // stats[3], stats[4] = min(u) - 40 / beta, max(u) + 40 / beta (one block)
 __global__ void __launch_bounds__(1024) dm_bracket_kernel(const double* __restrict__ u, int64_t N, double beta,
@@ -210,17 +212,26 @@ __global__ void __launch_bounds__(kEtaCand* kEtaLanes) dm_eta_partial_kernel(con
   }
 }
 
-// volumes of the candidates (fixed-order sums over the blocks); the new bracket is the sub-interval
-// where the volume crosses v (the volume decreases with eta). One block of kEtaCand threads.
-__global__ void dm_eta_select_kernel(const double* part, int nblocks, int64_t N, double v, double* stats) {
+// volumes of the candidates (fixed-order sums over the blocks: lane q of candidate j sums blocks q, q + 16,
+// ..., then the 16 lane sums are added in order); the new bracket is the sub-interval where the volume
+// crosses v (the volume decreases with eta). One block of kEtaCand x kEtaLanes threads.
+__global__ void __launch_bounds__(kEtaCand* kEtaLanes) dm_eta_select_kernel(const double* part, int nblocks,
+                                                                            int64_t N, double v, double* stats) {
   __shared__ int above[kEtaCand];
-  const int j = threadIdx.x;
+  __shared__ double acc[kEtaLanes][kEtaCand];
+  const int j = threadIdx.x % kEtaCand, q = threadIdx.x / kEtaCand;
   const double lo = stats[3], hi = stats[4];
-  double t = 0.0;
-  for (int b = 0; b < nblocks; ++b) t += part[(int64_t)b * kEtaCand + j];
-  above[j] = t / (double)N > v ? 1 : 0;
+  double s = 0.0;
+  for (int b = q; b < nblocks; b += kEtaLanes) s += part[(int64_t)b * kEtaCand + j];
+  acc[q][j] = s;
   __syncthreads();
-  if (j == 0) {
+  if (q == 0) {
+    double t = 0.0;
+    for (int r = 0; r < kEtaLanes; ++r) t += acc[r][j];
+    above[j] = t / (double)N > v ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
     int k = 0;  // number of candidates with volume above v: the root lies after candidate k - 1
     while (k < kEtaCand && above[k]) ++k;
     const double nlo = k == 0 ? lo : eta_candidate(lo, hi, k - 1);
@@ -252,15 +263,22 @@ __global__ void __launch_bounds__(256) dm_newton_partial_kernel(const double* __
   }
 }
 
-// one Newton step eta -= (mean s - v) / (-mean ds) (guard as the reference), one thread
-__global__ void dm_newton_step_kernel(const double* part, int nblocks, int64_t N, double v, double* stats) {
+// one Newton step eta -= (mean s - v) / (-mean ds) (guard as the reference), one block
+// (fixed-order sums: thread t adds blocks t, t + 256, ..., then block_sum)
+__global__ void __launch_bounds__(256) dm_newton_step_kernel(const double* part, int nblocks, int64_t N, double v,
+                                                             double* stats) {
+  __shared__ double red[4];
   double s = 0.0, ds = 0.0;
-  for (int b = 0; b < nblocks; ++b) {
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
     s += part[2 * (int64_t)b];
     ds += part[2 * (int64_t)b + 1];
   }
-  const double dv = -ds / (double)N;
-  stats[2] = stats[2] - (s / (double)N - v) / (dv != 0.0 ? dv : -1.0);
+  const double ts = block_sum(s, red);
+  const double tds = block_sum(ds, red);
+  if (threadIdx.x == 0) {
+    const double dv = -tds / (double)N;
+    stats[2] = stats[2] - (ts / (double)N - v) / (dv != 0.0 ? dv : -1.0);
+  }
 }
 
 // rho = sigmoid(beta (u - eta)), d = rho (1 - rho)
@@ -387,13 +405,13 @@ int dm_forward(nlfh_dmap* dm, const double* phi, double beta, double* rho, hipSt
     for (int pass = 0; pass < 10; ++pass) {
       hipLaunchKernelGGL(dm_eta_partial_kernel, dim3(nb), dim3(kEtaCand * kEtaLanes), 0, st, dm->d_u, N, beta,
                          dm->d_stats, dm->d_part);
-      hipLaunchKernelGGL(dm_eta_select_kernel, dim3(1), dim3(kEtaCand), 0, st, dm->d_part, nb, N, dm->v,
-                         dm->d_stats);
+      hipLaunchKernelGGL(dm_eta_select_kernel, dim3(1), dim3(kEtaCand * kEtaLanes), 0, st, dm->d_part, nb, N,
+                         dm->v, dm->d_stats);
     }
     for (int it = 0; it < 3; ++it) {
       hipLaunchKernelGGL(dm_newton_partial_kernel, dim3(nb), dim3(256), 0, st, dm->d_u, N, beta, dm->d_stats,
                          dm->d_part);
-      hipLaunchKernelGGL(dm_newton_step_kernel, dim3(1), dim3(1), 0, st, dm->d_part, nb, N, dm->v, dm->d_stats);
+      hipLaunchKernelGGL(dm_newton_step_kernel, dim3(1), dim3(256), 0, st, dm->d_part, nb, N, dm->v, dm->d_stats);
     }
     hipLaunchKernelGGL(dm_sigmoid_kernel, dim3(grid_for(N)), dim3(256), 0, st, dm->d_u, N, beta, dm->d_stats, rho,
                        dm->d_d);
