@@ -1,0 +1,43 @@
+"""Host-side logic that needs no GPU: the packaged gfx950 tiling table and the batched-tuning keys."""
+import ast
+import json
+import os
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TABLE = os.path.join(ROOT, "nonlinear-fiber-homogenization_amd", "data", "tiling_gfx950.json")
+
+
+def _entries():
+    with open(TABLE) as f:
+        table = json.load(f)
+    for key, e in table.items():
+        parts = key.split("|")
+        shape = ast.literal_eval(parts[0])
+        yield key, tuple(shape), parts[1], int(parts[2]), len(parts) == 4 and parts[3] == "batch", e
+
+
+def test_packaged_tilings_are_valid_candidates(pkg):
+    """Every tiling the package ships is one the autotuner could have picked for that shape (tile rows,
+    x-chunk and z values per lane that fit the 512-thread block and the LDS of the FP64 kernels)."""
+    from nlfh_amd import autotune as at
+    n = 0
+    for key, shape, prec, m, batched, e in _entries():
+        assert prec in ("f32", "f64") and m == 6, key
+        assert tuple(e["best"]) in at.candidates(shape, 8), key
+        for ru in e.get("runner_up", []):
+            assert tuple(ru) in at.candidates(shape, 8), key
+        n += 1
+    assert n >= 10
+
+
+def test_batched_keys_only_for_shapes_that_batch(pkg):
+    """Batched entries exist only for shapes whose representative batch has several cells (a 128^3 cell fills
+    the GPU alone and keeps its single-cell entry), and the key format is the autotuner's."""
+    from nlfh_amd import autotune as at
+    for key, shape, prec, m, batched, e in _entries():
+        assert key == at._key(shape, prec, m, batched)
+        if batched:
+            assert at.batch_cells(shape) > 1, key
+            assert e.get("cells") in (None, at.batch_cells(shape)), key
+    assert at.batch_cells((128, 128, 128)) == 1
+    assert at.batch_cells((64, 64, 64)) == 8
