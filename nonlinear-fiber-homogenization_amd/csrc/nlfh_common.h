@@ -69,7 +69,17 @@ __device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) { return {a.re - b
 __device__ __forceinline__ double wave_sum(const double* part, int nitems, int row, const int* range, int cell) {
   const int lane = threadIdx.x & 63;
   double acc = 0.0;
-  for (int w = range[2 * cell] + lane; w < range[2 * cell + 1]; w += 64) acc += part[(int64_t)row * nitems + w];
+  const int end = range[2 * cell + 1];
+  const double* pr = part + (int64_t)row * nitems;
+  // four strides' loads in flight at once; the additions keep the order of the plain strided loop
+  for (int w0 = range[2 * cell] + lane; w0 < end; w0 += 256) {
+    double v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = w0 + 64 * u < end ? pr[w0 + 64 * u] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (w0 + 64 * u < end) acc += v[u];
+  }
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   return acc;
 }

@@ -344,7 +344,6 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   constexpr int RS = StageCfg<T, N>::RS;
   static_assert(StageCfg<T, N>::fits, "staged pass A needs one round of lines and room for the plane");
   using VT = Vec16<T>;
-  using V = typename VT::type;
   constexpr int VN = VT::n;
   extern __shared__ double smem_d[];
   __shared__ double red[16];  // one per wavefront (up to 1024 threads)
@@ -359,8 +358,27 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   const CellDesc cd = a.cells[pi.cell];
   const int rhs = blockIdx.y;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  // twiddles: loaded into a register here, stored to LDS with the plane (one wait for all loads)
+  static_assert(BT >= N, "one twiddle per thread");
+  cplx<T> twv{};
+  if (tid < N) twv = twg[tid];
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
+  using V = typename VT::type;
+  constexpr int NV = N * N / VN;
+  // without the real-space update (UPD 0 and 4) a thread's whole share of the plane is loaded into
+  // registers first, so that its loads are in flight together (a plain loop waits for each load before
+  // the LDS store that follows it); they are issued ahead of the other loads, which are consumed later
+  constexpr bool BATCH = !UPDATE;
+  constexpr int LPT = BATCH ? (NV + BT - 1) / BT : 1;
+  const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + fo);
+  V lbuf[LPT];
+  if constexpr (BATCH) {
+#pragma unroll
+    for (int j = 0; j < LPT; ++j) {
+      const int i = tid + j * BT;
+      if (i < NV) lbuf[j] = Rv[i];
+    }
+  }
   T alpha = T(0);
   if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
   __shared__ double alpha_sh;
@@ -394,15 +412,30 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   }
   // load (and update) the plane with 16-byte vectors into rp
   {
-    const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + fo);
     V* ROv = reinterpret_cast<V*>(a.r_out + fo);
     const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
     V* Xv = reinterpret_cast<V*>(a.x + fo);
     const V* Pv = reinterpret_cast<const V*>(a.p + fo);
-    constexpr int NV = N * N / VN;
-    for (int i = tid; i < NV; i += BT) {
-      V rv = Rv[i];
-      if (UPDATE) {
+    if (tid < N) tw[tid] = twv;
+    // vector i of the plane into the staged real plane (and its norm)
+    auto put = [&](int i, const V& rv) {
+      const int e = i * VN;
+      const int y = e / N, z = e - (e / N) * N;
+      T* dst = rp + y * RS + z;
+#pragma unroll
+      for (int k = 0; k < VN; ++k) {
+        const T v = VT::get(rv, k);
+        if (!SPEC) acc += (double)v * (double)v;
+        dst[k] = v;
+      }
+    };
+    if constexpr (BATCH) {
+#pragma unroll
+      for (int j = 0; j < LPT; ++j)
+        if (tid + j * BT < NV) put(tid + j * BT, lbuf[j]);
+    } else {
+      for (int i = tid; i < NV; i += BT) {
+        V rv = Rv[i];
         const V ap = APv[i];
         if constexpr (VN == 4) {
           rv.x -= alpha * ap.x; rv.y -= alpha * ap.y; rv.z -= alpha * ap.z; rv.w -= alpha * ap.w;
@@ -420,15 +453,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
           }
           Xv[i] = xv;
         }
-      }
-      const int e = i * VN;
-      const int y = e / N, z = e - (e / N) * N;
-      T* dst = rp + y * RS + z;
-#pragma unroll
-      for (int k = 0; k < VN; ++k) {
-        const T v = VT::get(rv, k);
-        if (!SPEC) acc += (double)v * (double)v;
-        dst[k] = v;
+        put(i, rv);
       }
     }
   }
@@ -591,7 +616,25 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   const CellDesc cd = a.cells[pi.cell];
   const int rhs = blockIdx.y;
   const int tid = threadIdx.x;
-  for (int i = tid; i < N; i += BT) tw[i] = twg[i];
+  // twiddles: loaded into a register here, stored to LDS with the plane (one wait for all loads)
+  static_assert(BT >= N, "one twiddle per thread");
+  cplx<T> twv{};
+  if (tid < N) twv = twg[tid];
+  // up to 64 points a thread's share of the spectrum plane is loaded into registers first, ahead of the
+  // other loads (its loads in flight together; a plain loop waits for each load before the LDS store that
+  // follows it); at 128 the direction variants have no registers to spare and keep the plain loop
+  constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
+  constexpr int NVS = N * NH / CPV;
+  constexpr bool SBATCH = N <= 64;
+  constexpr int SPT = SBATCH ? (NVS + BT - 1) / BT : 1;
+  const V* const ssrc = reinterpret_cast<const V*>(a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec +
+                                                   (int64_t)pi.x * N * NH);
+  V sbuf[SPT];
+  if constexpr (SBATCH) {
+#pragma unroll
+    for (int j = 0; j < SPT; ++j)
+      if (tid + j * BT < NVS) sbuf[j] = ssrc[tid + j * BT];
+  }
   __shared__ double beta_sh;
   if (DIR && tid < 64) {
     const double rz1 = wave_sum(a.part_bdot, a.nlines, rhs, a.line_range, pi.cell);
@@ -622,12 +665,22 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     }
   }
   {
-    const cplx<T>* so = a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH;
-    constexpr int CPV = 16 / sizeof(cplx<T>);
-    constexpr int NVS = N * NH / CPV;
-    const V* src = reinterpret_cast<const V*>(so);
     V* dst = reinterpret_cast<V*>(plane);
-    for (int i = tid; i < NVS; i += BT) dst[i] = src[i];
+    if (tid < N) tw[tid] = twv;
+    if constexpr (SBATCH) {
+#pragma unroll
+      for (int j = 0; j < SPT; ++j)
+        if (tid + j * BT < NVS) {
+          V* d = dst + tid + j * BT;
+          if constexpr (CPV == 2) {
+            d->x = sbuf[j].x; d->y = sbuf[j].y; d->z = sbuf[j].z; d->w = sbuf[j].w;
+          } else {
+            d->x = sbuf[j].x; d->y = sbuf[j].y;
+          }
+        }
+    } else {
+      for (int i = tid; i < NVS; i += BT) dst[i] = ssrc[i];
+    }
   }
   const int l = tid / L, t = tid - (tid / L) * L;
   __syncthreads();
