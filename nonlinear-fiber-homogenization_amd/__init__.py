@@ -5,6 +5,7 @@ The directory name is not a Python identifier; load it under the module name `nl
   LinearHomogenization, Plan            (linear.py: the reference's class on libnlfh.so)
   BatchedHomogenization                 (batch.py: many cells x 6 load cases per launch)
   cells                                 (synthetic cubic-symmetric unit cells)
+  mech                                  (finite-strain Cell / Mixed Newton-CG: nlfiber/mech.py)
   cubic_invariants, cubic_leakage, hashin_shtrikman, iso_D, lame, unit_strain, ...
 """
 from . import cells  # noqa: F401
@@ -17,4 +18,4 @@ from .linear import (  # noqa: F401
 __version__ = "0.1.0"
 from .batch import BatchedHomogenization  # noqa: E402,F401
 from . import learned  # noqa: E402,F401
-from . import density, fiber  # noqa: E402,F401
+from . import density, fiber, mech  # noqa: E402,F401

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libnlfh.so")
 SOURCES = ["nlfh_matvec.hip", "nlfh_fft.hip", "nlfh_fft_fast.hip", "nlfh_blas.hip", "nlfh_capi.hip", "nlfh_net.hip",
-           "nlfh_density.hip"]
+           "nlfh_density.hip", "nlfh_finite.hip"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = [

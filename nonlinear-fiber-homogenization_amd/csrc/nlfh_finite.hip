@@ -1,0 +1,435 @@
+// Finite-strain cell operators on voxel grids (FP64): the cell energy, its gradient in the periodic
+// displacement w and in the mean deformation gradient Fbar, and the Hessian-vector product with the
+// tangent at a state, applied matrix-free.
+//
+// Follows nlfiber/mech.py: Q1 hexahedra with 2x2x2 Gauss points (mech.py:124-132, the element
+// gradients of linear.py:33-42), the energy W = wq sum_e s_e sum_g psi_eff(Fbar + H_g) with wq = 1/(8N)
+// (mech.py:141-154), the Wang-Lazarov-Sigmund-Jensen void interpolation psi_eff (mech.py:134-139,
+// 107-112), the neo-Hookean and linear materials (mech.py:39-50) and the stored-tangent product of
+// Mixed.hvp_A (mech.py:227-233; tangent_g mech.py:156-167).
+//
+// The reference stores the 3x3x3x3 tangent of every Gauss point (81 doubles x 8 points per voxel) and
+// contracts it in each Hessian-vector product; here the product recomputes the state's deformation
+// gradient from w and applies the tangent in closed form,
+//   neo-Hookean:  dP = mu dF + lam tr(F^-1 dF) F^-T + (mu - lam ln J) (F^-1 dF F^-1)^T
+//   linear:       dP = lam tr(dF) I + mu (dF + dF^T),
+// with the void interpolation dP = gamma^2 (dP_nl(Fg) - dP_lin) + dP_lin, Fg = I + gamma (F - I), so
+// each product reads two displacement fields instead of 5 KiB of tangent per voxel.
+//
+// Layout: w and v are the reference's (nx, ny, nz, 3) C-order arrays (component fastest); element e =
+// voxel (i, j, k), e = (i ny + j) nz + k, spans nodes (i + a, j + b, k + c) mod n, node index
+// a + 2b + 4c. One thread per element writes its 24 nodal forces to a (24, N) buffer; a second kernel
+// gathers the 8 contributions of every node in a fixed order, so results are deterministic.
+#include <hip/hip_runtime.h>
+
+#include "nlfh_common.h"
+
+namespace nlfh {
+namespace {
+
+constexpr int kFsThreads = 256;
+constexpr int kFsParts = 10;  // per-block partial sums: energy (or 0), then the 9 entries of sum P (or sum dP)
+
+constexpr double kGaussOff = 0.28867513459481288225;  // 1 / (2 sqrt 3)
+
+__host__ __device__ constexpr double gauss_coord(int bit) { return bit ? 0.5 + kGaussOff : 0.5 - kGaussOff; }
+
+// d N_a / d x_d at Gauss point g of the unit element (linear.py:44-53): g and a count x fastest
+__host__ __device__ constexpr double dshape(int g, int a, int d) {
+  const double x = gauss_coord(g & 1), y = gauss_coord((g >> 1) & 1), z = gauss_coord((g >> 2) & 1);
+  const int i = a & 1, j = (a >> 1) & 1, k = (a >> 2) & 1;
+  const double fx = i ? x : 1.0 - x, fy = j ? y : 1.0 - y, fz = k ? z : 1.0 - z;
+  const double sx = i ? 1.0 : -1.0, sy = j ? 1.0 : -1.0, sz = k ? 1.0 : -1.0;
+  return d == 0 ? sx * fy * fz : (d == 1 ? fx * sy * fz : fx * fy * sz);
+}
+
+struct FsMat {
+  double lam, mu;
+  double emin, p;     // s = emin + (1 - emin) clip01(rho)^p
+  double wls_b, wls_r0;
+  int neo;            // 1 neo-Hookean, 0 linear
+  int wls;            // void interpolation on
+};
+
+__device__ __forceinline__ double clip01(double x) { return x > 1.0 ? 1.0 : (x < 0.0 ? 0.0 : x); }
+
+// psi_linear(F) and its first Piola stress (mech.py:39-43)
+__device__ __forceinline__ double lin_psi_P(const double (&F)[3][3], double lam, double mu, double (&P)[3][3]) {
+  double e[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) e[i][j] = 0.5 * ((F[i][j] - (i == j)) + (F[j][i] - (j == i)));
+  const double tr = e[0][0] + e[1][1] + e[2][2];
+  double ee = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ee += e[i][j] * e[i][j];
+      P[i][j] = 2.0 * mu * e[i][j] + (i == j ? lam * tr : 0.0);
+    }
+  return 0.5 * lam * tr * tr + mu * ee;
+}
+
+__device__ __forceinline__ void lin_dP(const double (&dF)[3][3], double lam, double mu, double (&dP)[3][3]) {
+  const double tr = dF[0][0] + dF[1][1] + dF[2][2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dP[i][j] = mu * (dF[i][j] + dF[j][i]) + (i == j ? lam * tr : 0.0);
+}
+
+// J = det F and F^-1 (adjugate / J)
+__device__ __forceinline__ double inv3(const double (&F)[3][3], double (&Fi)[3][3]) {
+  const double c00 = F[1][1] * F[2][2] - F[1][2] * F[2][1];
+  const double c01 = F[1][2] * F[2][0] - F[1][0] * F[2][2];
+  const double c02 = F[1][0] * F[2][1] - F[1][1] * F[2][0];
+  const double J = F[0][0] * c00 + F[0][1] * c01 + F[0][2] * c02;
+  const double r = 1.0 / J;
+  Fi[0][0] = c00 * r;
+  Fi[1][0] = c01 * r;
+  Fi[2][0] = c02 * r;
+  Fi[0][1] = (F[0][2] * F[2][1] - F[0][1] * F[2][2]) * r;
+  Fi[1][1] = (F[0][0] * F[2][2] - F[0][2] * F[2][0]) * r;
+  Fi[2][1] = (F[0][1] * F[2][0] - F[0][0] * F[2][1]) * r;
+  Fi[0][2] = (F[0][1] * F[1][2] - F[0][2] * F[1][1]) * r;
+  Fi[1][2] = (F[0][2] * F[1][0] - F[0][0] * F[1][2]) * r;
+  Fi[2][2] = (F[0][0] * F[1][1] - F[0][1] * F[1][0]) * r;
+  return J;
+}
+
+// psi_neohooke(F) (mech.py:46-50) and P = mu F + (lam ln J - mu) F^-T; returns psi, fills Fi and lnJ
+__device__ __forceinline__ double neo_psi_P(const double (&F)[3][3], double lam, double mu, double (&P)[3][3],
+                                            double (&Fi)[3][3], double& lnJ) {
+  const double J = inv3(F, Fi);
+  lnJ = log(J);  // NaN for J <= 0, as in the reference (the line search rejects non-finite energies)
+  double I1 = 0.0;
+  const double c = lam * lnJ - mu;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      I1 += F[i][j] * F[i][j];
+      P[i][j] = mu * F[i][j] + c * Fi[j][i];
+    }
+  return 0.5 * mu * (I1 - 3.0) - mu * lnJ + 0.5 * lam * lnJ * lnJ;
+}
+
+// tangent of the neo-Hookean energy at F (given F^-1, ln J) applied to dF
+__device__ __forceinline__ void neo_dP(const double (&Fi)[3][3], double lnJ, const double (&dF)[3][3], double lam,
+                                       double mu, double (&dP)[3][3]) {
+  double A[3][3];  // F^-1 dF
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A[i][j] = Fi[i][0] * dF[0][j] + Fi[i][1] * dF[1][j] + Fi[i][2] * dF[2][j];
+  const double tr = A[0][0] + A[1][1] + A[2][2];
+  const double c = mu - lam * lnJ;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double B = A[j][0] * Fi[0][i] + A[j][1] * Fi[1][i] + A[j][2] * Fi[2][i];  // (F^-1 dF F^-1)_ji
+      dP[i][j] = mu * dF[i][j] + lam * tr * Fi[j][i] + c * B;
+    }
+}
+
+// psi_eff and its stress at F for one element's material (mech.py:134-139)
+__device__ __forceinline__ double eff_psi_P(const double (&F)[3][3], const FsMat& m, double gam, double (&P)[3][3]) {
+  if (!m.wls) {
+    if (m.neo) {
+      double Fi[3][3], lnJ;
+      return neo_psi_P(F, m.lam, m.mu, P, Fi, lnJ);
+    }
+    return lin_psi_P(F, m.lam, m.mu, P);
+  }
+  double Fg[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fg[i][j] = (i == j) + gam * (F[i][j] - (i == j));
+  double Pn[3][3], Pg[3][3], Pl[3][3];
+  double psi_n;
+  if (m.neo) {
+    double Fi[3][3], lnJ;
+    psi_n = neo_psi_P(Fg, m.lam, m.mu, Pn, Fi, lnJ);
+  } else {
+    psi_n = lin_psi_P(Fg, m.lam, m.mu, Pn);
+  }
+  const double psi_g = lin_psi_P(Fg, m.lam, m.mu, Pg);
+  const double psi_l = lin_psi_P(F, m.lam, m.mu, Pl);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = gam * (Pn[i][j] - Pg[i][j]) + Pl[i][j];
+  return psi_n - psi_g + psi_l;
+}
+
+// tangent of psi_eff at F applied to dF
+__device__ __forceinline__ void eff_dP(const double (&F)[3][3], const FsMat& m, double gam, const double (&dF)[3][3],
+                                       double (&dP)[3][3]) {
+  double Dl[3][3];
+  lin_dP(dF, m.lam, m.mu, Dl);
+  if (!m.neo) {  // psi_nl = psi_lin: the interpolation cancels
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dP[i][j] = Dl[i][j];
+    return;
+  }
+  const double g = m.wls ? gam : 1.0;
+  double Fg[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Fg[i][j] = m.wls ? (i == j) + g * (F[i][j] - (i == j)) : F[i][j];
+  double Fi[3][3];
+  const double lnJ = log(inv3(Fg, Fi));
+  double Dn[3][3];
+  neo_dP(Fi, lnJ, dF, m.lam, m.mu, Dn);
+  if (!m.wls) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) dP[i][j] = Dn[i][j];
+    return;
+  }
+  const double g2 = g * g;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dP[i][j] = g2 * (Dn[i][j] - Dl[i][j]) + Dl[i][j];
+}
+
+struct FsArgs {
+  int nx, ny, nz;
+  int64_t n;
+  const double* rho;
+  FsMat m;
+  double wq;
+  const double* w;   // state (N, 3)
+  const double* v;   // direction (N, 3) of the product, or null (zero)
+  double Fbar[9];
+  double dFbar[9];
+  double* fe;        // (24, N) element nodal forces, or null
+  double* part;      // (blocks, kFsParts) partial sums
+};
+
+// MODE 0: energy; 1: energy, element forces of the gradient, sum of P; 2: Hessian-vector product
+// (element forces, sum of dP)
+template <int MODE>
+__global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
+  __shared__ double red[kFsThreads / 64];
+  const int64_t e = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
+  const bool live = e < a.n;
+  double acc[kFsParts];
+#pragma unroll
+  for (int q = 0; q < kFsParts; ++q) acc[q] = 0.0;
+  if (live) {
+    const int k = (int)(e % a.nz);
+    const int j = (int)((e / a.nz) % a.ny);
+    const int i = (int)(e / ((int64_t)a.nz * a.ny));
+    const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
+    int64_t node[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
+      node[c] = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+    }
+    double wn[8][3];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) wn[c][d] = a.w[node[c] * 3 + d];
+    double vn[8][3];
+    if (MODE == 2) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vn[c][d] = a.v ? a.v[node[c] * 3 + d] : 0.0;
+    }
+    const double r = clip01(a.rho[e]);
+    const double vp = pow(r, a.m.p);
+    const double s = a.m.emin + (1.0 - a.m.emin) * vp;
+    double gam = 1.0;
+    if (a.m.wls) {  // mech.py:107-112
+      const double b = a.m.wls_b, r0 = a.m.wls_r0;
+      gam = (tanh(b * r0) + tanh(b * (vp - r0))) / (tanh(b * r0) + tanh(b * (1.0 - r0)));
+    }
+    const double sc = a.wq * s;
+    double f[8][3];
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f[c][d] = 0.0;
+#pragma unroll 1
+    for (int g = 0; g < 8; ++g) {
+      double F[3][3];
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          double h = a.Fbar[ci * 3 + dj];
+#pragma unroll
+          for (int c = 0; c < 8; ++c) h += wn[c][ci] * dshape(g, c, dj);
+          F[ci][dj] = h;
+        }
+      double P[3][3];
+      if (MODE == 2) {
+        double dF[3][3];
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+          for (int dj = 0; dj < 3; ++dj) {
+            double h = a.dFbar[ci * 3 + dj];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) h += vn[c][ci] * dshape(g, c, dj);
+            dF[ci][dj] = h;
+          }
+        eff_dP(F, a.m, gam, dF, P);
+      } else {
+        const double psi = eff_psi_P(F, a.m, gam, P);
+        acc[0] += psi;
+      }
+      if (MODE != 0) {
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+          for (int dj = 0; dj < 3; ++dj) acc[1 + ci * 3 + dj] += P[ci][dj];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+#pragma unroll
+          for (int ci = 0; ci < 3; ++ci)
+            f[c][ci] += P[ci][0] * dshape(g, c, 0) + P[ci][1] * dshape(g, c, 1) + P[ci][2] * dshape(g, c, 2);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kFsParts; ++q) acc[q] *= sc;
+    if (MODE != 0) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) a.fe[(int64_t)(c * 3 + ci) * a.n + e] = sc * f[c][ci];
+    }
+  }
+  const int nq = MODE == 0 ? 1 : kFsParts;
+  for (int q = 0; q < nq; ++q) {
+    const double t = block_sum(acc[q], red);
+    if (threadIdx.x == 0) a.part[(int64_t)blockIdx.x * kFsParts + q] = t;
+    __syncthreads();
+  }
+}
+
+// out(node, c) = sum over the node's 8 elements of their force on it (fixed order: corner 0..7)
+__global__ void __launch_bounds__(kFsThreads) fs_gather_kernel(const FsArgs a, double* __restrict__ out) {
+  const int64_t nd = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
+  if (nd >= a.n) return;
+  const int k = (int)(nd % a.nz);
+  const int j = (int)((nd / a.nz) % a.ny);
+  const int i = (int)(nd / ((int64_t)a.nz * a.ny));
+  const int i0 = i == 0 ? a.nx - 1 : i - 1, j0 = j == 0 ? a.ny - 1 : j - 1, k0 = k == 0 ? a.nz - 1 : k - 1;
+  double o[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    // the element whose corner c is this node starts one voxel back along each set bit of c
+    const int ii = (c & 1) ? i0 : i, jj = (c & 2) ? j0 : j, kk = (c & 4) ? k0 : k;
+    const int64_t el = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) o[d] += a.fe[(int64_t)(c * 3 + d) * a.n + el];
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) out[nd * 3 + d] = o[d];
+}
+
+// out[q] = sum over blocks of part[b][q], lane-strided then a fixed butterfly (deterministic)
+__global__ void __launch_bounds__(kFsThreads) fs_reduce_kernel(const double* __restrict__ part, int nblk,
+                                                               double* __restrict__ out) {
+  __shared__ double red[kFsThreads / 64];
+  const int q = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kFsThreads) acc += part[(int64_t)b * kFsParts + q];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) out[q] = t;
+}
+
+}  // namespace
+}  // namespace nlfh
+
+using namespace nlfh;
+
+extern "C" {
+
+long long nlfh_fs_work_doubles(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) return -1;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int64_t nblk = (n + kFsThreads - 1) / kFsThreads;
+  return (long long)(24 * n + nblk * kFsParts);
+}
+
+int nlfh_fs_eval(const int* shape, const double* rho, const double* mat, int flags, const double* w,
+                 const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
+                 double* work, void* stream) {
+  if (shape == nullptr || mat == nullptr || Fbar == nullptr) {
+    set_error("nlfh_fs_eval: shape, mat and Fbar are required");
+    return NLFH_EINVAL;
+  }
+  const int nx = shape[0], ny = shape[1], nz = shape[2];
+  if (nx < 1 || ny < 1 || nz < 1) {
+    set_error("nlfh_fs_eval: cell dimensions must be positive");
+    return NLFH_EINVAL;
+  }
+  if (mode < 0 || mode > 2) {
+    set_error("nlfh_fs_eval: mode must be 0 (energy), 1 (energy and gradient) or 2 (Hessian-vector product)");
+    return NLFH_EINVAL;
+  }
+  if (rho == nullptr || w == nullptr || scalars == nullptr || work == nullptr || (mode >= 1 && grad == nullptr) ||
+      (mode == 2 && dFbar == nullptr)) {
+    set_error("nlfh_fs_eval: a required pointer is null");
+    return NLFH_EINVAL;
+  }
+  FsArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.nz = nz;
+  a.n = (int64_t)nx * ny * nz;
+  a.rho = rho;
+  a.m.lam = mat[0];
+  a.m.mu = mat[1];
+  a.m.emin = mat[2];
+  a.m.p = mat[3];
+  a.m.wls_b = mat[4];
+  a.m.wls_r0 = mat[5];
+  a.m.neo = (flags & 1) ? 1 : 0;
+  a.m.wls = (flags & 2) ? 1 : 0;
+  a.wq = 1.0 / (8.0 * (double)a.n);
+  a.w = w;
+  a.v = mode == 2 ? v : nullptr;
+  for (int q = 0; q < 9; ++q) {
+    a.Fbar[q] = Fbar[q];
+    a.dFbar[q] = mode == 2 ? dFbar[q] : 0.0;
+  }
+  const int64_t nblk = (a.n + kFsThreads - 1) / kFsThreads;
+  if (nblk > INT32_MAX) {
+    set_error("nlfh_fs_eval: cell too large");
+    return NLFH_EINVAL;
+  }
+  a.fe = work;
+  a.part = work + 24 * a.n;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)nblk);
+  if (mode == 0) hipLaunchKernelGGL(fs_element_kernel<0>, grid, dim3(kFsThreads), 0, st, a);
+  else if (mode == 1) hipLaunchKernelGGL(fs_element_kernel<1>, grid, dim3(kFsThreads), 0, st, a);
+  else hipLaunchKernelGGL(fs_element_kernel<2>, grid, dim3(kFsThreads), 0, st, a);
+  HIP_TRY(hipGetLastError());
+  if (mode >= 1) {
+    hipLaunchKernelGGL(fs_gather_kernel, grid, dim3(kFsThreads), 0, st, a, grad);
+    HIP_TRY(hipGetLastError());
+  }
+  hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsThreads), 0, st, a.part, (int)nblk,
+                     scalars);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,321 @@
+"""Periodic finite-strain homogenization of voxel cells on the GPU (drop-in for nlfiber/mech.py's Cell and Mixed).
+
+The energy, its gradients and the Hessian-vector product run in libnlfh.so (`nlfh_fs_eval`, FP64); the
+preconditioner is the linear path's FFT Green operator of the homogeneous solid (`nlfh_precond`); the CG
+vectors stay on the device and the Newton / line-search / forcing logic runs on the host as in the reference.
+
+Kept from the reference (nlfiber/mech.py):
+  * Cell(rho, E, nu, material, wls, emin, p): stiffness scale s = emin + (1 - emin) rho^p, the void
+    interpolation gamma of Wang et al. (mech.py:91-112), energy(w, Fbar), mean_stress(w, Fbar),
+    precond_w(r) (mech.py:141-181);
+  * Mixed(cell, free, forcing): unknowns z = (w, free components of Fbar), z0, unpack, proj, tangent,
+    hvp_A, macro_diag, precond, solve_linear and the Newton-CG solve with Eisenstat-Walker forcing and the
+    Armijo / gradient-decrease line search (mech.py:183-330).
+Differences of form, not of result: the tangent is applied matrix-free (`tangent` returns the state it is
+taken at instead of 81 numbers per Gauss point); z, w and Fbar are torch / numpy float64 arrays instead of
+JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
+Not built: the adjoint grad_rho, the stability analysis (LOBPCG, effective tangent, rank-one ellipticity)
+and the third-order constants (mech.py:332-515).
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+from .linear import LinearHomogenization, VOIGT, lame
+
+MATERIALS = ("linear", "neohooke")
+STALL_RTOL = 1e-6
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def voigt_tensor(I):
+    """Symmetric 3x3 tensor of the Voigt unit strain I (engineering shear: off-diagonals 1/2) (mech.py:57-65)."""
+    H = np.zeros((3, 3))
+    i, j = VOIGT[I]
+    if i == j:
+        H[i, i] = 1.0
+    else:
+        H[i, j] = H[j, i] = 0.5
+    return H
+
+
+def to_voigt(M):
+    """Voigt vector (engineering shear) of the symmetric part of a 3x3 matrix (mech.py:68-71)."""
+    M = np.asarray(M, dtype=np.float64)
+    S = 0.5 * (M + M.T)
+    return np.array([S[0, 0], S[1, 1], S[2, 2], 2 * S[1, 2], 2 * S[0, 2], 2 * S[0, 1]])
+
+
+def _linesearch(E, gnorm, x, dx, E0, slope, gn):
+    """Backtracking from t = 1: Armijo decrease, or (at the round-off floor of E) a smaller gradient
+    norm; None when t falls below 1e-6 (mech.py:74-88)."""
+    tol = 64 * np.finfo(np.float64).eps * max(abs(E0), 1e-300)
+    t = 1.0
+    while t > 1e-6:
+        Et = E(x + t * dx)
+        if math.isfinite(Et):
+            if Et <= E0 + 1e-4 * t * slope:
+                return t
+            if Et <= E0 + tol and gnorm(x + t * dx) < gn:
+                return t
+        t *= 0.5
+    return None
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Cell:
+    """A periodic voxel cell. rho: design density (nx, ny, nz) or (n, n) in [0, 1] (mech.py:91-121)."""
+
+    def __init__(self, rho, E=1.0, nu=0.3, material="neohooke", wls=(500.0, 0.01), emin=1e-3, p=3.0):
+        if material not in MATERIALS:
+            raise ValueError(f"material must be one of {MATERIALS}")
+        self._dev = torch.device("cuda", torch.cuda.current_device())
+        rho = torch.as_tensor(rho, dtype=torch.float64, device=self._dev)
+        if rho.dim() == 2:
+            rho = rho[..., None]
+        self.rho, self.shape = rho.contiguous(), tuple(int(n) for n in rho.shape)
+        self.N = int(np.prod(self.shape))
+        self.E, self.nu, self.material, self.wls, self.emin, self.p = E, nu, material, wls, emin, p
+        self.lam, self.mu = lame(E, nu)
+        self.wq = 1.0 / (8.0 * self.N)
+        self.s = self.scale_of(self.rho)
+        self.gamma = self.gamma_of(self.rho)
+        self._lib = _lib.load()
+        b, r0 = wls if wls is not None else (0.0, 0.0)
+        self._mat = np.array([self.lam, self.mu, emin, p, b, r0], dtype=np.float64)
+        self._flags = (1 if material == "neohooke" else 0) | (2 if wls is not None else 0)
+        self._shape_c = (ctypes.c_int * 3)(*self.shape)
+        nwork = int(self._lib.nlfh_fs_work_doubles(*self.shape))
+        self._work = torch.empty(nwork, dtype=torch.float64, device=self._dev)
+        self._scal = torch.empty(10, dtype=torch.float64, device=self._dev)
+        self._lh = None
+
+    def scale_of(self, rho):
+        rho = torch.as_tensor(rho, dtype=torch.float64)
+        return self.emin + (1.0 - self.emin) * rho.clamp(0.0, 1.0) ** self.p
+
+    def gamma_of(self, rho):
+        if self.wls is None:
+            return None
+        b, r0 = self.wls
+        v = torch.as_tensor(rho, dtype=torch.float64).clamp(0.0, 1.0) ** self.p
+        return (math.tanh(b * r0) + torch.tanh(b * (v - r0))) / (math.tanh(b * r0) + math.tanh(b * (1 - r0)))
+
+    def with_rho(self, rho):
+        return Cell(rho, self.E, self.nu, self.material, self.wls, self.emin, self.p)
+
+    # --- operators ---------------------------------------------------------------------------------------
+    def _eval(self, w, Fbar, mode, v=None, dFbar=None):
+        w = torch.as_tensor(w, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
+        if w.numel() != 3 * self.N:
+            raise ValueError(f"w must hold {3 * self.N} values (shape {self.shape + (3,)})")
+        F = np.ascontiguousarray(np.asarray(Fbar, dtype=np.float64).reshape(9))
+        grad = torch.empty_like(w) if mode >= 1 else None
+        if v is not None:
+            v = torch.as_tensor(v, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
+            if v.numel() != 3 * self.N:
+                raise ValueError(f"v must hold {3 * self.N} values (shape {self.shape + (3,)})")
+        dF = None if dFbar is None else np.ascontiguousarray(np.asarray(dFbar, dtype=np.float64).reshape(9))
+        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self.rho), self._mat.ctypes.data_as(_dp), self._flags,
+                                          _ptr(w), F.ctypes.data_as(_dp), _ptr(v),
+                                          dF.ctypes.data_as(_dp) if dF is not None else None, mode, _ptr(grad),
+                                          _ptr(self._scal), _ptr(self._work), _stream()))
+        return grad, self._scal.cpu().numpy()
+
+    def energy(self, w, Fbar):
+        """W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + grad w at g) (mech.py:141-154), a Python float."""
+        return float(self._eval(w, Fbar, 0)[1][0])
+
+    def gradient(self, w, Fbar):
+        """(dW/dw with w's shape, Pbar = dW/dFbar (3, 3)) (the reference's Cell._g and Cell._P)."""
+        g, sc = self._eval(w, Fbar, 1)
+        return g.reshape(self.shape + (3,)), sc[1:].reshape(3, 3)
+
+    def mean_stress(self, w, Fbar):
+        """Pbar = dW/dFbar (mech.py:180-181)."""
+        return self.gradient(w, Fbar)[1]
+
+    def hessian_product(self, w, Fbar, v, dFbar):
+        """Cell Hessian at (w, Fbar) applied to (v, dFbar): (w part, dW/dFbar part (3, 3))."""
+        g, sc = self._eval(w, Fbar, 2, v, np.zeros((3, 3)) if dFbar is None else dFbar)
+        return g.reshape(self.shape + (3,)), sc[1:].reshape(3, 3)
+
+    def precond_w(self, r):
+        """Inverse symbol of the homogeneous linear operator with unit stiffness (mech.py:175-181). That
+        operator is the linear path's K(1) divided by N (its energy carries wq = 1/(8N) instead of the
+        element matrix's 1/8), so this is N times the linear preconditioner."""
+        if self._lh is None:
+            self._lh = LinearHomogenization(self.shape, self.E, self.nu, self.emin, self.p)
+        r = torch.as_tensor(r, dtype=torch.float64, device=self._dev).reshape(self.shape + (3,))
+        return self._lh.precond(r) * float(self.N)
+
+
+class Mixed:
+    """Mixed macroscopic control: Fbar components in `free` are unknowns with zero conjugate stress, the rest
+    are prescribed; free=[] is full control. Unknown z = (w, Fbar_free); solve min_z W(z) (mech.py:183-330)."""
+
+    def __init__(self, cell, free=((1, 1), (2, 2)), forcing="ew"):
+        self.c = cell
+        self.free = [tuple(int(i) for i in f) for f in free]
+        self.forcing = forcing
+        self.shape = cell.shape
+        self.nw = 3 * cell.N
+        self.k = len(self.free)
+        self.idx = tuple(np.array(self.free).T) if self.k else None
+        self._dev = cell._dev
+
+    # --- unknowns ----------------------------------------------------------------------------------------
+    def z0(self, Fb):
+        Fb = np.asarray(Fb, dtype=np.float64)
+        mac = torch.as_tensor(Fb[self.idx] if self.k else np.zeros(0), dtype=torch.float64, device=self._dev)
+        return torch.cat([torch.zeros(self.nw, dtype=torch.float64, device=self._dev), mac])
+
+    def unpack(self, z, Fb):
+        w = z[: self.nw].reshape(self.shape + (3,))
+        F = np.array(Fb, dtype=np.float64)
+        if self.k:
+            F[self.idx] = z[self.nw:].cpu().numpy()
+        return w, F
+
+    def proj(self, v):
+        w = v[: self.nw].reshape(-1, 3)
+        return torch.cat([(w - w.mean(dim=0)).reshape(-1), v[self.nw:]])
+
+    def _E(self, z, Fb):
+        return self.c.energy(*self.unpack(z, Fb))
+
+    def _g(self, z, Fb):
+        w, F = self.unpack(z, Fb)
+        g, P = self.c.gradient(w, F)
+        mac = torch.as_tensor(P[self.idx] if self.k else np.zeros(0), dtype=torch.float64, device=self._dev)
+        return torch.cat([g.reshape(-1), mac])
+
+    def tangent(self, z, Fb):
+        """The state (w, Fbar) the Hessian is taken at: hvp_A applies its tangent matrix-free."""
+        w, F = self.unpack(z, Fb)
+        return (w.clone(), F)
+
+    def hvp_A(self, A, v):
+        w, F = A
+        dF = np.zeros((3, 3))
+        if self.k:
+            vm = v[self.nw:].cpu().numpy()
+            for j, (a, b) in enumerate(self.free):
+                dF[a, b] += vm[j]
+        hw, hP = self.c.hessian_product(w, F, v[: self.nw], dF)
+        mac = torch.as_tensor(hP[self.idx] if self.k else np.zeros(0), dtype=torch.float64, device=self._dev)
+        return torch.cat([hw.reshape(-1), mac])
+
+    def macro_diag(self, A, Fb):
+        """Diagonal of the Hessian in the macro unknowns (for the preconditioner) (mech.py:240-246)."""
+        d = []
+        for j in range(self.k):
+            e = torch.zeros(self.nw + self.k, dtype=torch.float64, device=self._dev)
+            e[self.nw + j] = 1.0
+            d.append(float(self.hvp_A(A, e)[self.nw + j]))
+        return np.abs(np.array(d)) + 1e-12
+
+    def precond(self, r, dmac, swscale):
+        w = self.c.precond_w(r[: self.nw]) / swscale
+        mac = r[self.nw:] / torch.as_tensor(dmac, dtype=torch.float64, device=self._dev)
+        return torch.cat([w.reshape(-1), mac])
+
+    # --- linear solves (mech.py:253-288) ----------------------------------------------------------------
+    def _cg(self, A, b, dmac, sws, tol, maxit):
+        """Preconditioned CG on the mean-free space with the reference's negative-curvature exit: a
+        non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration)."""
+        proj = self.proj
+
+        def M(r):
+            return proj(self.precond(r, dmac, sws))
+
+        b = proj(b)
+        bn = float(torch.linalg.norm(b)) + 1e-300
+        z0 = M(b)
+        x, r, p = torch.zeros_like(b), b, z0
+        rz = float(torch.dot(b, z0))
+        it, done, conv = 0, False, False
+        while not done and it < maxit:
+            Ap = proj(self.hvp_A(A, p))
+            pAp = float(torch.dot(p, Ap))
+            neg = pAp <= 0
+            a = 0.0 if neg else rz / pAp
+            x1 = x + a * p
+            r1 = r - a * Ap
+            conv1 = float(torch.linalg.norm(r1)) <= tol * bn
+            z1 = M(r1)
+            rz1 = float(torch.dot(r1, z1))
+            p1 = z1 + (rz1 / (1.0 if rz == 0 else rz)) * p
+            if neg:
+                x1 = z0 if it == 0 else x
+            x, r, p, rz, it = x1, r1, p1, rz1, it + 1
+            done, conv = neg or conv1, conv1 and not neg
+        return x, it, conv, done and not conv
+
+    def _sws(self):
+        # scale of the preconditioner: geometric mean of the stiffness (contrast is handled by CG)
+        return float(torch.exp(torch.mean(torch.log(self.c.s))))
+
+    def solve_linear(self, A, b, tol=1e-10, maxit=20000):
+        """Solve H x = b with the tangent at state A (mean-free in w). Returns (x, iterations, converged)."""
+        dmac = self.macro_diag(A, None) if self.k else np.zeros(0)
+        x, it, conv, _ = self._cg(A, b, dmac, self._sws(), tol, int(maxit))
+        return x, int(it), bool(conv)
+
+    # --- Newton-CG (mech.py:290-330) --------------------------------------------------------------------
+    def solve(self, Fbar, z0=None, rtol=1e-9, maxnewton=30, cg_tol=1e-6, cg_maxit=20000, g_ref=None):
+        """Newton-CG. Fbar: prescribed components (free ones are the initial guess if z0 is None).
+        Returns (w, Fbar_full, log) with log['z'] the state."""
+        Fb = np.asarray(Fbar, dtype=np.float64)
+        z = self.z0(Fb) if z0 is None else torch.as_tensor(z0, dtype=torch.float64, device=self._dev)
+        g0, log = g_ref, dict(newton=0, cg=0, converged=False, gnorm=[], negative_curvature=False)
+        eta, gprev = None, None
+        sws = self._sws()
+        for k in range(maxnewton):
+            g = self.proj(self._g(z, Fb))
+            gn = float(torch.linalg.norm(g))
+            g0 = gn if g0 is None else g0
+            log["gnorm"].append(gn)
+            if gn <= rtol * max(g0, 1e-300) or gn < 1e-14:
+                log["converged"] = True
+                break
+            tol = cg_tol
+            if self.forcing == "ew":
+                e_new = 0.1 if gprev is None else 0.9 * (gn / gprev) ** 2
+                if eta is not None and 0.9 * eta**2 > 0.1:
+                    e_new = max(e_new, 0.9 * eta**2)
+                eta = min(e_new, 0.1)
+                tol = max(eta, cg_tol, 0.5 * rtol * g0 / gn) if gprev is not None else max(eta, cg_tol)
+            gprev = gn
+            A = self.tangent(z, Fb)
+            dmac = self.macro_diag(A, Fb) if self.k else np.zeros(0)
+            dz, its, ok, neg = self._cg(A, -g, dmac, sws, tol, int(cg_maxit))
+            log["cg"] += int(its)
+            log["negative_curvature"] |= bool(neg)
+            E0, slope = self._E(z, Fb), float(torch.dot(g, dz))
+            if slope >= 0:
+                dz = -self.proj(self.precond(g, dmac, sws))
+                slope = float(torch.dot(g, dz))
+            t = _linesearch(lambda x: self._E(x, Fb), lambda x: float(torch.linalg.norm(self.proj(self._g(x, Fb)))),
+                            z, dz, E0, slope, gn)
+            log["newton"] = k + 1
+            if t is None:
+                log["linesearch_failed"] = True
+                if gn <= STALL_RTOL * g0:
+                    log["converged"] = log["roundoff_floor"] = True
+                break
+            z = z + t * dz
+        w, F = self.unpack(z, Fb)
+        log["z"], log["g0"] = z, g0
+        return w, F, log

@@ -1,0 +1,106 @@
+"""Finite-strain cell operators and Newton-CG on the GPU (nlfh_fs_eval, package mech.Cell / mech.Mixed)
+against the oracle (oracle/mech_ref.py, pinned in tests/test_oracle_mech.py)."""
+import numpy as np
+import pytest
+
+from oracle import linear_ref as LO
+from oracle import mech_ref as MR
+
+pytestmark = pytest.mark.gpu
+
+CASES = [("neohooke", (500.0, 0.01)), ("neohooke", None), ("linear", (500.0, 0.01)), ("linear", None)]
+
+
+def _rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+@pytest.fixture(scope="module")
+def M(gpu_pkg):
+    import torch
+    torch.cuda.set_device(0)
+    return gpu_pkg.mech
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 3), (8, 8, 8), (6, 6, 1)])
+@pytest.mark.parametrize("material,wls", CASES)
+def test_energy_gradient_and_hessian_product_match_oracle(M, shape, material, wls):
+    import torch
+    rng = np.random.default_rng(11)
+    rho = rng.uniform(0.0, 1.0, shape)
+    ref = MR.CellRef(rho, material=material, wls=wls)
+    cell = M.Cell(rho, material=material, wls=wls)
+    w = 0.05 * rng.standard_normal(shape + (3,))
+    Fb = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    wd = torch.as_tensor(w, device="cuda")
+    print(f"fs_eval shape={shape} material={material} wls={wls}", flush=True)
+    assert cell.energy(wd, Fb) == pytest.approx(ref.energy(w, Fb), rel=1e-12)
+    g, P = cell.gradient(wd, Fb)
+    gr, Pr = ref.stress(w, Fb)
+    assert _rel(g.cpu().numpy(), gr) < 1e-11
+    assert _rel(P, Pr) < 1e-12
+    v = rng.standard_normal(shape + (3,))
+    dF = rng.standard_normal((3, 3))
+    hw, hP = cell.hessian_product(wd, Fb, torch.as_tensor(v, device="cuda"), dF)
+    hwr, hPr = ref.hvp(ref.tangent(w, Fb), v, dF)
+    assert _rel(hw.cpu().numpy(), hwr) < 1e-11
+    assert _rel(hP, hPr) < 1e-11
+    # a null direction leaves only the macro part
+    hw0, hP0 = cell.hessian_product(wd, Fb, None, dF)
+    hwr0, hPr0 = ref.hvp(ref.tangent(w, Fb), np.zeros_like(v), dF)
+    assert _rel(hw0.cpu().numpy(), hwr0) < 1e-11 and _rel(hP0, hPr0) < 1e-11
+
+
+def test_precond_is_N_times_the_linear_preconditioner(M):
+    import torch
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.0, 1.0, (6, 5, 4))
+    cell = M.Cell(rho)
+    r = rng.standard_normal((6, 5, 4, 3))
+    z = cell.precond_w(torch.as_tensor(r, device="cuda")).cpu().numpy()
+    lin = LO.LinearRef((6, 5, 4))
+    zr = lin.precond(r[None])[0] * cell.N
+    assert _rel(z, zr) < 1e-12
+
+
+@pytest.mark.parametrize("free", [[], [(1, 1), (2, 2)]])
+def test_newton_cg_reaches_the_oracle_equilibrium(M, free):
+    rng = np.random.default_rng(2)
+    rho = rng.uniform(0.2, 1.0, (4, 4, 4))
+    ref = MR.CellRef(rho)
+    cell = M.Cell(rho)
+    Fb = np.diag([1.03, 1.0, 1.0]) + 0.01 * np.array([[0, 1, 0], [0, 0, 0.5], [0.2, 0, 0]])
+    print(f"newton free={free}", flush=True)
+    w, F, log = M.Mixed(cell, free=free).solve(Fb, rtol=1e-11, cg_tol=1e-10)
+    assert log["converged"] and not log.get("linesearch_failed")
+    wr, Fr = MR.newton_dense(ref, Fb, free=free, rtol=1e-12)
+    np.testing.assert_allclose(F, Fr, atol=1e-10)
+    P = cell.mean_stress(w, F)
+    _, Pr = ref.stress(wr, Fr)
+    assert _rel(P, Pr) < 1e-8
+    for a, b in free:  # zero conjugate stress on the free components
+        assert abs(P[a, b]) < 1e-8 * np.abs(P).max()
+    assert _rel(w.cpu().numpy(), wr) < 1e-7
+
+
+def test_small_strain_limit_is_the_linear_C(M, gpu_pkg):
+    """Pbar(I + t H) = t C:H + O(t^2) with C from the linear GPU solve (mech.py:1-17)."""
+    import torch
+    rng = np.random.default_rng(4)
+    rho = rng.uniform(0.2, 1.0, (8, 8, 8))
+    lh = gpu_pkg.LinearHomogenization((8, 8, 8), tol=1e-10)
+    C, _, _ = lh.C_full(torch.as_tensor(rho, device="cuda"))
+    C = C.cpu().numpy()
+    n = np.array([1.0, -0.3, 0.2, 0.4, -0.2, 0.1])
+    H = sum(n[I] * M.voigt_tensor(I) for I in range(6))
+    cell = M.Cell(rho)
+    full = M.Mixed(cell, free=[])
+    errs = []
+    for t in (1e-3, 5e-4):
+        w, F, log = full.solve(np.eye(3) + t * H, rtol=1e-11, cg_tol=1e-10)
+        assert log["converged"]
+        Pv = M.to_voigt(cell.mean_stress(w, F))
+        Pv[3:] *= 0.5  # engineering shear of the stress vector -> tensor components
+        errs.append(_rel(Pv, t * C @ n))
+    assert errs[0] < 5e-3 and errs[1] < 0.6 * errs[0]
