@@ -41,8 +41,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber"])
+    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber", "finite"])
     ap.add_argument("--fiber-n-o", type=int, default=16, help="octant side of the fiber workload (cell 2 n_o)")
+    ap.add_argument("--finite-n", type=int, default=64, help="cell side of the finite-strain workload")
+    ap.add_argument("--finite-strain", type=float, default=0.05, help="uniaxial stretch - 1 of the finite workload")
     ap.add_argument("--cells", type=int, default=8, help="64^3 cells per step (cell64)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--learned", default=None, help="warm-start weights (npz)")
@@ -92,6 +94,8 @@ def main():
         sys.exit(2)
     if args.workload == "fiber":
         return fiber_bench(args)
+    if args.workload == "finite":
+        return finite_bench(args)
     jobs, emin, wname = workload(args)
     tol = 1e-8
     if wname == "stress128":
@@ -339,6 +343,106 @@ def fiber_bench(args):
         "config": {"workload": f"Fiber(n_o={n_o}) on a {n}^3 cell, beta {beta}, v {v}, r {r}, emin 1e-3, tol 1e-9, "
                                f"warm-started along phi_k = phi_0 + 0.02 k d", "pcg_iterations": its},
         "roofline": roof, "cpu_baseline": cpu}), flush=True)
+
+
+FS_FLOP_PER_VOXEL = 6016  # Hessian-vector product, neo-Hookean with void interpolation (8 Gauss points x 752)
+FS_HVP_BYTES_PER_VOXEL = 80  # read w, v (2 x 24), rho (8); write the product (24)
+
+
+def finite_bench(args):
+    """Finite-strain uniaxial-stress solve (Mixed with the lateral stretches free, neo-Hookean with the void
+    interpolation, the reference's defaults: rtol 1e-9, cg_tol 1e-6, Eisenstat-Walker forcing) of one n^3 cell,
+    from a zero start each step."""
+    import torch
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    pkg.load_library()
+    torch.cuda.set_device(0)
+    from nlfh_amd import mech
+    n, e = args.finite_n, args.finite_strain
+    rho, _ = pkg.cells.random_cell(n, 0)
+    cell = mech.Cell(rho)
+    mix = mech.Mixed(cell, free=[(1, 1), (2, 2)])
+    Fb = np.diag([1.0 + e, 1.0, 1.0])
+    logs = []
+    for _ in range(args.warmup):
+        mix.solve(Fb)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        w, F, log = mix.solve(Fb)
+        logs.append(log)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    # the Hessian-vector product kernels (element pass + node gather) at the final state, HIP events on
+    # the stream they run on
+    st = torch.cuda.current_stream()
+    v = torch.randn(cell.N * 3, dtype=torch.float64, device="cuda")
+    dF = np.diag([0.0, 1.0, 1.0])
+    for _ in range(3):
+        cell.hessian_product(w, F, v, dF)
+    reps = 20
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        cell._eval(w, F, 2, v, dF)
+    e1.record(st)
+    torch.cuda.synchronize()
+    t_h = e0.elapsed_time(e1) / reps * 1e-3
+    flops = FS_FLOP_PER_VOXEL * cell.N
+    roof = {"bound": "fp64-valu", "achieved": round(flops / t_h / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
+            "frac": round(flops / t_h / 1e12 / 78.6, 4), "traffic": None,
+            "kernel": f"fs_element_kernel<2> + fs_gather_kernel + fs_reduce_kernel (one Hessian-vector product, "
+                      f"{n}^3)", "flops_per_voxel": FS_FLOP_PER_VOXEL, "avg_us": round(t_h * 1e6, 2),
+            "hbm_algorithmic_GBs": round(FS_HVP_BYTES_PER_VOXEL * cell.N / t_h / 1e9, 1)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = finite_cpu(rho, Fb, logs[0], n)
+    print(json.dumps({
+        "metric": "finite-strain uniaxial-stress solves/sec (Newton-CG to rtol 1e-9, neo-Hookean, void interpolation)",
+        "value": round(args.steps / dt, 4), "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Mixed(Cell(random cell {n}^3 seed 0), free=[(1,1),(2,2)]).solve(diag(1+{e},1,1))",
+                   "newton": logs[0]["newton"], "cg": logs[0]["cg"], "converged": logs[0]["converged"],
+                   "Pbar_xx": float(cell.mean_stress(w, F)[0, 0]), "F_lateral": [float(F[1, 1]), float(F[2, 2])]},
+        "roofline": roof, "cpu_baseline": cpu}), flush=True)
+
+
+def finite_cpu(rho, Fb, log, n):
+    """The oracle's operators for the same solve, timed on a bounded sample and scaled: per Newton step one
+    gradient, one tangent, the line-search energies; per CG iteration one stored-tangent product and one FFT
+    preconditioner application (on the same cell), times the GPU solve's Newton and CG counts."""
+    from oracle import linear_ref as O
+    from oracle import mech_ref as MR
+    cores = os.cpu_count() or 1
+    O.set_fft_workers(cores)
+    ref = MR.CellRef(rho)
+    lin = O.LinearRef(rho.shape)
+    w = np.zeros(rho.shape + (3,))
+    t0 = time.perf_counter()
+    ref.stress(w, Fb)
+    t1 = time.perf_counter()
+    ref.energy(w, Fb)
+    t2 = time.perf_counter()
+    A = ref.tangent(w, Fb)
+    t3 = time.perf_counter()
+    v = np.random.default_rng(0).standard_normal(w.shape)
+    k = 3
+    for _ in range(k):
+        hw, _ = ref.hvp(A, v, np.zeros((3, 3)))
+        lin.precond(hw[None])
+    t4 = time.perf_counter()
+    t_g, t_e, t_a, t_cg = t1 - t0, t2 - t1, t3 - t2, (t4 - t3) / k
+    # per Newton step: gradient, tangent, 2 macro-diagonal products, energy + one line-search energy
+    total = log["newton"] * (t_g + t_a + 2 * t_cg + 2 * t_e) + log["cg"] * t_cg
+    t_newton = t_g + t_e + t_a
+    return {"value": round(1.0 / total, 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"numpy oracle on the {n}^3 cell: one gradient + energy + tangent ({t_newton:.2f} s), {k} "
+                      f"tangent products + FFT preconditioner ({t_cg:.2f} s each); scaled to the GPU solve's "
+                      f"{log['newton']} Newton steps and {log['cg']} CG iterations (extrapolated)",
+            "seconds_sampled": round(t4 - t0, 2)}
 
 
 def fiber_cpu(phi, beta, v, r):

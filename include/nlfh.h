@@ -196,14 +196,18 @@ int nlfh_dmap_rho_vjp(nlfh_dmap* dm, const double* phi, double beta, const doubl
 /* ---- Finite-strain cell operators (nlfiber/mech.py) -------------------------------------------------
  * FP64, device memory. rho (nx, ny, nz); w, v, grad: (nx, ny, nz, 3) C-order, component fastest.
  * mat = {lam, mu, emin, p, wls_b, wls_r0}; flags bit 0: neo-Hookean (else linear), bit 1: void interpolation.
- * Fbar, dFbar: host 3x3 row-major. work: nlfh_fs_work_doubles(nx, ny, nz) doubles of device memory. */
+ * Fbar: host 3x3 row-major; dFbar: device 3x3 row-major (so that a CG loop can form it from its vectors
+ * without a host round trip). work: nlfh_fs_work_doubles(nx, ny, nz) doubles of device memory. */
 long long nlfh_fs_work_doubles(int nx, int ny, int nz);
+/* sg (2, N) = (s_e / (8N), gamma_e): the stiffness scale and void interpolation factor of every element
+ * (Cell.scale_of / Cell.gamma_of, mech.py:104-112), computed once per density. */
+int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int flags, double* sg, void* stream);
 /* mode 0: scalars[0] = W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + H_g) (Cell.energy, mech.py:141-154);
  * mode 1: also grad = dW/dw (Cell._g, mech.py:172) and scalars[1..9] = Pbar = dW/dFbar (Cell._P, mech.py:173);
  * mode 2: grad = d^2W/dw dz [v; dFbar] and scalars[1..9] = sum of the stress increments, i.e. the product of
  *         the cell Hessian at the state (w, Fbar) with the direction (v, dFbar) (Mixed.hvp_A with the tangent of
  *         Mixed.tangent, mech.py:227-236; the tangent is applied matrix-free). v may be null (zero). */
-int nlfh_fs_eval(const int* shape, const double* rho, const double* mat, int flags, const double* w,
+int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
                  double* work, void* stream);
 

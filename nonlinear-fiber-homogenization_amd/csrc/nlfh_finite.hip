@@ -18,8 +18,9 @@
 //
 // Layout: w and v are the reference's (nx, ny, nz, 3) C-order arrays (component fastest); element e =
 // voxel (i, j, k), e = (i ny + j) nz + k, spans nodes (i + a, j + b, k + c) mod n, node index
-// a + 2b + 4c. One thread per element writes its 24 nodal forces to a (24, N) buffer; a second kernel
-// gathers the 8 contributions of every node in a fixed order, so results are deterministic.
+// a + 2b + 4c. Eight lanes per element (one per Gauss point) write the element's 24 nodal forces to a
+// (24, N) buffer; a second kernel gathers the 8 contributions of every node in a fixed order, so results
+// are deterministic.
 #include <hip/hip_runtime.h>
 
 #include "nlfh_common.h"
@@ -205,112 +206,141 @@ __device__ __forceinline__ void eff_dP(const double (&F)[3][3], const FsMat& m, 
 struct FsArgs {
   int nx, ny, nz;
   int64_t n;
-  const double* rho;
+  const double* sg;  // (2, N): wq s_e and gamma_e of every element (fs_prepare_kernel)
   FsMat m;
   double wq;
   const double* w;   // state (N, 3)
   const double* v;   // direction (N, 3) of the product, or null (zero)
   double Fbar[9];
-  double dFbar[9];
+  const double* dFbar;  // device, 9 values (mode 2)
   double* fe;        // (24, N) element nodal forces, or null
   double* part;      // (blocks, kFsParts) partial sums
 };
 
+// Eight consecutive lanes per element, lane g at Gauss point g: the lane builds F_g (and dF_g) from the
+// element's 8 nodes, evaluates the stress (or its increment), and the element's 24 nodal forces
+// sum_g P_g dN_g,a are formed by a reduce-scatter over the 8 lanes (xor 4, 2, 1: each step keeps the
+// half of the nodes whose index bit matches the lane's and adds the partner's half), after which lane g
+// holds node a = g. The order of the additions is fixed, so results are deterministic.
 // MODE 0: energy; 1: energy, element forces of the gradient, sum of P; 2: Hessian-vector product
-// (element forces, sum of dP)
+// (element forces, sum of dP).
+constexpr int kFsElemPerBlock = kFsThreads / 8;
+
 template <int MODE>
 __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
   __shared__ double red[kFsThreads / 64];
-  const int64_t e = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
+  const int g = threadIdx.x & 7;
+  const int64_t e = (int64_t)blockIdx.x * kFsElemPerBlock + (threadIdx.x >> 3);
   const bool live = e < a.n;
   double acc[kFsParts];
 #pragma unroll
   for (int q = 0; q < kFsParts; ++q) acc[q] = 0.0;
+  // this lane's Gauss point and the shape-function factors there
+  const double gx = gauss_coord(g & 1), gy = gauss_coord((g >> 1) & 1), gz = gauss_coord((g >> 2) & 1);
+  auto dn = [&](int c, double (&d)[3]) {  // gradient of shape function c at this lane's point
+    const double fx = (c & 1) ? gx : 1.0 - gx, fy = (c & 2) ? gy : 1.0 - gy, fz = (c & 4) ? gz : 1.0 - gz;
+    d[0] = ((c & 1) ? fy : -fy) * fz;
+    d[1] = ((c & 2) ? fx : -fx) * fz;
+    d[2] = ((c & 4) ? fx : -fx) * fy;
+  };
+  double P[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) P[i][j] = 0.0;
+  double sc = 0.0;
   if (live) {
     const int k = (int)(e % a.nz);
     const int j = (int)((e / a.nz) % a.ny);
     const int i = (int)(e / ((int64_t)a.nz * a.ny));
     const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
-    int64_t node[8];
+    double F[3][3], dF[3][3];
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+      for (int dj = 0; dj < 3; ++dj) {
+        F[ci][dj] = a.Fbar[ci * 3 + dj];
+        dF[ci][dj] = MODE == 2 ? a.dFbar[ci * 3 + dj] : 0.0;
+      }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
       const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
-      node[c] = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+      const int64_t nd = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+      double d[3];
+      dn(c, d);
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) {
+        const double wv = a.w[nd * 3 + ci];
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) F[ci][dj] += wv * d[dj];
+        if (MODE == 2 && a.v != nullptr) {
+          const double vv = a.v[nd * 3 + ci];
+#pragma unroll
+          for (int dj = 0; dj < 3; ++dj) dF[ci][dj] += vv * d[dj];
+        }
+      }
     }
-    double wn[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) wn[c][d] = a.w[node[c] * 3 + d];
-    double vn[8][3];
+    sc = a.sg[e];
+    const double gam = a.sg[a.n + e];
     if (MODE == 2) {
-#pragma unroll
-      for (int c = 0; c < 8; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) vn[c][d] = a.v ? a.v[node[c] * 3 + d] : 0.0;
+      eff_dP(F, a.m, gam, dF, P);
+    } else {
+      acc[0] = sc * eff_psi_P(F, a.m, gam, P);
     }
-    const double r = clip01(a.rho[e]);
-    const double vp = pow(r, a.m.p);
-    const double s = a.m.emin + (1.0 - a.m.emin) * vp;
-    double gam = 1.0;
-    if (a.m.wls) {  // mech.py:107-112
-      const double b = a.m.wls_b, r0 = a.m.wls_r0;
-      gam = (tanh(b * r0) + tanh(b * (vp - r0))) / (tanh(b * r0) + tanh(b * (1.0 - r0)));
-    }
-    const double sc = a.wq * s;
-    double f[8][3];
-#pragma unroll
-    for (int c = 0; c < 8; ++c)
-#pragma unroll
-      for (int d = 0; d < 3; ++d) f[c][d] = 0.0;
-#pragma unroll 1
-    for (int g = 0; g < 8; ++g) {
-      double F[3][3];
+    if (MODE != 0) {
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci)
 #pragma unroll
         for (int dj = 0; dj < 3; ++dj) {
-          double h = a.Fbar[ci * 3 + dj];
-#pragma unroll
-          for (int c = 0; c < 8; ++c) h += wn[c][ci] * dshape(g, c, dj);
-          F[ci][dj] = h;
+          P[ci][dj] *= sc;
+          acc[1 + ci * 3 + dj] = P[ci][dj];
         }
-      double P[3][3];
-      if (MODE == 2) {
-        double dF[3][3];
+    }
+  }
+  if (MODE != 0) {
+    // force of this lane's Gauss point on node c (zero for lanes past the end: P is zero there)
+    auto contrib = [&](int c, double (&f)[3]) {
+      double d[3];
+      dn(c, d);
 #pragma unroll
-        for (int ci = 0; ci < 3; ++ci)
+      for (int ci = 0; ci < 3; ++ci) f[ci] = P[ci][0] * d[0] + P[ci][1] * d[1] + P[ci][2] * d[2];
+    };
+    // xor 4: keep the 4 nodes whose bit 2 equals the lane's
+    const bool b2 = (g & 4) != 0, b1 = (g & 2) != 0, b0 = (g & 1) != 0;
+    double h4[4][3];
 #pragma unroll
-          for (int dj = 0; dj < 3; ++dj) {
-            double h = a.dFbar[ci * 3 + dj];
+    for (int q = 0; q < 4; ++q) {
+      double lo[3], hi[3];
+      contrib(q, lo);
+      contrib(q + 4, hi);
 #pragma unroll
-            for (int c = 0; c < 8; ++c) h += vn[c][ci] * dshape(g, c, dj);
-            dF[ci][dj] = h;
-          }
-        eff_dP(F, a.m, gam, dF, P);
-      } else {
-        const double psi = eff_psi_P(F, a.m, gam, P);
-        acc[0] += psi;
-      }
-      if (MODE != 0) {
-#pragma unroll
-        for (int ci = 0; ci < 3; ++ci)
-#pragma unroll
-          for (int dj = 0; dj < 3; ++dj) acc[1 + ci * 3 + dj] += P[ci][dj];
-#pragma unroll
-        for (int c = 0; c < 8; ++c)
-#pragma unroll
-          for (int ci = 0; ci < 3; ++ci)
-            f[c][ci] += P[ci][0] * dshape(g, c, 0) + P[ci][1] * dshape(g, c, 1) + P[ci][2] * dshape(g, c, 2);
+      for (int ci = 0; ci < 3; ++ci) {
+        const double send = b2 ? lo[ci] : hi[ci];
+        const double keep = b2 ? hi[ci] : lo[ci];
+        h4[q][ci] = keep + __shfl_xor(send, 4, 8);
       }
     }
+    // xor 2: of nodes (q = 0..3 -> c = q + 4 b2) keep those with bit 1 equal to the lane's
+    double h2[2][3];
 #pragma unroll
-    for (int q = 0; q < kFsParts; ++q) acc[q] *= sc;
-    if (MODE != 0) {
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int c = 0; c < 8; ++c)
+      for (int ci = 0; ci < 3; ++ci) {
+        const double send = b1 ? h4[q][ci] : h4[q + 2][ci];
+        const double keep = b1 ? h4[q + 2][ci] : h4[q][ci];
+        h2[q][ci] = keep + __shfl_xor(send, 2, 8);
+      }
+    // xor 1: keep the node with bit 0 equal to the lane's, i.e. node g
+    double h1[3];
 #pragma unroll
-        for (int ci = 0; ci < 3; ++ci) a.fe[(int64_t)(c * 3 + ci) * a.n + e] = sc * f[c][ci];
+    for (int ci = 0; ci < 3; ++ci) {
+      const double send = b0 ? h2[0][ci] : h2[1][ci];
+      const double keep = b0 ? h2[1][ci] : h2[0][ci];
+      h1[ci] = keep + __shfl_xor(send, 1, 8);
+    }
+    if (live) {
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci) a.fe[(int64_t)(g * 3 + ci) * a.n + e] = h1[ci];
     }
   }
   const int nq = MODE == 0 ? 1 : kFsParts;
@@ -319,6 +349,22 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
     if (threadIdx.x == 0) a.part[(int64_t)blockIdx.x * kFsParts + q] = t;
     __syncthreads();
   }
+}
+
+// sg = (wq s_e, gamma_e) per element: s = emin + (1 - emin) clip01(rho)^p (mech.py:104-105) and the void
+// interpolation factor gamma (mech.py:107-112; 1 without it)
+__global__ void __launch_bounds__(kFsThreads) fs_prepare_kernel(const double* __restrict__ rho, FsMat m, double wq,
+                                                                int64_t n, double* __restrict__ sg) {
+  const int64_t e = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
+  if (e >= n) return;
+  const double vp = pow(clip01(rho[e]), m.p);
+  double gam = 1.0;
+  if (m.wls) {
+    const double b = m.wls_b, r0 = m.wls_r0;
+    gam = (tanh(b * r0) + tanh(b * (vp - r0))) / (tanh(b * r0) + tanh(b * (1.0 - r0)));
+  }
+  sg[e] = wq * (m.emin + (1.0 - m.emin) * vp);
+  sg[n + e] = gam;
 }
 
 // out(node, c) = sum over the node's 8 elements of their force on it (fixed order: corner 0..7)
@@ -363,37 +409,28 @@ extern "C" {
 long long nlfh_fs_work_doubles(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) return -1;
   const int64_t n = (int64_t)nx * ny * nz;
-  const int64_t nblk = (n + kFsThreads - 1) / kFsThreads;
+  const int64_t nblk = (n + kFsElemPerBlock - 1) / kFsElemPerBlock;
   return (long long)(24 * n + nblk * kFsParts);
 }
 
-int nlfh_fs_eval(const int* shape, const double* rho, const double* mat, int flags, const double* w,
-                 const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
-                 double* work, void* stream) {
-  if (shape == nullptr || mat == nullptr || Fbar == nullptr) {
-    set_error("nlfh_fs_eval: shape, mat and Fbar are required");
+static int fs_setup(const int* shape, const double* mat, int flags, FsArgs& a) {
+  if (shape == nullptr || mat == nullptr) {
+    set_error("nlfh_fs: shape and mat are required");
     return NLFH_EINVAL;
   }
   const int nx = shape[0], ny = shape[1], nz = shape[2];
   if (nx < 1 || ny < 1 || nz < 1) {
-    set_error("nlfh_fs_eval: cell dimensions must be positive");
+    set_error("nlfh_fs: cell dimensions must be positive");
     return NLFH_EINVAL;
   }
-  if (mode < 0 || mode > 2) {
-    set_error("nlfh_fs_eval: mode must be 0 (energy), 1 (energy and gradient) or 2 (Hessian-vector product)");
-    return NLFH_EINVAL;
-  }
-  if (rho == nullptr || w == nullptr || scalars == nullptr || work == nullptr || (mode >= 1 && grad == nullptr) ||
-      (mode == 2 && dFbar == nullptr)) {
-    set_error("nlfh_fs_eval: a required pointer is null");
-    return NLFH_EINVAL;
-  }
-  FsArgs a{};
   a.nx = nx;
   a.ny = ny;
   a.nz = nz;
   a.n = (int64_t)nx * ny * nz;
-  a.rho = rho;
+  if ((a.n + kFsElemPerBlock - 1) / kFsElemPerBlock > INT32_MAX) {
+    set_error("nlfh_fs: cell too large");
+    return NLFH_EINVAL;
+  }
   a.m.lam = mat[0];
   a.m.mu = mat[1];
   a.m.emin = mat[2];
@@ -403,17 +440,46 @@ int nlfh_fs_eval(const int* shape, const double* rho, const double* mat, int fla
   a.m.neo = (flags & 1) ? 1 : 0;
   a.m.wls = (flags & 2) ? 1 : 0;
   a.wq = 1.0 / (8.0 * (double)a.n);
-  a.w = w;
-  a.v = mode == 2 ? v : nullptr;
-  for (int q = 0; q < 9; ++q) {
-    a.Fbar[q] = Fbar[q];
-    a.dFbar[q] = mode == 2 ? dFbar[q] : 0.0;
-  }
-  const int64_t nblk = (a.n + kFsThreads - 1) / kFsThreads;
-  if (nblk > INT32_MAX) {
-    set_error("nlfh_fs_eval: cell too large");
+  return NLFH_OK;
+}
+
+int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int flags, double* sg, void* stream) {
+  FsArgs a{};
+  const int rc = fs_setup(shape, mat, flags, a);
+  if (rc != NLFH_OK) return rc;
+  if (rho == nullptr || sg == nullptr) {
+    set_error("nlfh_fs_prepare: rho and sg are required");
     return NLFH_EINVAL;
   }
+  const int64_t gblk = (a.n + kFsThreads - 1) / kFsThreads;
+  hipLaunchKernelGGL(fs_prepare_kernel, dim3((unsigned)gblk), dim3(kFsThreads), 0, (hipStream_t)stream, rho, a.m, a.wq,
+                     a.n, sg);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
+                 const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
+                 double* work, void* stream) {
+  FsArgs a{};
+  const int rc = fs_setup(shape, mat, flags, a);
+  if (rc != NLFH_OK) return rc;
+  if (mode < 0 || mode > 2) {
+    set_error("nlfh_fs_eval: mode must be 0 (energy), 1 (energy and gradient) or 2 (Hessian-vector product)");
+    return NLFH_EINVAL;
+  }
+  if (sg == nullptr || w == nullptr || Fbar == nullptr || scalars == nullptr || work == nullptr ||
+      (mode >= 1 && grad == nullptr) || (mode == 2 && dFbar == nullptr)) {
+    set_error("nlfh_fs_eval: a required pointer is null");
+    return NLFH_EINVAL;
+  }
+  a.sg = sg;
+  a.w = w;
+  a.v = mode == 2 ? v : nullptr;
+  for (int q = 0; q < 9; ++q) a.Fbar[q] = Fbar[q];
+  a.dFbar = mode == 2 ? dFbar : nullptr;
+  const int64_t nblk = (a.n + kFsElemPerBlock - 1) / kFsElemPerBlock;  // element blocks (partials)
+  const int64_t gblk = (a.n + kFsThreads - 1) / kFsThreads;             // gather blocks
   a.fe = work;
   a.part = work + 24 * a.n;
   hipStream_t st = (hipStream_t)stream;
@@ -423,7 +489,7 @@ int nlfh_fs_eval(const int* shape, const double* rho, const double* mat, int fla
   else hipLaunchKernelGGL(fs_element_kernel<2>, grid, dim3(kFsThreads), 0, st, a);
   HIP_TRY(hipGetLastError());
   if (mode >= 1) {
-    hipLaunchKernelGGL(fs_gather_kernel, grid, dim3(kFsThreads), 0, st, a, grad);
+    hipLaunchKernelGGL(fs_gather_kernel, dim3((unsigned)gblk), dim3(kFsThreads), 0, st, a, grad);
     HIP_TRY(hipGetLastError());
   }
   hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsThreads), 0, st, a.part, (int)nblk,

@@ -99,6 +99,9 @@ class Cell:
         nwork = int(self._lib.nlfh_fs_work_doubles(*self.shape))
         self._work = torch.empty(nwork, dtype=torch.float64, device=self._dev)
         self._scal = torch.empty(10, dtype=torch.float64, device=self._dev)
+        self._sg = torch.empty(2 * self.N, dtype=torch.float64, device=self._dev)
+        _lib.check(self._lib.nlfh_fs_prepare(self._shape_c, _ptr(self.rho), self._mat.ctypes.data_as(_dp),
+                                             self._flags, _ptr(self._sg), _stream()))
         self._lh = None
 
     def scale_of(self, rho):
@@ -117,6 +120,8 @@ class Cell:
 
     # --- operators ---------------------------------------------------------------------------------------
     def _eval(self, w, Fbar, mode, v=None, dFbar=None):
+        """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).
+        dFbar (mode 2): a 3x3 host array or a device tensor of 9 values."""
         w = torch.as_tensor(w, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
         if w.numel() != 3 * self.N:
             raise ValueError(f"w must hold {3 * self.N} values (shape {self.shape + (3,)})")
@@ -126,12 +131,16 @@ class Cell:
             v = torch.as_tensor(v, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
             if v.numel() != 3 * self.N:
                 raise ValueError(f"v must hold {3 * self.N} values (shape {self.shape + (3,)})")
-        dF = None if dFbar is None else np.ascontiguousarray(np.asarray(dFbar, dtype=np.float64).reshape(9))
-        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self.rho), self._mat.ctypes.data_as(_dp), self._flags,
-                                          _ptr(w), F.ctypes.data_as(_dp), _ptr(v),
-                                          dF.ctypes.data_as(_dp) if dF is not None else None, mode, _ptr(grad),
+        dF = None
+        if mode == 2:
+            dF = torch.as_tensor(np.zeros(9) if dFbar is None else dFbar, dtype=torch.float64,
+                                 device=self._dev).reshape(-1).contiguous()
+            if dF.numel() != 9:
+                raise ValueError("dFbar must hold 9 values")
+        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self._sg), self._mat.ctypes.data_as(_dp), self._flags,
+                                          _ptr(w), F.ctypes.data_as(_dp), _ptr(v), _ptr(dF), mode, _ptr(grad),
                                           _ptr(self._scal), _ptr(self._work), _stream()))
-        return grad, self._scal.cpu().numpy()
+        return grad, self._scal
 
     def energy(self, w, Fbar):
         """W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + grad w at g) (mech.py:141-154), a Python float."""
@@ -140,7 +149,7 @@ class Cell:
     def gradient(self, w, Fbar):
         """(dW/dw with w's shape, Pbar = dW/dFbar (3, 3)) (the reference's Cell._g and Cell._P)."""
         g, sc = self._eval(w, Fbar, 1)
-        return g.reshape(self.shape + (3,)), sc[1:].reshape(3, 3)
+        return g.reshape(self.shape + (3,)), sc[1:].cpu().numpy().reshape(3, 3)
 
     def mean_stress(self, w, Fbar):
         """Pbar = dW/dFbar (mech.py:180-181)."""
@@ -148,8 +157,8 @@ class Cell:
 
     def hessian_product(self, w, Fbar, v, dFbar):
         """Cell Hessian at (w, Fbar) applied to (v, dFbar): (w part, dW/dFbar part (3, 3))."""
-        g, sc = self._eval(w, Fbar, 2, v, np.zeros((3, 3)) if dFbar is None else dFbar)
-        return g.reshape(self.shape + (3,)), sc[1:].reshape(3, 3)
+        g, sc = self._eval(w, Fbar, 2, v, dFbar)
+        return g.reshape(self.shape + (3,)), sc[1:].cpu().numpy().reshape(3, 3)
 
     def precond_w(self, r):
         """Inverse symbol of the homogeneous linear operator with unit stiffness (mech.py:175-181). That
@@ -174,6 +183,8 @@ class Mixed:
         self.k = len(self.free)
         self.idx = tuple(np.array(self.free).T) if self.k else None
         self._dev = cell._dev
+        # row-major flat index of each free component, for device-side gathers and scatters
+        self._flat = torch.tensor([3 * a + b for a, b in self.free], dtype=torch.long, device=self._dev)
 
     # --- unknowns ----------------------------------------------------------------------------------------
     def z0(self, Fb):
@@ -208,14 +219,11 @@ class Mixed:
 
     def hvp_A(self, A, v):
         w, F = A
-        dF = np.zeros((3, 3))
+        dF = torch.zeros(9, dtype=torch.float64, device=self._dev)
         if self.k:
-            vm = v[self.nw:].cpu().numpy()
-            for j, (a, b) in enumerate(self.free):
-                dF[a, b] += vm[j]
-        hw, hP = self.c.hessian_product(w, F, v[: self.nw], dF)
-        mac = torch.as_tensor(hP[self.idx] if self.k else np.zeros(0), dtype=torch.float64, device=self._dev)
-        return torch.cat([hw.reshape(-1), mac])
+            dF.index_add_(0, self._flat, v[self.nw:])
+        hw, sc = self.c._eval(w, F, 2, v[: self.nw], dF)
+        return torch.cat([hw, sc[1:][self._flat]])
 
     def macro_diag(self, A, Fb):
         """Diagonal of the Hessian in the macro unknowns (for the preconditioner) (mech.py:240-246)."""
@@ -234,7 +242,8 @@ class Mixed:
     # --- linear solves (mech.py:253-288) ----------------------------------------------------------------
     def _cg(self, A, b, dmac, sws, tol, maxit):
         """Preconditioned CG on the mean-free space with the reference's negative-curvature exit: a
-        non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration)."""
+        non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration). The scalars
+        stay on the device; one host read per iteration fetches p.Ap, ||r|| and r.z for the loop test."""
         proj = self.proj
 
         def M(r):
@@ -244,19 +253,23 @@ class Mixed:
         bn = float(torch.linalg.norm(b)) + 1e-300
         z0 = M(b)
         x, r, p = torch.zeros_like(b), b, z0
-        rz = float(torch.dot(b, z0))
+        rz = torch.dot(b, z0)
         it, done, conv = 0, False, False
+        one = torch.ones((), dtype=torch.float64, device=self._dev)
         while not done and it < maxit:
             Ap = proj(self.hvp_A(A, p))
-            pAp = float(torch.dot(p, Ap))
-            neg = pAp <= 0
-            a = 0.0 if neg else rz / pAp
+            pAp = torch.dot(p, Ap)
+            negt = pAp <= 0
+            a = torch.where(negt, torch.zeros_like(pAp), rz / torch.where(negt, one, pAp))
             x1 = x + a * p
             r1 = r - a * Ap
-            conv1 = float(torch.linalg.norm(r1)) <= tol * bn
+            rn = torch.linalg.norm(r1)
             z1 = M(r1)
-            rz1 = float(torch.dot(r1, z1))
-            p1 = z1 + (rz1 / (1.0 if rz == 0 else rz)) * p
+            rz1 = torch.dot(r1, z1)
+            p1 = z1 + (rz1 / torch.where(rz == 0, one, rz)) * p
+            pAp_h, rn_h = torch.stack([pAp, rn]).tolist()
+            neg = pAp_h <= 0
+            conv1 = rn_h <= tol * bn
             if neg:
                 x1 = z0 if it == 0 else x
             x, r, p, rz, it = x1, r1, p1, rz1, it + 1
