@@ -207,6 +207,9 @@ int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int 
  * mode 2: grad = d^2W/dw dz [v; dFbar] and scalars[1..9] = sum of the stress increments, i.e. the product of
  *         the cell Hessian at the state (w, Fbar) with the direction (v, dFbar) (Mixed.hvp_A with the tangent of
  *         Mixed.tangent, mech.py:227-236; the tangent is applied matrix-free). v may be null (zero). */
+/* w (nx, ny, nz, 3) -= its per-component mean, in place: the w part of Mixed.proj (mech.py:236-239).
+ * work: as for nlfh_fs_eval (its first 3 ceil(N / 256) + 3 doubles are overwritten). */
+int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);
 int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
                  double* work, void* stream);

@@ -230,8 +230,10 @@ template <int MODE>
 __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
   __shared__ double red[kFsThreads / 64];
   const int g = threadIdx.x & 7;
-  const int64_t e = (int64_t)blockIdx.x * kFsElemPerBlock + (threadIdx.x >> 3);
-  const bool live = e < a.n;
+  // 32-bit indexing: the host limits cells to 24 N < 2^31 (the force buffer's extent)
+  const int n = (int)a.n;
+  const int e = (int)blockIdx.x * kFsElemPerBlock + (int)(threadIdx.x >> 3);
+  const bool live = e < n;
   double acc[kFsParts];
 #pragma unroll
   for (int q = 0; q < kFsParts; ++q) acc[q] = 0.0;
@@ -250,9 +252,9 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
     for (int j = 0; j < 3; ++j) P[i][j] = 0.0;
   double sc = 0.0;
   if (live) {
-    const int k = (int)(e % a.nz);
-    const int j = (int)((e / a.nz) % a.ny);
-    const int i = (int)(e / ((int64_t)a.nz * a.ny));
+    const int k = e % a.nz;
+    const int j = (e / a.nz) % a.ny;
+    const int i = e / (a.nz * a.ny);
     const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
     double F[3][3], dF[3][3];
 #pragma unroll
@@ -262,10 +264,12 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         F[ci][dj] = a.Fbar[ci * 3 + dj];
         dF[ci][dj] = MODE == 2 ? a.dFbar[ci * 3 + dj] : 0.0;
       }
-#pragma unroll
+    // two nodes' loads in flight at a time: unrolling all eight holds 48 loaded values in registers
+    // and cuts the occupancy in half
+#pragma unroll 2
     for (int c = 0; c < 8; ++c) {
       const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
-      const int64_t nd = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+      const int nd = (ii * a.ny + jj) * a.nz + kk;
       double d[3];
       dn(c, d);
 #pragma unroll
@@ -281,7 +285,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
       }
     }
     sc = a.sg[e];
-    const double gam = a.sg[a.n + e];
+    const double gam = a.sg[n + e];
     if (MODE == 2) {
       eff_dP(F, a.m, gam, dF, P);
     } else {
@@ -340,7 +344,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
     }
     if (live) {
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) a.fe[(int64_t)(g * 3 + ci) * a.n + e] = h1[ci];
+      for (int ci = 0; ci < 3; ++ci) a.fe[(g * 3 + ci) * n + e] = h1[ci];
     }
   }
   const int nq = MODE == 0 ? 1 : kFsParts;
@@ -369,23 +373,53 @@ __global__ void __launch_bounds__(kFsThreads) fs_prepare_kernel(const double* __
 
 // out(node, c) = sum over the node's 8 elements of their force on it (fixed order: corner 0..7)
 __global__ void __launch_bounds__(kFsThreads) fs_gather_kernel(const FsArgs a, double* __restrict__ out) {
-  const int64_t nd = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
-  if (nd >= a.n) return;
-  const int k = (int)(nd % a.nz);
-  const int j = (int)((nd / a.nz) % a.ny);
-  const int i = (int)(nd / ((int64_t)a.nz * a.ny));
+  const int n = (int)a.n;  // 24 n < 2^31 (host-checked)
+  const int nd = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  if (nd >= n) return;
+  const int k = nd % a.nz;
+  const int j = (nd / a.nz) % a.ny;
+  const int i = nd / (a.nz * a.ny);
   const int i0 = i == 0 ? a.nx - 1 : i - 1, j0 = j == 0 ? a.ny - 1 : j - 1, k0 = k == 0 ? a.nz - 1 : k - 1;
   double o[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     // the element whose corner c is this node starts one voxel back along each set bit of c
     const int ii = (c & 1) ? i0 : i, jj = (c & 2) ? j0 : j, kk = (c & 4) ? k0 : k;
-    const int64_t el = ((int64_t)ii * a.ny + jj) * a.nz + kk;
+    const int el = (ii * a.ny + jj) * a.nz + kk;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) o[d] += a.fe[(int64_t)(c * 3 + d) * a.n + el];
+    for (int d = 0; d < 3; ++d) o[d] += a.fe[(c * 3 + d) * n + el];
   }
 #pragma unroll
   for (int d = 0; d < 3; ++d) out[nd * 3 + d] = o[d];
+}
+
+// Mean removal of an (N, 3) field: per-block component sums (one node per thread), their fixed-order
+// total, and the subtraction (mech.py:236-239, the w part of Mixed.proj).
+__global__ void __launch_bounds__(kFsThreads) fs_sum3_kernel(const double* __restrict__ w, int n,
+                                                             double* __restrict__ part) {
+  __shared__ double red[kFsThreads / 64];
+  const int nd = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  for (int c = 0; c < 3; ++c) {
+    const double t = block_sum(nd < n ? w[nd * 3 + c] : 0.0, red);
+    if (threadIdx.x == 0) part[blockIdx.x * 3 + c] = t;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_mean3_kernel(const double* __restrict__ part, int nblk, double inv_n,
+                                                              double* __restrict__ mean) {
+  __shared__ double red[kFsThreads / 64];
+  const int c = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kFsThreads) acc += part[b * 3 + c];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) mean[c] = t * inv_n;
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_sub3_kernel(double* __restrict__ w, int n,
+                                                             const double* __restrict__ mean) {
+  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  if (i < 3 * n) w[i] -= mean[i % 3];
 }
 
 // out[q] = sum over blocks of part[b][q], lane-strided then a fixed butterfly (deterministic)
@@ -427,8 +461,8 @@ static int fs_setup(const int* shape, const double* mat, int flags, FsArgs& a) {
   a.ny = ny;
   a.nz = nz;
   a.n = (int64_t)nx * ny * nz;
-  if ((a.n + kFsElemPerBlock - 1) / kFsElemPerBlock > INT32_MAX) {
-    set_error("nlfh_fs: cell too large");
+  if (24 * a.n >= (int64_t)INT32_MAX) {  // the kernels index the (24, N) force buffer in 32 bits
+    set_error("nlfh_fs: cell too large (24 N must stay below 2^31)");
     return NLFH_EINVAL;
   }
   a.m.lam = mat[0];
@@ -494,6 +528,29 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
   }
   hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsThreads), 0, st, a.part, (int)nblk,
                      scalars);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_project(const int* shape, double* w, double* work, void* stream) {
+  FsArgs a{};
+  const double mat[6] = {0, 0, 0, 1, 0, 0};
+  const int rc = fs_setup(shape, mat, 0, a);
+  if (rc != NLFH_OK) return rc;
+  if (w == nullptr || work == nullptr) {
+    set_error("nlfh_fs_project: w and work are required");
+    return NLFH_EINVAL;
+  }
+  const int n = (int)a.n;
+  const int nb = (n + kFsThreads - 1) / kFsThreads;
+  double* part = work;
+  double* mean = work + 3 * (int64_t)nb;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fs_sum3_kernel, dim3(nb), dim3(kFsThreads), 0, st, w, n, part);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(fs_mean3_kernel, dim3(3), dim3(kFsThreads), 0, st, part, nb, 1.0 / n, mean);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(fs_sub3_kernel, dim3((3 * n + kFsThreads - 1) / kFsThreads), dim3(kFsThreads), 0, st, w, n, mean);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -200,8 +200,10 @@ class Mixed:
         return w, F
 
     def proj(self, v):
-        w = v[: self.nw].reshape(-1, 3)
-        return torch.cat([(w - w.mean(dim=0)).reshape(-1), v[self.nw:]])
+        """Copy of v with the per-component mean of its w part removed (mech.py:236-239)."""
+        out = v.clone()
+        _lib.check(self.c._lib.nlfh_fs_project(self.c._shape_c, _ptr(out), _ptr(self.c._work), _stream()))
+        return out
 
     def _E(self, z, Fb):
         return self.c.energy(*self.unpack(z, Fb))

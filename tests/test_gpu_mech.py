@@ -104,3 +104,15 @@ def test_small_strain_limit_is_the_linear_C(M, gpu_pkg):
         Pv[3:] *= 0.5  # engineering shear of the stress vector -> tensor components
         errs.append(_rel(Pv, t * C @ n))
     assert errs[0] < 5e-3 and errs[1] < 0.6 * errs[0]
+
+
+def test_projection_removes_the_component_means(M):
+    import torch
+    rng = np.random.default_rng(9)
+    cell = M.Cell(rng.uniform(0.0, 1.0, (7, 5, 3)))
+    mix = M.Mixed(cell)
+    v = rng.standard_normal(3 * cell.N + 2) + 0.3
+    out = mix.proj(torch.as_tensor(v, device="cuda")).cpu().numpy()
+    w = v[:-2].reshape(-1, 3)
+    np.testing.assert_allclose(out[:-2].reshape(-1, 3), w - w.mean(axis=0), atol=1e-14)
+    assert out[-2:].tolist() == v[-2:].tolist()
