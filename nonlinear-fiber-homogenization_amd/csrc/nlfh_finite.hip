@@ -217,14 +217,26 @@ struct FsArgs {
   double* part;      // (blocks, kFsParts) partial sums
 };
 
-// Eight consecutive lanes per element, lane g at Gauss point g: the lane builds F_g (and dF_g) from the
-// element's 8 nodes, evaluates the stress (or its increment), and the element's 24 nodal forces
-// sum_g P_g dN_g,a are formed by a reduce-scatter over the 8 lanes (xor 4, 2, 1: each step keeps the
-// half of the nodes whose index bit matches the lane's and adds the partner's half), after which lane g
-// holds node a = g. The order of the additions is fixed, so results are deterministic.
+// Eight consecutive lanes per element, lane g at Gauss point g. Lane g handles node c under the label
+// j = c ^ g: the shape gradient of node c at point g is then sigma_g (.) T_j, where T_j depends on j only
+// (the weight of a node along an axis is 1/2 + 1/(2 sqrt 3) when node and point sit on the same side, whatever
+// the side) and sigma_g flips the sign of axis d when bit d of g is set. The element's 24 nodal forces
+// sum_g P_g grad N_c(g) come out of a reduce-scatter over the 8 lanes in which every lane keeps labels
+// j < 4 (then j < 2, then j = 0) and adds the partner's j + 4 (j + 2, j + 1): the partner's label for the
+// same node (xor 4, 2, 1 of the lane) is exactly that. Lane g ends with label 0, node g. The order of the
+// additions is fixed, so results are deterministic.
 // MODE 0: energy; 1: energy, element forces of the gradient, sum of P; 2: Hessian-vector product
 // (element forces, sum of dP).
 constexpr int kFsElemPerBlock = kFsThreads / 8;
+constexpr double kGaussSame = 0.5 + kGaussOff;   // weight of a node on the point's side of an axis
+constexpr double kGaussOther = 0.5 - kGaussOff;  // on the other side
+
+// T_j[d]: gradient of the shape function of label j along axis d at the point labelled 0
+__host__ __device__ constexpr double tgrad(int j, int d) {
+  const double fx = (j & 1) ? kGaussOther : kGaussSame, fy = (j & 2) ? kGaussOther : kGaussSame,
+               fz = (j & 4) ? kGaussOther : kGaussSame;
+  return d == 0 ? ((j & 1) ? fy * fz : -fy * fz) : (d == 1 ? ((j & 2) ? fx * fz : -fx * fz) : ((j & 4) ? fx * fy : -fx * fy));
+}
 
 template <int MODE>
 __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
@@ -237,20 +249,12 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
   double acc[kFsParts];
 #pragma unroll
   for (int q = 0; q < kFsParts; ++q) acc[q] = 0.0;
-  // this lane's Gauss point and the shape-function factors there
-  const double gx = gauss_coord(g & 1), gy = gauss_coord((g >> 1) & 1), gz = gauss_coord((g >> 2) & 1);
-  auto dn = [&](int c, double (&d)[3]) {  // gradient of shape function c at this lane's point
-    const double fx = (c & 1) ? gx : 1.0 - gx, fy = (c & 2) ? gy : 1.0 - gy, fz = (c & 4) ? gz : 1.0 - gz;
-    d[0] = ((c & 1) ? fy : -fy) * fz;
-    d[1] = ((c & 2) ? fx : -fx) * fz;
-    d[2] = ((c & 4) ? fx : -fx) * fy;
-  };
+  const double sig[3] = {(g & 1) ? -1.0 : 1.0, (g & 2) ? -1.0 : 1.0, (g & 4) ? -1.0 : 1.0};
   double P[3][3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) P[i][j] = 0.0;
-  double sc = 0.0;
   if (live) {
     const int k = e % a.nz;
     const int j = (e / a.nz) % a.ny;
@@ -264,14 +268,17 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         F[ci][dj] = a.Fbar[ci * 3 + dj];
         dF[ci][dj] = MODE == 2 ? a.dFbar[ci * 3 + dj] : 0.0;
       }
-    // two nodes' loads in flight at a time: unrolling all eight holds 48 loaded values in registers
-    // and cuts the occupancy in half
+    // the loads go node by node, the same node for the 8 lanes of an element (one address per element and
+    // instruction); the lane's coefficient for node c is sigma_g (.) T_(c^g). Two nodes' loads in flight at
+    // a time: unrolling all eight holds 48 loaded values in registers and cuts the occupancy in half.
 #pragma unroll 2
     for (int c = 0; c < 8; ++c) {
       const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
       const int nd = (ii * a.ny + jj) * a.nz + kk;
-      double d[3];
-      dn(c, d);
+      const int jl = c ^ g;
+      const double fx = (jl & 1) ? kGaussOther : kGaussSame, fy = (jl & 2) ? kGaussOther : kGaussSame,
+                   fz = (jl & 4) ? kGaussOther : kGaussSame;
+      const double d[3] = {((c & 1) ? fy : -fy) * fz, ((c & 2) ? fx : -fx) * fz, ((c & 4) ? fx : -fx) * fy};
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci) {
         const double wv = a.w[nd * 3 + ci];
@@ -284,7 +291,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         }
       }
     }
-    sc = a.sg[e];
+    const double sc = a.sg[e];
     const double gam = a.sg[n + e];
     if (MODE == 2) {
       eff_dP(F, a.m, gam, dF, P);
@@ -302,46 +309,28 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
     }
   }
   if (MODE != 0) {
-    // force of this lane's Gauss point on node c (zero for lanes past the end: P is zero there)
-    auto contrib = [&](int c, double (&f)[3]) {
-      double d[3];
-      dn(c, d);
+    // P (.) sigma: the forces in labelled form use the constants T_j (zero for lanes past the end)
+    double Ps[3][3];
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) f[ci] = P[ci][0] * d[0] + P[ci][1] * d[1] + P[ci][2] * d[2];
+    for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+      for (int dj = 0; dj < 3; ++dj) Ps[ci][dj] = P[ci][dj] * sig[dj];
+    auto contrib = [&](int jl, int ci) {
+      return Ps[ci][0] * tgrad(jl, 0) + Ps[ci][1] * tgrad(jl, 1) + Ps[ci][2] * tgrad(jl, 2);
     };
-    // xor 4: keep the 4 nodes whose bit 2 equals the lane's
-    const bool b2 = (g & 4) != 0, b1 = (g & 2) != 0, b0 = (g & 1) != 0;
     double h4[4][3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      double lo[3], hi[3];
-      contrib(q, lo);
-      contrib(q + 4, hi);
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) {
-        const double send = b2 ? lo[ci] : hi[ci];
-        const double keep = b2 ? hi[ci] : lo[ci];
-        h4[q][ci] = keep + __shfl_xor(send, 4, 8);
-      }
-    }
-    // xor 2: of nodes (q = 0..3 -> c = q + 4 b2) keep those with bit 1 equal to the lane's
+      for (int ci = 0; ci < 3; ++ci) h4[q][ci] = contrib(q, ci) + __shfl_xor(contrib(q + 4, ci), 4, 8);
     double h2[2][3];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) {
-        const double send = b1 ? h4[q][ci] : h4[q + 2][ci];
-        const double keep = b1 ? h4[q + 2][ci] : h4[q][ci];
-        h2[q][ci] = keep + __shfl_xor(send, 2, 8);
-      }
-    // xor 1: keep the node with bit 0 equal to the lane's, i.e. node g
+      for (int ci = 0; ci < 3; ++ci) h2[q][ci] = h4[q][ci] + __shfl_xor(h4[q + 2][ci], 2, 8);
     double h1[3];
 #pragma unroll
-    for (int ci = 0; ci < 3; ++ci) {
-      const double send = b0 ? h2[0][ci] : h2[1][ci];
-      const double keep = b0 ? h2[1][ci] : h2[0][ci];
-      h1[ci] = keep + __shfl_xor(send, 1, 8);
-    }
+    for (int ci = 0; ci < 3; ++ci) h1[ci] = h2[0][ci] + __shfl_xor(h2[1][ci], 1, 8);
     if (live) {
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci) a.fe[(g * 3 + ci) * n + e] = h1[ci];
