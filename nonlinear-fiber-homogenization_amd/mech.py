@@ -118,6 +118,19 @@ class Cell:
     def with_rho(self, rho):
         return Cell(rho, self.E, self.nu, self.material, self.wls, self.emin, self.p)
 
+    def tiled(self, k):
+        """k x k x k supercell (2D: k x k x 1) (mech.py:118-121)."""
+        reps = (k, k, 1) if self.shape[2] == 1 else (k, k, k)
+        return Cell(self.rho.repeat(*reps), self.E, self.nu, self.material, self.wls, self.emin, self.p)
+
+    @property
+    def Kinv(self):
+        """Inverse symbol of the unit-stiffness linear operator of this cell, (nx, ny, nz//2+1, 3, 3) complex
+        (mech.py:175-178): N times the linear path's symbol."""
+        if self._lh is None:
+            self._lh = LinearHomogenization(self.shape, self.E, self.nu, self.emin, self.p)
+        return self._lh.Kinv * float(self.N)
+
     # --- operators ---------------------------------------------------------------------------------------
     def _eval(self, w, Fbar, mode, v=None, dFbar=None):
         """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).

@@ -116,3 +116,19 @@ def test_projection_removes_the_component_means(M):
     w = v[:-2].reshape(-1, 3)
     np.testing.assert_allclose(out[:-2].reshape(-1, 3), w - w.mean(axis=0), atol=1e-14)
     assert out[-2:].tolist() == v[-2:].tolist()
+
+
+def test_tiled_cell_has_the_same_energy_density(M):
+    """A k^3 supercell of a periodic cell under the same uniform state has the same energy per volume
+    (mech.py:118-121)."""
+    rng = np.random.default_rng(8)
+    rho = rng.uniform(0.0, 1.0, (4, 3, 5))
+    cell = M.Cell(rho)
+    big = cell.tiled(2)
+    assert big.shape == (8, 6, 10)
+    Fb = np.eye(3) + 0.03 * rng.standard_normal((3, 3))
+    w0 = np.zeros(cell.shape + (3,))
+    w1 = np.zeros(big.shape + (3,))
+    assert big.energy(w1, Fb) == pytest.approx(cell.energy(w0, Fb), rel=1e-12)
+    np.testing.assert_allclose(big.mean_stress(w1, Fb), cell.mean_stress(w0, Fb), rtol=1e-11, atol=1e-15)
+    assert cell.Kinv.shape == (4, 3, 3, 3, 3)
