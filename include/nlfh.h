@@ -199,14 +199,18 @@ int nlfh_dmap_rho_vjp(nlfh_dmap* dm, const double* phi, double beta, const doubl
  * Fbar: host 3x3 row-major; dFbar: device 3x3 row-major (so that a CG loop can form it from its vectors
  * without a host round trip). work: nlfh_fs_work_doubles(nx, ny, nz) doubles of device memory. */
 long long nlfh_fs_work_doubles(int nx, int ny, int nz);
-/* sg (2, N) = (s_e / (8N), gamma_e): the stiffness scale and void interpolation factor of every element
- * (Cell.scale_of / Cell.gamma_of, mech.py:104-112), computed once per density. */
+/* sg (4, N) = (s_e / (8N), gamma_e, (ds_e/drho_e) / (8N), dgamma_e/drho_e): the stiffness scale and void
+ * interpolation factor of every element and their density derivatives (Cell.scale_of / Cell.gamma_of,
+ * mech.py:104-112, with clip01's derivative convention), computed once per density. */
 int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int flags, double* sg, void* stream);
 /* mode 0: scalars[0] = W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + H_g) (Cell.energy, mech.py:141-154);
  * mode 1: also grad = dW/dw (Cell._g, mech.py:172) and scalars[1..9] = Pbar = dW/dFbar (Cell._P, mech.py:173);
  * mode 2: grad = d^2W/dw dz [v; dFbar] and scalars[1..9] = sum of the stress increments, i.e. the product of
  *         the cell Hessian at the state (w, Fbar) with the direction (v, dFbar) (Mixed.hvp_A with the tangent of
- *         Mixed.tangent, mech.py:227-236; the tangent is applied matrix-free). v may be null (zero). */
+ *         Mixed.tangent, mech.py:227-236; the tangent is applied matrix-free). v may be null (zero);
+ * mode 3: grad[e] (N values) = d/drho_e of sum_g wq s_e P_eff(F_g; gamma_e) : (dFbar + grad v at g), the density
+ *         coupling of Mixed.grad_rho (mech.py:349-363) and the explicit density derivative of Pbar : dFbar;
+ *         scalars unused (may be null). */
 /* w (nx, ny, nz, 3) -= its per-component mean, in place: the w part of Mixed.proj (mech.py:236-239).
  * work: as for nlfh_fs_eval (its first 3 ceil(N / 256) + 3 doubles are overwritten). */
 int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);

@@ -1,6 +1,7 @@
 // Finite-strain cell operators on voxel grids (FP64): the cell energy, its gradient in the periodic
-// displacement w and in the mean deformation gradient Fbar, and the Hessian-vector product with the
-// tangent at a state, applied matrix-free.
+// displacement w and in the mean deformation gradient Fbar, the Hessian-vector product with the
+// tangent at a state, applied matrix-free, and the per-element density derivative of the stress work
+// <P, dF> that the adjoint density gradient needs (Mixed.grad_rho, mech.py:348-363).
 //
 // Follows nlfiber/mech.py: Q1 hexahedra with 2x2x2 Gauss points (mech.py:124-132, the element
 // gradients of linear.py:33-42), the energy W = wq sum_e s_e sum_g psi_eff(Fbar + H_g) with wq = 1/(8N)
@@ -203,10 +204,41 @@ __device__ __forceinline__ void eff_dP(const double (&F)[3][3], const FsMat& m, 
     for (int j = 0; j < 3; ++j) dP[i][j] = g2 * (Dn[i][j] - Dl[i][j]) + Dl[i][j];
 }
 
+// P_eff at F and its derivative in the interpolation factor gamma (mech.py:134-139):
+// dP/dgamma = (P_nl(Fg) - P_lin(Fg)) + gamma (A_nl(Fg) - A_lin) : (F - I), Fg = I + gamma (F - I); zero for the
+// linear material (the interpolation cancels) and without the interpolation (gamma is not a function of rho)
+__device__ __forceinline__ void eff_P_dgamma(const double (&F)[3][3], const FsMat& m, double gam, double (&P)[3][3],
+                                             double (&Pg)[3][3]) {
+  (void)eff_psi_P(F, m, gam, P);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Pg[i][j] = 0.0;
+  if (!m.wls || !m.neo) return;
+  double H[3][3], Fg[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      H[i][j] = F[i][j] - (i == j);
+      Fg[i][j] = (i == j) + gam * H[i][j];
+    }
+  double Pn[3][3], Pl[3][3], Fi[3][3], lnJ;
+  (void)neo_psi_P(Fg, m.lam, m.mu, Pn, Fi, lnJ);
+  (void)lin_psi_P(Fg, m.lam, m.mu, Pl);
+  double Dn[3][3], Dl[3][3];
+  neo_dP(Fi, lnJ, H, m.lam, m.mu, Dn);
+  lin_dP(H, m.lam, m.mu, Dl);
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Pg[i][j] = (Pn[i][j] - Pl[i][j]) + gam * (Dn[i][j] - Dl[i][j]);
+}
+
 struct FsArgs {
   int nx, ny, nz;
   int64_t n;
-  const double* sg;  // (2, N): wq s_e and gamma_e of every element (fs_prepare_kernel)
+  const double* sg;  // (4, N): wq s_e, gamma_e, wq ds_e/drho_e, dgamma_e/drho_e of every element (fs_prepare_kernel)
   FsMat m;
   double wq;
   const double* w;   // state (N, 3)
@@ -266,7 +298,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
 #pragma unroll
       for (int dj = 0; dj < 3; ++dj) {
         F[ci][dj] = a.Fbar[ci * 3 + dj];
-        dF[ci][dj] = MODE == 2 ? a.dFbar[ci * 3 + dj] : 0.0;
+        dF[ci][dj] = MODE >= 2 ? a.dFbar[ci * 3 + dj] : 0.0;
       }
     // the loads go node by node, the same node for the 8 lanes of an element (one address per element and
     // instruction); the lane's coefficient for node c is sigma_g (.) T_(c^g). Two nodes' loads in flight at
@@ -284,7 +316,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         const double wv = a.w[nd * 3 + ci];
 #pragma unroll
         for (int dj = 0; dj < 3; ++dj) F[ci][dj] += wv * d[dj];
-        if (MODE == 2 && a.v != nullptr) {
+        if (MODE >= 2 && a.v != nullptr) {
           const double vv = a.v[nd * 3 + ci];
 #pragma unroll
           for (int dj = 0; dj < 3; ++dj) dF[ci][dj] += vv * d[dj];
@@ -293,12 +325,25 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
     }
     const double sc = a.sg[e];
     const double gam = a.sg[n + e];
-    if (MODE == 2) {
+    if (MODE == 3) {
+      // d/drho_e of wq s_e sum_g P(F_g; gamma_e) : dF_g, this lane's point: summed over the 8 lanes below
+      double Pv[3][3], Pgam[3][3];
+      eff_P_dgamma(F, a.m, gam, Pv, Pgam);
+      double t = 0.0, u = 0.0;
+#pragma unroll
+      for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+        for (int dj = 0; dj < 3; ++dj) {
+          t += Pv[ci][dj] * dF[ci][dj];
+          u += Pgam[ci][dj] * dF[ci][dj];
+        }
+      acc[0] = a.sg[2 * n + e] * t + sc * a.sg[3 * n + e] * u;
+    } else if (MODE == 2) {
       eff_dP(F, a.m, gam, dF, P);
     } else {
       acc[0] = sc * eff_psi_P(F, a.m, gam, P);
     }
-    if (MODE != 0) {
+    if (MODE == 1 || MODE == 2) {
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci)
 #pragma unroll
@@ -307,6 +352,15 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
           acc[1 + ci * 3 + dj] = P[ci][dj];
         }
     }
+  }
+  if (MODE == 3) {
+    // per-element sum over the 8 Gauss points (fixed butterfly); no block partials
+    double t = acc[0];
+    t += __shfl_xor(t, 4, 8);
+    t += __shfl_xor(t, 2, 8);
+    t += __shfl_xor(t, 1, 8);
+    if (live && g == 0) a.fe[e] = t;
+    return;
   }
   if (MODE != 0) {
     // P (.) sigma: the forces in labelled form use the constants T_j (zero for lanes past the end)
@@ -344,20 +398,30 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
   }
 }
 
-// sg = (wq s_e, gamma_e) per element: s = emin + (1 - emin) clip01(rho)^p (mech.py:104-105) and the void
-// interpolation factor gamma (mech.py:107-112; 1 without it)
+// sg = (wq s_e, gamma_e, wq ds_e/drho_e, dgamma_e/drho_e) per element: s = emin + (1 - emin) clip01(rho)^p
+// (mech.py:104-105) and the void interpolation factor gamma (mech.py:107-112; 1 without it), with the
+// derivative convention of the reference's clip01 (1 on [0, 1] including the bounds, 0 outside; mech.py:31-35)
 __global__ void __launch_bounds__(kFsThreads) fs_prepare_kernel(const double* __restrict__ rho, FsMat m, double wq,
                                                                 int64_t n, double* __restrict__ sg) {
   const int64_t e = (int64_t)blockIdx.x * kFsThreads + threadIdx.x;
   if (e >= n) return;
-  const double vp = pow(clip01(rho[e]), m.p);
-  double gam = 1.0;
+  const double r = rho[e];
+  const double c = clip01(r);
+  const double dc = (r > 1.0 || r < 0.0) ? 0.0 : 1.0;
+  const double vp = pow(c, m.p);
+  const double dvp = dc * m.p * pow(c, m.p - 1.0);
+  double gam = 1.0, dgam = 0.0;
   if (m.wls) {
     const double b = m.wls_b, r0 = m.wls_r0;
-    gam = (tanh(b * r0) + tanh(b * (vp - r0))) / (tanh(b * r0) + tanh(b * (1.0 - r0)));
+    const double den = tanh(b * r0) + tanh(b * (1.0 - r0));
+    const double th = tanh(b * (vp - r0));
+    gam = (tanh(b * r0) + th) / den;
+    dgam = b * (1.0 - th * th) / den * dvp;
   }
   sg[e] = wq * (m.emin + (1.0 - m.emin) * vp);
   sg[n + e] = gam;
+  sg[2 * n + e] = wq * (1.0 - m.emin) * dvp;
+  sg[3 * n + e] = dgam;
 }
 
 // out(node, c) = sum over the node's 8 elements of their force on it (fixed order: corner 0..7)
@@ -487,26 +551,33 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
   FsArgs a{};
   const int rc = fs_setup(shape, mat, flags, a);
   if (rc != NLFH_OK) return rc;
-  if (mode < 0 || mode > 2) {
-    set_error("nlfh_fs_eval: mode must be 0 (energy), 1 (energy and gradient) or 2 (Hessian-vector product)");
+  if (mode < 0 || mode > 3) {
+    set_error("nlfh_fs_eval: mode must be 0 (energy), 1 (energy and gradient), 2 (Hessian-vector product) or 3 "
+              "(density sensitivity)");
     return NLFH_EINVAL;
   }
-  if (sg == nullptr || w == nullptr || Fbar == nullptr || scalars == nullptr || work == nullptr ||
-      (mode >= 1 && grad == nullptr) || (mode == 2 && dFbar == nullptr)) {
+  if (sg == nullptr || w == nullptr || Fbar == nullptr || work == nullptr || (mode <= 2 && scalars == nullptr) ||
+      (mode >= 1 && grad == nullptr) || (mode >= 2 && dFbar == nullptr)) {
     set_error("nlfh_fs_eval: a required pointer is null");
     return NLFH_EINVAL;
   }
   a.sg = sg;
   a.w = w;
-  a.v = mode == 2 ? v : nullptr;
+  a.v = mode >= 2 ? v : nullptr;
   for (int q = 0; q < 9; ++q) a.Fbar[q] = Fbar[q];
-  a.dFbar = mode == 2 ? dFbar : nullptr;
+  a.dFbar = mode >= 2 ? dFbar : nullptr;
   const int64_t nblk = (a.n + kFsElemPerBlock - 1) / kFsElemPerBlock;  // element blocks (partials)
   const int64_t gblk = (a.n + kFsThreads - 1) / kFsThreads;             // gather blocks
   a.fe = work;
   a.part = work + 24 * a.n;
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)nblk);
+  if (mode == 3) {  // per-element output straight into grad (N values)
+    a.fe = grad;
+    hipLaunchKernelGGL(fs_element_kernel<3>, grid, dim3(kFsThreads), 0, st, a);
+    HIP_TRY(hipGetLastError());
+    return NLFH_OK;
+  }
   if (mode == 0) hipLaunchKernelGGL(fs_element_kernel<0>, grid, dim3(kFsThreads), 0, st, a);
   else if (mode == 1) hipLaunchKernelGGL(fs_element_kernel<1>, grid, dim3(kFsThreads), 0, st, a);
   else hipLaunchKernelGGL(fs_element_kernel<2>, grid, dim3(kFsThreads), 0, st, a);

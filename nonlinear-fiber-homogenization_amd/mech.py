@@ -14,8 +14,10 @@ Kept from the reference (nlfiber/mech.py):
 Differences of form, not of result: the tangent is applied matrix-free (`tangent` returns the state it is
 taken at instead of 81 numbers per Gauss point); z, w and Fbar are torch / numpy float64 arrays instead of
 JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
-Not built: the adjoint grad_rho, the stability analysis (LOBPCG, effective tangent, rank-one ellipticity)
-and the third-order constants (mech.py:332-515).
+  * Mixed.grad_rho, the adjoint density gradient of a quantity of the equilibrium state (mech.py:348-363), with
+    q a torch function (torch autograd stands in for jax.grad; Cell.mean_stress_t is the differentiable Pbar).
+Not built: the stability analysis (LOBPCG, effective tangent, rank-one ellipticity) and the third-order
+constants (mech.py:365-515).
 """
 import ctypes
 import math
@@ -74,6 +76,34 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else None
 
 
+class _MeanStressFn(torch.autograd.Function):
+    """Pbar(w, Fbar; rho) with the backward of the cell Hessian: w-bar = H_wF G, Fbar-bar = H_FF G (one
+    Hessian-vector product with direction (0, G)), rho-bar = the density sensitivity of Pbar : G."""
+
+    @staticmethod
+    def forward(ctx, w, F, rho, cell):
+        Fh = F.detach().to("cpu", torch.float64).numpy().reshape(3, 3)
+        _, sc = cell._eval(w.detach(), Fh, 1)
+        ctx.cell, ctx.Fh = cell, Fh
+        ctx.save_for_backward(w.detach())
+        return sc[1:].clone().reshape(3, 3)
+
+    @staticmethod
+    def backward(ctx, gP):
+        cell, Fh = ctx.cell, ctx.Fh
+        (w,) = ctx.saved_tensors
+        dF = gP.detach().reshape(-1).to(torch.float64).contiguous()
+        gw = gF = gr = None
+        if ctx.needs_input_grad[0] or ctx.needs_input_grad[1]:
+            hw, sc = cell._eval(w, Fh, 2, None, dF)
+            gw = hw.reshape(w.shape)
+            gF = sc[1:].clone().reshape(3, 3)
+        if ctx.needs_input_grad[2]:
+            g3, _ = cell._eval(w, Fh, 3, None, dF)
+            gr = g3.reshape(cell.shape)
+        return gw, gF, gr, None
+
+
 class Cell:
     """A periodic voxel cell. rho: design density (nx, ny, nz) or (n, n) in [0, 1] (mech.py:91-121)."""
 
@@ -99,7 +129,7 @@ class Cell:
         nwork = int(self._lib.nlfh_fs_work_doubles(*self.shape))
         self._work = torch.empty(nwork, dtype=torch.float64, device=self._dev)
         self._scal = torch.empty(10, dtype=torch.float64, device=self._dev)
-        self._sg = torch.empty(2 * self.N, dtype=torch.float64, device=self._dev)
+        self._sg = torch.empty(4 * self.N, dtype=torch.float64, device=self._dev)
         _lib.check(self._lib.nlfh_fs_prepare(self._shape_c, _ptr(self.rho), self._mat.ctypes.data_as(_dp),
                                              self._flags, _ptr(self._sg), _stream()))
         self._lh = None
@@ -134,18 +164,20 @@ class Cell:
     # --- operators ---------------------------------------------------------------------------------------
     def _eval(self, w, Fbar, mode, v=None, dFbar=None):
         """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).
-        dFbar (mode 2): a 3x3 host array or a device tensor of 9 values."""
+        dFbar (modes 2, 3): a 3x3 host array or a device tensor of 9 values. Mode 3 returns the per-element
+        density sensitivity (N values) as grad."""
         w = torch.as_tensor(w, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
         if w.numel() != 3 * self.N:
             raise ValueError(f"w must hold {3 * self.N} values (shape {self.shape + (3,)})")
         F = np.ascontiguousarray(np.asarray(Fbar, dtype=np.float64).reshape(9))
-        grad = torch.empty_like(w) if mode >= 1 else None
+        grad = (torch.empty(self.N, dtype=torch.float64, device=self._dev) if mode == 3 else torch.empty_like(w)) \
+            if mode >= 1 else None
         if v is not None:
             v = torch.as_tensor(v, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
             if v.numel() != 3 * self.N:
                 raise ValueError(f"v must hold {3 * self.N} values (shape {self.shape + (3,)})")
         dF = None
-        if mode == 2:
+        if mode >= 2:
             dF = torch.as_tensor(np.zeros(9) if dFbar is None else dFbar, dtype=torch.float64,
                                  device=self._dev).reshape(-1).contiguous()
             if dF.numel() != 9:
@@ -172,6 +204,20 @@ class Cell:
         """Cell Hessian at (w, Fbar) applied to (v, dFbar): (w part, dW/dFbar part (3, 3))."""
         g, sc = self._eval(w, Fbar, 2, v, dFbar)
         return g.reshape(self.shape + (3,)), sc[1:].cpu().numpy().reshape(3, 3)
+
+    def rho_sensitivity(self, w, Fbar, v, dFbar):
+        """d/drho_e of sum_e wq s_e sum_g P_eff(F_g; gamma_e) : (dFbar + grad v at g), per element (nx, ny, nz):
+        the density term of Mixed.grad_rho's coupling (mech.py:360-362)."""
+        g, _ = self._eval(w, Fbar, 3, v, dFbar)
+        return g.reshape(self.shape)
+
+    def mean_stress_t(self, w, Fbar, rho=None):
+        """Pbar (3, 3) as a device tensor differentiable (torch autograd) in w, Fbar (a torch 3x3 or host array)
+        and this cell's density rho (pass the tensor whose gradient you want; its values must be the cell's): the
+        counterpart of differentiating the reference's Cell._P / grad(energy, argnums=1) with jax.grad."""
+        F = Fbar if isinstance(Fbar, torch.Tensor) else torch.as_tensor(Fbar, dtype=torch.float64, device=self._dev)
+        r = self.rho if rho is None else rho
+        return _MeanStressFn.apply(torch.as_tensor(w, dtype=torch.float64, device=self._dev), F, r, self)
 
     def precond_w(self, r):
         """Inverse symbol of the homogeneous linear operator with unit stiffness (mech.py:175-181). That
@@ -210,6 +256,15 @@ class Mixed:
         F = np.array(Fb, dtype=np.float64)
         if self.k:
             F[self.idx] = z[self.nw:].cpu().numpy()
+        return w, F
+
+    def unpack_t(self, z, Fb):
+        """unpack as torch tensors (w, Fbar 3x3 on the device), differentiable in z (for quantities q(z, Fb, rho)
+        handed to grad_rho)."""
+        w = z[: self.nw].reshape(self.shape + (3,))
+        F = torch.as_tensor(np.asarray(Fb, dtype=np.float64), device=self._dev)
+        if self.k:
+            F = F.reshape(-1).index_put((self._flat,), z[self.nw:]).reshape(3, 3)
         return w, F
 
     def proj(self, v):
@@ -300,6 +355,31 @@ class Mixed:
         dmac = self.macro_diag(A, None) if self.k else np.zeros(0)
         x, it, conv, _ = self._cg(A, b, dmac, self._sws(), tol, int(maxit))
         return x, int(it), bool(conv)
+
+    # --- adjoint (mech.py:348-363) -------------------------------------------------------------------------
+    def grad_rho(self, z, Fb, q, cg_tol=1e-10, cg_maxit=20000):
+        """d q / d rho for a scalar q(z, Fb, rho) of the equilibrium state z(rho) at prescribed Fb:
+        H lam = dq/dz, dq/drho = dq/drho|_z - d/drho <lam, grad_z W(z; rho)>. q is a torch function of z (a
+        device tensor) and rho (the cell's density tensor), e.g. of the macro unknowns z[nw:] or of
+        cell.mean_stress_t(*mix.unpack_t(z, Fb), rho); its derivatives come from torch autograd."""
+        Fb = np.asarray(Fb, dtype=np.float64)
+        z = torch.as_tensor(z, dtype=torch.float64, device=self._dev)
+        zz = z.detach().clone().requires_grad_(True)
+        rr = self.c.rho.detach().clone().requires_grad_(True)
+        with torch.enable_grad():
+            qv = q(zz, Fb, rr)
+            gz, gr = torch.autograd.grad(qv, (zz, rr), allow_unused=True)
+        gz = torch.zeros_like(z) if gz is None else gz.detach()
+        gr = torch.zeros_like(self.c.rho) if gr is None else gr.detach()
+        dq_dz = self.proj(gz)
+        A = self.tangent(z, Fb)
+        lam, its, ok = self.solve_linear(A, dq_dz, cg_tol, cg_maxit)
+        w, F = A
+        dF = torch.zeros(9, dtype=torch.float64, device=self._dev)
+        if self.k:
+            dF.index_add_(0, self._flat, lam[self.nw:])
+        coupling = self.c.rho_sensitivity(w, F, lam[: self.nw], dF)
+        return gr - coupling, dict(cg=its, converged=ok)
 
     # --- Newton-CG (mech.py:290-330) --------------------------------------------------------------------
     def solve(self, Fbar, z0=None, rtol=1e-9, maxnewton=30, cg_tol=1e-6, cg_maxit=20000, g_ref=None):

@@ -16,7 +16,8 @@ Restated, in numpy float64:
 
 Pins (tests/test_oracle_mech.py): the stresses against central differences of the energies; the
 tangent against central differences of the stresses; the cell gradient and Hessian product against
-central differences of the cell energy and gradient; the neo-Hookean tangent at F = I against the
+central differences of the cell energy and gradient; the density sensitivity against central differences in rho;
+the neo-Hookean tangent at F = I against the
 linear one; for the linear material, W at the linear corrector u_I and Fbar = I + E_I equals C_II / 2
 of the linear oracle, with zero gradient in w (the corrector is the equilibrium).
 """
@@ -114,12 +115,18 @@ class CellRef:
         self.mat = MatRef(material, self.lam, self.mu)
         self.wq = 1.0 / (8.0 * self.N)
         v = clip01(rho) ** p
+        # derivatives with the reference's clip01 convention (1 on [0, 1], bounds included; mech.py:31-35)
+        dv = np.where((rho > 1.0) | (rho < 0.0), 0.0, p * clip01(rho) ** (p - 1.0))
         self.s = emin + (1.0 - emin) * v
+        self.ds = (1.0 - emin) * dv
         if wls is None:
             self.gamma = None
+            self.dgamma = np.zeros_like(rho)
         else:
             b, r0 = wls
-            self.gamma = (np.tanh(b * r0) + np.tanh(b * (v - r0))) / (np.tanh(b * r0) + np.tanh(b * (1 - r0)))
+            den = np.tanh(b * r0) + np.tanh(b * (1 - r0))
+            self.gamma = (np.tanh(b * r0) + np.tanh(b * (v - r0))) / den
+            self.dgamma = b * (1.0 - np.tanh(b * (v - r0)) ** 2) / den * dv
 
     def nodal(self, w):
         return np.stack([np.roll(w, shift=tuple(-int(o) for o in off), axis=(0, 1, 2)) for off in NODES])
@@ -155,6 +162,35 @@ class CellRef:
         gam = None if self.gamma is None else np.broadcast_to(self.gamma, F.shape[:-2])
         _, _, A = self.mat.eval(F, gam)
         return self.wq * self.s[None, ..., None, None, None, None] * A
+
+    def work(self, w, Fbar, v, dFbar):
+        """sum_e wq s_e sum_g P_eff(F_g; gamma_e) : dL_g with dL = dFbar + grad v: the pairing whose density
+        derivative Mixed.grad_rho's coupling term takes (mech.py:360-362)."""
+        F = np.asarray(Fbar) + self.grad_w(w)
+        gam = None if self.gamma is None else np.broadcast_to(self.gamma, F.shape[:-2])
+        _, P, _ = self.mat.eval(F, gam)
+        dL = np.asarray(dFbar) + self.grad_w(v)
+        return self.wq * float(np.sum(self.s * np.sum(P * dL, axis=(-2, -1))))
+
+    def rho_sensitivity(self, w, Fbar, v, dFbar):
+        """d work / d rho_e per element (nx, ny, nz), from the explicit tangent: s' P : dL + s gamma' dP/dgamma : dL
+        with dP/dgamma = P_nl(Fg) - P_lin(Fg) + gamma (A_nl(Fg) - A_lin) : (F - I)."""
+        F = np.asarray(Fbar) + self.grad_w(w)
+        dL = np.asarray(dFbar) + self.grad_w(v)
+        if self.gamma is None:
+            _, P, _ = self.mat.eval(F, None)
+            return self.wq * self.ds * np.sum(P * dL, axis=(0, -2, -1))
+        gam = np.broadcast_to(self.gamma, F.shape[:-2])
+        _, P, _ = self.mat.eval(F, gam)
+        H = F - I3
+        Fg = I3 + gam[..., None, None] * H
+        _, Pn, An = self.mat._nl(Fg)
+        Pl = P_lin(Fg, self.lam, self.mu)
+        Al = A_lin(self.lam, self.mu)
+        dPg = Pn - Pl + gam[..., None, None] * np.einsum("...abcd,...cd->...ab", An - Al, H)
+        t = np.sum(P * dL, axis=(0, -2, -1))
+        u = np.sum(dPg * dL, axis=(0, -2, -1))
+        return self.wq * (self.ds * t + self.s * self.dgamma * u)
 
     def hvp(self, A, v, dFbar):
         """Hessian product (w part, sum of dP) with the stored tangent A (Mixed.hvp_A, mech.py:227-231)."""

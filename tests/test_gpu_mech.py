@@ -132,3 +132,68 @@ def test_tiled_cell_has_the_same_energy_density(M):
     assert big.energy(w1, Fb) == pytest.approx(cell.energy(w0, Fb), rel=1e-12)
     np.testing.assert_allclose(big.mean_stress(w1, Fb), cell.mean_stress(w0, Fb), rtol=1e-11, atol=1e-15)
     assert cell.Kinv.shape == (4, 3, 3, 3, 3)
+
+
+@pytest.mark.parametrize("material,wls", CASES)
+def test_density_sensitivity_matches_oracle(M, material, wls):
+    import torch
+    rng = np.random.default_rng(13)
+    rho = rng.uniform(0.0, 1.0, (5, 4, 3))
+    ref = MR.CellRef(rho, material=material, wls=wls)
+    cell = M.Cell(rho, material=material, wls=wls)
+    w = 0.05 * rng.standard_normal(rho.shape + (3,))
+    v = rng.standard_normal(rho.shape + (3,))
+    Fb = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    dF = rng.standard_normal((3, 3))
+    g = cell.rho_sensitivity(torch.as_tensor(w, device="cuda"), Fb, torch.as_tensor(v, device="cuda"), dF)
+    assert _rel(g.cpu().numpy(), ref.rho_sensitivity(w, Fb, v, dF)) < 1e-11
+
+
+def _fd_solves(M, rho, free, Fb, d, h, quantity):
+    vals = []
+    for sgn in (1.0, -1.0):
+        c = M.Cell(rho + sgn * h * d)
+        w, F, log = M.Mixed(c, free=free).solve(Fb, rtol=1e-12, cg_tol=1e-10)
+        assert log["converged"]
+        vals.append(quantity(c, w, F))
+    return (vals[0] - vals[1]) / (2 * h)
+
+
+def test_grad_rho_of_poisson_ratio_matches_central_differences(M):
+    """The reference's test_grad_rho_of_poisson_ratio (tests/test_mech.py:95-113) on a 4^3 cell."""
+    import torch
+    rng = np.random.default_rng(6)
+    rho = rng.uniform(0.3, 1.0, (4, 4, 4))
+    free = [(1, 1), (2, 2)]
+    e = 0.04
+    Fb = np.diag([1 + e, 1.0, 1.0])
+    cell = M.Cell(rho)
+    mix = M.Mixed(cell, free=free)
+    w, F, log = mix.solve(Fb, rtol=1e-12, cg_tol=1e-10)
+    assert log["converged"]
+    g, info = mix.grad_rho(log["z"], Fb, lambda z, F_, r: -(z[mix.nw] - 1.0) / e)
+    assert info["converged"]
+    d = rng.standard_normal(rho.shape) * (rho * (1 - rho))
+    fd = _fd_solves(M, rho, free, Fb, d, 1e-5, lambda c, w_, F_: -(F_[1, 1] - 1.0) / e)
+    assert float(torch.sum(g * torch.as_tensor(d, device="cuda"))) == pytest.approx(fd, rel=1e-4, abs=1e-9)
+
+
+def test_grad_rho_of_mean_stress_matches_central_differences(M):
+    """q = Pbar_xx at the uniaxial-stress equilibrium (the StressPath loss of nlfiber/steer/losses.py:86-101):
+    both the explicit density term and the adjoint coupling are exercised."""
+    import torch
+    rng = np.random.default_rng(10)
+    rho = rng.uniform(0.3, 1.0, (4, 4, 4))
+    free = [(1, 1), (2, 2)]
+    Fb = np.diag([1.03, 1.0, 1.0])
+    cell = M.Cell(rho)
+    mix = M.Mixed(cell, free=free)
+    w, F, log = mix.solve(Fb, rtol=1e-12, cg_tol=1e-10)
+
+    def q(z, F_, r):
+        return cell.mean_stress_t(*mix.unpack_t(z, F_), r)[0, 0]
+
+    g, info = mix.grad_rho(log["z"], Fb, q)
+    d = rng.standard_normal(rho.shape) * (rho * (1 - rho))
+    fd = _fd_solves(M, rho, free, Fb, d, 1e-5, lambda c, w_, F_: float(c.mean_stress(w_, F_)[0, 0]))
+    assert float(torch.sum(g * torch.as_tensor(d, device="cuda"))) == pytest.approx(fd, rel=1e-5)

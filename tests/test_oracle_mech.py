@@ -105,3 +105,19 @@ def test_dense_newton_small_strain_matches_linear_C():
         Pv = np.array([P[i, j] if i == j else 0.5 * (P[i, j] + P[j, i]) for i, j in LO.VOIGT])
         errs.append(np.abs(Pv - t * C @ n).max() / (t * np.abs(C @ n).max()))
     assert errs[0] < 5e-3 and errs[1] < 0.6 * errs[0]  # first-order convergence in t
+
+
+@pytest.mark.parametrize("material,wls", [("neohooke", (500.0, 0.01)), ("neohooke", (20.0, 0.3)), ("linear", None)])
+def test_density_sensitivity_is_the_rho_derivative(material, wls):
+    rng = np.random.default_rng(12)
+    rho = rng.uniform(0.05, 0.95, (3, 3, 2))
+    w = 0.05 * rng.standard_normal(rho.shape + (3,))
+    v = rng.standard_normal(rho.shape + (3,))
+    Fb = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    dF = rng.standard_normal((3, 3))
+    g = M.CellRef(rho, material=material, wls=wls).rho_sensitivity(w, Fb, v, dF)
+    d = rng.standard_normal(rho.shape)
+    h = 1e-6
+    num = (M.CellRef(rho + h * d, material=material, wls=wls).work(w, Fb, v, dF) -
+           M.CellRef(rho - h * d, material=material, wls=wls).work(w, Fb, v, dF)) / (2 * h)
+    assert np.sum(g * d) == pytest.approx(num, rel=1e-6, abs=1e-12)
