@@ -16,8 +16,10 @@ taken at instead of 81 numbers per Gauss point); z, w and Fbar are torch / numpy
 JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
   * Mixed.grad_rho, the adjoint density gradient of a quantity of the equilibrium state (mech.py:348-363), with
     q a torch function (torch autograd stands in for jax.grad; Cell.mean_stress_t is the differentiable Pbar).
-Not built: the stability analysis (LOBPCG, effective tangent, rank-one ellipticity) and the third-order
-constants (mech.py:365-515).
+  * effective_tangent (9 linear solves with the matrix-free Hessian), fibonacci_sphere and rank_one_min
+    (mech.py:411-451).
+Not built: Mixed.lowest_eigs (LOBPCG on the scaled Hessian) and the third-order constants toec
+(mech.py:365-409, 454-515).
 """
 import ctypes
 import math
@@ -427,3 +429,50 @@ class Mixed:
         w, F = self.unpack(z, Fb)
         log["z"], log["g0"] = z, g0
         return w, F, log
+
+
+# --- effective tangent and strong ellipticity (mech.py:411-451) ---------------------------------------------------
+def effective_tangent(cell, w, Fbar, cg_tol=1e-10):
+    """A_abcd = d Pbar_ab / d Fbar_cd at the equilibrium (w, Fbar) under full control (mech.py:411-427): for each
+    unit direction E_cd, the displacement response dw solves H_ww dw = -H_wF E_cd, and A[:, :, c, d] is the stress
+    part of the Hessian product with (dw, E_cd). 9 linear solves; a numpy (3, 3, 3, 3) array."""
+    full = Mixed(cell, free=[])
+    Fb = np.asarray(Fbar, dtype=np.float64)
+    z = torch.as_tensor(w, dtype=torch.float64, device=cell._dev).reshape(-1)
+    Atan = full.tangent(z, Fb)
+    A = np.zeros((3, 3, 3, 3))
+    for c in range(3):
+        for d in range(3):
+            E = np.zeros((3, 3))
+            E[c, d] = 1.0
+            hw, _ = cell.hessian_product(Atan[0], Fb, None, E)
+            dz, _, _ = full.solve_linear(Atan, -hw.reshape(-1), cg_tol)
+            _, dP = cell.hessian_product(Atan[0], Fb, dz, E)
+            A[:, :, c, d] = dP
+    return A
+
+
+def fibonacci_sphere(m):
+    """m nearly uniform unit vectors (golden-angle spiral; mech.py:430-434)."""
+    t = (np.arange(m) + 0.5) / m
+    polar = np.arccos(1.0 - 2.0 * t)
+    azim = np.pi * (1.0 + np.sqrt(5.0)) * np.arange(0.5, m, 1.0)
+    return np.stack([np.cos(azim) * np.sin(polar), np.sin(azim) * np.sin(polar), np.cos(polar)], axis=1)
+
+
+def rank_one_min(A, m=1000, plane=False):
+    """min over unit n of the smallest eigenvalue of the acoustic tensor Q_ik = A_ijkl n_j n_l (symmetrised);
+    strong ellipticity is lost when it is <= 0. 2D cells: n in the x-y plane. Returns (value, n)
+    (mech.py:437-451)."""
+    if plane:
+        th = np.linspace(0.0, np.pi, m, endpoint=False)
+        ns = np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], axis=1)
+    else:
+        ns = fibonacci_sphere(m)
+    Q = np.einsum("ijkl,nj,nl->nik", np.asarray(A, dtype=np.float64), ns, ns)
+    Q = 0.5 * (Q + np.swapaxes(Q, 1, 2))
+    if plane:
+        Q = Q[:, :2, :2]
+    lo = np.linalg.eigvalsh(Q)[:, 0]
+    i = int(np.argmin(lo))
+    return float(lo[i]), ns[i]

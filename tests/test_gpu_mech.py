@@ -197,3 +197,40 @@ def test_grad_rho_of_mean_stress_matches_central_differences(M):
     d = rng.standard_normal(rho.shape) * (rho * (1 - rho))
     fd = _fd_solves(M, rho, free, Fb, d, 1e-5, lambda c, w_, F_: float(c.mean_stress(w_, F_)[0, 0]))
     assert float(torch.sum(g * torch.as_tensor(d, device="cuda"))) == pytest.approx(fd, rel=1e-5)
+
+
+def test_effective_tangent_at_identity_is_linear_C_and_elliptic(M, gpu_pkg):
+    """The reference's test_effective_tangent_at_identity (tests/test_mech.py:43-47) on an 8^3 cell."""
+    import torch
+    rng = np.random.default_rng(14)
+    rho = rng.uniform(0.2, 1.0, (8, 8, 8))
+    lh = gpu_pkg.LinearHomogenization((8, 8, 8), tol=1e-11)
+    C, _, _ = lh.C_full(torch.as_tensor(rho, device="cuda"))
+    C = C.cpu().numpy()
+    cell = M.Cell(rho)
+    A = M.effective_tangent(cell, torch.zeros(cell.shape + (3,), dtype=torch.float64, device="cuda"), np.eye(3))
+    V = np.zeros((6, 6))
+    for I, (i, j) in enumerate(LO.VOIGT):
+        for J, (k, l) in enumerate(LO.VOIGT):
+            V[I, J] = A[i, j, k, l]
+    np.testing.assert_allclose(V, C, atol=1e-7 * np.abs(C).max())
+    val, n = M.rank_one_min(A, m=200)
+    assert val > 0 and abs(np.linalg.norm(n) - 1.0) < 1e-12
+
+
+def test_effective_tangent_matches_central_differences_of_solves(M):
+    import torch
+    rng = np.random.default_rng(15)
+    rho = rng.uniform(0.3, 1.0, (4, 4, 4))
+    cell = M.Cell(rho)
+    full = M.Mixed(cell, free=[])
+    Fb = np.diag([1.04, 0.99, 1.0]) + 0.01 * np.array([[0, 1, 0], [0.5, 0, 0], [0, 0, 0]])
+    w, F, log = full.solve(Fb, rtol=1e-12, cg_tol=1e-10)
+    A = M.effective_tangent(cell, w, F)
+    h = 1e-5
+    for c, d in [(0, 0), (1, 2), (2, 1)]:
+        E = np.zeros((3, 3))
+        E[c, d] = h
+        Pp = cell.mean_stress(*full.solve(F + E, rtol=1e-12, cg_tol=1e-10)[:2])
+        Pm = cell.mean_stress(*full.solve(F - E, rtol=1e-12, cg_tol=1e-10)[:2])
+        np.testing.assert_allclose(A[:, :, c, d], (Pp - Pm) / (2 * h), rtol=1e-5, atol=1e-8 * np.abs(A).max())
