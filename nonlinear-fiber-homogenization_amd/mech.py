@@ -18,8 +18,9 @@ JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
     q a torch function (torch autograd stands in for jax.grad; Cell.mean_stress_t is the differentiable Pbar).
   * effective_tangent (9 linear solves with the matrix-free Hessian), fibonacci_sphere and rank_one_min
     (mech.py:411-451).
-Not built: Mixed.lowest_eigs (LOBPCG on the scaled Hessian) and the third-order constants toec
-(mech.py:365-409, 454-515).
+  * Mixed.lowest_eigs: scipy's LOBPCG on the Jacobi-scaled Hessian, the block products on the device
+    (mech.py:365-409).
+Not built: the third-order constants toec (mech.py:454-515).
 """
 import ctypes
 import math
@@ -382,6 +383,67 @@ class Mixed:
             dF.index_add_(0, self._flat, lam[self.nw:])
         coupling = self.c.rho_sensitivity(w, F, lam[: self.nw], dF)
         return gr - coupling, dict(cg=its, converged=ok)
+
+    # --- stability (mech.py:365-409) ------------------------------------------------------------------------
+    def lowest_eigs(self, z, Fb, k=4, X=None, tol=1e-5, maxiter=2000):
+        """Lowest eigenvalues of D^-1/2 H D^-1/2 (D the Hessian diagonal at the undeformed state, macro unknowns
+        included), rigid translations excluded; same inertia as H (Sylvester). scipy's LOBPCG drives it on the
+        host, each block product being one matrix-free Hessian product per column on the device.
+        Returns (values, vectors in z coordinates, iterations)."""
+        from scipy.sparse.linalg import LinearOperator, lobpcg
+        Fb = np.asarray(Fb, dtype=np.float64)
+        A = self.tangent(torch.as_tensor(z, dtype=torch.float64, device=self._dev), Fb)
+        dsc = self._diag_scale()
+        nz = self.nw + self.k
+        T = np.zeros((nz, 3))
+        for i in range(3):
+            T[i:self.nw:3, i] = 1.0
+        Y, _ = np.linalg.qr(T / dsc[:, None])
+        dsc_t = torch.as_tensor(dsc, device=self._dev)
+
+        def mm(v):
+            v = np.asarray(v, dtype=np.float64)
+            V = v.reshape(nz, -1)
+            out = np.empty_like(V)
+            for c in range(V.shape[1]):
+                x = torch.as_tensor(np.ascontiguousarray(V[:, c]), device=self._dev) * dsc_t
+                out[:, c] = (self.hvp_A(A, x) * dsc_t).cpu().numpy()
+            return out.reshape(v.shape)
+
+        op = LinearOperator((nz, nz), matvec=mm, matmat=mm, dtype=np.float64)
+        if X is None:
+            X = np.random.default_rng(0).standard_normal((nz, k))
+        vals, vecs, hist = lobpcg(op, X, Y=Y, largest=False, tol=tol, maxiter=maxiter, retLambdaHistory=True)
+        o = np.argsort(vals)
+        return vals[o], vecs[:, o] * dsc[:, None], len(hist)
+
+    def _diag_scale(self):
+        """1 / sqrt|diag H| at the undeformed state: the w block probed with the 8 colours of the 2x2x2
+        colouring x 3 components (nodes of one colour do not share an element), the macro entries directly."""
+        if not hasattr(self, "_dsc"):
+            Fb = np.eye(3)
+            A = self.tangent(self.z0(Fb), Fb)
+            d = np.zeros(self.nw + self.k)
+            dw = d[: self.nw].reshape(self.shape + (3,))
+            col = np.indices(self.shape) % 2
+            col = col[0] + 2 * col[1] + 4 * col[2]
+            for cc in range(8):
+                mask = col == cc
+                if not mask.any():
+                    continue
+                for comp in range(3):
+                    e = np.zeros(self.shape + (3,))
+                    e[..., comp] = mask
+                    v = torch.cat([torch.as_tensor(e.ravel(), device=self._dev),
+                                   torch.zeros(self.k, dtype=torch.float64, device=self._dev)])
+                    hv = self.hvp_A(A, v)[: self.nw].reshape(self.shape + (3,)).cpu().numpy()
+                    dw[..., comp][mask] = hv[..., comp][mask]
+            for j in range(self.k):
+                e = torch.zeros(self.nw + self.k, dtype=torch.float64, device=self._dev)
+                e[self.nw + j] = 1.0
+                d[self.nw + j] = float(self.hvp_A(A, e)[self.nw + j])
+            self._dsc = 1.0 / np.sqrt(np.abs(d) + 1e-30)
+        return self._dsc
 
     # --- Newton-CG (mech.py:290-330) --------------------------------------------------------------------
     def solve(self, Fbar, z0=None, rtol=1e-9, maxnewton=30, cg_tol=1e-6, cg_maxit=20000, g_ref=None):

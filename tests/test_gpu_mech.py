@@ -234,3 +234,39 @@ def test_effective_tangent_matches_central_differences_of_solves(M):
         Pp = cell.mean_stress(*full.solve(F + E, rtol=1e-12, cg_tol=1e-10)[:2])
         Pm = cell.mean_stress(*full.solve(F - E, rtol=1e-12, cg_tol=1e-10)[:2])
         np.testing.assert_allclose(A[:, :, c, d], (Pp - Pm) / (2 * h), rtol=1e-5, atol=1e-8 * np.abs(A).max())
+
+
+def test_lobpcg_matches_dense_spectrum(M):
+    """The reference's test_lobpcg_matches_dense_spectrum (tests/test_mech.py:115-135): the dense Hessian
+    here comes from the oracle's stored-tangent products at the GPU solve's state."""
+    rng = np.random.default_rng(16)
+    rho = rng.uniform(0.2, 1.0, (4, 4, 4))
+    cell = M.Cell(rho)
+    free = [(1, 1), (2, 2)]
+    mix = M.Mixed(cell, free)
+    w, F, log = mix.solve(np.diag([0.95, 1.0, 1.0]), rtol=1e-12)
+    z = log["z"]
+    vals, _, _ = mix.lowest_eigs(z, F, k=3, tol=1e-8)
+    ref = MR.CellRef(rho)
+    wz = z[: mix.nw].cpu().numpy().reshape(rho.shape + (3,))
+    At = ref.tangent(wz, F)
+    nz = mix.nw + 2
+    H = np.empty((nz, nz))
+    for c in range(nz):
+        v = np.zeros(nz)
+        v[c] = 1.0
+        dF = np.zeros((3, 3))
+        for j, (a, b) in enumerate(free):
+            dF[a, b] = v[mix.nw + j]
+        hw, hP = ref.hvp(At, v[: mix.nw].reshape(rho.shape + (3,)), dF)
+        H[:, c] = np.concatenate([hw.ravel(), [hP[a, b] for a, b in free]])
+    dsc = mix._diag_scale()
+    Hs = H * dsc[:, None] * dsc[None, :]
+    Tm = np.zeros((nz, 3))
+    for i in range(3):
+        Tm[i:mix.nw:3, i] = 1.0
+    Y, _ = np.linalg.qr(Tm / dsc[:, None])
+    Pm = np.eye(nz) - Y @ Y.T
+    ev = np.linalg.eigvalsh(Pm @ Hs @ Pm)
+    ev = ev[np.abs(ev) > 1e-12 * np.abs(ev).max()]
+    np.testing.assert_allclose(vals, np.sort(ev)[:3], rtol=1e-5)
