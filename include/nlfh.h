@@ -211,6 +211,12 @@ int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int 
  * mode 3: grad[e] (N values) = d/drho_e of sum_g wq s_e P_eff(F_g; gamma_e) : (dFbar + grad v at g), the density
  *         coupling of Mixed.grad_rho (mech.py:349-363) and the explicit density derivative of Pbar : dFbar;
  *         scalars unused (may be null). */
+/* Second and third derivatives of f(s) = <psi_eff(I + sum_I s_I (e_I + grad chi_I))> at s = 0 from the linear
+ * correctors chi (6, nx, ny, nz, 3) of the unit Voigt strains: out[0..21) = d2f_IJ for I <= J (row-major over
+ * the upper triangle), out[21..77) = d3f_IJK for I <= J <= K (lexicographic). toec (mech.py:454-490) forms C and
+ * the third-order constants from these. work: as for nlfh_fs_eval. */
+int nlfh_fs_toec(const int* shape, const double* sg, const double* mat, int flags, const double* chi, double* out,
+                 double* work, void* stream);
 /* w (nx, ny, nz, 3) -= its per-component mean, in place: the w part of Mixed.proj (mech.py:236-239).
  * work: as for nlfh_fs_eval (its first 3 ceil(N / 256) + 3 doubles are overwritten). */
 int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);

@@ -475,6 +475,129 @@ __global__ void __launch_bounds__(kFsThreads) fs_sub3_kernel(double* __restrict_
   if (i < 3 * n) w[i] -= mean[i % 3];
 }
 
+// Third-order elastic constants of the cell from the linear correctors chi_I (6 fields (N, 3)), per
+// Gauss point H_I = e_I + grad chi_I with e_I the symmetric Voigt unit strain (engineering shear: 1/2
+// off the diagonal), at F = I (toec, mech.py:454-490), with f(s) = <psi_eff(I + sum_I s_I H_I)>:
+//   d2f_IJ  = sum_e wq s_e sum_g D2psi_eff(I)[H_I, H_J],  D2psi_eff(I) = D2psi_lin (the neo-Hookean tangent at I
+//            is the linear one, so the void interpolation drops out): lam tr a tr b + mu (a : b + a : b^T);
+//   d3f_IJK = sum_e wq s_e gamma_e^3 sum_g D3psi_nl(I)[H_I, H_J, H_K] (psi_lin is quadratic), with
+//            D3psi_neo(I)[a, b, c] = -mu (tr abc + tr acb) - lam (tr a tr bc + tr b tr ac + tr c tr ab), 0 linear.
+// Eight lanes per element (one per Gauss point); each of the 21 + 56 unique values is block-summed in a
+// fixed order into part[block][77] (pairs I <= J first, then triples I <= J <= K).
+constexpr int kToecVals = 77;
+
+__device__ __forceinline__ double tr_prod3(const double (&a)[3][3], const double (&b)[3][3], const double (&c)[3][3]) {
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double ab = a[i][0] * b[0][j] + a[i][1] * b[1][j] + a[i][2] * b[2][j];
+      t += ab * c[j][i];
+    }
+  return t;
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_toec_kernel(const FsArgs a, const double* __restrict__ chi,
+                                                             double* __restrict__ part) {
+  __shared__ double red[kFsThreads / 64];
+  const int g = threadIdx.x & 7;
+  const int n = (int)a.n;
+  const int e = (int)blockIdx.x * kFsElemPerBlock + (int)(threadIdx.x >> 3);
+  const bool live = e < n;
+  double H[6][3][3];
+#pragma unroll
+  for (int I = 0; I < 6; ++I)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) H[I][i][j] = 0.0;
+  double ws = 0.0, wg3 = 0.0;
+  if (live) {
+    const int k = e % a.nz;
+    const int j = (e / a.nz) % a.ny;
+    const int i = e / (a.nz * a.ny);
+    const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
+    for (int c = 0; c < 8; ++c) {
+      const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
+      const int nd = (ii * a.ny + jj) * a.nz + kk;
+      const int jl = c ^ g;
+      const double fx = (jl & 1) ? kGaussOther : kGaussSame, fy = (jl & 2) ? kGaussOther : kGaussSame,
+                   fz = (jl & 4) ? kGaussOther : kGaussSame;
+      const double d[3] = {((c & 1) ? fy : -fy) * fz, ((c & 2) ? fx : -fx) * fz, ((c & 4) ? fx : -fx) * fy};
+#pragma unroll
+      for (int I = 0; I < 6; ++I)
+#pragma unroll
+        for (int ci = 0; ci < 3; ++ci) {
+          const double x = chi[((size_t)I * n + nd) * 3 + ci];
+#pragma unroll
+          for (int dj = 0; dj < 3; ++dj) H[I][ci][dj] += x * d[dj];
+        }
+    }
+    // + e_I: Voigt order xx yy zz yz xz xy
+    H[0][0][0] += 1.0;
+    H[1][1][1] += 1.0;
+    H[2][2][2] += 1.0;
+    H[3][1][2] += 0.5;
+    H[3][2][1] += 0.5;
+    H[4][0][2] += 0.5;
+    H[4][2][0] += 0.5;
+    H[5][0][1] += 0.5;
+    H[5][1][0] += 0.5;
+    ws = a.sg[e];
+    const double gam = a.sg[n + e];
+    wg3 = a.m.neo ? ws * gam * gam * gam : 0.0;
+  }
+  double tr[6];
+#pragma unroll
+  for (int I = 0; I < 6; ++I) tr[I] = H[I][0][0] + H[I][1][1] + H[I][2][2];
+  const double lam = a.m.lam, mu = a.m.mu;
+  int q = 0;
+  for (int I = 0; I < 6; ++I)
+    for (int J = I; J < 6; ++J, ++q) {
+      double dd = 0.0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) dd += H[I][r][c] * (H[J][r][c] + H[J][c][r]);
+      const double t = block_sum(ws * (lam * tr[I] * tr[J] + mu * dd), red);
+      if (threadIdx.x == 0) part[(size_t)blockIdx.x * kToecVals + q] = t;
+      __syncthreads();
+    }
+  for (int I = 0; I < 6; ++I)
+    for (int J = I; J < 6; ++J)
+      for (int K = J; K < 6; ++K, ++q) {
+        double v = 0.0;
+        if (wg3 != 0.0) {
+          double tIJ = 0.0, tIK = 0.0, tJK = 0.0;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              tIJ += H[I][r][c] * H[J][c][r];
+              tIK += H[I][r][c] * H[K][c][r];
+              tJK += H[J][r][c] * H[K][c][r];
+            }
+          const double d3 = -mu * (tr_prod3(H[I], H[J], H[K]) + tr_prod3(H[I], H[K], H[J])) -
+                            lam * (tr[I] * tJK + tr[J] * tIK + tr[K] * tIJ);
+          v = wg3 * d3;
+        }
+        const double t = block_sum(v, red);
+        if (threadIdx.x == 0) part[(size_t)blockIdx.x * kToecVals + q] = t;
+        __syncthreads();
+      }
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_toec_reduce_kernel(const double* __restrict__ part, int nblk,
+                                                                    double* __restrict__ out) {
+  __shared__ double red[kFsThreads / 64];
+  const int q = blockIdx.x;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += kFsThreads) acc += part[(size_t)b * kToecVals + q];
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) out[q] = t;
+}
+
 // out[q] = sum over blocks of part[b][q], lane-strided then a fixed butterfly (deterministic)
 __global__ void __launch_bounds__(kFsThreads) fs_reduce_kernel(const double* __restrict__ part, int nblk,
                                                                double* __restrict__ out) {
@@ -588,6 +711,29 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
   }
   hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsThreads), 0, st, a.part, (int)nblk,
                      scalars);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_toec(const int* shape, const double* sg, const double* mat, int flags, const double* chi, double* out,
+                 double* work, void* stream) {
+  FsArgs a{};
+  const int rc = fs_setup(shape, mat, flags, a);
+  if (rc != NLFH_OK) return rc;
+  if (sg == nullptr || chi == nullptr || out == nullptr || work == nullptr) {
+    set_error("nlfh_fs_toec: sg, chi, out and work are required");
+    return NLFH_EINVAL;
+  }
+  if (6 * 3 * a.n >= (int64_t)INT32_MAX) {
+    set_error("nlfh_fs_toec: cell too large");
+    return NLFH_EINVAL;
+  }
+  a.sg = sg;
+  const int nblk = (int)((a.n + kFsElemPerBlock - 1) / kFsElemPerBlock);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fs_toec_kernel, dim3(nblk), dim3(kFsThreads), 0, st, a, chi, work);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(fs_toec_reduce_kernel, dim3(kToecVals), dim3(kFsThreads), 0, st, work, nblk, out);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -20,7 +20,8 @@ JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
     (mech.py:411-451).
   * Mixed.lowest_eigs: scipy's LOBPCG on the Jacobi-scaled Hessian, the block products on the device
     (mech.py:365-409).
-Not built: the third-order constants toec (mech.py:454-515).
+  * toec and cubic_toec: second and third energy derivatives along the linear correctors from one kernel pass
+    (mech.py:454-515); not differentiable in rho (the reference's is, through chi).
 """
 import ctypes
 import math
@@ -538,3 +539,58 @@ def rank_one_min(A, m=1000, plane=False):
     lo = np.linalg.eigvalsh(Q)[:, 0]
     i = int(np.argmin(lo))
     return float(lo[i]), ns[i]
+
+
+# --- third-order elastic constants (mech.py:454-515) -----------------------------------------------------------
+def toec(cell, chi):
+    """C (6x6) and Lagrangian-strain third-order constants T (6x6x6) of the cell from the linear correctors chi
+    (6, nx, ny, nz, 3) of unit Voigt strains (same stiffness scale and E, nu) (mech.py:454-490): with
+    f(s) = <psi(I + sum_I s_I (e_I + grad chi_I))>, which equals Wbar to O(s^4), C_IJ = f''_IJ and
+    T_IJK = f'''_IJK - 3 Sym C[e_I, e_J e_K]. The second and third derivatives come from one kernel pass
+    (`nlfh_fs_toec`, closed-form derivatives of the energy at F = I); numpy arrays (not differentiable)."""
+    chi_t = torch.as_tensor(chi, dtype=torch.float64, device=cell._dev).reshape(6, -1).contiguous()
+    if chi_t.shape[1] != 3 * cell.N:
+        raise ValueError(f"chi must have shape (6,) + {cell.shape + (3,)}")
+    out = torch.empty(77, dtype=torch.float64, device=cell._dev)
+    _lib.check(cell._lib.nlfh_fs_toec(cell._shape_c, _ptr(cell._sg), cell._mat.ctypes.data_as(_dp), cell._flags,
+                                      _ptr(chi_t), _ptr(out), _ptr(cell._work), _stream()))
+    v = out.cpu().numpy()
+    C = np.zeros((6, 6))
+    T3 = np.zeros((6, 6, 6))
+    q = 0
+    for I in range(6):
+        for J in range(I, 6):
+            C[I, J] = C[J, I] = v[q]
+            q += 1
+    for I in range(6):
+        for J in range(I, 6):
+            for K in range(J, 6):
+                for a, b, c in {(I, J, K), (I, K, J), (J, I, K), (J, K, I), (K, I, J), (K, J, I)}:
+                    T3[a, b, c] = v[q]
+                q += 1
+    # geometric part: 3 Sym_IJK C(e_I, voigt(e_J e_K))
+    X = np.zeros((6, 6, 6))
+    for J in range(6):
+        for K in range(6):
+            X[:, J, K] = to_voigt(voigt_tensor(J) @ voigt_tensor(K))
+    CX = np.einsum("ia,ajk->ijk", C, X)
+    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+    sym = sum(np.transpose(CX, p) for p in perms) / 6.0
+    return C, T3 - 3.0 * sym
+
+
+CUBIC_TOEC = {
+    "C111": [(0, 0, 0), (1, 1, 1), (2, 2, 2)],
+    "C112": [(0, 0, 1), (0, 0, 2), (1, 1, 0), (1, 1, 2), (2, 2, 0), (2, 2, 1)],
+    "C123": [(0, 1, 2)],
+    "C144": [(0, 3, 3), (1, 4, 4), (2, 5, 5)],
+    "C155": [(0, 4, 4), (0, 5, 5), (1, 3, 3), (1, 5, 5), (2, 3, 3), (2, 4, 4)],
+    "C456": [(3, 4, 5)],
+}
+
+
+def cubic_toec(T):
+    """The six cubic third-order constants (C111, C112, C123, C144, C155, C456), each the mean of its equivalent
+    entries (Voigt indices 1..6 = xx yy zz yz xz xy) (mech.py:493-515)."""
+    T = np.asarray(T)
+    return {k: float(np.mean([T[i] for i in idx])) for k, idx in CUBIC_TOEC.items()}

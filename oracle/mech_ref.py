@@ -244,3 +244,37 @@ def newton_dense(cell, Fbar, free=(), rtol=1e-12, maxnewton=30):
         for j, (a, b) in enumerate(free):
             Fb[a, b] += dz[nw + j]
     return w, Fb
+
+
+# ---- third-order constants (mech.py:454-490) ---------------------------------------------------------------
+def _voigt_tensor(I):
+    from .linear_ref import VOIGT
+    H = np.zeros((3, 3))
+    i, j = VOIGT[I]
+    if i == j:
+        H[i, i] = 1.0
+    else:
+        H[i, j] = H[j, i] = 0.5
+    return H
+
+
+def toec_derivatives(cell, chi, h=1e-4):
+    """(d2f (6, 6), d3f (6, 6, 6)) of f(s) = wq sum_e s_e sum_g psi_eff(I + sum_I s_I H_I,g), H_I = e_I + grad chi_I:
+    d2f from the explicit tangent at F = I, d3f by central differences of that second derivative along each
+    e_K (O(h^2))."""
+    Hs = np.stack([_voigt_tensor(I) + cell.grad_w(np.asarray(chi[I])) for I in range(6)])  # (6, 8, nx, ny, nz, 3, 3)
+    gam = None if cell.gamma is None else np.broadcast_to(cell.gamma, Hs.shape[1:-2])
+
+    def d2(sv):
+        F = I3 + np.tensordot(sv, Hs, axes=1)
+        _, _, A = cell.mat.eval(F, gam)
+        AH = np.einsum("gxyzabcd,Jgxyzcd->Jgxyzab", A, Hs)
+        return cell.wq * np.einsum("xyz,Igxyzab,Jgxyzab->IJ", cell.s, Hs, AH)
+
+    C = d2(np.zeros(6))
+    T3 = np.zeros((6, 6, 6))
+    for K in range(6):
+        e = np.zeros(6)
+        e[K] = h
+        T3[:, :, K] = (d2(e) - d2(-e)) / (2 * h)
+    return C, T3

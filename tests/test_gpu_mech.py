@@ -270,3 +270,55 @@ def test_lobpcg_matches_dense_spectrum(M):
     ev = np.linalg.eigvalsh(Pm @ Hs @ Pm)
     ev = ev[np.abs(ev) > 1e-12 * np.abs(ev).max()]
     np.testing.assert_allclose(vals, np.sort(ev)[:3], rtol=1e-5)
+
+
+@pytest.mark.parametrize("material,wls", [("neohooke", (500.0, 0.01)), ("neohooke", None), ("linear", None)])
+def test_toec_derivatives_match_oracle(M, material, wls):
+    import torch
+    rng = np.random.default_rng(18)
+    rho = rng.uniform(0.1, 1.0, (4, 3, 5))
+    chi = 0.3 * rng.standard_normal((6,) + rho.shape + (3,))
+    cell = M.Cell(rho, material=material, wls=wls)
+    C, T = M.toec(cell, torch.as_tensor(chi, device="cuda"))
+    C2, T3 = MR.toec_derivatives(MR.CellRef(rho, material=material, wls=wls), chi)
+    assert _rel(C, C2) < 1e-12
+    X = np.zeros((6, 6, 6))
+    for J in range(6):
+        for K in range(6):
+            X[:, J, K] = M.to_voigt(M.voigt_tensor(J) @ M.voigt_tensor(K))
+    CX = np.einsum("ia,ajk->ijk", C2, X)
+    sym = sum(np.transpose(CX, p) for p in [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]) / 6
+    assert np.abs(T - (T3 - 3 * sym)).max() < 1e-6 * np.abs(T3 - 3 * sym).max()
+
+
+def test_toec_against_linear_C_and_nonlinear_solves(M, gpu_pkg):
+    """The reference's test_toec_second_order_is_linear_C and test_toec_against_nonlinear_solves
+    (tests/test_mech.py:36-68) on a 4^3 cell: C from toec equals the linear C; g(t) = Wbar(I + t n) has
+    g''' from central differences of Pbar along the path equal to T(n, n, n) + 3 n.C.voigt(Hn Hn)."""
+    import torch
+    rng = np.random.default_rng(19)
+    rho = rng.uniform(0.3, 1.0, (4, 4, 4))
+    lh = gpu_pkg.LinearHomogenization((4, 4, 4), tol=1e-12)
+    Cl, chi, _ = lh.C_full(torch.as_tensor(rho, device="cuda"))
+    Cl = Cl.cpu().numpy()
+    cell = M.Cell(rho)
+    C2, T = M.toec(cell, chi.contiguous())
+    np.testing.assert_allclose(C2, Cl, atol=1e-9 * np.abs(Cl).max())
+    np.testing.assert_allclose(T, np.transpose(T, (1, 0, 2)), atol=1e-12 * np.abs(T).max())
+    np.testing.assert_allclose(T, np.transpose(T, (0, 2, 1)), atol=1e-12 * np.abs(T).max())
+    n = np.array([1.0, -0.3, 0.2, 0.4, -0.2, 0.1])
+    Hn = sum(n[I] * M.voigt_tensor(I) for I in range(6))
+    X = M.to_voigt(Hn @ Hn)
+    pred = np.einsum("ijk,i,j,k", T, n, n, n) + 3 * n @ C2 @ X
+    full = M.Mixed(cell, free=[])
+    h = 0.004
+
+    def dg(t):
+        w, F, log = full.solve(np.eye(3) + t * Hn, rtol=1e-12, cg_tol=1e-10)
+        assert log["converged"]
+        return float(np.sum(cell.mean_stress(w, F) * Hn))
+
+    g3 = (dg(h) - 2 * dg(0.0) + dg(-h)) / h**2
+    assert g3 == pytest.approx(pred, rel=2e-3)
+    c = M.cubic_toec(T)
+    assert set(c) == {"C111", "C112", "C123", "C144", "C155", "C456"}

@@ -121,3 +121,16 @@ def test_density_sensitivity_is_the_rho_derivative(material, wls):
     num = (M.CellRef(rho + h * d, material=material, wls=wls).work(w, Fb, v, dF) -
            M.CellRef(rho - h * d, material=material, wls=wls).work(w, Fb, v, dF)) / (2 * h)
     assert np.sum(g * d) == pytest.approx(num, rel=1e-6, abs=1e-12)
+
+
+def test_toec_derivatives_second_order_is_linear_C_and_symmetric():
+    """toec's second derivative equals the linear C at the linear correctors (tests/test_mech.py:36-41);
+    the third derivative, from central differences, is symmetric in all three indices."""
+    rng = np.random.default_rng(17)
+    rho = rng.uniform(0.2, 1.0, (3, 3, 3))
+    lin = LO.LinearRef(rho.shape, tol=1e-12, maxit=4000)
+    C, chi, _ = lin.C_full(rho)
+    C2, T3 = M.toec_derivatives(M.CellRef(rho), chi)
+    np.testing.assert_allclose(C2, C, atol=1e-9 * np.abs(C).max())
+    np.testing.assert_allclose(T3, np.transpose(T3, (1, 0, 2)), atol=1e-7 * np.abs(T3).max())
+    np.testing.assert_allclose(T3, np.transpose(T3, (0, 2, 1)), atol=1e-7 * np.abs(T3).max())
