@@ -32,8 +32,31 @@ import torch
 from . import _lib
 from .linear import LinearHomogenization, VOIGT, lame
 
-MATERIALS = ("linear", "neohooke")
 STALL_RTOL = 1e-6
+
+
+def clip01(x):
+    """Clamp to [0, 1] (mech.py:31-35); the kernels differentiate it with slope 1 on [0, 1], bounds included."""
+    return torch.where(x > 1.0, torch.ones_like(x), torch.where(x < 0.0, torch.zeros_like(x), x))
+
+
+def psi_linear(F, lam, mu):
+    """Linear-elastic energy density of the small strain sym(F - I), for (..., 3, 3) tensors (mech.py:39-43)."""
+    H = F - torch.eye(3, dtype=F.dtype, device=F.device)
+    eps = 0.5 * (H + H.transpose(-1, -2))
+    tr = eps.diagonal(dim1=-2, dim2=-1).sum(-1)
+    return 0.5 * lam * tr**2 + mu * (eps * eps).sum(dim=(-2, -1))
+
+
+def psi_neohooke(F, lam, mu):
+    """Compressible neo-Hookean energy density, for (..., 3, 3) tensors (mech.py:46-50)."""
+    J = torch.linalg.det(F)
+    lnJ = torch.log(J)
+    return 0.5 * mu * ((F * F).sum(dim=(-2, -1)) - 3.0) - mu * lnJ + 0.5 * lam * lnJ**2
+
+
+# the kernels evaluate these materials (and their derivatives) in closed form; the functions above state them
+MATERIALS = {"linear": psi_linear, "neohooke": psi_neohooke}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -113,7 +136,7 @@ class Cell:
 
     def __init__(self, rho, E=1.0, nu=0.3, material="neohooke", wls=(500.0, 0.01), emin=1e-3, p=3.0):
         if material not in MATERIALS:
-            raise ValueError(f"material must be one of {MATERIALS}")
+            raise ValueError(f"material must be one of {tuple(MATERIALS)}")
         self._dev = torch.device("cuda", torch.cuda.current_device())
         rho = torch.as_tensor(rho, dtype=torch.float64, device=self._dev)
         if rho.dim() == 2:
@@ -122,6 +145,7 @@ class Cell:
         self.N = int(np.prod(self.shape))
         self.E, self.nu, self.material, self.wls, self.emin, self.p = E, nu, material, wls, emin, p
         self.lam, self.mu = lame(E, nu)
+        self.psi = MATERIALS[material]
         self.wq = 1.0 / (8.0 * self.N)
         self.s = self.scale_of(self.rho)
         self.gamma = self.gamma_of(self.rho)

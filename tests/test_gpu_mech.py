@@ -322,3 +322,16 @@ def test_toec_against_linear_C_and_nonlinear_solves(M, gpu_pkg):
     assert g3 == pytest.approx(pred, rel=2e-3)
     c = M.cubic_toec(T)
     assert set(c) == {"C111", "C112", "C123", "C144", "C155", "C456"}
+
+
+def test_material_functions_match_the_kernel_energy(M):
+    """psi_linear / psi_neohooke (torch) against the kernels' energy of a uniform state (w = 0)."""
+    import torch
+    rng = np.random.default_rng(20)
+    rho = np.ones((3, 3, 3))
+    Fb = np.eye(3) + 0.1 * rng.standard_normal((3, 3))
+    for material in ("linear", "neohooke"):
+        cell = M.Cell(rho, material=material, wls=None, emin=0.0)
+        W = cell.energy(torch.zeros(cell.shape + (3,), dtype=torch.float64, device="cuda"), Fb)
+        psi = float(cell.psi(torch.as_tensor(Fb), cell.lam, cell.mu))
+        assert W == pytest.approx(psi, rel=1e-13)
