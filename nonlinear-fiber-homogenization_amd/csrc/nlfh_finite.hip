@@ -272,7 +272,6 @@ __host__ __device__ constexpr double tgrad(int j, int d) {
 
 template <int MODE>
 __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
-  __shared__ double red[kFsThreads / 64];
   const int g = threadIdx.x & 7;
   // 32-bit indexing: the host limits cells to 24 N < 2^31 (the force buffer's extent)
   const int n = (int)a.n;
@@ -390,11 +389,23 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
       for (int ci = 0; ci < 3; ++ci) a.fe[(g * 3 + ci) * n + e] = h1[ci];
     }
   }
-  const int nq = MODE == 0 ? 1 : kFsParts;
+  // partial sums of the block: a fixed xor butterfly per wavefront for every value, then one barrier and a
+  // fixed-order sum over the wavefronts; stored value-major (part[q][block]) for coalesced final sums
+  constexpr int nq = MODE == 0 ? 1 : kFsParts;
+  __shared__ double redq[kFsThreads / 64][kFsParts];
+#pragma unroll
   for (int q = 0; q < nq; ++q) {
-    const double t = block_sum(acc[q], red);
-    if (threadIdx.x == 0) a.part[(int64_t)blockIdx.x * kFsParts + q] = t;
-    __syncthreads();
+    double v = acc[q];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) redq[threadIdx.x >> 6][q] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nq) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < kFsThreads / 64; ++w) t += redq[w][threadIdx.x];
+    a.part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
   }
 }
 
@@ -599,12 +610,15 @@ __global__ void __launch_bounds__(kFsThreads) fs_toec_reduce_kernel(const double
 }
 
 // out[q] = sum over blocks of part[b][q], lane-strided then a fixed butterfly (deterministic)
-__global__ void __launch_bounds__(kFsThreads) fs_reduce_kernel(const double* __restrict__ part, int nblk,
-                                                               double* __restrict__ out) {
-  __shared__ double red[kFsThreads / 64];
+constexpr int kFsReduceThreads = 1024;
+
+__global__ void __launch_bounds__(kFsReduceThreads) fs_reduce_kernel(const double* __restrict__ part, int nblk,
+                                                                     double* __restrict__ out) {
+  __shared__ double red[kFsReduceThreads / 64];
   const int q = blockIdx.x;
+  const double* pq = part + (int64_t)q * nblk;  // value-major: the partials of value q are contiguous
   double acc = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kFsThreads) acc += part[(int64_t)b * kFsParts + q];
+  for (int b = threadIdx.x; b < nblk; b += kFsReduceThreads) acc += pq[b];
   const double t = block_sum(acc, red);
   if (threadIdx.x == 0) out[q] = t;
 }
@@ -709,7 +723,7 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
     hipLaunchKernelGGL(fs_gather_kernel, dim3((unsigned)gblk), dim3(kFsThreads), 0, st, a, grad);
     HIP_TRY(hipGetLastError());
   }
-  hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsThreads), 0, st, a.part, (int)nblk,
+  hipLaunchKernelGGL(fs_reduce_kernel, dim3(mode == 0 ? 1 : kFsParts), dim3(kFsReduceThreads), 0, st, a.part, (int)nblk,
                      scalars);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
