@@ -217,6 +217,13 @@ int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int 
  * the third-order constants from these. work: as for nlfh_fs_eval. */
 int nlfh_fs_toec(const int* shape, const double* sg, const double* mat, int flags, const double* chi, double* out,
                  double* work, void* stream);
+/* CG steps of Mixed._cg (mech.py:266-283) on len unknowns, scalars as device values: alpha = rz / pAp (0 if
+ * pAp <= 0), x_out = x + alpha p, r_out = r - alpha Ap, rn2 = |r_out|^2 (fixed-order sum; work: len / 256 + 1
+ * doubles); and p_out = z + (rz_new / rz_old) p (rz_old = 0 taken as 1). */
+int nlfh_fs_cg_update(int len, double* x_out, double* r_out, const double* x, const double* r, const double* p,
+                      const double* Ap, const double* rz, const double* pAp, double* rn2, double* work, void* stream);
+int nlfh_fs_cg_direction(int len, double* p_out, const double* z, const double* p, const double* rz_new,
+                         const double* rz_old, void* stream);
 /* w (nx, ny, nz, 3) -= its per-component mean, in place: the w part of Mixed.proj (mech.py:236-239).
  * work: as for nlfh_fs_eval (its first 3 ceil(N / 256) + 3 doubles are overwritten). */
 int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);

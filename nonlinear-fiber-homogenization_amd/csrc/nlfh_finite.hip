@@ -609,6 +609,43 @@ __global__ void __launch_bounds__(kFsThreads) fs_toec_reduce_kernel(const double
   if (threadIdx.x == 0) out[q] = t;
 }
 
+// CG vector updates of the Newton-CG driver (Mixed._cg, mech.py:266-283) on the flat unknowns z = (w, Fbar_free):
+// alpha = rz / pAp (0 when pAp <= 0: the negative-curvature exit keeps x), x' = x + alpha p, r' = r - alpha Ap,
+// with the block partial sums of r'^2; and p' = z + (rz_new / rz_old) p (rz_old = 0 -> 1).
+__global__ void __launch_bounds__(kFsThreads) fs_cg_update_kernel(int len, double* __restrict__ x_out,
+                                                                  double* __restrict__ r_out, const double* __restrict__ x,
+                                                                  const double* __restrict__ r,
+                                                                  const double* __restrict__ p,
+                                                                  const double* __restrict__ Ap,
+                                                                  const double* __restrict__ rz,
+                                                                  const double* __restrict__ pAp,
+                                                                  double* __restrict__ part) {
+  __shared__ double red[kFsThreads / 64];
+  const double pa = *pAp;
+  const double al = pa <= 0.0 ? 0.0 : *rz / pa;
+  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  double v = 0.0;
+  if (i < len) {
+    x_out[i] = x[i] + al * p[i];
+    const double rn = r[i] - al * Ap[i];
+    r_out[i] = rn;
+    v = rn * rn;
+  }
+  const double t = block_sum(v, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_cg_direction_kernel(int len, double* __restrict__ p_out,
+                                                                     const double* __restrict__ z,
+                                                                     const double* __restrict__ p,
+                                                                     const double* __restrict__ rz_new,
+                                                                     const double* __restrict__ rz_old) {
+  const double ro = *rz_old;
+  const double beta = *rz_new / (ro == 0.0 ? 1.0 : ro);
+  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  if (i < len) p_out[i] = z[i] + beta * p[i];
+}
+
 // out[q] = sum over blocks of part[b][q], lane-strided then a fixed butterfly (deterministic)
 constexpr int kFsReduceThreads = 1024;
 
@@ -748,6 +785,36 @@ int nlfh_fs_toec(const int* shape, const double* sg, const double* mat, int flag
   hipLaunchKernelGGL(fs_toec_kernel, dim3(nblk), dim3(kFsThreads), 0, st, a, chi, work);
   HIP_TRY(hipGetLastError());
   hipLaunchKernelGGL(fs_toec_reduce_kernel, dim3(kToecVals), dim3(kFsThreads), 0, st, work, nblk, out);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_cg_update(int len, double* x_out, double* r_out, const double* x, const double* r, const double* p,
+                      const double* Ap, const double* rz, const double* pAp, double* rn2, double* work, void* stream) {
+  if (len < 1 || x_out == nullptr || r_out == nullptr || x == nullptr || r == nullptr || p == nullptr ||
+      Ap == nullptr || rz == nullptr || pAp == nullptr || rn2 == nullptr || work == nullptr) {
+    set_error("nlfh_fs_cg_update: a required pointer is null or len < 1");
+    return NLFH_EINVAL;
+  }
+  const int nb = (len + kFsThreads - 1) / kFsThreads;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fs_cg_update_kernel, dim3(nb), dim3(kFsThreads), 0, st, len, x_out, r_out, x, r, p, Ap, rz, pAp,
+                     work);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(fs_reduce_kernel, dim3(1), dim3(kFsReduceThreads), 0, st, work, nb, rn2);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_cg_direction(int len, double* p_out, const double* z, const double* p, const double* rz_new,
+                         const double* rz_old, void* stream) {
+  if (len < 1 || p_out == nullptr || z == nullptr || p == nullptr || rz_new == nullptr || rz_old == nullptr) {
+    set_error("nlfh_fs_cg_direction: a required pointer is null or len < 1");
+    return NLFH_EINVAL;
+  }
+  const int nb = (len + kFsThreads - 1) / kFsThreads;
+  hipLaunchKernelGGL(fs_cg_direction_kernel, dim3(nb), dim3(kFsThreads), 0, (hipStream_t)stream, len, p_out, z, p,
+                     rz_new, rz_old);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -247,14 +247,14 @@ class Cell:
         r = self.rho if rho is None else rho
         return _MeanStressFn.apply(torch.as_tensor(w, dtype=torch.float64, device=self._dev), F, r, self)
 
-    def precond_w(self, r):
+    def precond_w(self, r, scale=1.0):
         """Inverse symbol of the homogeneous linear operator with unit stiffness (mech.py:175-181). That
         operator is the linear path's K(1) divided by N (its energy carries wq = 1/(8N) instead of the
         element matrix's 1/8), so this is N times the linear preconditioner."""
         if self._lh is None:
             self._lh = LinearHomogenization(self.shape, self.E, self.nu, self.emin, self.p)
         r = torch.as_tensor(r, dtype=torch.float64, device=self._dev).reshape(self.shape + (3,))
-        return self._lh.precond(r) * float(self.N)
+        return self._lh.precond(r) * (float(self.N) * scale)
 
 
 class Mixed:
@@ -333,16 +333,20 @@ class Mixed:
         return np.abs(np.array(d)) + 1e-12
 
     def precond(self, r, dmac, swscale):
-        w = self.c.precond_w(r[: self.nw]) / swscale
+        w = self.c.precond_w(r[: self.nw], 1.0 / swscale)
         mac = r[self.nw:] / torch.as_tensor(dmac, dtype=torch.float64, device=self._dev)
         return torch.cat([w.reshape(-1), mac])
 
     # --- linear solves (mech.py:253-288) ----------------------------------------------------------------
     def _cg(self, A, b, dmac, sws, tol, maxit):
         """Preconditioned CG on the mean-free space with the reference's negative-curvature exit: a
-        non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration). The scalars
-        stay on the device; one host read per iteration fetches p.Ap, ||r|| and r.z for the loop test."""
+        non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration). The scalars stay on
+        the device (the x, r and p updates are two fused kernels); one host read per iteration fetches p.Ap
+        and |r|^2 for the loop test."""
         proj = self.proj
+        lib, c = self.c._lib, self.c
+
+        dmac = torch.as_tensor(dmac, dtype=torch.float64, device=self._dev)  # one copy, not one per iteration
 
         def M(r):
             return proj(self.precond(r, dmac, sws))
@@ -350,29 +354,29 @@ class Mixed:
         b = proj(b)
         bn = float(torch.linalg.norm(b)) + 1e-300
         z0 = M(b)
+        L = b.numel()
         x, r, p = torch.zeros_like(b), b, z0
-        rz = torch.dot(b, z0)
+        bufs = [torch.empty_like(b) for _ in range(6)]  # two sets of (x, r, p), alternated
+        rz = torch.dot(b, z0).reshape(1)
+        rn2 = torch.empty(1, dtype=torch.float64, device=self._dev)
         it, done, conv = 0, False, False
-        one = torch.ones((), dtype=torch.float64, device=self._dev)
         while not done and it < maxit:
+            x1, r1, p1 = bufs[0:3] if it % 2 == 0 else bufs[3:6]
             Ap = proj(self.hvp_A(A, p))
-            pAp = torch.dot(p, Ap)
-            negt = pAp <= 0
-            a = torch.where(negt, torch.zeros_like(pAp), rz / torch.where(negt, one, pAp))
-            x1 = x + a * p
-            r1 = r - a * Ap
-            rn = torch.linalg.norm(r1)
+            pAp = torch.dot(p, Ap).reshape(1)
+            _lib.check(lib.nlfh_fs_cg_update(L, _ptr(x1), _ptr(r1), _ptr(x), _ptr(r), _ptr(p), _ptr(Ap), _ptr(rz),
+                                             _ptr(pAp), _ptr(rn2), _ptr(c._work), _stream()))
             z1 = M(r1)
-            rz1 = torch.dot(r1, z1)
-            p1 = z1 + (rz1 / torch.where(rz == 0, one, rz)) * p
-            pAp_h, rn_h = torch.stack([pAp, rn]).tolist()
+            rz1 = torch.dot(r1, z1).reshape(1)
+            _lib.check(lib.nlfh_fs_cg_direction(L, _ptr(p1), _ptr(z1), _ptr(p), _ptr(rz1), _ptr(rz), _stream()))
+            pAp_h, rn2_h = torch.cat([pAp, rn2]).tolist()
             neg = pAp_h <= 0
-            conv1 = rn_h <= tol * bn
+            conv1 = math.sqrt(max(rn2_h, 0.0)) <= tol * bn
             if neg:
                 x1 = z0 if it == 0 else x
             x, r, p, rz, it = x1, r1, p1, rz1, it + 1
             done, conv = neg or conv1, conv1 and not neg
-        return x, it, conv, done and not conv
+        return x.clone(), it, conv, done and not conv
 
     def _sws(self):
         # scale of the preconditioner: geometric mean of the stiffness (contrast is handled by CG)
