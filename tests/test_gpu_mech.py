@@ -335,3 +335,19 @@ def test_material_functions_match_the_kernel_energy(M):
         W = cell.energy(torch.zeros(cell.shape + (3,), dtype=torch.float64, device="cuda"), Fb)
         psi = float(cell.psi(torch.as_tensor(Fb), cell.lam, cell.mu))
         assert W == pytest.approx(psi, rel=1e-13)
+
+
+def test_plane_strain_cell_newton_matches_oracle(M):
+    """A 2D density (n, n) is one periodic element layer (exact plane strain, mech.py:93-96)."""
+    rng = np.random.default_rng(21)
+    rho = rng.uniform(0.3, 1.0, (5, 4))
+    cell = M.Cell(rho)
+    assert cell.shape == (5, 4, 1)
+    free = [(1, 1)]
+    Fb = np.diag([1.03, 1.0, 1.0])
+    w, F, log = M.Mixed(cell, free=free).solve(Fb, rtol=1e-11, cg_tol=1e-10)
+    assert log["converged"]
+    ref = MR.CellRef(rho)
+    wr, Fr = MR.newton_dense(ref, Fb, free=free, rtol=1e-12)
+    np.testing.assert_allclose(F, Fr, atol=1e-10)
+    assert _rel(cell.mean_stress(w, F), ref.stress(wr, Fr)[1]) < 1e-8
