@@ -620,12 +620,12 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   static_assert(BT >= N, "one twiddle per thread");
   cplx<T> twv{};
   if (tid < N) twv = twg[tid];
-  // up to 64 points a thread's share of the spectrum plane is loaded into registers first, ahead of the
+  // a thread's share of the spectrum plane is loaded into registers first, ahead of the
   // other loads (its loads in flight together; a plain loop waits for each load before the LDS store that
-  // follows it); at 128 the direction variants have no registers to spare and keep the plain loop
+  // follows it); at 128 the odd-iteration direction variant has no registers to spare and keeps the plain loop
   constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
   constexpr int NVS = N * NH / CPV;
-  constexpr bool SBATCH = N <= 64;
+  constexpr bool SBATCH = N <= 64 || DIR != 2;
   constexpr int SPT = SBATCH ? (NVS + BT - 1) / BT : 1;
   const V* const ssrc = reinterpret_cast<const V*>(a.spec + cd.off_spec + ((int64_t)rhs * 3 + pi.comp) * cd.nspec +
                                                    (int64_t)pi.x * N * NH);
