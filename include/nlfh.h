@@ -3,8 +3,8 @@
  * Drop-in boundary for nlfiber/linear.py (class LinearHomogenization, nlfiber/linear.py:99-264).
  * The reference has no FFI of its own: its interface is that Python class. Each entry point below
  * replaces one of its methods (cited per function); the Python class of this package keeps the
- * reference's signature on top of these calls, and INTEGRATION.md shows the ctypes binding a
- * maintainer would add.
+ * reference's signature on top of these calls through a ctypes binding (_lib.py). The finite-strain
+ * entry points (nlfh_fs_*) likewise serve the package's counterpart of nlfiber/mech.py.
  *
  * Conventions
  *  - All array arguments are DEVICE pointers (HIP), stream-ordered on `stream` (a hipStream_t;
