@@ -190,7 +190,7 @@ class Cell:
         return self._lh.Kinv * float(self.N)
 
     # --- operators ---------------------------------------------------------------------------------------
-    def _eval(self, w, Fbar, mode, v=None, dFbar=None):
+    def _eval(self, w, Fbar, mode, v=None, dFbar=None, out=None):
         """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).
         dFbar (modes 2, 3): a 3x3 host array or a device tensor of 9 values. Mode 3 returns the per-element
         density sensitivity (N values) as grad."""
@@ -199,7 +199,7 @@ class Cell:
             raise ValueError(f"w must hold {3 * self.N} values (shape {self.shape + (3,)})")
         F = np.ascontiguousarray(np.asarray(Fbar, dtype=np.float64).reshape(9))
         grad = (torch.empty(self.N, dtype=torch.float64, device=self._dev) if mode == 3 else torch.empty_like(w)) \
-            if mode >= 1 else None
+            if mode >= 1 and out is None else out
         if v is not None:
             v = torch.as_tensor(v, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
             if v.numel() != 3 * self.N:
@@ -315,13 +315,19 @@ class Mixed:
         w, F = self.unpack(z, Fb)
         return (w.clone(), F)
 
-    def hvp_A(self, A, v):
+    def hvp_A(self, A, v, out=None):
+        """Hessian at the state A = (w, Fbar) applied to v (into `out` when given, a contiguous vector of
+        nw + k values)."""
         w, F = A
         dF = torch.zeros(9, dtype=torch.float64, device=self._dev)
         if self.k:
             dF.index_add_(0, self._flat, v[self.nw:])
-        hw, sc = self.c._eval(w, F, 2, v[: self.nw], dF)
-        return torch.cat([hw, sc[1:][self._flat]])
+        if out is None:
+            out = torch.empty(self.nw + self.k, dtype=torch.float64, device=self._dev)
+        _, sc = self.c._eval(w, F, 2, v[: self.nw], dF, out=out[: self.nw])
+        if self.k:
+            out[self.nw:].copy_(sc[1:][self._flat])
+        return out
 
     def macro_diag(self, A, Fb):
         """Diagonal of the Hessian in the macro unknowns (for the preconditioner) (mech.py:240-246)."""
@@ -332,10 +338,27 @@ class Mixed:
             d.append(float(self.hvp_A(A, e)[self.nw + j]))
         return np.abs(np.array(d)) + 1e-12
 
-    def precond(self, r, dmac, swscale):
-        w = self.c.precond_w(r[: self.nw], 1.0 / swscale)
-        mac = r[self.nw:] / torch.as_tensor(dmac, dtype=torch.float64, device=self._dev)
-        return torch.cat([w.reshape(-1), mac])
+    def precond(self, r, dmac, swscale, out=None):
+        """The reference's preconditioner (mech.py:248-251): the FFT Green operator of the unit-stiffness solid,
+        scaled by 1/swscale, on w, the inverse macro diagonal on the macro unknowns (into `out` when given)."""
+        c = self.c
+        if c._lh is None:
+            c._lh = LinearHomogenization(c.shape, c.E, c.nu, c.emin, c.p)
+        if out is None:
+            out = torch.empty(self.nw + self.k, dtype=torch.float64, device=self._dev)
+        rp = r[: self.nw].reshape(c.N, 3).t().contiguous()  # component-planar (3, N) for the FFT passes
+        zp = torch.empty_like(rp)
+        _lib.check(c._lib.nlfh_precond(c._lh._plan.handle, _ptr(rp), _ptr(zp), 1, 0, _stream()))
+        # back to (N, 3) with the scale N / swscale in one pass
+        torch.mul(zp.t(), float(c.N) / swscale, out=out[: self.nw].view(c.N, 3))
+        if self.k:
+            torch.div(r[self.nw:], torch.as_tensor(dmac, dtype=torch.float64, device=self._dev), out=out[self.nw:])
+        return out
+
+    def _proj_(self, v):
+        """proj in place on a vector this class owns."""
+        _lib.check(self.c._lib.nlfh_fs_project(self.c._shape_c, _ptr(v), _ptr(self.c._work), _stream()))
+        return v
 
     # --- linear solves (mech.py:253-288) ----------------------------------------------------------------
     def _cg(self, A, b, dmac, sws, tol, maxit):
@@ -348,9 +371,11 @@ class Mixed:
 
         dmac = torch.as_tensor(dmac, dtype=torch.float64, device=self._dev)  # one copy, not one per iteration
 
-        def M(r):
-            return proj(self.precond(r, dmac, sws))
+        def M(r, out=None):
+            return self._proj_(self.precond(r, dmac, sws, out))
 
+        Apb = torch.empty_like(b)
+        zb = torch.empty_like(b)
         b = proj(b)
         bn = float(torch.linalg.norm(b)) + 1e-300
         z0 = M(b)
@@ -362,11 +387,11 @@ class Mixed:
         it, done, conv = 0, False, False
         while not done and it < maxit:
             x1, r1, p1 = bufs[0:3] if it % 2 == 0 else bufs[3:6]
-            Ap = proj(self.hvp_A(A, p))
+            Ap = self._proj_(self.hvp_A(A, p, Apb))
             pAp = torch.dot(p, Ap).reshape(1)
             _lib.check(lib.nlfh_fs_cg_update(L, _ptr(x1), _ptr(r1), _ptr(x), _ptr(r), _ptr(p), _ptr(Ap), _ptr(rz),
                                              _ptr(pAp), _ptr(rn2), _ptr(c._work), _stream()))
-            z1 = M(r1)
+            z1 = M(r1, zb)
             rz1 = torch.dot(r1, z1).reshape(1)
             _lib.check(lib.nlfh_fs_cg_direction(L, _ptr(p1), _ptr(z1), _ptr(p), _ptr(rz1), _ptr(rz), _stream()))
             pAp_h, rn2_h = torch.cat([pAp, rn2]).tolist()
