@@ -1,27 +1,29 @@
-"""Periodic finite-strain homogenization of voxel cells on the GPU (drop-in for nlfiber/mech.py's Cell and Mixed).
+"""Periodic finite-strain homogenization of voxel cells on the GPU (drop-in for nlfiber/mech.py).
 
-The energy, its gradients and the Hessian-vector product run in libnlfh.so (`nlfh_fs_eval`, FP64); the
-preconditioner is the linear path's FFT Green operator of the homogeneous solid (`nlfh_precond`); the CG
-vectors stay on the device and the Newton / line-search / forcing logic runs on the host as in the reference.
+The energy, its gradients, the Hessian-vector product, the density sensitivity and the third-order energy
+derivatives run in libnlfh.so (`nlfh_fs_*`, FP64); the preconditioner is the linear path's FFT Green operator of
+the homogeneous solid (`nlfh_precond`); the CG vectors stay on the device and the Newton / line-search / forcing
+logic runs on the host as in the reference.
 
 Kept from the reference (nlfiber/mech.py):
   * Cell(rho, E, nu, material, wls, emin, p): stiffness scale s = emin + (1 - emin) rho^p, the void
-    interpolation gamma of Wang et al. (mech.py:91-112), energy(w, Fbar), mean_stress(w, Fbar),
-    precond_w(r) (mech.py:141-181);
+    interpolation gamma of Wang et al. (mech.py:91-121), energy(w, Fbar), mean_stress(w, Fbar),
+    precond_w(r) (mech.py:141-181), tiled, with_rho, and the material functions (mech.py:31-54);
   * Mixed(cell, free, forcing): unknowns z = (w, free components of Fbar), z0, unpack, proj, tangent,
     hvp_A, macro_diag, precond, solve_linear and the Newton-CG solve with Eisenstat-Walker forcing and the
-    Armijo / gradient-decrease line search (mech.py:183-330).
+    Armijo / gradient-decrease line search (mech.py:183-346);
+  * Mixed.grad_rho, the adjoint density gradient of a quantity of the equilibrium state (mech.py:348-363), with
+    q a torch function (torch autograd stands in for jax.grad; Cell.mean_stress_t is the differentiable Pbar);
+  * Mixed.lowest_eigs: scipy's LOBPCG on the Jacobi-scaled Hessian, the block products on the device
+    (mech.py:365-409);
+  * effective_tangent (9 linear solves with the matrix-free Hessian), fibonacci_sphere and rank_one_min
+    (mech.py:411-451);
+  * toec and cubic_toec: second and third energy derivatives along the linear correctors from one kernel pass
+    (mech.py:454-515).
 Differences of form, not of result: the tangent is applied matrix-free (`tangent` returns the state it is
 taken at instead of 81 numbers per Gauss point); z, w and Fbar are torch / numpy float64 arrays instead of
-JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays.
-  * Mixed.grad_rho, the adjoint density gradient of a quantity of the equilibrium state (mech.py:348-363), with
-    q a torch function (torch autograd stands in for jax.grad; Cell.mean_stress_t is the differentiable Pbar).
-  * effective_tangent (9 linear solves with the matrix-free Hessian), fibonacci_sphere and rank_one_min
-    (mech.py:411-451).
-  * Mixed.lowest_eigs: scipy's LOBPCG on the Jacobi-scaled Hessian, the block products on the device
-    (mech.py:365-409).
-  * toec and cubic_toec: second and third energy derivatives along the linear correctors from one kernel pass
-    (mech.py:454-515); not differentiable in rho (the reference's is, through chi).
+JAX arrays; 3x3 results (the returned Fbar, mean_stress) and toec's C and T are numpy arrays; toec is not
+differentiable in rho (the reference's is, through chi).
 """
 import ctypes
 import math
