@@ -195,7 +195,8 @@ int nlfh_dmap_rho_vjp(nlfh_dmap* dm, const double* phi, double beta, const doubl
 
 /* ---- Finite-strain cell operators (nlfiber/mech.py) -------------------------------------------------
  * FP64, device memory. rho (nx, ny, nz); w, v, grad: (nx, ny, nz, 3) C-order, component fastest.
- * mat = {lam, mu, emin, p, wls_b, wls_r0}; flags bit 0: neo-Hookean (else linear), bit 1: void interpolation.
+ * mat = {lam, mu, emin, p, wls_b, wls_r0}; flags bit 0: neo-Hookean (else linear), bit 1: void interpolation;
+ * nlfh_fs_eval only: bit 2 set = form only the scalars whose bit 8 + q is set (the others are returned as 0).
  * Fbar: host 3x3 row-major; dFbar: device 3x3 row-major (so that a CG loop can form it from its vectors
  * without a host round trip). work: nlfh_fs_work_doubles(nx, ny, nz) doubles of device memory. */
 long long nlfh_fs_work_doubles(int nx, int ny, int nz);

@@ -246,7 +246,8 @@ struct FsArgs {
   double Fbar[9];
   const double* dFbar;  // device, 9 values (mode 2)
   double* fe;        // (24, N) element nodal forces, or null
-  double* part;      // (blocks, kFsParts) partial sums
+  double* part;      // (kFsParts, blocks) partial sums
+  unsigned pmask;    // partial sums to form (bit q: scalar q); the others are zero
 };
 
 // Eight consecutive lanes per element, lane g at Gauss point g. Lane g handles node c under the label
@@ -391,10 +392,12 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
   }
   // partial sums of the block: a fixed xor butterfly per wavefront for every value, then one barrier and a
   // fixed-order sum over the wavefronts; stored value-major (part[q][block]) for coalesced final sums
+  // only the sums the caller asked for (a.pmask; the others are stored as zero)
   constexpr int nq = MODE == 0 ? 1 : kFsParts;
   __shared__ double redq[kFsThreads / 64][kFsParts];
 #pragma unroll
   for (int q = 0; q < nq; ++q) {
+    if (!((a.pmask >> q) & 1u)) continue;  // uniform over the launch
     double v = acc[q];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -403,8 +406,10 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
   __syncthreads();
   if (threadIdx.x < nq) {
     double t = 0.0;
+    if ((a.pmask >> threadIdx.x) & 1u) {
 #pragma unroll
-    for (int w = 0; w < kFsThreads / 64; ++w) t += redq[w][threadIdx.x];
+      for (int w = 0; w < kFsThreads / 64; ++w) t += redq[w][threadIdx.x];
+    }
     a.part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
   }
 }
@@ -738,6 +743,8 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
   a.sg = sg;
   a.w = w;
   a.v = mode >= 2 ? v : nullptr;
+  // flags bit 2: form only the scalars named in bits 8..17 (bit 8 + q: scalar q)
+  a.pmask = (flags & 4) ? ((unsigned)flags >> 8) & 0x3FFu : 0x3FFu;
   for (int q = 0; q < 9; ++q) a.Fbar[q] = Fbar[q];
   a.dFbar = mode >= 2 ? dFbar : nullptr;
   const int64_t nblk = (a.n + kFsElemPerBlock - 1) / kFsElemPerBlock;  // element blocks (partials)

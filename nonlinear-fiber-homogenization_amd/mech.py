@@ -192,8 +192,9 @@ class Cell:
         return self._lh.Kinv * float(self.N)
 
     # --- operators ---------------------------------------------------------------------------------------
-    def _eval(self, w, Fbar, mode, v=None, dFbar=None, out=None):
+    def _eval(self, w, Fbar, mode, v=None, dFbar=None, out=None, scalars=None):
         """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).
+        scalars: optional bit mask of the scalars to form (bit q: scalar q; the others come back as 0).
         dFbar (modes 2, 3): a 3x3 host array or a device tensor of 9 values. Mode 3 returns the per-element
         density sensitivity (N values) as grad."""
         w = torch.as_tensor(w, dtype=torch.float64, device=self._dev).reshape(-1).contiguous()
@@ -212,7 +213,8 @@ class Cell:
                                  device=self._dev).reshape(-1).contiguous()
             if dF.numel() != 9:
                 raise ValueError("dFbar must hold 9 values")
-        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self._sg), self._mat.ctypes.data_as(_dp), self._flags,
+        flags = self._flags if scalars is None else self._flags | 4 | (int(scalars) << 8)
+        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self._sg), self._mat.ctypes.data_as(_dp), flags,
                                           _ptr(w), F.ctypes.data_as(_dp), _ptr(v), _ptr(dF), mode, _ptr(grad),
                                           _ptr(self._scal), _ptr(self._work), _stream()))
         return grad, self._scal
@@ -274,6 +276,8 @@ class Mixed:
         self._dev = cell._dev
         # row-major flat index of each free component, for device-side gathers and scatters
         self._flat = torch.tensor([3 * a + b for a, b in self.free], dtype=torch.long, device=self._dev)
+        # the Hessian products need only the stress sums of the free components (scalar 1 + 3a + b)
+        self._smask = sum(1 << (1 + 3 * a + b) for a, b in set(self.free))
 
     # --- unknowns ----------------------------------------------------------------------------------------
     def z0(self, Fb):
@@ -326,7 +330,7 @@ class Mixed:
             dF.index_add_(0, self._flat, v[self.nw:])
         if out is None:
             out = torch.empty(self.nw + self.k, dtype=torch.float64, device=self._dev)
-        _, sc = self.c._eval(w, F, 2, v[: self.nw], dF, out=out[: self.nw])
+        _, sc = self.c._eval(w, F, 2, v[: self.nw], dF, out=out[: self.nw], scalars=self._smask)
         if self.k:
             out[self.nw:].copy_(sc[1:][self._flat])
         return out
