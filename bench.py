@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber", "finite"])
     ap.add_argument("--fiber-n-o", type=int, default=16, help="octant side of the fiber workload (cell 2 n_o)")
+    ap.add_argument("--fiber-precision", default="fp64", choices=["fp64", "mixed"],
+                    help="linear solver of the fiber workload (fp64: the reference's PCG)")
     ap.add_argument("--finite-n", type=int, default=64, help="cell side of the finite-strain workload")
     ap.add_argument("--finite-strain", type=float, default=0.05, help="uniaxial stretch - 1 of the finite workload")
     ap.add_argument("--cells", type=int, default=8, help="64^3 cells per step (cell64)")
@@ -296,7 +298,7 @@ def fiber_bench(args):
     n_o, beta, v, r = args.fiber_n_o, 8.0, 0.3, 1.5
     phi0 = random_start(n_o, seed=0)
     d = random_start(n_o, seed=1)
-    fib = Fiber(n_o, v=v, r=r)  # the reference's defaults: emin 1e-3, tol 1e-9
+    fib = Fiber(n_o, v=v, r=r, precision=args.fiber_precision)  # the reference's defaults: emin 1e-3, tol 1e-9
     phis = [torch.tensor(phi0 + 0.02 * k * d, device="cuda") for k in range(args.warmup + args.steps)]
     for k in range(args.warmup):
         fib.value_jac(phis[k], beta)
@@ -339,8 +341,10 @@ def fiber_bench(args):
         "metric": "Fiber.value_jac evaluations/sec (density map + 2-RHS PCG + Jacobian rows)",
         "value": round(value, 3), "unit": "value_jac/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * dt / args.steps, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"Fiber(n_o={n_o}) on a {n}^3 cell, beta {beta}, v {v}, r {r}, emin 1e-3, tol 1e-9, "
+        "vs_baseline": None, "dtype": "f64" if args.fiber_precision == "fp64" else "f32-inner/f64-refinement",
+        "data": "synthetic",
+        "config": {"solver_precision": args.fiber_precision,
+                   "workload": f"Fiber(n_o={n_o}) on a {n}^3 cell, beta {beta}, v {v}, r {r}, emin 1e-3, tol 1e-9, "
                                f"warm-started along phi_k = phi_0 + 0.02 k d", "pcg_iterations": its},
         "roofline": roof, "cpu_baseline": cpu}), flush=True)
 

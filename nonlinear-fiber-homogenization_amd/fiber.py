@@ -18,10 +18,11 @@ from .linear import LinearHomogenization, cubic_invariants, cubic_leakage  # noq
 
 
 class Fiber:
-    def __init__(self, n_o, d=3, v=0.3, r=1.5, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-9):
+    def __init__(self, n_o, d=3, v=0.3, r=1.5, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-9, precision="fp64"):
+        """precision (not in the reference): the linear solver's, "fp64" (the reference's PCG) or "mixed"."""
         self.n_o, self.d, self.v = n_o, d, v
         self.dm = DensityMap(n_o, d, r, v)
-        self.lh = LinearHomogenization(self.dm.shape, E=E, nu=nu, emin=emin, p=p, tol=tol)
+        self.lh = LinearHomogenization(self.dm.shape, E=E, nu=nu, emin=emin, p=p, tol=tol, precision=precision)
         self._mat = dict(E=E, nu=nu, emin=emin, p=p, tol=tol)
         self._lh_bin = {}
         self.u = None  # warm start of the correctors
