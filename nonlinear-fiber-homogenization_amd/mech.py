@@ -192,7 +192,16 @@ class Cell:
         return self._lh.Kinv * float(self.N)
 
     # --- operators ---------------------------------------------------------------------------------------
-    def _eval(self, w, Fbar, mode, v=None, dFbar=None, out=None, scalars=None):
+    def _sg_for(self, rho):
+        """Per-element scale / interpolation factors of another density on this cell's grid (the reference's
+        energy(w, Fbar, rho), mech.py:141-145, evaluates the cell at a density other than its own)."""
+        r = torch.as_tensor(rho, dtype=torch.float64, device=self._dev).reshape(self.shape).contiguous()
+        sg = torch.empty(4 * self.N, dtype=torch.float64, device=self._dev)
+        _lib.check(self._lib.nlfh_fs_prepare(self._shape_c, _ptr(r), self._mat.ctypes.data_as(_dp), self._flags,
+                                             _ptr(sg), _stream()))
+        return sg
+
+    def _eval(self, w, Fbar, mode, v=None, dFbar=None, out=None, scalars=None, sg=None):
         """One nlfh_fs_eval call; returns (grad or None, the 10 scalars as a device tensor, unsynchronised).
         scalars: optional bit mask of the scalars to form (bit q: scalar q; the others come back as 0).
         dFbar (modes 2, 3): a 3x3 host array or a device tensor of 9 values. Mode 3 returns the per-element
@@ -214,18 +223,21 @@ class Cell:
             if dF.numel() != 9:
                 raise ValueError("dFbar must hold 9 values")
         flags = self._flags if scalars is None else self._flags | 4 | (int(scalars) << 8)
-        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self._sg), self._mat.ctypes.data_as(_dp), flags,
+        _lib.check(self._lib.nlfh_fs_eval(self._shape_c, _ptr(self._sg if sg is None else sg),
+                                          self._mat.ctypes.data_as(_dp), flags,
                                           _ptr(w), F.ctypes.data_as(_dp), _ptr(v), _ptr(dF), mode, _ptr(grad),
                                           _ptr(self._scal), _ptr(self._work), _stream()))
         return grad, self._scal
 
-    def energy(self, w, Fbar):
-        """W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + grad w at g) (mech.py:141-154), a Python float."""
-        return float(self._eval(w, Fbar, 0)[1][0])
+    def energy(self, w, Fbar, rho=None):
+        """W(w, Fbar) = wq sum_e s_e sum_g psi_eff(Fbar + grad w at g) (mech.py:141-154), a Python float; rho:
+        another density on this grid (default: the cell's)."""
+        return float(self._eval(w, Fbar, 0, sg=None if rho is None else self._sg_for(rho))[1][0])
 
-    def gradient(self, w, Fbar):
-        """(dW/dw with w's shape, Pbar = dW/dFbar (3, 3)) (the reference's Cell._g and Cell._P)."""
-        g, sc = self._eval(w, Fbar, 1)
+    def gradient(self, w, Fbar, rho=None):
+        """(dW/dw with w's shape, Pbar = dW/dFbar (3, 3)) (the reference's Cell._g and Cell._P); rho as in
+        energy."""
+        g, sc = self._eval(w, Fbar, 1, sg=None if rho is None else self._sg_for(rho))
         return g.reshape(self.shape + (3,)), sc[1:].cpu().numpy().reshape(3, 3)
 
     def mean_stress(self, w, Fbar):
@@ -309,6 +321,11 @@ class Mixed:
 
     def _E(self, z, Fb):
         return self.c.energy(*self.unpack(z, Fb))
+
+    def Ez(self, z, Fb, rho=None):
+        """W at the unknowns z (mech.py:201-203); rho as in Cell.energy."""
+        w, F = self.unpack(z, Fb)
+        return self.c.energy(w, F, rho)
 
     def _g(self, z, Fb):
         w, F = self.unpack(z, Fb)

@@ -351,3 +351,19 @@ def test_plane_strain_cell_newton_matches_oracle(M):
     wr, Fr = MR.newton_dense(ref, Fb, free=free, rtol=1e-12)
     np.testing.assert_allclose(F, Fr, atol=1e-10)
     assert _rel(cell.mean_stress(w, F), ref.stress(wr, Fr)[1]) < 1e-8
+
+
+def test_energy_at_another_density(M):
+    """Cell.energy(w, Fbar, rho) (mech.py:141-145) equals the energy of a cell built on that density."""
+    import torch
+    rng = np.random.default_rng(22)
+    rho = rng.uniform(0.0, 1.0, (4, 3, 3))
+    rho2 = rng.uniform(0.0, 1.0, (4, 3, 3))
+    w = torch.as_tensor(0.05 * rng.standard_normal(rho.shape + (3,)), device="cuda")
+    Fb = np.eye(3) + 0.05 * rng.standard_normal((3, 3))
+    a, b = M.Cell(rho), M.Cell(rho2)
+    assert a.energy(w, Fb, rho2) == b.energy(w, Fb)
+    assert a.energy(w, Fb) == pytest.approx(MR.CellRef(rho).energy(w.cpu().numpy(), Fb), rel=1e-12)
+    np.testing.assert_array_equal(a.gradient(w, Fb, rho2)[1], b.gradient(w, Fb)[1])
+    mix = M.Mixed(a, free=[])
+    assert mix.Ez(w.reshape(-1), Fb, rho2) == b.energy(w, Fb)
