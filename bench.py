@@ -395,8 +395,16 @@ def finite_bench(args):
     torch.cuda.synchronize()
     t_h = e0.elapsed_time(e1) / reps * 1e-3
     flops = FS_FLOP_PER_VOXEL * cell.N
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "finite_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            tr = json.load(f)
+        key = f"hvp voxels={cell.N}"  # counters of the same launch shape only
+        if key in tr:
+            traffic = tr[key]["hbm_bytes"]
     roof = {"bound": "fp64-valu", "achieved": round(flops / t_h / 1e12, 2), "peak": 78.6, "unit": "TFLOP/s",
-            "frac": round(flops / t_h / 1e12 / 78.6, 4), "traffic": None,
+            "frac": round(flops / t_h / 1e12 / 78.6, 4), "traffic": traffic,
             "kernel": f"fs_element_kernel<2> + fs_gather_kernel + fs_reduce_kernel (one Hessian-vector product, "
                       f"{n}^3)", "flops_per_voxel": FS_FLOP_PER_VOXEL, "avg_us": round(t_h * 1e6, 2),
             "hbm_algorithmic_GBs": round(FS_HVP_BYTES_PER_VOXEL * cell.N / t_h / 1e9, 1)}
