@@ -41,3 +41,15 @@ def test_batched_keys_only_for_shapes_that_batch(pkg):
             assert e.get("cells") in (None, at.batch_cells(shape)), key
     assert at.batch_cells((128, 128, 128)) == 1
     assert at.batch_cells((64, 64, 64)) == 8
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The bench runs on one GPU: --gpus above 1 exits with status 2 and a JSON line that says so, before any
+    GPU work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 2
+    line = json.loads(r.stdout.strip().splitlines()[-1])
+    assert line["n_gpus"] == 2 and "one GPU" in line["error"]
