@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .linear import Plan, _device, _es_host, _ptr, _stream
+from .linear import Plan, _device, _es_host, _ptr, _stream, check_grid
 
 
 class BatchedHomogenization:
@@ -34,6 +34,8 @@ class BatchedHomogenization:
             raise ValueError("precision must be 'fp64' or 'mixed'")
         self.precision = precision
         self.m = m
+        for sh in dict.fromkeys(self.shapes):
+            check_grid(sh)
         self._dev = _device()
         self._plan = Plan(self.shapes, E, nu, m_layout=m)
         self._lib = _lib.load()

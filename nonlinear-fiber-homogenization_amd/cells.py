@@ -86,7 +86,7 @@ def hat_kernel_fft(n, r):
 def periodic_filter(x, r, kind="gaussian"):
     n = x.shape[0]
     kf = gaussian_kernel_fft(n, r) if kind == "gaussian" else hat_kernel_fft(n, r)
-    return np.fft.irfftn(np.fft.rfftn(x) * kf, s=x.shape)
+    return np.fft.irfftn(np.fft.rfftn(x) * kf, s=x.shape, axes=tuple(range(x.ndim)))
 
 
 _CANON = {}

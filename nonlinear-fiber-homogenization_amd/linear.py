@@ -112,13 +112,66 @@ def _es_host(Es):
     return Es, Es.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
+def _check_dtype(dtype):
+    """The reference's `dtype` (nlfiber/linear.py:102,107-111) sets the precision of its operators. This
+    class computes and returns float64 (FP32 enters only as the inner solve of precision="mixed"), so any
+    other dtype is refused rather than silently replaced."""
+    name = str(getattr(dtype, "name", None) or getattr(dtype, "__name__", None) or dtype)
+    if "float64" not in name and name not in ("double", "f8", "<f8"):
+        raise ValueError(f"dtype {dtype!r} is not supported: LinearHomogenization computes in float64 "
+                         "(precision='mixed' runs FP32 inner solves inside FP64 refinement)")
+
+
+# Grids the FFT preconditioner and the operator kernels take (the plan refuses others)
+MAX_NX = 512
+MAX_NY = 512
+MAX_NZ = 256
+
+
+def check_grid(shape):
+    """Raise ValueError for a cell the library cannot take: every side must be a product of the radices 2,
+    3 and 5, with nx, ny <= 512 and nz <= 256 (the preconditioner's line transforms are mixed-radix
+    2/3/4/5/8 codelets), and nz must split into at most 64 lanes of 1-4 consecutive z values (the operator
+    keeps a row of nz nodes in one wavefront)."""
+    if len(shape) != 3:
+        raise ValueError(f"cell shape {tuple(shape)} must have three sides")
+    for n, lim, ax in zip(shape, (MAX_NX, MAX_NY, MAX_NZ), "xyz"):
+        m = int(n)
+        if m < 1 or m > lim:
+            raise ValueError(f"cell side n{ax} = {n} is outside [1, {lim}]")
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m != 1:
+            raise ValueError(f"cell side n{ax} = {n} has a prime factor other than 2, 3, 5, which the FFT "
+                             f"preconditioner does not take (the nearest supported sides are "
+                             f"{_nearest_smooth(int(n))})")
+    nz = int(shape[2])
+    if not any(nz % zpt == 0 and nz // zpt <= 64 for zpt in (1, 2, 3, 4)):
+        raise ValueError(f"cell side nz = {nz} must be a multiple of 1, 2, 3 or 4 with at most 64 lanes "
+                         "(nz = lanes x z values per lane, lanes <= 64)")
+
+
+def _nearest_smooth(n):
+    def smooth(k):
+        for p in (2, 3, 5):
+            while k % p == 0:
+                k //= p
+        return k == 1
+    lo = next(k for k in range(n, 0, -1) if smooth(k))
+    hi = next(k for k in range(n, 4 * n + 2) if smooth(k))
+    return f"{lo} and {hi}"
+
+
 class Plan:
     """Owner of one nlfh_plan (cells of given shapes, material E, nu)."""
 
     def __init__(self, shapes, E=1.0, nu=0.3, m_layout=6):
+        self.shapes = [tuple(int(v) for v in s) for s in shapes]
+        for sh in dict.fromkeys(self.shapes):
+            check_grid(sh)
         lib = _lib.load()
         self.device = _device()
-        self.shapes = [tuple(int(v) for v in s) for s in shapes]
         arr = (ctypes.c_int * (3 * len(self.shapes)))(*[v for s in self.shapes for v in s])
         h = ctypes.c_void_p()
         _lib.check(lib.nlfh_plan_create(arr, len(self.shapes), float(E), float(nu), int(m_layout), ctypes.byref(h)))
@@ -245,6 +298,8 @@ class LinearHomogenization:
         self.plane = len(shape) == 2
         self.shape = shape + (1,) if self.plane else shape
         self.E, self.nu, self.emin, self.p = E, nu, emin, p
+        _check_dtype(dtype)
+        check_grid(self.shape)
         self.dtype, self.tol, self.maxit = torch.float64, tol, maxit
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
@@ -277,6 +332,19 @@ class LinearHomogenization:
         Es_np, Es_p = _es_host(Es)
         m = Es_np.shape[0]
         s = s.detach().contiguous()
+        if m > 6:
+            # the library forms at most 6 x 6 at once: every pair (I, J) from a 2-strain call
+            C = torch.empty((m, m), dtype=torch.float64)
+            dcds = torch.empty((m, m) + self.shape, dtype=torch.float64, device=self._dev) if want_grad else None
+            for I in range(m):
+                for J in range(I, m):
+                    idx = [I, J] if I != J else [I]
+                    Cp, dp = self._bilinear_raw(s, Es_np[idx], u_planar[idx].contiguous(), want_grad)
+                    C[I, J] = C[J, I] = Cp[0, -1]
+                    if want_grad:
+                        dcds[I, J] = dp[0, -1]
+                        dcds[J, I] = dp[-1, 0]
+            return C, dcds
         C = torch.empty((m, m), dtype=torch.float64, device=self._dev)
         dcds = torch.empty((m, m) + self.shape, dtype=torch.float64, device=self._dev) if want_grad else None
         _lib.check(self._lib.nlfh_bilinear(self._plan.handle, _ptr(s), Es_p, _ptr(u_planar), m, _ptr(C), _ptr(dcds),
@@ -284,17 +352,28 @@ class LinearHomogenization:
         return C.cpu(), dcds
 
     def _project_raw(self, v):
-        _lib.check(self._lib.nlfh_project_mean(self._plan.handle, _ptr(v), v.shape[0], _stream()))
+        for i in range(0, v.shape[0], 6):
+            blk = v[i:i + 6]
+            _lib.check(self._lib.nlfh_project_mean(self._plan.handle, _ptr(blk), blk.shape[0], _stream()))
 
     def _solve_rhs_raw(self, s, b_planar):
+        """x with K x = b for every field of b (m, 3, nx, ny, nz), in blocks of at most 6 right-hand sides
+        (FP64 PCG from zero, the reference's _solve_rhs); last_info holds the largest iteration count."""
         m = b_planar.shape[0]
         x = torch.empty_like(b_planar)
-        it = (ctypes.c_int * 1)()
-        cv = (ctypes.c_int * 1)()
-        rr = (ctypes.c_double * m)()
-        _lib.check(self._lib.nlfh_solve_rhs(self._plan.handle, _ptr(s.contiguous()), _ptr(b_planar), m, _ptr(x),
-                                            float(self.tol), int(self.maxit), 0, it, cv, rr, _stream()))
-        self.last_info = dict(it=it[0], converged=bool(cv[0]), relres=list(rr))
+        its, conv, rel = [], [], []
+        for i in range(0, m, 6):
+            k = min(6, m - i)
+            it = (ctypes.c_int * 1)()
+            cv = (ctypes.c_int * 1)()
+            rr = (ctypes.c_double * k)()
+            _lib.check(self._lib.nlfh_solve_rhs(self._plan.handle, _ptr(s.contiguous()), _ptr(b_planar[i:i + k]), k,
+                                                _ptr(x[i:i + k]), float(self.tol), int(self.maxit), 0, it, cv, rr,
+                                                _stream()))
+            its.append(it[0])
+            conv.append(bool(cv[0]))
+            rel += list(rr)
+        self.last_info = dict(it=max(its), converged=all(conv), relres=rel)
         return x
 
     def _s(self, s):
@@ -331,18 +410,27 @@ class LinearHomogenization:
         return out[0] if single else out
 
     def load(self, s, Es):
-        Es_np, Es_p = _es_host(Es)
+        Es_np, _ = _es_host(Es)
         m = Es_np.shape[0]
         f = torch.empty((m, 3) + self.shape, dtype=torch.float64, device=self._dev)
-        _lib.check(self._lib.nlfh_load(self._plan.handle, _ptr(self._s(s).detach()), Es_p, m, _ptr(f), _stream()))
+        sd = self._s(s).detach()
+        for i in range(0, m, 6):
+            blk, blk_p = _es_host(Es_np[i:i + 6])
+            _lib.check(self._lib.nlfh_load(self._plan.handle, _ptr(sd), blk_p, blk.shape[0], _ptr(f[i:i + 6]),
+                                           _stream()))
         return from_planar(f)
 
     def residual(self, s, Es, u):
-        Es_np, Es_p = _es_host(Es)
+        Es_np, _ = _es_host(Es)
         up = to_planar(u, self.shape)
+        if up.shape[0] != Es_np.shape[0]:
+            raise ValueError(f"{up.shape[0]} fields for {Es_np.shape[0]} strains")
         r = torch.empty_like(up)
-        _lib.check(self._lib.nlfh_residual(self._plan.handle, _ptr(self._s(s).detach()), Es_p, _ptr(up),
-                                           up.shape[0], _ptr(r), _stream()))
+        sd = self._s(s).detach()
+        for i in range(0, up.shape[0], 6):
+            blk, blk_p = _es_host(Es_np[i:i + 6])
+            _lib.check(self._lib.nlfh_residual(self._plan.handle, _ptr(sd), blk_p, _ptr(up[i:i + 6]), blk.shape[0],
+                                               _ptr(r[i:i + 6]), _stream()))
         self._project_raw(r)
         return from_planar(r)
 

@@ -64,7 +64,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 
 def voigt_tensor(I):
-    """Symmetric 3x3 tensor of the Voigt unit strain I (engineering shear: off-diagonals 1/2) (mech.py:57-65)."""
+    """3x3 strain tensor whose Voigt vector is the I-th unit vector; a unit engineering shear puts 1/2 in both
+    off-diagonal slots (mech.py:57-65)."""
     H = np.zeros((3, 3))
     i, j = VOIGT[I]
     if i == j:
@@ -75,15 +76,16 @@ def voigt_tensor(I):
 
 
 def to_voigt(M):
-    """Voigt vector (engineering shear) of the symmetric part of a 3x3 matrix (mech.py:68-71)."""
+    """Six-vector xx, yy, zz, yz, xz, xy of sym(M), shears doubled as engineering strains (mech.py:68-71)."""
     M = np.asarray(M, dtype=np.float64)
     S = 0.5 * (M + M.T)
     return np.array([S[0, 0], S[1, 1], S[2, 2], 2 * S[1, 2], 2 * S[0, 2], 2 * S[0, 1]])
 
 
 def _linesearch(E, gnorm, x, dx, E0, slope, gn):
-    """Backtracking from t = 1: Armijo decrease, or (at the round-off floor of E) a smaller gradient
-    norm; None when t falls below 1e-6 (mech.py:74-88)."""
+    """Step length by halving from 1 (mech.py:74-88): accept the first t with a sufficient energy decrease
+    (Armijo, c = 1e-4), or, when the energy change is below its round-off level, with a smaller gradient
+    norm than at t = 0. Gives up (None) once t would drop below 1e-6."""
     tol = 64 * np.finfo(np.float64).eps * max(abs(E0), 1e-300)
     t = 1.0
     while t > 1e-6:
@@ -427,7 +429,8 @@ class Mixed:
         return x.clone(), it, conv, done and not conv
 
     def _sws(self):
-        # scale of the preconditioner: geometric mean of the stiffness (contrast is handled by CG)
+        # the FFT preconditioner is the homogeneous operator times one number: exp(mean log s), the log-average
+        # element stiffness; the spread of s around it is left to the Krylov iteration
         return float(torch.exp(torch.mean(torch.log(self.c.s))))
 
     def solve_linear(self, A, b, tol=1e-10, maxit=20000):
@@ -438,10 +441,11 @@ class Mixed:
 
     # --- adjoint (mech.py:348-363) -------------------------------------------------------------------------
     def grad_rho(self, z, Fb, q, cg_tol=1e-10, cg_maxit=20000):
-        """d q / d rho for a scalar q(z, Fb, rho) of the equilibrium state z(rho) at prescribed Fb:
-        H lam = dq/dz, dq/drho = dq/drho|_z - d/drho <lam, grad_z W(z; rho)>. q is a torch function of z (a
-        device tensor) and rho (the cell's density tensor), e.g. of the macro unknowns z[nw:] or of
-        cell.mean_stress_t(*mix.unpack_t(z, Fb), rho); its derivatives come from torch autograd."""
+        """Total derivative in rho of a scalar q(z, Fb, rho) evaluated at equilibrium z(rho) (mech.py:348-363).
+        One adjoint solve with the Hessian, H lam = dq/dz, then the explicit part dq/drho at fixed z minus the
+        density derivative of lam . grad_z W. q is a torch function of the device unknowns z and of the cell's
+        density, for instance of the macro unknowns z[nw:] or of cell.mean_stress_t(*mix.unpack_t(z, Fb),
+        rho); torch autograd differentiates it."""
         Fb = np.asarray(Fb, dtype=np.float64)
         z = torch.as_tensor(z, dtype=torch.float64, device=self._dev)
         zz = z.detach().clone().requires_grad_(True)
@@ -524,8 +528,10 @@ class Mixed:
 
     # --- Newton-CG (mech.py:290-330) --------------------------------------------------------------------
     def solve(self, Fbar, z0=None, rtol=1e-9, maxnewton=30, cg_tol=1e-6, cg_maxit=20000, g_ref=None):
-        """Newton-CG. Fbar: prescribed components (free ones are the initial guess if z0 is None).
-        Returns (w, Fbar_full, log) with log['z'] the state."""
+        """Equilibrium by Newton's method with CG inner solves (mech.py:302-346). Fbar gives the fixed macro
+        components, and its free components seed the iteration when no z0 is passed. Returns the fluctuation
+        w, the full macro gradient and a log dict (Newton and CG counts, gradient norms, flags; 'z' holds
+        the final unknown vector)."""
         Fb = np.asarray(Fbar, dtype=np.float64)
         z = self.z0(Fb) if z0 is None else torch.as_tensor(z0, dtype=torch.float64, device=self._dev)
         g0, log = g_ref, dict(newton=0, cg=0, converged=False, gnorm=[], negative_curvature=False)

@@ -79,7 +79,7 @@ def hat_kernel(shape, r):
 def filt(x, r):
     """Periodic convolution with the hat kernel through rfftn (nlfiber/density.py:34-36)."""
     kf = np.fft.rfftn(hat_kernel(x.shape, r))
-    return np.fft.irfftn(np.fft.rfftn(x) * kf, s=x.shape)
+    return np.fft.irfftn(np.fft.rfftn(x) * kf, s=x.shape, axes=tuple(range(x.ndim)))
 
 
 def sigmoid(t):

@@ -59,3 +59,38 @@ def test_corrector_vjp_vs_fd(gpu_pkg):
     un = u.detach().cpu().numpy()
     gr = ref.correctors_vjp(s0.cpu().numpy(), Es, un, 2 * un)
     assert np.abs(g.cpu().numpy() - gr).max() < 1e-6 * np.abs(gr).max()
+
+
+def test_eight_strains_correctors_vjp_and_bilinear(gpu_pkg):
+    """More than six macroscopic strains: the solve, the load, the residual, the adjoint solve and C run in
+    blocks of at most six (the library's limit per call); the reference's custom_vjp takes any number
+    (nlfiber/linear.py:118-134). The gradient of sum(u^2) + sum(C) through the correctors and the bilinear
+    form, and C itself, are checked against the oracle."""
+    rho = _cell(gpu_pkg, seed=9)
+    lh = gpu_pkg.LinearHomogenization(rho.shape, tol=1e-12)
+    s0 = lh.stiffness_scale(rho)
+    rng = np.random.default_rng(10)
+    Es = np.concatenate([np.eye(6), rng.standard_normal((2, 6))])
+    s = s0.clone().requires_grad_(True)
+    u = lh.correctors(s, Es, None)
+    assert u.shape == (8,) + rho.shape + (3,)
+    C = lh.bilinear(s, Es, u)
+    f = (u ** 2).sum()
+    (g,) = torch.autograd.grad(f, s, retain_graph=True)
+    (gC,) = torch.autograd.grad(C.sum(), s)
+    ref = O.LinearRef(rho.shape, tol=1e-12)
+    sn = s0.cpu().numpy()
+    ur, _ = ref.solve(sn, Es)
+    un = u.detach().cpu().numpy()
+    assert np.abs(un - ur).max() < 1e-8 * np.abs(ur).max()
+    Cr = ref.bilinear(sn, Es, un)
+    assert np.abs(np.asarray(C.detach()) - Cr).max() < 1e-10 * np.abs(Cr).max()
+    gr = ref.correctors_vjp(sn, Es, un, 2 * un)
+    assert np.abs(g.cpu().numpy() - gr).max() < 1e-6 * np.abs(gr).max()
+    b = ref.element_energy(sn, Es, un)
+    gCr = b.sum(axis=(0, 1)) / b[0, 0].size  # d sum_IJ C_IJ / ds_e with the correctors fixed
+    assert np.abs(gC.cpu().numpy() - gCr).max() < 1e-10 * np.abs(gCr).max()
+    # load and residual in blocks: the residual of the correctors is at the solver tolerance
+    r = lh.residual(s0, Es, u.detach())
+    bnorm = lh.load(s0, Es).flatten(1).norm(dim=1)
+    assert (r.flatten(1).norm(dim=1) / bnorm).max() < 1e-9

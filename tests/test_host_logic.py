@@ -53,3 +53,36 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_grid_limits_raise_value_errors(pkg):
+    """Cells the FFT preconditioner or the operator cannot take are refused with a ValueError before any GPU
+    work (the reference's jnp.fft takes any grid; this library takes sides that are products of 2, 3 and 5,
+    nx, ny <= 512, nz <= 256 and nz split over at most 64 lanes of 1-4 values)."""
+    import pytest
+    from nlfh_amd.linear import check_grid
+    for ok in ((64, 64, 64), (48, 48, 48), (30, 45, 75), (512, 512, 256), (8, 8, 1), (5, 9, 27)):
+        check_grid(ok)
+    for bad, what in (((14, 14, 14), "prime factor"), ((7, 8, 8), "prime factor"), ((1024, 8, 8), "outside"),
+                      ((8, 8, 512), "outside"), ((8, 8, 250), "lanes"), ((8, 8, 0), "outside")):
+        with pytest.raises(ValueError, match=what):
+            check_grid(bad)
+    # the classes check before touching the device (no HIP device here)
+    with pytest.raises(ValueError, match="prime factor"):
+        pkg.LinearHomogenization((7, 7, 7))
+    with pytest.raises(ValueError, match="prime factor"):
+        pkg.BatchedHomogenization([(8, 8, 8), (14, 14, 14)])
+
+
+def test_dtype_other_than_float64_is_refused(pkg):
+    """The reference's dtype argument (nlfiber/linear.py:102) sets its operators' precision; this class computes
+    in float64 and refuses other dtypes instead of ignoring them."""
+    import numpy as np
+    import pytest
+    import torch
+    for bad in (torch.float32, np.float32, "float32", torch.bfloat16):
+        with pytest.raises(ValueError, match="not supported"):
+            pkg.LinearHomogenization((8, 8, 8), dtype=bad)
+    from nlfh_amd.linear import _check_dtype
+    for ok in (torch.float64, np.float64, "float64", np.dtype("float64")):
+        _check_dtype(ok)
