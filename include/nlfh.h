@@ -231,6 +231,19 @@ int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);
 int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
                  double* work, void* stream);
+/* nlfh_fs_eval without the work argument, for callers that cannot allocate device memory (code under HIP graph
+ * capture, a Newton-CG loop that must not call hipMalloc between steps, harnesses that forbid allocation). No
+ * counterpart in the reference. The workspace is a static array of the library, one per GPU, held until the
+ * process ends: 24 N + 10 ceil(N / 32) doubles for cells of N <= nlfh_fs_noalloc_max_voxels() = 32768 voxels
+ * (32^3; 6.1 MiB). Larger cells are refused (NLFH_EUNSUPPORTED). The function itself calls no allocator; its
+ * first call on a device looks up the array's address there. Every call on one GPU shares the array: calls that
+ * overlap in time on one GPU (other streams, other host threads) race; calls ordered on one stream do not.
+ * Every value the kernels read from it is written by the same call first, so nothing needs clearing.
+ * Results are bitwise those of nlfh_fs_eval (same kernels, same order of sums). */
+long long nlfh_fs_noalloc_max_voxels(void);
+int nlfh_fs_eval_noalloc(const int* shape, const double* sg, const double* mat, int flags, const double* w,
+                         const double* Fbar, const double* v, const double* dFbar, int mode, double* grad,
+                         double* scalars, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,6 +27,9 @@ SIGNATURES = [
     ("nlfh_fs_toec", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _vp, _vp, _vp]),
     ("nlfh_fs_eval", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _c_double_p, _vp, _vp,
                                     ctypes.c_int, _vp, _vp, _vp, _vp]),
+    ("nlfh_fs_noalloc_max_voxels", ctypes.c_longlong, []),
+    ("nlfh_fs_eval_noalloc", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _c_double_p, _vp, _vp,
+                                            ctypes.c_int, _vp, _vp, _vp]),
     ("nlfh_last_error", ctypes.c_char_p, []),
     ("nlfh_version", ctypes.c_int, []),
     ("nlfh_plan_create", ctypes.c_int, [_c_int_p, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,

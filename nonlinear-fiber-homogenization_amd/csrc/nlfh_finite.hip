@@ -24,6 +24,9 @@
 // are deterministic.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "nlfh_common.h"
 
 namespace nlfh {
@@ -724,9 +727,48 @@ int nlfh_fs_prepare(const int* shape, const double* rho, const double* mat, int 
   return NLFH_OK;
 }
 
-int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
+}  // extern "C"
+
+namespace nlfh {
+namespace {
+
+// The workspace of the no-allocation form of nlfh_fs_eval: the (24, N) element forces and the block partials
+// of a cell of up to kNoAllocMaxVoxels voxels (32^3), in the library's own device memory. Host code reaches it
+// only through the per-device address from hipGetSymbolAddress (the array's name is a host-side stand-in).
+constexpr int64_t kNoAllocMaxVoxels = 32768;
+constexpr int64_t kNoAllocDoubles =
+    24 * kNoAllocMaxVoxels + ((kNoAllocMaxVoxels + kFsElemPerBlock - 1) / kFsElemPerBlock) * kFsParts;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch array's address on the current device, looked up once per device
+int scratch_on_current_device(double** out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("nlfh_fs_eval_noalloc: device index out of range");
+    return NLFH_EUNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr) {
+      set_error(std::string("nlfh_fs_eval_noalloc: hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+      return NLFH_EHIP;
+    }
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return NLFH_OK;
+}
+
+int fs_eval_impl(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
-                 double* work, void* stream) {
+                 double* work, int64_t max_voxels, void* stream) {
   FsArgs a{};
   const int rc = fs_setup(shape, mat, flags, a);
   if (rc != NLFH_OK) return rc;
@@ -739,6 +781,11 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
       (mode >= 1 && grad == nullptr) || (mode >= 2 && dFbar == nullptr)) {
     set_error("nlfh_fs_eval: a required pointer is null");
     return NLFH_EINVAL;
+  }
+  if (max_voxels > 0 && a.n > max_voxels) {
+    set_error("nlfh_fs_eval_noalloc: the cell has " + std::to_string(a.n) + " voxels, more than the " +
+              std::to_string(max_voxels) + " its static workspace holds (use nlfh_fs_eval with a workspace)");
+    return NLFH_EUNSUPPORTED;
   }
   a.sg = sg;
   a.w = w;
@@ -771,6 +818,29 @@ int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flag
                      scalars);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
+}
+
+}  // namespace
+}  // namespace nlfh
+
+extern "C" {
+
+int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
+                 const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
+                 double* work, void* stream) {
+  return fs_eval_impl(shape, sg, mat, flags, w, Fbar, v, dFbar, mode, grad, scalars, work, 0, stream);
+}
+
+long long nlfh_fs_noalloc_max_voxels(void) { return (long long)kNoAllocMaxVoxels; }
+
+int nlfh_fs_eval_noalloc(const int* shape, const double* sg, const double* mat, int flags, const double* w,
+                         const double* Fbar, const double* v, const double* dFbar, int mode, double* grad,
+                         double* scalars, void* stream) {
+  double* work = nullptr;
+  const int rc = scratch_on_current_device(&work);
+  if (rc != NLFH_OK) return rc;
+  return fs_eval_impl(shape, sg, mat, flags, w, Fbar, v, dFbar, mode, grad, scalars, work, kNoAllocMaxVoxels,
+                      stream);
 }
 
 int nlfh_fs_toec(const int* shape, const double* sg, const double* mat, int flags, const double* chi, double* out,
