@@ -63,6 +63,19 @@ int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* x
 int nlfh_ritz_start(nlfh_plan* plan, const double* s, const double* Es_host, int m, const double* u_in,
                     double* u_out, double* coef_out_host, void* stream);
 
+/* Chebyshev-Jacobi smoothing of a start (no counterpart in the reference: it conditions the learned prediction
+ * that feeds the u0 argument of the solve, nlfiber/linear.py:212-216). For every cell and right-hand side I,
+ * `degree` steps of the Chebyshev semi-iteration for K u = -load(s, E_I) preconditioned by D = diag K(s), for the
+ * spectral interval [lo, hi] of D^-1 K: x_1 = x_0 + r_0 / theta, x_{k+1} = x_k + c1_k (x_k - x_{k-1}) +
+ * c2_k D^-1 r_k with r = -(K x + f), in FP32 (one fused operator launch per step). It damps the error components
+ * of D^-1 K above lo, which the reference-medium preconditioner resolves slowest in the soft phase. hi <= 0 takes
+ * the bound lambda_max(Ke) / Ke_00 of lambda_max(D^-1 K(s)), valid for every s (5.318 at nu = 0.3); hi_used
+ * (host, nullable) receives the value used. u_in and u_out are FP64 (m fields) and may be the same array. */
+/* lambda_max(Ke) / Ke_00 for material (E, nu): the default hi of nlfh_cheb_smooth (-1 for invalid E, nu). */
+double nlfh_jacobi_bound(double E, double nu);
+int nlfh_cheb_smooth(nlfh_plan* plan, const double* s, const double* Es_host, int m, const double* u_in, double* u_out,
+                     int degree, double lo, double hi, double* hi_used_host, void* stream);
+
 /* Loosest relative tolerance of an FP32 inner solve of NLFH_PREC_MIXED (default 1e-2); each inner
  * solve targets max(inner_tol, 0.5 * tol * ||b|| / ||r||), the reduction its refinement step needs.
  * No counterpart in the reference (which has no mixed precision). */

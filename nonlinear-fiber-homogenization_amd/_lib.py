@@ -48,6 +48,9 @@ SIGNATURES = [
     ("nlfh_solve_rhs", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       _c_int_p, _c_int_p, _c_double_p, _vp]),
     ("nlfh_ritz_start", ctypes.c_int, [_vp, _vp, _c_double_p, ctypes.c_int, _vp, _vp, _c_double_p, _vp]),
+    ("nlfh_jacobi_bound", ctypes.c_double, [ctypes.c_double, ctypes.c_double]),
+    ("nlfh_cheb_smooth", ctypes.c_int, [_vp, _vp, _c_double_p, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_double, _c_double_p, _vp]),
     ("nlfh_bilinear", ctypes.c_int, [_vp, _vp, _c_double_p, _vp, ctypes.c_int, _vp, _vp, _vp]),
     ("nlfh_project_mean", ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
     ("nlfh_symbol", ctypes.c_int, [_vp, ctypes.c_int, _c_double_p]),

@@ -27,7 +27,7 @@ class BatchedHomogenization:
     (the unit strains 0..m-1). precision: "fp64" or "mixed"."""
 
     def __init__(self, shapes, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-8, maxit=4000, precision="mixed", m=6,
-                 tune=False, warm_start=None, streams=2):
+                 tune=False, warm_start=None, streams=2, warm_smoothing=(128, 900.0)):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.E, self.nu, self.emin, self.p, self.tol, self.maxit = E, nu, emin, p, tol, maxit
         if precision not in ("fp64", "mixed"):
@@ -47,6 +47,7 @@ class BatchedHomogenization:
             from .learned import WarmStartNet
             warm_start = WarmStartNet.load(warm_start)
         self.warm_start = warm_start
+        self.warm_smoothing = warm_smoothing
         self.streams = max(1, int(streams))
         self._groups = self._partition(min(self.streams, len(self.shapes)))
         self._subs = None  # (plan, stream) per group, made at the first solve
@@ -122,8 +123,9 @@ class BatchedHomogenization:
 
     def warm_start_fields(self, rho_packed, s, max_batch=32):
         """Learned start for every cell: the network's prediction per group of equal cubic shapes
-        (cells it cannot take start from zero), then the per-cell energy-optimal combination
-        (nlfh_ritz_start). Returns a packed field buffer."""
+        (cells it cannot take start from zero), smoothed by `warm_smoothing` = (degree, hi / lo) steps of
+        Chebyshev-Jacobi iteration (nlfh_cheb_smooth, hi the spectral bound of D^-1 K; None: no smoothing),
+        then the per-cell energy-optimal combination (nlfh_ritz_start). Returns a packed field buffer."""
         if self.m != 6:
             raise ValueError("the learned start predicts all six unit-strain correctors (m = 6)")
         P = torch.zeros(self.tot * 3 * self.m, dtype=torch.float64, device=self._dev)
@@ -142,6 +144,11 @@ class BatchedHomogenization:
                     a = int(self.off[i]) * 3 * self.m
                     P[a:a + planar.shape[1]] = planar[j]
         Es_np, Es_p = _es_host(np.eye(6))
+        if self.warm_smoothing:
+            degree, ratio = self.warm_smoothing
+            hi = float(self._lib.nlfh_jacobi_bound(float(self.E), float(self.nu)))
+            _lib.check(self._lib.nlfh_cheb_smooth(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(P), _ptr(P),
+                                                  int(degree), hi / float(ratio), hi, None, _stream()))
         _lib.check(self._lib.nlfh_ritz_start(self._plan.handle, _ptr(s), Es_p, self.m, _ptr(P), _ptr(P), None,
                                              _stream()))
         return P

@@ -431,6 +431,44 @@ hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_f2d(const float* in, double* out, int64_t n, hipStream_t st) {
+  hipLaunchKernelGGL((convert_kernel<float, double>), dim3(nblocks(n, 256)), dim3(256), 0, st, in, out, n);
+  return hipGetLastError();
+}
+
+// dinv[v] = 1 / (kdiag * sum of s over the 8 elements that share node v), per cell (periodic): the inverse of the
+// diagonal of K(s), which is the same for the three components since every diagonal entry of Ke is kdiag
+__global__ void jacobi_dinv_kernel(const CellDesc* __restrict__ cells, int ncells, const double* __restrict__ s,
+                                   double kdiag, float* __restrict__ dinv, int64_t tot) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= tot) return;
+  int lo = 0, hi = ncells - 1;  // the cell holding voxel v: the last with off_vox <= v
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (cells[mid].off_vox <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  const CellDesc cd = cells[lo];
+  const int64_t l = v - cd.off_vox;
+  const int k = (int)(l % cd.nz), j = (int)((l / cd.nz) % cd.ny), i = (int)(l / ((int64_t)cd.nz * cd.ny));
+  const double* sc = s + cd.off_vox;
+  double t = 0.0;
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const int ii = (a & 1) ? (i == 0 ? cd.nx - 1 : i - 1) : i;
+    const int jj = (a & 2) ? (j == 0 ? cd.ny - 1 : j - 1) : j;
+    const int kk = (a & 4) ? (k == 0 ? cd.nz - 1 : k - 1) : k;
+    t += sc[((int64_t)ii * cd.ny + jj) * cd.nz + kk];
+  }
+  dinv[v] = (float)(1.0 / (kdiag * t));
+}
+
+hipError_t launch_jacobi_dinv(const CellDesc* cells, int ncells, const double* s, double kdiag, float* dinv,
+                              int64_t tot, hipStream_t st) {
+  hipLaunchKernelGGL(jacobi_dinv_kernel, dim3(nblocks(tot, 256)), dim3(256), 0, st, cells, ncells, s, kdiag, dinv, tot);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy_d(const double* in, double* out, int64_t n, hipStream_t st) {
   hipLaunchKernelGGL((convert_kernel<double, double>), dim3(nblocks(n, 256)), dim3(256), 0, st, in, out, n);
   return hipGetLastError();

@@ -203,6 +203,7 @@ struct nlfh_plan {
   double* d_part_b = nullptr;
   double* d_part_e = nullptr;
   double* d_ritz = nullptr;    // Ritz start: G, H (ncells*36 each) and coefficients (ncells*36)
+  float* d_dinv32 = nullptr;   // Chebyshev smoothing: 1 / diag K(s) per voxel
   double* d_strain = nullptr;
   double* d_params = nullptr;  // per cell: [0, ncells) FP64 tolerance, [ncells, 2 ncells) inner FP32 tolerance
   int* d_iparams = nullptr;    // maxit64, maxit32
@@ -250,12 +251,11 @@ namespace {
 // Real symmetric symbol of K(1) from the 27-point stencil, inverted with the reference's
 // regularisation (nlfiber/linear.py:154-165): Kh(k) = sum_o S_o exp(-2 pi i k.o / n); Kh(0) = I;
 // Kinv = inv(Kh + 1e-12 |tr Kh| I); Kinv(0) = 0. Stored as (k00, k11, k22, k12, k02, k01).
-void build_symbol(const Shape& sh, double E, double nu, std::vector<double>& out) {
-  // element matrix Ke via the Walsh-mode operator: Ke = W^T L W (unit cube, 2x2x2 Gauss)
-  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
-  // dense Ke from B matrices at the Gauss points (the reference's construction, nlfiber/linear.py:57-71,108-112)
+// dense Ke from B matrices at the Gauss points (the reference's construction, nlfiber/linear.py:57-71,108-112)
+void element_matrix(double lam, double mu, double (&Ke)[24][24]) {
   double gp[2] = {0.5 - 1.0 / (2.0 * std::sqrt(3.0)), 0.5 + 1.0 / (2.0 * std::sqrt(3.0))};
-  double Ke[24][24] = {};
+  for (int i = 0; i < 24; ++i)
+    for (int j = 0; j < 24; ++j) Ke[i][j] = 0.0;
   double D[6][6] = {};
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) D[i][j] = lam;
@@ -289,6 +289,38 @@ void build_symbol(const Shape& sh, double E, double nu, std::vector<double>& out
         Ke[i][j] += t / 8.0;
       }
   }
+}
+
+// An upper bound of the spectrum of D^-1 K(s), D = diag K(s), for every stiffness field s > 0: with
+// K(s) = sum_e s_e P_e^T Ke P_e and every diagonal entry of Ke equal to Ke_00, u^T D u = sum_e s_e Ke_00 |P_e u|^2,
+// so the Rayleigh quotient u^T K u / u^T D u is an s-weighted mean of element quotients, each at most
+// lambda_max(Ke) / Ke_00 (5.318 at nu = 0.3). lambda_max by power iteration on the 24 x 24 PSD matrix.
+double jacobi_bound(double lam, double mu) {
+  double Ke[24][24];
+  element_matrix(lam, mu, Ke);
+  double v[24], w[24], lmax = 0.0;
+  for (int i = 0; i < 24; ++i) v[i] = 1.0 + 0.01 * i * (i % 3 + 1) * ((i & 4) ? -1.0 : 1.0);
+  for (int it = 0; it < 2000; ++it) {
+    double nw = 0.0;
+    for (int i = 0; i < 24; ++i) {
+      double t = 0.0;
+      for (int j = 0; j < 24; ++j) t += Ke[i][j] * v[j];
+      w[i] = t;
+      nw += t * t;
+    }
+    nw = std::sqrt(nw);
+    double nv = 0.0;
+    for (int i = 0; i < 24; ++i) nv += v[i] * v[i];
+    lmax = nw / std::sqrt(nv);
+    for (int i = 0; i < 24; ++i) v[i] = w[i] / nw;
+  }
+  return lmax / Ke[0][0];
+}
+
+void build_symbol(const Shape& sh, double E, double nu, std::vector<double>& out) {
+  const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+  double Ke[24][24];
+  element_matrix(lam, mu, Ke);
   // stencil: S[o][a][c] = response at node offset o (in {-1,0,1}^3) of component a to a unit
   // displacement of component c at the origin = sum over elements containing both nodes
   double S[27][3][3] = {};
@@ -529,7 +561,38 @@ int run_matvec(nlfh_plan* p, const T* u, const T* p_old, T* p_new, T* out, const
     a.lam16 = (T)(p->lam / 16.0);
     a.mu16 = (T)(p->mu / 16.0);
     a.scale_out = scale;
-    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, st));
+    HIP_TRY(launch_matvec<T>(a, g.count, sh.zpt, g.max_ty, strain, pupdate, dot, 0, st));
+  }
+  return NLFH_OK;
+}
+
+// One Chebyshev-Jacobi step for K x = -f on every cell: out = x + c1 (x - x_prev) - c2 D^-1 (K x + f).
+int run_cheb_step(nlfh_plan* p, const float* x, const float* x_prev, float* out, const float* s32, const float* dinv,
+                  float c1, float c2, int m, hipStream_t st) {
+  for (const MvGroup& g : p->groups) {
+    const Shape& sh = p->shapes[g.shape];
+    MatvecArgs<float> a{};
+    a.cells = p->d_cells;
+    a.items = p->d_mv + g.begin;
+    a.u = x;
+    a.out = out;
+    a.s = s32;
+    a.strain = p->d_strain;
+    a.m = m;
+    a.nitems = (int)p->mv.size();
+    a.item_base = g.begin;
+    a.LZ = sh.nz / sh.zpt;
+    int lzs = 1;
+    while (lzs < a.LZ) lzs <<= 1;
+    a.LZS = lzs;
+    a.lam16 = (float)(p->lam / 16.0);
+    a.mu16 = (float)(p->mu / 16.0);
+    a.scale_out = 1.0f;
+    a.x_prev = x_prev;
+    a.dinv = dinv;
+    a.c1 = c1;
+    a.c2 = c2;
+    HIP_TRY(launch_matvec<float>(a, g.count, sh.zpt, g.max_ty, 1, 0, 0, 1, st));
   }
   return NLFH_OK;
 }
@@ -1501,6 +1564,56 @@ int nlfh_ritz_start(nlfh_plan* p, const double* s, const double* Es, int m, cons
   HIP_TRY(launch_combine(va, w.X, A, u_out, st));
   HIP_TRY(hipStreamSynchronize(st));  // coef (host) must outlive the copy
   if (coef_out) std::memcpy(coef_out, coef.data(), coef.size() * sizeof(double));
+  return leave(p, stream);
+}
+
+double nlfh_jacobi_bound(double E, double nu) {
+  if (!(E > 0.0) || !(nu > -1.0 && nu < 0.5)) return -1.0;
+  return jacobi_bound(E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu)));
+}
+
+int nlfh_cheb_smooth(nlfh_plan* p, const double* s, const double* Es, int m, const double* u_in, double* u_out,
+                     int degree, double lo, double hi, double* hi_used, void* stream) {
+  CHECK_ARG(p && s && Es && u_in && u_out, "null argument");
+  TRY(check_m(p, m));
+  CHECK_ARG(degree >= 0 && degree <= 4096, "degree must be in 0..4096");
+  const double bound = jacobi_bound(p->lam, p->mu);
+  if (!(hi > 0.0)) hi = bound;
+  CHECK_ARG(lo > 0.0 && lo < hi && std::isfinite(hi), "the interval must satisfy 0 < lo < hi");
+  if (hi_used) *hi_used = hi;
+  TRY(enter(p, stream));
+  hipStream_t st = p->stream;
+  const int64_t F = p->field_len_m(m);
+  if (degree == 0) {
+    if (u_out != u_in) HIP_TRY(launch_copy_d(u_in, u_out, F, st));
+    return leave(p, stream);
+  }
+  TRY(get_ready32(p));
+  if (!p->d_dinv32) TRY(p->alloc(&p->d_dinv32, p->tot_vox));
+  Work<float>& w = p->w32;
+  TRY(set_strain(p, Es, m, st));
+  HIP_TRY(launch_d2f(s, w.s, p->tot_vox, st));
+  HIP_TRY(launch_jacobi_dinv(p->d_cells, p->ncells, s, (p->lam + 4.0 * p->mu) / 9.0, p->d_dinv32, p->tot_vox, st));
+  // three FP32 iterates in rotation: x_{k-1}, x_k, x_{k+1}
+  float* buf[3] = {w.P[0], w.P[1], w.Ap};
+  HIP_TRY(launch_d2f(u_in, buf[0], F, st));
+  const double th = 0.5 * (hi + lo), de = 0.5 * (hi - lo), sig = th / de;
+  double rho0 = 1.0 / sig;
+  int prev = 0, cur = 0;
+  for (int k = 0; k < degree; ++k) {
+    double c1 = 0.0, c2 = 1.0 / th;
+    if (k > 0) {
+      const double rho1 = 1.0 / (2.0 * sig - rho0);
+      c1 = rho1 * rho0;
+      c2 = 2.0 * rho1 / de;
+      rho0 = rho1;
+    }
+    const int nxt = (cur + 1) % 3 == prev ? (cur + 2) % 3 : (cur + 1) % 3;
+    TRY(run_cheb_step(p, buf[cur], buf[prev], buf[nxt], w.s, p->d_dinv32, (float)c1, (float)c2, m, st));
+    prev = cur;
+    cur = nxt;
+  }
+  HIP_TRY(launch_f2d(buf[cur], u_out, F, st));
   return leave(p, stream);
 }
 

@@ -27,6 +27,10 @@ struct MatvecArgs {
   int LZS;  // lanes reserved per row: power of two >= LZ (lanes LZ..LZS-1 idle)
   T lam16, mu16;  // lambda / 16, mu / 16
   T scale_out;    // output multiplier (-1 gives b = -load)
+  // CHEB (with STRAIN): one Chebyshev-Jacobi step, out = u + c1 (u - x_prev) - c2 dinv (K u + f)
+  const T* x_prev;  // previous iterate (same layout as u)
+  const T* dinv;    // 1 / diag K(s) per voxel (the diagonal is the same for the 3 components)
+  T c1, c2;
 };
 
 struct FftLen {
@@ -126,7 +130,7 @@ struct EnergyArgs {
 
 template <typename T>
 hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
-                         hipStream_t st);
+                         int cheb, hipStream_t st);
 template <typename T>
 size_t prec_plane_lds(int ny, int nz, int rows_per_batch);
 template <typename T>
@@ -142,6 +146,9 @@ hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, do
                                   double p, hipStream_t st);
 hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);
 hipError_t launch_copy_d(const double* in, double* out, int64_t n, hipStream_t st);
+hipError_t launch_f2d(const float* in, double* out, int64_t n, hipStream_t st);
+hipError_t launch_jacobi_dinv(const CellDesc* cells, int ncells, const double* s, double kdiag, float* dinv,
+                              int64_t tot, hipStream_t st);
 template <typename T>
 hipError_t launch_fill(T* out, int64_t n, T v, hipStream_t st);
 template <typename T>

@@ -150,8 +150,11 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
 // wave-uniform too.
 // S64: the stiffness is read in FP64 (a.s64) and rounded to T as it is loaded (the standalone FP32
 // operator, whose caller holds s in FP64; the solver converts s once per solve and reads a.s).
-template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP, bool S64 = false>
+// CHEB (STRAIN only): the owned node values become one step of the Chebyshev-Jacobi iteration for K u = -f,
+// u + c1 (u - x_prev) - c2 dinv (K u + f); x_prev and dinv of the plane stored at step s are loaded at step s - 1.
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP, bool S64 = false, bool CHEB = false>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
+  static_assert(!CHEB || (STRAIN && !PUPDATE && !DOT && !S64), "the Chebyshev step is a strain-term launch");
   extern __shared__ double smem_d[];
   T* const lds = reinterpret_cast<T*>(smem_d);
   __shared__ double red[8];
@@ -194,6 +197,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     PNc[c] = PUPDATE ? a.p_new + fbase + c * nvox : nullptr;
     Oc[c] = a.out + fbase + c * nvox;
   }
+  const T* Xp[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) Xp[c] = CHEB ? a.x_prev + fbase + c * nvox : nullptr;
+  const T* __restrict__ DI = CHEB ? a.dinv + cd.off_vox : nullptr;
+  const T c1 = CHEB ? a.c1 : T(0), c2 = CHEB ? a.c2 : T(0);
   const T* __restrict__ S = S64 ? nullptr : a.s + cd.off_vox;
   const double* __restrict__ S8 = S64 ? a.s64 + cd.off_vox : nullptr;
   auto load_s = [&](unsigned i) -> T {
@@ -261,6 +269,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   // in flight): the next x-plane (u and p) and the stiffness of the next element layer
   T nxt_u[2][3][ZPT], nxt_p[2][3][ZPT], sv[2][ZPT];
   T pc[2][3][ZPT];  // node values of the planes awaiting output (by layer parity)
+  T xq[2][3][ZPT], dq[2][ZPT];  // CHEB: previous iterate and 1 / diagonal of the next plane to store
   T Flo[3][2][ZPT];  // own half of the y-reverse of the last element layer
   double dot = 0.0;
 #pragma unroll
@@ -272,6 +281,8 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       Flo[c][1][q] = T(0);
       pc[0][c][q] = T(0);
       pc[1][c][q] = T(0);
+      xq[0][c][q] = xq[1][c][q] = T(0);
+      if (c == 0) dq[0][q] = dq[1][q] = T(0);
     }
 
   // x stage of layer L_t (parity PT = t & 1) from planes `up` (L_t) and `un` (L_t + 1)
@@ -324,6 +335,16 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       const unsigned so = plane_off(x0 + (s < xc - 1 ? s : xc - 1)) + roff_ld;
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) sv[PS][q] = load_s(so + q);
+      if constexpr (CHEB) {
+        // node plane x0 - 1 + s, stored by part (a) of step s + 1 (the last steps repeat plane x0 + xc - 1)
+        const unsigned xo = plane_off(x0 - 1 + (s < xc + 1 ? s : xc)) + roff_ld;
+#pragma unroll
+        for (int q = 0; q < ZPT; ++q) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xq[PS][c][q] = at(Xp[c], xo + q);
+          dq[PS][q] = at(DI, xo + q);
+        }
+      }
     }
     // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
     if (s >= 1 && owns) {
@@ -340,6 +361,10 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           if (s >= 2) {
             T v = carry[c][q] + lo;
             if (STRAIN) v *= scale_out;  // only the load launches scale (by -1); the others pass 1
+            if constexpr (CHEB) {
+              const T xk = pc[PQ][c][q];
+              v = xk + c1 * (xk - xq[PQ][c][q]) - c2 * dq[PQ][q] * v;
+            }
             at(Oc[c], oo + q) = v;
             if (DOT) {
               if constexpr (sizeof(T) == 8) dot += pc[PQ][c][q] * v;
@@ -353,6 +378,11 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     if (s > xc) return true;
     // (b) element layer L_s
     settle(sv[PQ]);
+    if constexpr (CHEB) {  // consumed by part (a) above
+#pragma unroll
+      for (int c = 0; c < 3; ++c) settle(xq[PQ][c]);
+      settle(dq[PQ]);
+    }
     if (active && erow) {
       const T* SD = lds + PS * buf_sz + lbase;
       T* UY = lds + (2 + PS) * buf_sz + lbase;
@@ -464,8 +494,16 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 // Host-side launcher -------------------------------------------------------------------------
 
 template <typename T, int ZPT, bool DPP>
-static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, dim3 grid, int threads,
-                                size_t lds, hipStream_t st) {
+static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, int cheb, dim3 grid,
+                                int threads, size_t lds, hipStream_t st) {
+  if (cheb) {
+    if (!strain || pupdate || dot || a.s64 != nullptr) return hipErrorInvalidValue;
+    if (lds > 65536)
+      (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, true, false, false, DPP, false, true>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((matvec_kernel<T, ZPT, true, false, false, DPP, false, true>), grid, dim3(threads), lds, st, a);
+    return hipGetLastError();
+  }
 #define NLFH_MV(S_, P_, D_)                                                                           \
   if (lds > 65536)                                                                                  \
     (void)hipFuncSetAttribute((const void*)matvec_kernel<T, ZPT, S_, P_, D_, DPP>,                     \
@@ -499,25 +537,25 @@ static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate,
 
 template <typename T>
 hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, int strain, int pupdate, int dot,
-                         hipStream_t st) {
+                         int cheb, hipStream_t st) {
   // max_ty + 2 rows of LZS lanes per block
   const int threads = ((a.LZS * (max_ty + 2) + 63) / 64) * 64;
   const size_t lds = (size_t)4 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
   dim3 grid(nblk, a.m, 1);
   const bool dpp = a.LZ == 64;  // a row is a whole wavefront
   switch (zpt * 2 + (dpp ? 1 : 0)) {
-    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, grid, threads, lds, st);
-    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, grid, threads, lds, st);
+    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
     default: return hipErrorInvalidValue;
   }
 }
 
-template hipError_t launch_matvec<double>(const MatvecArgs<double>&, int, int, int, int, int, int, hipStream_t);
-template hipError_t launch_matvec<float>(const MatvecArgs<float>&, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_matvec<double>(const MatvecArgs<double>&, int, int, int, int, int, int, int, hipStream_t);
+template hipError_t launch_matvec<float>(const MatvecArgs<float>&, int, int, int, int, int, int, int, hipStream_t);
 
 }  // namespace nlfh

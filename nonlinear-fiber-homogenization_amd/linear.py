@@ -293,7 +293,7 @@ class LinearHomogenization:
     not converge."""
 
     def __init__(self, shape, E=1.0, nu=0.3, emin=1e-3, p=3.0, dtype=torch.float64, tol=1e-9, maxit=4000,
-                 precision="fp64", warm_start=None):
+                 precision="fp64", warm_start=None, warm_smoothing=(128, 900.0)):
         shape = tuple(int(n) for n in shape)
         self.plane = len(shape) == 2
         self.shape = shape + (1,) if self.plane else shape
@@ -318,6 +318,7 @@ class LinearHomogenization:
             from .learned import WarmStartNet
             warm_start = WarmStartNet.load(warm_start)
         self.warm_start = warm_start
+        self.warm_smoothing = warm_smoothing
 
     # --- internals ------------------------------------------------------------------------
     @property
@@ -498,6 +499,20 @@ class LinearHomogenization:
         up = to_planar(u.detach() if isinstance(u, torch.Tensor) else u, self.shape)
         return _BilinearFn.apply(s.contiguous(), self, Es, up)
 
+    def cheb_smooth(self, s, Es, u, degree=128, ratio=900.0, hi=None):
+        """`degree` Chebyshev-Jacobi steps for K u = -load(s, Es) from u (m, nx, ny, nz, 3) on the spectral interval
+        [hi / ratio, hi] of D^-1 K (nlfh_cheb_smooth; hi defaults to the bound lambda_max(Ke) / Ke_00)."""
+        Es_np, Es_p = _es_host(Es)
+        m = Es_np.shape[0]
+        up = to_planar(u, self.shape)
+        out = torch.empty_like(up)
+        if hi is None:
+            hi = float(self._lib.nlfh_jacobi_bound(float(self.E), float(self.nu)))
+        _lib.check(self._lib.nlfh_cheb_smooth(self._plan.handle, _ptr(self._s(s).detach()), Es_p, m, _ptr(up),
+                                              _ptr(out), int(degree), float(hi) / float(ratio), float(hi), None,
+                                              _stream()))
+        return from_planar(out)
+
     def ritz_start(self, s, Es, P):
         """Energy-optimal combination of the predicted fields P (m, nx, ny, nz, 3) for each right-hand
         side (nlfh_ritz_start); returns the start in the reference layout and the (m, m) weights."""
@@ -518,6 +533,8 @@ class LinearHomogenization:
         if u0 is None and self.warm_start is not None and not self.plane:
             r = rho.detach() if isinstance(rho, torch.Tensor) else torch.as_tensor(np.asarray(rho))
             P = self.warm_start.predict(r.to(self._dev, torch.float64))[0]
+            if self.warm_smoothing:
+                P = self.cheb_smooth(s.detach(), Es, P, *self.warm_smoothing)
             u_w, _ = self.ritz_start(s.detach(), Es, P)
             u, it = self.solve(s.detach(), Es, u_w)
             info = dict(self.last_info, warm_start=True, fallback=False)

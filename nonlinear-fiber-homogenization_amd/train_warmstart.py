@@ -8,9 +8,10 @@ Cells are the seeded random cubic-symmetric cells of `cells.random_cell` (traini
 `--seed0`, held-out seeds from `--heldout-seed0`; the ranges must not overlap). Their densities
 are generated by a pool of
 CPU workers while the GPU solves; training targets are the six unit-strain correctors of the batched
-mixed-precision solver at `--data-tol`. Evaluation solves every held-out cell at `--tol` (or at each of `--tols`) from zero, from the
-network's prediction combined by energy-optimal Ritz weights (inference time included in the warm timing)
-and from the raw prediction, and reports iterations, wall-clock solves/s and the largest C difference; for
+mixed-precision solver at `--data-tol`. Evaluation solves every held-out cell at `--tol` (or at each of
+`--tols`) from zero, from the learned start (the network's prediction, Chebyshev-Jacobi smoothing and
+energy-optimal Ritz weights; inference and smoothing time included in the warm timing) and from the unsmoothed
+Ritz start, and reports iterations, wall-clock solves/s and the largest C difference; for
 the first `--start-diagnostics` cells it also reports how good a start the prediction is (relative
 residual, which the stopping rule measures, and relative energy-norm error, per right-hand side).
 
@@ -68,23 +69,26 @@ def to_target(u_packed, B, n):
 
 
 def evaluate(net, held, n, bs, emin, tol):
-    """Zero start against the learned start (prediction + Ritz weights) and the raw prediction as u0, on
-    the held-out cells in batches of `bs`, mixed precision to relative tolerance `tol`."""
+    """Zero start against the learned start (prediction + Chebyshev-Jacobi smoothing + Ritz weights, the
+    default of the warm_start option) and, for comparison, the unsmoothed Ritz start, on the held-out cells
+    in batches of `bs`, mixed precision to relative tolerance `tol`. The learned solve's time includes the
+    network inference, the smoothing and the Ritz step."""
     import torch
     from nlfh_amd import BatchedHomogenization
 
     def plans(B):
         kw = dict(emin=emin, tol=tol, precision="mixed")
         return (BatchedHomogenization([(n, n, n)] * B, **kw),
-                BatchedHomogenization([(n, n, n)] * B, warm_start=net, **kw))
+                BatchedHomogenization([(n, n, n)] * B, warm_start=net, **kw),
+                BatchedHomogenization([(n, n, n)] * B, warm_start=net, warm_smoothing=None, **kw))
 
     cold_it, warm_it, raw_it, cold_t, warm_t, dC, fallbacks = [], [], [], 0.0, 0.0, [], 0
-    bh, bw = plans(bs)
+    bh, bw, bu = plans(bs)
     for k in range(0, len(held), bs):
         buf = [rho for _, rho in held[k:k + bs]]
         B = len(buf)
         if B != bs:
-            bh, bw = plans(B)
+            bh, bw, bu = plans(B)
         rho_d = bh.pack(buf)
         torch.cuda.synchronize()
         t = time.time()
@@ -92,21 +96,21 @@ def evaluate(net, held, n, bs, emin, tol):
         torch.cuda.synchronize()
         cold_t += time.time() - t
         t = time.time()
-        C1, _, it1 = bw.C_full(rho_d)  # prediction + Ritz weights + solve
+        C1, _, it1 = bw.C_full(rho_d)  # prediction + smoothing + Ritz weights + solve
         torch.cuda.synchronize()
         warm_t += time.time() - t
         fallbacks += int(bw.last_info["fallback"]) * B
-        # the unprojected prediction as u0, for comparison
-        _, _, it2 = bh.C_full(rho_d, u0=pack_prediction(net.predict(rho_d.view(B, n, n, n))))
+        # the unsmoothed Ritz start, for comparison
+        _, _, it2 = bu.C_full(rho_d)
         cold_it.extend(it0.tolist())
         warm_it.extend(it1.tolist())
         raw_it.extend(it2.tolist())
         c0, c1 = np.asarray(C0), np.asarray(C1)
         dC.append(float(np.max(np.abs(c1 - c0) / np.abs(c0).max(axis=(1, 2), keepdims=True))))
-        _log(f"tol {tol:g} eval {k + B}/{len(held)}: mean it cold {np.mean(cold_it):.1f} learned+ritz "
-             f"{np.mean(warm_it):.1f} learned raw {np.mean(raw_it):.1f}")
+        _log(f"tol {tol:g} eval {k + B}/{len(held)}: mean it cold {np.mean(cold_it):.1f} learned "
+             f"{np.mean(warm_it):.1f} learned unsmoothed {np.mean(raw_it):.1f}")
     return dict(cold_mean_it=float(np.mean(cold_it)), warm_mean_it=float(np.mean(warm_it)),
-                raw_mean_it=float(np.mean(raw_it)), cold_solves_per_s=len(held) / cold_t,
+                unsmoothed_mean_it=float(np.mean(raw_it)), cold_solves_per_s=len(held) / cold_t,
                 warm_solves_per_s=len(held) / warm_t, it_ratio=float(np.mean(warm_it) / np.mean(cold_it)),
                 max_rel_dC=float(max(dC)), fallback_cells=fallbacks, cold_seconds=cold_t, warm_seconds=warm_t,
                 warm_better_frac=float(np.mean(np.asarray(warm_it) < np.asarray(cold_it))))

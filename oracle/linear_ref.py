@@ -318,6 +318,45 @@ class LinearRef:
         return g
 
 
+# --- Chebyshev-Jacobi smoothing of a start (no reference counterpart; feeds u0 of nlfiber/linear.py:212-216) ---
+def jacobi_diag(s, E=1.0, nu=0.3):
+    """diag K(s) per node (nx, ny, nz): Ke_00 times the sum of s over the 8 elements sharing the node (every
+    diagonal entry of Ke equals Ke_00 = (lambda + 4 mu) / 9)."""
+    lam, mu = lame(E, nu)
+    t = sum(np.roll(s, (a, b, c), (0, 1, 2)) for a in (0, 1) for b in (0, 1) for c in (0, 1))
+    return (lam + 4.0 * mu) / 9.0 * t
+
+
+def jacobi_bound(E=1.0, nu=0.3):
+    """lambda_max(Ke) / Ke_00: an upper bound of the spectrum of D^-1 K(s) for every s > 0."""
+    Ke = element_matrices(E, nu)[0]
+    return float(np.linalg.eigvalsh(Ke).max() / Ke[0, 0])
+
+
+def cheb_smooth(ref, s, Es, u0, degree, lo, hi):
+    """`degree` steps of the Chebyshev semi-iteration for K u = -load(s, E_I) with the Jacobi preconditioner and
+    the spectral interval [lo, hi] of D^-1 K (three-term form): x_1 = x_0 + D^-1 r_0 / theta and
+    x_{k+1} = x_k + rho_k rho_{k-1} (x_k - x_{k-1}) + (2 rho_k / delta) D^-1 r_k, r_k = -(K x_k + f),
+    rho_0 = delta / theta, rho_k = 1 / (2 theta / delta - rho_{k-1})."""
+    Es = np.asarray(Es, np.float64)
+    f = ref.load(s, Es)
+    dinv = 1.0 / jacobi_diag(s, ref.E, ref.nu)[None, ..., None]
+    th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
+    sig = th / de
+    x_prev = x = np.asarray(u0, np.float64).copy()
+    rho0 = 1.0 / sig
+    for k in range(degree):
+        r = -(ref.apply_K(s, x) + f)
+        if k == 0:
+            x_new = x + dinv * r / th
+        else:
+            rho1 = 1.0 / (2.0 * sig - rho0)
+            x_new = x + rho1 * rho0 * (x - x_prev) + (2.0 * rho1 / de) * dinv * r
+            rho0 = rho1
+        x_prev, x = x, x_new
+    return x
+
+
 def cubic_invariants(C):
     """nlfiber/linear.py:267-271."""
     C = np.asarray(C)

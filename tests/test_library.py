@@ -36,3 +36,13 @@ def test_error_path_without_device(pkg):
     assert lib.nlfh_version() == 1
     assert lib.nlfh_plan_bytes(None) == 0
     del shapes
+
+
+def test_jacobi_bound_is_the_element_bound(pkg):
+    """nlfh_jacobi_bound (host arithmetic only) is lambda_max(Ke) / Ke_00 of the oracle's Ke for several Poisson
+    ratios, and rejects invalid materials."""
+    from oracle import linear_ref as O
+    lib = pkg.load_library()
+    for nu in (0.0, 0.2, 0.3, 0.45):
+        assert abs(lib.nlfh_jacobi_bound(2.0, nu) - O.jacobi_bound(2.0, nu)) < 1e-11 * O.jacobi_bound(2.0, nu)
+    assert lib.nlfh_jacobi_bound(1.0, 0.5) == -1.0 and lib.nlfh_jacobi_bound(-1.0, 0.3) == -1.0

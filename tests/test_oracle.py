@@ -236,3 +236,64 @@ def test_density_exact_volume_and_vjp():
     d2 = rng.standard_normal((4, 4, 4))
     fd2 = (np.sum(DR.rho(phi + h * d2, beta, 1.5, 0.27) ** 3) - np.sum(DR.rho(phi - h * d2, beta, 1.5, 0.27) ** 3)) / (2 * h)
     assert abs(np.vdot(g, d2) - fd2) < 1e-7 * abs(fd2) + 1e-10
+
+
+def test_jacobi_diagonal_and_spectral_bound():
+    """jacobi_diag is diag K(s) of the dense assembly (every component), and jacobi_bound = lambda_max(Ke) / Ke_00
+    dominates the spectrum of D^-1 K(s) for random, high-contrast and single-stiff-voxel fields (the last comes
+    close to it: the Rayleigh quotient of an element's top mode in a stiff voxel surrounded by soft ones)."""
+    rng = np.random.default_rng(3)
+    Ke = O.Ke_tensor_product()
+    bound = O.jacobi_bound()
+    assert abs(bound - np.linalg.eigvalsh(Ke).max() / Ke[0, 0]) < 1e-12
+    tops = []
+    for shape, kind in (((3, 4, 3), "uniform01"), ((4, 4, 2), "contrast"), ((4, 4, 4), "voxel")):
+        if kind == "uniform01":
+            s = rng.uniform(0.1, 1.0, shape)
+        elif kind == "contrast":
+            s = np.where(rng.uniform(size=shape) < 0.5, 1.0, 1e-4)
+        else:
+            s = np.full(shape, 1e-6)
+            s[1, 2, 1] = 1.0
+        K = O.dense_K(shape, s, Ke)
+        d = O.jacobi_diag(s).reshape(-1)
+        np.testing.assert_allclose(np.diag(K).reshape(-1, 3), np.repeat(d[:, None], 3, axis=1), rtol=1e-13)
+        dh = 1.0 / np.sqrt(np.repeat(d, 3))
+        lam = np.linalg.eigvalsh(dh[:, None] * K * dh[None, :])
+        assert lam.min() > -1e-12 and lam.max() <= bound * (1 + 1e-12)
+        tops.append(lam.max())
+    assert tops[2] > 0.9 * bound  # nearly attained by an isolated stiff voxel
+
+
+def test_cheb_smooth_is_the_chebyshev_residual_polynomial():
+    """The three-term recurrence of cheb_smooth against the closed form: the error after k steps is
+    P_k(D^-1 K) e_0 with P_k(t) = T_k((theta - t) / delta) / T_k(theta / delta), evaluated through the
+    eigendecomposition of D^-1/2 K D^-1/2 of the dense assembly (e = x - u* for any solution u* of K u = -f)."""
+    rng = np.random.default_rng(4)
+    shape = (3, 3, 2)
+    s = rng.uniform(0.05, 1.0, shape)
+    ref = O.LinearRef(shape)
+    Es = np.stack([O.unit_strain(0), O.unit_strain(4)])
+    u0 = rng.standard_normal((2,) + shape + (3,))
+    K = O.dense_K(shape, s, O.Ke_tensor_product())
+    d = np.repeat(O.jacobi_diag(s).reshape(-1), 3)
+    w, Q = np.linalg.eigh(K / np.sqrt(d)[:, None] / np.sqrt(d)[None, :])
+    f = ref.load(s, Es).reshape(2, -1)
+    ustar = np.stack([np.linalg.lstsq(K, -fi, rcond=None)[0] for fi in f])
+    hi = O.jacobi_bound()
+    for degree, lo in ((1, hi / 10), (5, hi / 30), (17, hi / 100)):
+        x = O.cheb_smooth(ref, s, Es, u0, degree, lo, hi).reshape(2, -1)
+        th, de = 0.5 * (hi + lo), 0.5 * (hi - lo)
+        T = np.polynomial.chebyshev.Chebyshev.basis(degree)
+        P = T((th - w) / de) / T(th / de)
+        for I in range(2):
+            e0 = u0[I].reshape(-1) - ustar[I]
+            ek = (Q @ (P * (Q.T @ (np.sqrt(d) * e0)))) / np.sqrt(d)
+            np.testing.assert_allclose(x[I] - ustar[I], ek, rtol=0, atol=1e-10 * np.abs(e0).max())
+    # on [lo, hi] the polynomial is at most 1 / T_k(theta / delta): the energy error of high modes falls
+    x = O.cheb_smooth(ref, s, Es, u0, 17, hi / 100, hi).reshape(2, -1)
+    e0, ek = u0[0].reshape(-1) - ustar[0], x[0] - ustar[0]
+    hi_modes = w > hi / 100
+    c0 = (Q.T @ (np.sqrt(d) * e0))[hi_modes]
+    ck = (Q.T @ (np.sqrt(d) * ek))[hi_modes]
+    assert np.linalg.norm(ck) < 0.5 * np.linalg.norm(c0)
