@@ -9,9 +9,10 @@ Workloads (`--workload`):
   stress128  16 cells at 128^3 with emin = 1e-6, volume 0.15, beta = 16.
 For every cell: the 6 unit-strain correctors by PCG to a true FP64 relative residual of 1e-8 and the
 6x6 C. Inputs (rho) are resident in HBM before the timed region; the timed region is rho -> C on the
-device (plan, symbol and tiling selection excluded, as the reference's __init__ is). With
-`--learned WEIGHTS` the network's prediction (inference included in the timed region) starts the
-solve.
+device (plan, symbol and tiling selection excluded, as the reference's __init__ is). The learned start
+(the default of the cell64 workload: the packaged network's prediction, smoothed by Chebyshev-Jacobi steps
+and combined by Ritz weights; inference and smoothing included in the timed region) starts the solve;
+`--no-learned` starts from zero, `--learned WEIGHTS` picks other weights.
 
 Prints one JSON line. `roofline` is for the matvec kernel of the timed solver precision over the
 step's cells, timed live with HIP events on the stream it runs on (algorithmic bytes
@@ -52,7 +53,11 @@ def parse():
     ap.add_argument("--finite-strain", type=float, default=0.05, help="uniaxial stretch - 1 of the finite workload")
     ap.add_argument("--cells", type=int, default=8, help="64^3 cells per step (cell64)")
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
-    ap.add_argument("--learned", default=None, help="warm-start weights (npz)")
+    ap.add_argument("--learned", default=None,
+                    help="warm-start weights (npz); the cell64 workload uses the packaged weights unless --no-learned")
+    ap.add_argument("--no-learned", action="store_true", help="zero start (the cell64 default is the learned start)")
+    ap.add_argument("--smoothing", default="128,900",
+                    help="DEGREE,RATIO of the learned start's Chebyshev-Jacobi smoothing (0,1: none)")
     ap.add_argument("--no-tune", action="store_true", help="default tiling instead of the autotuner")
     ap.add_argument("--tiling", default=None, help="TY,XC,ZPT matvec tiling for every shape (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison")
@@ -126,11 +131,16 @@ def main():
     torch.cuda.set_device(0)
     dev = torch.device("cuda", 0)
     net = None
-    if args.learned:
+    learned = args.learned
+    if learned is None and args.workload == "cell64" and not args.no_learned:
+        learned = os.path.join(ROOT, "nonlinear-fiber-homogenization_amd", "data", "warmstart64.npz")
+    if learned and not args.no_learned:
         from nlfh_amd.learned import WarmStartNet
-        net = WarmStartNet.load(args.learned)
+        net = WarmStartNet.load(learned)
+    deg, ratio = (float(v) for v in args.smoothing.split(","))
+    smoothing = (int(deg), ratio) if deg > 0 else None
     bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
-                                   warm_start=net, streams=args.streams)
+                                   warm_start=net, streams=args.streams, warm_smoothing=smoothing)
     t_tune = time.perf_counter()
     if args.tiling:
         ty, xc, zpt = (int(v) for v in args.tiling.split(","))
@@ -192,6 +202,22 @@ def main():
     if args.precision == "mixed" and not args.no_fp64_check:
         # after the timed region: every timed cell solved again by the batched plain FP64 PCG, C compared
         extra["fp64_check"] = fp64_check(pkg, shapes, rho_d, emin, tol, np.asarray(C), args.streams)
+    zero_start = None
+    if net is not None:
+        # the same batched mixed solver from zero on the same cells (after the timed region), for the gain
+        bz = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
+                                       streams=args.streams)
+        for i in range(len(set(shapes))):
+            bz._plan.set_tiling(i, *bh._plan.get_tiling(i))
+        bz.C_full(rho_d)
+        torch.cuda.synchronize()
+        tz = time.perf_counter()
+        for _ in range(args.steps):
+            _, _, itz = bz.C_full(rho_d)
+        torch.cuda.synchronize()
+        tz = time.perf_counter() - tz
+        zero_start = {"value": round(ncell * args.steps / tz, 3), "unit": "homogenizations/s",
+                      "pcg_iterations": itz.tolist(), "learned_speedup": round(value / (ncell * args.steps / tz), 4)}
     plain = None
     if not args.no_plain and args.workload == "cell64":
         plain = plain_fp64(pkg, rhos, emin, tol, dev)
@@ -229,6 +255,7 @@ def main():
             "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
                        "ty_xc_zpt": tiling}, "tune_seconds": round(t_tune, 2),
             "learned_start": bool(net), "learned_start_fallback_steps": fallbacks,
+            "learned_start_smoothing": list(smoothing) if (net and smoothing) else None,
             "fp64_fallback_cells": fallback_fp64[-1] if fallback_fp64 else [],
             "pcg_iterations": its_all[-1] if its_all else [],
             "cells": metas if ncell <= 16 else f"{ncell} cells (seeds in the workload description)",
@@ -237,6 +264,7 @@ def main():
         "solver_bandwidth": solver_bw,
         "cpu_baseline": cpu,
         "plain_fp64_gpu": plain,
+        "mixed_zero_start": zero_start,
     }
     res.update(extra)
     if args.cpu_full and not args.no_cpu:
