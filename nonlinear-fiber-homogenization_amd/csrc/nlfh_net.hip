@@ -101,6 +101,140 @@ __global__ void __launch_bounds__(256) conv3d_kernel(const ConvArgs p) {
   }
 }
 
+// LDS-staged form of the same convolution for cells whose side the brick tiles: a block owns a brick of
+// 4 x BY x BZ = 256 output voxels (BZ = 16 when 16 | n, else 8 with BY = 8) and all output channels; per
+// chunk of KC input channels (32 when both sources split into 32-channel chunks, else 16) it stages the
+// brick with its one-voxel periodic halo in LDS (nearest upsampling of a half-resolution source applied
+// as it is staged), then runs the 27 taps as K = KC steps of v_mfma_f32_16x16x32_bf16 (KC = 32) or
+// v_mfma_f32_16x16x16_bf16 (KC = 16). A wavefront holds 4 M-tiles of 16 voxels (z-rows of the brick) x NB
+// output-channel blocks; each weight fragment is loaded once per tap and used by the 4 M-tiles.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
+
+template <int KC>
+struct KFrag;
+template <>
+struct KFrag<32> {
+  typedef bf16x8 T;
+  __device__ static f32x4 mfma(T a, T b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+template <>
+struct KFrag<16> {
+  typedef bf16x4 T;
+  __device__ static f32x4 mfma(T a, T b, f32x4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+};
+
+template <int BZ>
+struct Brick {
+  static constexpr int BX = 4, BY = 256 / (4 * BZ);
+  static constexpr int HX = BX + 2, HY = BY + 2, HZ = BZ + 2;
+  static constexpr int HV = HX * HY * HZ;
+  // voxel r (0..15) of M-tile mt (0..15) inside the brick
+  __device__ static void voxel(int mt, int r, int& x, int& y, int& z) {
+    x = mt >> 2;
+    if (BZ == 16) {
+      y = mt & 3;
+      z = r;
+    } else {
+      y = (mt & 3) * 2 + (r >> 3);
+      z = r & 7;
+    }
+  }
+};
+
+template <int KC, int NB, int BZ>
+__global__ void __launch_bounds__(256) conv3d_lds_kernel(const ConvArgs p) {
+  typedef Brick<BZ> Bk;
+  typedef typename KFrag<KC>::T Frag;
+  constexpr int KL = KC / 4;  // bf16 values per lane and fragment
+  extern __shared__ uint4 lds_u4[];
+  uint16_t* lds = reinterpret_cast<uint16_t*>(lds_u4);
+  const int n = p.n;
+  const int nbz = n / BZ, nby = n / Bk::BY, nbx = n / Bk::BX;
+  int id = blockIdx.x;
+  const int bz = id % nbz; id /= nbz;
+  const int by = id % nby; id /= nby;
+  const int bx = id % nbx;
+  const int bidx = id / nbx;
+  const int x0 = bx * Bk::BX, y0 = by * Bk::BY, z0 = bz * BZ;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  const int cin = p.ca + p.cb;
+  const int na = p.a_half ? n / 2 : n;
+  // LDS offset (in bf16) of the A-fragment voxel of each of this lane's M-tiles, tap (0, 0, 0)
+  int abase[4];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) {
+    int x, y, z;
+    Bk::voxel(wave * 4 + mt, row, x, y, z);
+    abase[mt] = (((x + 1) * Bk::HY + (y + 1)) * Bk::HZ + (z + 1)) * KC + kg * KL;
+  }
+  f32x4 acc[4][NB];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[mt][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  constexpr int UPV = KC / 8;  // 16-byte units per staged voxel
+  for (int c0 = 0; c0 < cin; c0 += KC) {
+    const bool fromA = c0 < p.ca;
+    for (int u = threadIdx.x; u < Bk::HV * UPV; u += 256) {
+      const int hv = u / UPV, q = u - hv * UPV;
+      const int hz = hv % Bk::HZ, hy = (hv / Bk::HZ) % Bk::HY, hx = hv / (Bk::HZ * Bk::HY);
+      int X = x0 - 1 + hx, Y = y0 - 1 + hy, Z = z0 - 1 + hz;
+      X = X < 0 ? X + n : (X >= n ? X - n : X);
+      Y = Y < 0 ? Y + n : (Y >= n ? Y - n : Y);
+      Z = Z < 0 ? Z + n : (Z >= n ? Z - n : Z);
+      uint4 val;
+      if (fromA) {
+        const int ax = p.a_half ? X >> 1 : X, ay = p.a_half ? Y >> 1 : Y, az = p.a_half ? Z >> 1 : Z;
+        const int64_t ai = (((int64_t)bidx * na + ax) * na + ay) * na + az;
+        val = *reinterpret_cast<const uint4*>(p.a + ai * p.ca + c0 + q * 8);
+      } else {
+        const int64_t bi = (((int64_t)bidx * n + X) * n + Y) * n + Z;
+        val = *reinterpret_cast<const uint4*>(p.b + bi * p.cb + (c0 - p.ca) + q * 8);
+      }
+      lds_u4[hv * UPV + q] = val;
+    }
+    __syncthreads();
+    const uint16_t* wc = p.w + c0 + kg * KL;
+    for (int t = 0; t < 27; ++t) {
+      int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+      if (p.flip) { dx = -dx; dy = -dy; dz = -dz; }
+      const int toff = ((dx * Bk::HY + dy) * Bk::HZ + dz) * KC;
+      Frag bw[NB];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        bw[j] = *reinterpret_cast<const Frag*>(wc + ((int64_t)t * p.coutp + j * 16 + row) * cin);
+#pragma unroll
+      for (int mt = 0; mt < 4; ++mt) {
+        const Frag av = *reinterpret_cast<const Frag*>(lds + abase[mt] + toff);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[mt][j] = KFrag<KC>::mfma(av, bw[j], acc[mt][j]);
+      }
+    }
+    __syncthreads();
+  }
+  // D: lane holds voxels 4 kg + e of each M-tile, output channel j * 16 + row
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    const int co = j * 16 + row;
+    if (co >= p.cout) continue;
+    const float bias = p.bias ? p.bias[co] : 0.f;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        int x, y, z;
+        Bk::voxel(wave * 4 + mt, 4 * kg + e, x, y, z);
+        const int64_t v = (((int64_t)bidx * n + x0 + x) * n + y0 + y) * n + z0 + z;
+        float yv = acc[mt][j][e] + bias;
+        if (p.act == 1) yv = yv > 0.f ? yv : 0.1f * yv;
+        if (p.out_f) p.out_f[v * p.cout + co] = yv;
+        if (p.out_h) p.out_h[v * p.cout + co] = f2bf(yv);
+      }
+  }
+}
+
 struct WgradArgs {
   const uint16_t* g;  // [batch][n^3][cg] output-side gradient (bf16)
   const uint16_t* a;  // source A [batch][na^3][ca]
@@ -333,9 +467,32 @@ int nlfh_net_conv(const uint16_t* a, int ca, int a_half, const uint16_t* b, int 
     return NLFH_EINVAL;
   }
   ConvArgs p{a, b, w, bias, out_f, out_h, ca, cb, a_half, n, batch, cout, coutp, act, flip};
+  hipStream_t st = (hipStream_t)stream;
+  const int bz = n % 16 == 0 ? 16 : (n % 8 == 0 ? 8 : 0);
+  if (bz) {  // the LDS-staged brick kernel
+    const int kc = (ca % 32 == 0 && cb % 32 == 0) ? 32 : 16;
+    const int64_t nblk = (int64_t)batch * (n / 4) * (n / (256 / (4 * bz))) * (n / bz);
+    const size_t lds = (size_t)(4 + 2) * (256 / (4 * bz) + 2) * (bz + 2) * kc * sizeof(uint16_t);
+    const int nb = coutp / 16;
+#define NLFH_CONV_LDS(KC_, BZ_)                                                                            \
+  switch (nb) {                                                                                          \
+    case 1: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 1, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+    case 2: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 2, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+    case 3: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 3, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+    case 4: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 4, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+    case 5: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 5, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+    default: hipLaunchKernelGGL((conv3d_lds_kernel<KC_, 6, BZ_>), dim3((unsigned)nblk), dim3(256), lds, st, p); break; \
+  }
+    if (kc == 32 && bz == 16) { NLFH_CONV_LDS(32, 16) }
+    else if (kc == 32) { NLFH_CONV_LDS(32, 8) }
+    else if (bz == 16) { NLFH_CONV_LDS(16, 16) }
+    else { NLFH_CONV_LDS(16, 8) }
+#undef NLFH_CONV_LDS
+    HIP_TRY(hipGetLastError());
+    return NLFH_OK;
+  }
   const int64_t tiles = ((int64_t)n * n * n * batch + 15) / 16;
   const int blocks = (int)((tiles + 3) / 4);
-  hipStream_t st = (hipStream_t)stream;
   switch (coutp / 16) {
     case 1: hipLaunchKernelGGL(conv3d_kernel<1>, dim3(blocks), dim3(256), 0, st, p); break;
     case 2: hipLaunchKernelGGL(conv3d_kernel<2>, dim3(blocks), dim3(256), 0, st, p); break;

@@ -48,6 +48,66 @@ def test_forward_and_gradients_match_reference(gpu_pkg):
         assert eb < 5e-2, (name, eb)
 
 
+def _bf16(x):
+    return x.to(torch.bfloat16).to(torch.float64)
+
+
+@pytest.mark.parametrize("n,ca,cb,a_half,cout,coutp,act,flip", [
+    (16, 16, 0, 0, 16, 16, 1, 0),    # brick 4x4x16, 16-channel chunks
+    (32, 32, 0, 0, 32, 32, 1, 1),    # 32-channel chunks, flipped taps
+    (16, 64, 32, 1, 32, 32, 1, 0),   # half-resolution source + skip source (decoder)
+    (8, 32, 16, 1, 16, 16, 0, 0),    # brick 4x8x8, mixed chunk width falls to 16
+    (8, 32, 0, 0, 96, 96, 0, 1),     # six output blocks
+    (16, 16, 0, 0, 18, 32, 0, 0),    # head: padded output channels
+    (12, 16, 16, 0, 32, 32, 1, 0),   # a side the bricks do not tile: the generic kernel
+    (64, 16, 0, 0, 16, 16, 1, 0),    # full-resolution layer of a 64^3 cell
+])
+def test_conv_layer_matches_torch(gpu_pkg, n, ca, cb, a_half, cout, coutp, act, flip):
+    """nlfh_net_conv against torch's conv3d in float64 on the same bf16-rounded operands (circular padding;
+    flip = 1 reads the taps mirrored, as the data gradient does): FP32 accumulation only, so the outputs
+    agree to 1e-4 of their largest entry."""
+    import ctypes
+    import torch.nn.functional as F
+    lib = gpu_pkg.load_library()
+    rng = np.random.default_rng(n + ca + cb + cout)
+    B = 2
+    na = n // 2 if a_half else n
+    A = _bf16(torch.tensor(rng.standard_normal((B, na, na, na, ca))))
+    Bs = _bf16(torch.tensor(rng.standard_normal((B, n, n, n, cb)))) if cb else None
+    W = _bf16(torch.tensor(rng.standard_normal((27, coutp, ca + cb)) * 0.1))
+    bias = torch.tensor(rng.standard_normal(cout) * 0.1)
+    # reference: nearest upsampling of A, concatenation, periodic 3x3x3 taps
+    Af = A.repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3) if a_half else A
+    X = torch.cat([Af, Bs], dim=-1) if cb else Af
+    Wk = W.reshape(3, 3, 3, coutp, ca + cb)
+    if flip:
+        Wk = Wk.flip(0, 1, 2)
+    Wk = Wk.permute(3, 4, 0, 1, 2)
+    xt = F.pad(X.permute(0, 4, 1, 2, 3), (1, 1, 1, 1, 1, 1), mode="circular")
+    ref = (F.conv3d(xt, Wk) + F.pad(bias, (0, coutp - cout))[None, :, None, None, None]).permute(0, 2, 3, 4, 1)
+    ref = ref[..., :cout]
+    if act:
+        ref = F.leaky_relu(ref, 0.1)
+
+    def dev16(t):
+        return t.float().to(torch.bfloat16).view(torch.int16).cuda().contiguous()
+
+    a_d, w_d = dev16(A), dev16(W)
+    b_d = dev16(Bs) if cb else None
+    bias_d = bias.float().cuda()
+    out = torch.full((B, n, n, n, cout), float("nan"), dtype=torch.float32, device="cuda")
+    outh = torch.empty((B, n, n, n, cout), dtype=torch.int16, device="cuda")
+    P = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    gpu_pkg._lib.check(lib.nlfh_net_conv(P(a_d), ca, a_half, P(b_d), cb, P(w_d), P(bias_d), n, B, cout, coutp, act,
+                                         flip, P(out), P(outh), st))
+    o = out.cpu().double()
+    assert torch.isfinite(o).all()
+    assert float((o - ref).abs().max() / ref.abs().max()) < 1e-4
+    oh = outh.cpu().view(torch.bfloat16).double()
+    assert torch.equal(oh, out.cpu().to(torch.bfloat16).double())  # the bf16 copy is the rounded output
+
+
 def test_training_reduces_loss(gpu_pkg):
     """Adam steps on correctors produced by the solver lower the loss; the prediction used as u0
     gives the same C as a cold start."""
