@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber", "finite"])
+    ap.add_argument("--workload", default="cell64", choices=["cell64", "batch256", "stress128", "fiber", "finite", "net"])
     ap.add_argument("--fiber-n-o", type=int, default=16, help="octant side of the fiber workload (cell 2 n_o)")
     ap.add_argument("--fiber-precision", default="fp64", choices=["fp64", "mixed"],
                     help="linear solver of the fiber workload (fp64: the reference's PCG)")
@@ -110,6 +110,8 @@ def main():
         return fiber_bench(args)
     if args.workload == "finite":
         return finite_bench(args)
+    if args.workload == "net":
+        return net_bench(args)
     jobs, emin, wname = workload(args)
     tol = 1e-8
     if wname == "stress128":
@@ -389,6 +391,71 @@ def fiber_bench(args):
                    "workload": f"Fiber(n_o={n_o}) on a {n}^3 cell, beta {beta}, v {v}, r {r}, emin 1e-3, tol 1e-9, "
                                f"warm-started along phi_k = phi_0 + 0.02 k d", "pcg_iterations": its},
         "roofline": roof, "cpu_baseline": cpu}), flush=True)
+
+
+def net_flops_per_voxel():
+    """Multiply-add flops per input voxel of one forward pass of the warm-start network (2 * 27 * cin * coutp per
+    output voxel of every convolution, padded channel counts, at its level's resolution)."""
+    from nlfh_amd.learned import LAYERS
+    return sum(2 * 27 * (ca + cb) * coutp / 8 ** lvl for _, ca, _, cb, _, coutp, _, lvl in LAYERS)
+
+
+def net_bench(args):
+    """The learned start's network on 64^3 cells: inference of 32 cells (the learned start's batch) and training
+    steps of 4 cells (forward, MSE loss, backward, Adam), synthetic densities and targets, HIP events around
+    repeated calls on the current stream. The roofline is MFMA-bound: algorithmic flops (forward 1x, training
+    3x: forward, data and weight gradients) over time against the 2.5 PFLOP/s dense BF16 peak."""
+    import torch
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    pkg.load_library()
+    torch.cuda.set_device(0)
+    from nlfh_amd.learned import WarmStartNet
+    net = WarmStartNet.load(os.path.join(ROOT, "nonlinear-fiber-homogenization_amd", "data", "warmstart64.npz"))
+    n, bi, bt = 64, 32, 4
+    g = torch.Generator(device="cuda").manual_seed(0)
+    rho = torch.rand((bi, n, n, n), dtype=torch.float64, device="cuda", generator=g)
+    X = net.input(rho[:bt])
+    Y = 0.01 * torch.randn((bt, n, n, n, 18), device="cuda", generator=g)
+
+    def infer():
+        net.predict(rho)
+
+    def train():
+        out = net.forward(X, keep=True)
+        _, dout = net.loss_and_grad(out, Y)
+        net.adam(net.backward(dout), lr=1e-6)
+
+    st = torch.cuda.current_stream()
+    res = {}
+    for name, f, reps in (("inference", infer, max(args.steps, 5)), ("training_step", train, max(args.steps, 5))):
+        for _ in range(max(args.warmup, 1)):
+            f()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            f()
+        e1.record(st)
+        torch.cuda.synchronize()
+        res[name] = e0.elapsed_time(e1) / reps * 1e-3
+    fv = net_flops_per_voxel()
+    f_inf = fv * n ** 3 * bi
+    f_tr = 3 * fv * n ** 3 * bt
+    tf_inf, tf_tr = f_inf / res["inference"] / 1e12, f_tr / res["training_step"] / 1e12
+    print(json.dumps({
+        "metric": "warm-start network: 64^3 cells predicted per second (inference)", "value": round(bi / res["inference"], 1),
+        "unit": "cells/s", "n_gpus": 1, "steps": max(args.steps, 5), "warmup": max(args.warmup, 1),
+        "ms_per_step": round(1e3 * res["inference"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulation)", "data": "synthetic",
+        "config": {"workload": f"inference of {bi} cells and training steps of {bt} cells at 64^3 (3 levels, widths "
+                               f"16/32/64, 18 outputs)", "flops_per_voxel_forward": fv,
+                   "training_step_ms": round(1e3 * res["training_step"], 3),
+                   "training_tflops": round(tf_tr, 1)},
+        "roofline": {"bound": "mfma", "achieved": round(tf_inf, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tf_inf / 2500.0, 4), "traffic": None,
+                     "kernel": "conv3d_lds_kernel (all convolutions of one inference, 32 x 64^3)"},
+        "cpu_baseline": None}), flush=True)
 
 
 FS_FLOP_PER_VOXEL = 6016  # Hessian-vector product, neo-Hookean with void interpolation (8 Gauss points x 752)

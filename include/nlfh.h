@@ -165,9 +165,10 @@ int nlfh_net_wgrad(const unsigned short* g, int cg, const unsigned short* a, int
 /* g = bf16(dy * act'(y)) (act 1: leaky ReLU with y its output, 0: identity) */
 int nlfh_net_act_grad(const float* dy, const unsigned short* y, unsigned short* g, long long len, int act,
                       void* stream);
-/* db[c] = sum_v dy[v][c] act'(y[v][c]) */
-int nlfh_net_bias_grad(const float* dy, const unsigned short* y, long long nvox, int c, int act, float* db,
-                       void* stream);
+/* db[c] = sum_v dy[v][c] act'(y[v][c]); part (nullable) is scratch of part_len floats for per-block partial sums
+ * (fixed-order reduction; with null part one block per channel sums alone) */
+int nlfh_net_bias_grad(const float* dy, const unsigned short* y, long long nvox, int c, int act, float* part,
+                       long long part_len, float* db, void* stream);
 /* 2x2x2 average pooling (bf16) */
 int nlfh_net_pool(const unsigned short* in, unsigned short* out, int n, int batch, int c, void* stream);
 /* coarse (+)= scale * sum of the 8 fine children of channels [c0, c0 + c) of fine (cf channels) */

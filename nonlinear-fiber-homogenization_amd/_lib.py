@@ -62,7 +62,8 @@ SIGNATURES = [
     ("nlfh_net_wgrad", ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int,
                                       ctypes.c_int, ctypes.c_int, _vp, ctypes.c_longlong, _vp, _vp]),
     ("nlfh_net_act_grad", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong, ctypes.c_int, _vp]),
-    ("nlfh_net_bias_grad", ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    ("nlfh_net_bias_grad", ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _vp,
+                                          ctypes.c_longlong, _vp, _vp]),
     ("nlfh_net_pool", ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]),
     ("nlfh_net_upsum", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_float, ctypes.c_int, _vp]),

@@ -297,6 +297,130 @@ __global__ void __launch_bounds__(64) wgrad_kernel(const WgradArgs p) {
   for (int r = 0; r < 4; ++r) out[(int64_t)(cob * 16 + 4 * kg + r) * cin + ci] = acc[r];
 }
 
+// Weight gradient over bricks of 4 x 4 x 16 voxels (cells with 16 | n): a block takes a run of bricks of one
+// cell batch, one 16-channel input block (ci0) and one 16-channel output-gradient block (co0); per brick it
+// stages g[v][co0..+16) (256 voxels) and the input's halo brick [6 x 6 x 18][ci0..+16) in LDS, and every
+// wavefront accumulates D_t[co][ci] = sum_v g[v][co] in[v + d_t][ci] for its taps (t = wave, wave + 4, ...)
+// with v_mfma_f32_16x16x16_bf16, K = the 16 voxels of one z-row of the brick. The g fragments of the 16 rows
+// are read once per brick and reused by every tap. Partial sums per run: part[run][t][co][ci].
+__global__ void __launch_bounds__(256) wgrad_lds_kernel(const WgradArgs p, int bricks_per_run) {
+  typedef Brick<16> Bk;
+  __shared__ uint4 sg[256 * 2];       // g brick: [voxel][16 channels]
+  __shared__ uint4 sx[Bk::HV * 2];    // input halo brick: [halo voxel][16 channels]
+  const uint16_t* g16 = reinterpret_cast<const uint16_t*>(sg);
+  const uint16_t* x16 = reinterpret_cast<const uint16_t*>(sx);
+  const int cin = p.ca + p.cb;
+  const int nci = cin / 16, nco = p.cg / 16;
+  int id = blockIdx.x;
+  const int cib = id % nci; id /= nci;
+  const int cob = id % nco; id /= nco;
+  const int run = id;
+  const int n = p.n;
+  const int na = p.a_half ? n / 2 : n;
+  const int nbz = n / 16, nby = n / 4, nbx = n / 4;
+  const int64_t per_cell = (int64_t)nbz * nby * nbx;
+  const int64_t nbr = per_cell * p.batch;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  const int ci0 = cib * 16, co0 = cob * 16;
+  const bool fromA = ci0 < p.ca;
+  f32x4 acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  for (int bi = 0; bi < bricks_per_run; ++bi) {
+    const int64_t br = (int64_t)run * bricks_per_run + bi;
+    if (br >= nbr) break;  // uniform over the block
+    const int bidx = (int)(br / per_cell);
+    int64_t r = br - (int64_t)bidx * per_cell;
+    const int bz = (int)(r % nbz); r /= nbz;
+    const int by = (int)(r % nby);
+    const int bx = (int)(r / nby);
+    const int x0 = bx * 4, y0 = by * 4, z0 = bz * 16;
+    // stage: g rows (two 16-byte units per voxel), input halo (two units per halo voxel)
+    for (int u = threadIdx.x; u < 512; u += 256) {
+      const int v = u >> 1, q = u & 1;
+      const int x = v >> 6, y = (v >> 4) & 3, z = v & 15;  // brick voxel order: M-tile (x, y), z
+      const int64_t gv = (((int64_t)bidx * n + x0 + x) * n + y0 + y) * n + z0 + z;
+      sg[u] = *reinterpret_cast<const uint4*>(p.g + gv * p.cg + co0 + q * 8);
+    }
+    for (int u = threadIdx.x; u < Bk::HV * 2; u += 256) {
+      const int hv = u >> 1, q = u & 1;
+      const int hz = hv % Bk::HZ, hy = (hv / Bk::HZ) % Bk::HY, hx = hv / (Bk::HZ * Bk::HY);
+      int X = x0 - 1 + hx, Y = y0 - 1 + hy, Z = z0 - 1 + hz;
+      X = X < 0 ? X + n : (X >= n ? X - n : X);
+      Y = Y < 0 ? Y + n : (Y >= n ? Y - n : Y);
+      Z = Z < 0 ? Z + n : (Z >= n ? Z - n : Z);
+      uint4 val;
+      if (fromA) {
+        const int ax = p.a_half ? X >> 1 : X, ay = p.a_half ? Y >> 1 : Y, az = p.a_half ? Z >> 1 : Z;
+        val = *reinterpret_cast<const uint4*>(p.a + ((((int64_t)bidx * na + ax) * na + ay) * na + az) * p.ca + ci0 +
+                                              q * 8);
+      } else {
+        val = *reinterpret_cast<const uint4*>(p.b + ((((int64_t)bidx * n + X) * n + Y) * n + Z) * p.cb +
+                                              (ci0 - p.ca) + q * 8);
+      }
+      sx[u] = val;
+    }
+    __syncthreads();
+    // A fragments (M = output channel, K = 4 voxels of a z-row) for the 16 rows of the brick
+    bf16x4 ga[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) ga[m][e] = (short)g16[(m * 16 + kg * 4 + e) * 16 + row];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int t = wave + 4 * k;
+      if (t < 27) {
+        const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+          const int x = m >> 2, y = m & 3;
+          const int hb = ((x + 1 + dx) * Bk::HY + (y + 1 + dy)) * Bk::HZ + (1 + dz) + kg * 4;
+          bf16x4 xb;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) xb[e] = (short)x16[(hb + e) * 16 + row];
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ga[m], xb, acc[k], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // D rows = output channel 4 kg + e, column = input channel `row`
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int t = wave + 4 * k;
+    if (t >= 27) continue;
+    float* out = p.part + (((int64_t)run * 27 + t) * p.cg) * cin;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) out[(int64_t)(co0 + 4 * kg + e) * cin + ci0 + row] = acc[k][e];
+  }
+}
+
+// Bias gradient, first level: block b sums voxels [b * chunk, (b + 1) * chunk) of dy * act'(y) per channel
+// (threads = vl voxel lanes x C channels, coalesced over channels-last rows), fixed-order sum over the
+// voxel lanes; part[b][c].
+__global__ void bias_part_kernel(const float* dy, const uint16_t* y, int64_t nvox, int C, int act, int64_t chunk,
+                                 float* part) {
+  __shared__ float red[256];
+  const int vl = blockDim.x / C;
+  const int c = threadIdx.x % C, vi = threadIdx.x / C;
+  const int64_t v0 = (int64_t)blockIdx.x * chunk, v1 = v0 + chunk < nvox ? v0 + chunk : nvox;
+  float s = 0.f;
+  for (int64_t v = v0 + vi; v < v1; v += vl) {
+    float d = dy[v * C + c];
+    if (act == 1 && bf2f(y[v * C + c]) <= 0.f) d *= 0.1f;
+    s += d;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (vi == 0) {
+    float t = 0.f;
+    for (int k = 0; k < vl; ++k) t += red[k * C + c];
+    part[(int64_t)blockIdx.x * C + c] = t;
+  }
+}
+
 // dW[i] = sum over chunks of part[chunk][i] (fixed order)
 __global__ void reduce_chunks_kernel(const float* part, int nchunk, int64_t len, float* out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < len; i += (int64_t)gridDim.x * blockDim.x) {
@@ -524,8 +648,28 @@ int nlfh_net_wgrad(const uint16_t* g, int cg, const uint16_t* a, int ca, int a_h
     set_error("nlfh_net_wgrad: partial buffer too small");
     return NLFH_EINVAL;
   }
-  WgradArgs p{g, a, b, part, ca, cb, a_half, cg, n, batch, chunk, (int)nchunk};
   hipStream_t st = (hipStream_t)stream;
+  if (n % 16 == 0) {  // bricks of 4 x 4 x 16 voxels, runs of bricks per block
+    const int64_t nbr = (int64_t)batch * (n / 16) * (n / 4) * (n / 4);
+    const int64_t pairs = (int64_t)(cin / 16) * (cg / 16);
+    int64_t per_run = 1;
+    while ((nbr + per_run - 1) / per_run * pairs > 2048 && per_run < nbr) per_run *= 2;  // about 2048 blocks
+    int64_t nrun = (nbr + per_run - 1) / per_run;
+    while (nrun * wlen > part_len && per_run < nbr) {
+      per_run *= 2;
+      nrun = (nbr + per_run - 1) / per_run;
+    }
+    if (nrun * wlen > part_len) {
+      set_error("nlfh_net_wgrad: partial buffer too small");
+      return NLFH_EINVAL;
+    }
+    WgradArgs p{g, a, b, part, ca, cb, a_half, cg, n, batch, 0, (int)nrun};
+    hipLaunchKernelGGL(wgrad_lds_kernel, dim3((unsigned)(nrun * pairs)), dim3(256), 0, st, p, (int)per_run);
+    hipLaunchKernelGGL(reduce_chunks_kernel, dim3(nb_for(wlen)), dim3(256), 0, st, part, (int)nrun, wlen, dw);
+    HIP_TRY(hipGetLastError());
+    return NLFH_OK;
+  }
+  WgradArgs p{g, a, b, part, ca, cb, a_half, cg, n, batch, chunk, (int)nchunk};
   const int64_t blocks = nchunk * 27 * (cg / 16) * (cin / 16);
   hipLaunchKernelGGL(wgrad_kernel, dim3((unsigned)blocks), dim3(64), 0, st, p);
   hipLaunchKernelGGL(reduce_chunks_kernel, dim3(nb_for(wlen)), dim3(256), 0, st, part, (int)nchunk, wlen, dw);
@@ -539,8 +683,26 @@ int nlfh_net_act_grad(const float* dy, const uint16_t* y, uint16_t* g, long long
   return NLFH_OK;
 }
 
-int nlfh_net_bias_grad(const float* dy, const uint16_t* y, long long nvox, int c, int act, float* db, void* stream) {
-  hipLaunchKernelGGL(bias_grad_kernel, dim3(c), dim3(256), 0, (hipStream_t)stream, dy, y, (int64_t)nvox, c, act, db);
+int nlfh_net_bias_grad(const float* dy, const uint16_t* y, long long nvox, int c, int act, float* part,
+                       long long part_len, float* db, void* stream) {
+  if (!dy || !db || (act && !y) || c < 1 || c > 256 || nvox < 1) {
+    set_error("nlfh_net_bias_grad: invalid arguments");
+    return NLFH_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  if (part == nullptr || part_len < (long long)c) {  // no scratch: one block per channel
+    hipLaunchKernelGGL(bias_grad_kernel, dim3(c), dim3(256), 0, st, dy, y, (int64_t)nvox, c, act, db);
+    HIP_TRY(hipGetLastError());
+    return NLFH_OK;
+  }
+  const int vl = 256 / c > 0 ? 256 / c : 1;
+  int64_t nblk = (nvox + 511) / 512;  // about 512 voxels per block: enough blocks to cover the load latency
+  if (nblk * c > part_len) nblk = part_len / c;
+  const int64_t chunk = (nvox + nblk - 1) / nblk;
+  nblk = (nvox + chunk - 1) / chunk;
+  hipLaunchKernelGGL(bias_part_kernel, dim3((unsigned)nblk), dim3(vl * c), 0, st, dy, y, (int64_t)nvox, c, act, chunk,
+                     part);
+  hipLaunchKernelGGL(reduce_chunks_kernel, dim3(nb_for(c)), dim3(256), 0, st, part, (int)nblk, (int64_t)c, db);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -168,7 +168,8 @@ class WarmStartNet:
             _lib.check(lib.nlfh_net_wgrad(_p(g), coutp, _p(a), ca, ah, _p(b), cb, s, B, _p(part), part.numel(), _p(dw),
                                           _st()))
             db = torch.empty_like(self.b[name])
-            _lib.check(lib.nlfh_net_bias_grad(_p(dy), _p(y) if act else None, nv, cout, act, _p(db), _st()))
+            _lib.check(lib.nlfh_net_bias_grad(_p(dy), _p(y) if act else None, nv, cout, act, _p(part), part.numel(),
+                                              _p(db), _st()))
             grads[name] = (dw, db)
             return g
 

@@ -108,6 +108,61 @@ def test_conv_layer_matches_torch(gpu_pkg, n, ca, cb, a_half, cout, coutp, act, 
     assert torch.equal(oh, out.cpu().to(torch.bfloat16).double())  # the bf16 copy is the rounded output
 
 
+@pytest.mark.parametrize("n,ca,cb,a_half,cg,act", [
+    (64, 32, 16, 1, 16, 1),   # runs of several bricks per block (three input blocks), half-resolution source
+    (16, 64, 0, 0, 64, 1),    # four output-gradient blocks
+    (12, 16, 16, 0, 32, 0),   # a side the bricks do not tile: the generic kernels
+])
+def test_weight_and_bias_gradients_match_torch(gpu_pkg, n, ca, cb, a_half, cg, act):
+    """nlfh_net_wgrad, dW[t][co][ci] = sum_v g[v][co] in[v + d_t][ci] (periodic, the concatenated and upsampled
+    input of nlfh_net_conv), and nlfh_net_bias_grad, db[c] = sum_v dy[v][c] act'(y[v][c]), against float64
+    torch on the same bf16 operands (FP32 accumulation: 1e-4 of the largest entry)."""
+    import ctypes
+    lib = gpu_pkg.load_library()
+    rng = np.random.default_rng(n + ca + cg)
+    B = 2
+    na = n // 2 if a_half else n
+    A = _bf16(torch.tensor(rng.standard_normal((B, na, na, na, ca))))
+    Bs = _bf16(torch.tensor(rng.standard_normal((B, n, n, n, cb)))) if cb else None
+    G = _bf16(torch.tensor(rng.standard_normal((B, n, n, n, cg))))
+    Af = A.repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3) if a_half else A
+    X = torch.cat([Af, Bs], dim=-1) if cb else Af
+    cin = ca + cb
+    ref = torch.empty((27, cg, cin), dtype=torch.float64)
+    for t in range(27):
+        d = (t // 9 - 1, (t // 3) % 3 - 1, t % 3 - 1)
+        Xs = torch.roll(X, shifts=(-d[0], -d[1], -d[2]), dims=(1, 2, 3))  # Xs[v] = X[v + d]
+        ref[t] = torch.einsum("bxyzo,bxyzi->oi", G, Xs)
+
+    def dev16(t):
+        return t.float().to(torch.bfloat16).view(torch.int16).cuda().contiguous()
+
+    P = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    part = torch.empty(16 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dw = torch.full((27, cg, cin), float("nan"), dtype=torch.float32, device="cuda")
+    a_d, g_d = dev16(A), dev16(G)
+    b_d = dev16(Bs) if cb else None
+    gpu_pkg._lib.check(lib.nlfh_net_wgrad(P(g_d), cg, P(a_d), ca, a_half, P(b_d), cb, n, B, P(part), part.numel(),
+                                          P(dw), st))
+    w = dw.cpu().double()
+    assert torch.isfinite(w).all()
+    assert float((w - ref).abs().max() / ref.abs().max()) < 1e-4
+    # bias gradient with and without the partial-sum scratch
+    dy = torch.tensor(rng.standard_normal((B * n ** 3, cg)), dtype=torch.float32)
+    y = _bf16(torch.tensor(rng.standard_normal((B * n ** 3, cg))))
+    gref = dy.double() * torch.where(y <= 0, 0.1, 1.0) if act else dy.double()
+    dbr = gref.sum(0)
+    for use_part in (True, False):
+        db = torch.full((cg,), float("nan"), dtype=torch.float32, device="cuda")
+        y_d = dev16(y)
+        gpu_pkg._lib.check(lib.nlfh_net_bias_grad(P(dy.cuda()), P(y_d) if act else None, B * n ** 3, cg, act,
+                                                  P(part) if use_part else None, part.numel() if use_part else 0,
+                                                  P(db), st))
+        torch.cuda.synchronize()
+        assert float((db.cpu().double() - dbr).abs().max() / dbr.abs().max()) < 1e-4
+
+
 def test_training_reduces_loss(gpu_pkg):
     """Adam steps on correctors produced by the solver lower the loss; the prediction used as u0
     gives the same C as a cold start."""

@@ -297,3 +297,34 @@ def test_cheb_smooth_is_the_chebyshev_residual_polynomial():
     c0 = (Q.T @ (np.sqrt(d) * e0))[hi_modes]
     ck = (Q.T @ (np.sqrt(d) * ek))[hi_modes]
     assert np.linalg.norm(ck) < 0.5 * np.linalg.norm(c0)
+
+
+def test_network_reference_gradients_match_central_differences():
+    """The float64 network reference (oracle/convnet_ref.py, unrounded) that the GPU gradients are checked
+    against: its autograd gradients of the MSE loss along random directions of every layer's weights and
+    biases equal central differences of the loss (h = 1e-6)."""
+    import torch
+    from oracle import convnet_ref as R
+    g = torch.Generator().manual_seed(0)
+    params = {}
+    for name, ca, ah, cb, cout, coutp, act, lvl in R.LAYERS:
+        w = torch.randn((27, coutp, ca + cb), generator=g, dtype=torch.float64) * (2.0 / (27 * (ca + cb))) ** 0.5
+        if name == "out":
+            w[:, 18:, :] = 0.0
+        params[name] = w.requires_grad_(True)
+        params[name + ".b"] = (0.1 * torch.randn(cout, generator=g, dtype=torch.float64)).requires_grad_(True)
+    rho = torch.rand((1, 8, 8, 8), generator=g, dtype=torch.float64)
+    target = torch.randn((1, 8, 8, 8, 18), generator=g, dtype=torch.float64)
+    L = R.loss(R.forward(params, rho, bf16=False), target)
+    grads = torch.autograd.grad(L, list(params.values()))
+    h = 1e-6
+    for (k, p), gk in zip(params.items(), grads):
+        d = torch.randn(p.shape, generator=g, dtype=torch.float64)
+        if k == "out":
+            d[:, 18:, :] = 0.0
+        with torch.no_grad():
+            pp = {kk: (vv + h * d if kk == k else vv) for kk, vv in params.items()}
+            pm = {kk: (vv - h * d if kk == k else vv) for kk, vv in params.items()}
+            fd = (R.loss(R.forward(pp, rho, bf16=False), target) - R.loss(R.forward(pm, rho, bf16=False), target)) / (2 * h)
+        an = float((gk * d).sum())
+        assert abs(an - float(fd)) <= 1e-5 * max(abs(an), 1e-12) + 1e-12, (k, an, float(fd))
