@@ -22,9 +22,14 @@ struct FastCfg {
   static constexpr int L = N1 > N2 ? N1 : N2;
   static constexpr int NH = N / 2 + 1;
   static constexpr int NP = N / 2;
-  // threads of passes A, C; FP32 at 128 takes 1024 so that one round of lines covers every row pair of a
-  // plane (the staged passes need that; 139 KiB of LDS, one block per CU)
-  static constexpr int BT = N == 128 ? (sizeof(T) == 8 ? 128 : 1024) : 256;
+  // threads of passes A, C; 128 takes 1024 so that one round of lines covers every row pair of a plane
+  // (the staged FP32 passes need that; 139 KiB of LDS, one block per CU)
+  static constexpr int BT = N == 128 ? 1024 : 256;
+  // FP64 at 128: the plane alone takes 130 KiB, so a line's exchange between its two steps happens in the
+  // line's own slots of the plane (a row pair or a column holds the line's N values; the L threads of a
+  // line are in one wavefront, which reads all its slots before it writes them), with a skewed index in
+  // place of padding; no separate exchange buffer
+  static constexpr bool XIN = sizeof(T) == 8 && N == 128;
   static constexpr int LPR = BT / L;                                   // lines per round
   static constexpr int XS = N1 * (N2 + 1) > N2 * (N1 + 1) ? N1 * (N2 + 1) : N2 * (N1 + 1);
   // pass B positions per block: 256 threads (128 for FP64 at 128), one component's exchange buffer in LDS
@@ -38,6 +43,17 @@ __device__ __forceinline__ cplx<T> cmulw(cplx<T> v, cplx<T> w, bool conj_w) {
   return {v.re * w.re - v.im * wi, v.re * wi + v.im * w.re};
 }
 
+// Exchange slot of element (k1, t) of a line between its two steps (k1 < N1 outputs of step 1 of thread t < N2;
+// step 2 thread t' reads (t', j2)): a padded row of a separate buffer, or, in place (XIN), position
+// k1 N2 + ((t + k1) mod N2) of the line's own N slots of the plane, `stride` apart (1 for a row pair,
+// NH for a column).
+template <typename T, int N>
+__device__ __forceinline__ int xslot(int k1, int t) {
+  using C = FastCfg<T, N>;
+  if constexpr (C::XIN) return k1 * C::N2 + ((t + k1) & (C::N2 - 1));
+  else return k1 * (C::N2 + 1) + t;
+}
+
 // Pass A ---------------------------------------------------------------------------------------
 template <typename T, int N, bool UPDATE>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
@@ -48,7 +64,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);  // [N]: e^{-2 pi i m / N}
   cplx<T>* plane = tw + N;                            // [N][NH]
   cplx<T>* xch = plane + N * NH;                      // [LPR][XS]
-  cplx<T>* dcn = xch + LPR * C::XS;                   // [N]: packed DC / Nyquist column transform
+  cplx<T>* dcn = xch + (C::XIN ? 0 : LPR * C::XS);     // [N]: packed DC / Nyquist column transform
   const int item = a.plane_base + blockIdx.x;
   const PlaneItem pi = a.planes[item];
   if (a.done != nullptr && a.done[pi.cell]) return;
@@ -73,7 +89,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
   for (int p0 = 0; p0 < NP; p0 += LPR) {
     const int p = p0 + l;
     const bool act = p < NP;
-    cplx<T>* xl = xch + l * C::XS;
+    cplx<T>* xl = C::XIN ? plane + (2 * p) * NH : xch + l * C::XS;
     if (act && t < N2) {
       cplx<T> v[N1];
       const int64_t r0 = fo + (int64_t)(2 * p) * N, r1 = r0 + N;
@@ -102,13 +118,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
       }
       Dft<T, N1, -1>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+      for (int k1 = 0; k1 < N1; ++k1) xl[xslot<T, N>(k1, t)] = cmulw(v[k1], tw[t * k1], false);
     }
     __syncthreads();
     if (act && t < N1) {
       cplx<T> w[N2];
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[xslot<T, N>(t, j2)];
       Dft<T, N2, -1>::run(w);
       cplx<T>* zp = plane + (2 * p) * NH;  // the pair's two rows hold Z[0..N-1] for now
 #pragma unroll
@@ -147,7 +163,8 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
   for (int c0 = 0; c0 < NP; c0 += LPR) {
     const int col = c0 + l;
     const bool act = col < NP;
-    cplx<T>* xl = xch + l * C::XS;
+    cplx<T>* xl = C::XIN ? plane + col : xch + l * C::XS;
+    constexpr int xs = C::XIN ? NH : 1;  // slot stride: a column of the plane in place
     if (act && t < N2) {
       cplx<T> v[N1];
 #pragma unroll
@@ -157,13 +174,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_fast(const PrecArg
       }
       Dft<T, N1, -1>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+      for (int k1 = 0; k1 < N1; ++k1) xl[xslot<T, N>(k1, t) * xs] = cmulw(v[k1], tw[t * k1], false);
     }
     __syncthreads();
     if (act && t < N1) {
       cplx<T> w[N2];
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[xslot<T, N>(t, j2) * xs];
       Dft<T, N2, -1>::run(w);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
@@ -210,7 +227,8 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
   for (int c0 = 0; c0 < NP; c0 += LPR) {
     const int col = c0 + l;
     const bool act = col < NP;
-    cplx<T>* xl = xch + l * C::XS;
+    cplx<T>* xl = C::XIN ? plane + col : xch + l * C::XS;
+    constexpr int xs = C::XIN ? NH : 1;  // slot stride: a column of the plane in place
     if (act && t < N2) {
       cplx<T> v[N1];
 #pragma unroll
@@ -225,13 +243,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
       }
       Dft<T, N1, 1>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+      for (int k1 = 0; k1 < N1; ++k1) xl[xslot<T, N>(k1, t) * xs] = cmulw(v[k1], tw[t * k1], true);
     }
     __syncthreads();
     if (act && t < N1) {
       cplx<T> w[N2];
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[xslot<T, N>(t, j2) * xs];
       Dft<T, N2, 1>::run(w);
 #pragma unroll
       for (int k2 = 0; k2 < N2; ++k2) {
@@ -254,7 +272,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
   for (int p0 = 0; p0 < NP; p0 += LPR) {
     const int p = p0 + l;
     const bool act = p < NP;
-    cplx<T>* xl = xch + l * C::XS;
+    cplx<T>* xl = C::XIN ? plane + (2 * p) * NH : xch + l * C::XS;
     if (act && t < N2) {
       cplx<T> v[N1];
       const cplx<T>* ra = plane + (2 * p) * NH;
@@ -276,13 +294,13 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_fast(const PrecArg
       }
       Dft<T, N1, 1>::run(v);
 #pragma unroll
-      for (int k1 = 0; k1 < N1; ++k1) xl[k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+      for (int k1 = 0; k1 < N1; ++k1) xl[xslot<T, N>(k1, t)] = cmulw(v[k1], tw[t * k1], true);
     }
     __syncthreads();
     if (act && t < N1) {
       cplx<T> w[N2];
 #pragma unroll
-      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[t * (N2 + 1) + j2];
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[xslot<T, N>(t, j2)];
       Dft<T, N2, 1>::run(w);
       const int64_t r0 = fo + (int64_t)(2 * p) * N, r1 = r0 + N;
 #pragma unroll
@@ -324,7 +342,7 @@ struct Vec16<double> {
 template <typename T, int N>
 struct StageCfg {
   static constexpr int RS = N + (sizeof(T) == 4 ? 4 : 2);
-  static constexpr bool fits = FastCfg<T, N>::LPR >= N / 2 &&
+  static constexpr bool fits = !FastCfg<T, N>::XIN && FastCfg<T, N>::LPR >= N / 2 &&
                                (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
 };
 
@@ -938,7 +956,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecAr
 template <typename T, int N>
 size_t fast_lds_plane() {
   using C = FastCfg<T, N>;
-  return ((size_t)N + (size_t)N * C::NH + (size_t)C::LPR * C::XS + N) * sizeof(cplx<T>);
+  return ((size_t)N + (size_t)N * C::NH + (C::XIN ? 0 : (size_t)C::LPR * C::XS) + N) * sizeof(cplx<T>);
 }
 template <typename T, int N>
 size_t fast_lds_line() {
