@@ -73,6 +73,8 @@ def parse():
                     help="host processes generating the cells (0: in this process, e.g. under a profiler)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="skip the timed solve: only the matvec roofline launches (for counter collection)")
+    ap.add_argument("--precond-only", action="store_true",
+                    help="skip the solve: time the preconditioner (FP64 and FP32, 6 RHS) on the workload's cells")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (HBM-counter calibration)")
     return ap.parse_args()
@@ -155,6 +157,9 @@ def main():
               for i, sh in enumerate(dict.fromkeys(shapes))}
     rho_d = bh.pack(rhos)
 
+    if args.precond_only:
+        print(json.dumps({"precond": precond_timing(pkg, bh, shapes)}), flush=True)
+        return
     if args.kernels_only:
         for prec in ("mixed", "fp64"):
             print(json.dumps({"roofline_" + prec: matvec_roofline(pkg, bh, rho_d, prec)}), flush=True)
@@ -327,6 +332,33 @@ def matvec_roofline(pkg, bh, rho_d, precision):
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": f"{kname} (K(s) u, 6 RHS, {bh.tot} voxels = {len(bh.shapes)} cells)",
             "algorithmic_bytes": bytes_mv, "avg_us": round(t_mv * 1e6, 2)}
+
+
+def precond_timing(pkg, bh, shapes, reps=5):
+    """nlfh_precond on all cells of the plan, 6 right-hand sides, FP64 then FP32, random residuals: HIP events over
+    `reps` applications after a warm-up; algorithmic bytes N 2 3 m w + Nh 6 w per distinct shape."""
+    import torch
+    lib = pkg.load_library()
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    out = {}
+    for dt, tdt, w in ((0, torch.float64, 8), (1, torch.float32, 4)):
+        r = torch.randn(bh.tot * 18, dtype=tdt, device="cuda")
+        z = torch.empty_like(r)
+        args = (bh._plan.handle, ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(z.data_ptr()), 6, dt, sp)
+        pkg._lib.check(lib.nlfh_precond(*args))
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            pkg._lib.check(lib.nlfh_precond(*args))
+        e1.record(st)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1) / reps * 1e-3
+        b = bh.tot * 2 * 3 * 6 * w + sum(sh[0] * sh[1] * (sh[2] // 2 + 1) * 6 * w for sh in set(shapes))
+        out["fp64" if dt == 0 else "fp32"] = {"ms": round(t * 1e3, 3), "algorithmic_bytes": b,
+                                              "GBs": round(b / t / 1e9, 1)}
+        del r, z
+    return out
 
 
 def fiber_bench(args):
