@@ -440,8 +440,8 @@ hipError_t launch_f2d(const float* in, double* out, int64_t n, hipStream_t st) {
 // diagonal of K(s), which is the same for the three components since every diagonal entry of Ke is kdiag
 __global__ void jacobi_dinv_kernel(const CellDesc* __restrict__ cells, int ncells, const double* __restrict__ s,
                                    double kdiag, float* __restrict__ dinv, int64_t tot) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= tot) return;
+  // grid-stride: the launch caps its grid at 65536 blocks
+  for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < tot; v += (int64_t)gridDim.x * blockDim.x) {
   int lo = 0, hi = ncells - 1;  // the cell holding voxel v: the last with off_vox <= v
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -461,6 +461,7 @@ __global__ void jacobi_dinv_kernel(const CellDesc* __restrict__ cells, int ncell
     t += sc[((int64_t)ii * cd.ny + jj) * cd.nz + kk];
   }
   dinv[v] = (float)(1.0 / (kdiag * t));
+  }
 }
 
 hipError_t launch_jacobi_dinv(const CellDesc* cells, int ncells, const double* s, double kdiag, float* dinv,

@@ -298,6 +298,39 @@ def test_cheb_smooth_batched_cells_match_single(gpu_pkg):
         assert float((xi - views_out[i]).abs().max()) <= 1e-5 * float(xi.abs().max())
 
 
+def test_cheb_first_step_beyond_2pow24_voxels(gpu_pkg):
+    """Regression: on a plan of more voxels than one launch of 65536 blocks x 256 threads covers (65 cells of
+    64^3, 17.0 M voxels) the Jacobi diagonal reaches every cell: one Chebyshev step from zero is
+    x_1 = -D^-1 f / theta in every cell (checked on the first and the last three cells against D from rolled sums
+    of s and the plan's load)."""
+    import ctypes
+    from nlfh_amd.linear import _es_host
+    n, nc = 64, 65
+    bh = gpu_pkg.BatchedHomogenization([(n, n, n)] * nc, emin=1e-3, streams=1)
+    assert bh.tot > 65536 * 256
+    g = torch.Generator(device="cuda").manual_seed(5)
+    rd = torch.rand(bh.tot, dtype=torch.float64, device="cuda", generator=g)
+    s = bh.stiffness_scale(rd)
+    lib = gpu_pkg.load_library()
+    Es_np, Es_p = _es_host(np.eye(6))
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    hi = float(lib.nlfh_jacobi_bound(1.0, 0.3))
+    lo = hi / 900.0
+    z = torch.zeros(bh.tot * 18, dtype=torch.float64, device="cuda")
+    f = torch.empty_like(z)
+    x1 = torch.empty_like(z)
+    gpu_pkg._lib.check(lib.nlfh_load(bh._plan.handle, P(s), Es_p, 6, P(f), st))
+    gpu_pkg._lib.check(lib.nlfh_cheb_smooth(bh._plan.handle, P(s), Es_p, 6, P(z), P(x1), 1, lo, hi, None, st))
+    lam, mu = 0.3 / (1.3 * 0.4), 1.0 / 2.6
+    fv, xv = bh.field_views(f), bh.field_views(x1)
+    for i in (0, nc - 3, nc - 2, nc - 1):
+        sc = s[bh.off[i]:bh.off[i + 1]].view(n, n, n)
+        D = (lam + 4 * mu) / 9 * sum(torch.roll(sc, (a, b, c), (0, 1, 2)) for a in (0, 1) for b in (0, 1) for c in (0, 1))
+        want = -fv[i] / D[None, ..., None] / (0.5 * (hi + lo))
+        assert float((xv[i] - want).abs().max() / want.abs().max()) < 1e-5, i
+
+
 def test_trained_network_on_heldout_cells(gpu_pkg):
     """The packaged network (trained on 5000 solver-generated 64^3 cells) on held-out cells: its prediction is
     a real approximation of the correctors (relative L2 error well below 1), the learned start (prediction,
