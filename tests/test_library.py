@@ -46,3 +46,22 @@ def test_jacobi_bound_is_the_element_bound(pkg):
     for nu in (0.0, 0.2, 0.3, 0.45):
         assert abs(lib.nlfh_jacobi_bound(2.0, nu) - O.jacobi_bound(2.0, nu)) < 1e-11 * O.jacobi_bound(2.0, nu)
     assert lib.nlfh_jacobi_bound(1.0, 0.5) == -1.0 and lib.nlfh_jacobi_bound(-1.0, 0.3) == -1.0
+
+
+def test_sanitized_host_abi_checks(tmp_path):
+    """The C ABI built with AddressSanitizer and UndefinedBehaviorSanitizer on its host code
+    (tests/sanitize/build.sh) runs the host check program tests/sanitize/abi_errors.c clean: argument refusals
+    of every entry-point family and the host arithmetic (the Jacobi spectral bound, workspace sizes), no device
+    calls, no sanitizer report, leak checking on."""
+    import shutil
+    import subprocess
+    import pytest
+    if shutil.which("hipcc") is None and not os.path.exists("/opt/rocm/bin/hipcc"):
+        pytest.skip("hipcc is not available")
+    b = subprocess.run(["bash", os.path.join(ROOT, "tests", "sanitize", "build.sh"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=900)
+    assert b.returncode == 0, b.stderr[-2000:]
+    r = subprocess.run([str(tmp_path / "abi_errors")], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-4000:]
+    assert "all checks passed" in r.stdout
+    assert "Sanitizer" not in r.stderr and "runtime error" not in r.stderr, r.stderr[-4000:]
