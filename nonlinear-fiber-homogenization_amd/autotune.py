@@ -39,7 +39,7 @@ def candidates(shape, elem_bytes=8):
         for ty in TY_CHOICES:
             if ty > ny or lzs * (ty + 2) > 512:
                 continue
-            if 4 * (ty + 2) * 6 * nz * elem_bytes > 159 * 1024:
+            if 3 * (ty + 2) * 6 * nz * elem_bytes > 159 * 1024:
                 continue
             for xc in XC_CHOICES:
                 if xc > nx:

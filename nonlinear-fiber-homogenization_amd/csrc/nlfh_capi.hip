@@ -383,8 +383,8 @@ void default_tiling(Shape& sh) {
   int lz = sh.nz / zpt, lzs = 1;
   while (lzs < lz) lzs <<= 1;
   int rows = std::max(3, 512 / lzs);
-  // four LDS buffers of (ty + 2) rows x 6 values x nz (float64) must fit
-  const int lds_rows = (int)(kLdsMax / ((size_t)4 * 6 * sh.nz * sizeof(double)));
+  // three LDS buffers of (ty + 2) rows x 6 values x nz (float64) must fit
+  const int lds_rows = (int)(kLdsMax / ((size_t)3 * 6 * sh.nz * sizeof(double)));
   rows = std::min(rows, lds_rows);
   // small tiles keep every CU busy: the autotuner's measurements on gfx950 pick ty = 2-4 and xc = 2 (nx <= 32),
   // 8 (nx <= 64) or 16 (larger) for cubic cells from 16^3 to 128^3
@@ -1168,7 +1168,7 @@ int nlfh_plan_set_tiling(nlfh_plan* p, int shape_index, int ty, int xc, int zpt)
   while (lzs < lz) lzs <<= 1;
   CHECK_ARG(ty >= 1 && lzs * (std::min(ty, sh.ny) + 2) <= 512, "ty too large for a 512-thread block");
   CHECK_ARG(xc >= 1, "xc must be positive");
-  const size_t lds = (size_t)4 * (std::min(ty, sh.ny) + 2) * 6 * sh.nz * sizeof(double);
+  const size_t lds = (size_t)3 * (std::min(ty, sh.ny) + 2) * 6 * sh.nz * sizeof(double);
   CHECK_ARG(lds <= kLdsMax, "tile exceeds LDS");
   sh.ty = std::min(ty, sh.ny);
   sh.xc = std::min(xc, sh.nx);
