@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--tiling", default=None, help="TY,XC,ZPT matvec tiling for every shape (skips the autotuner)")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 PCG comparison")
     ap.add_argument("--streams", type=int, default=2, help="stream groups of the batched solver")
+    ap.add_argument("--inner-restart", action="store_true",
+                    help="mixed precision: restart every FP32 inner PCG from p = z (default: carry the direction over)")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 re-solve of the timed cells (by default C is compared against it, outside "
                          "the timed region, and the run fails above 1e-6 of max|C|)")
@@ -144,7 +146,8 @@ def main():
     deg, ratio = (float(v) for v in args.smoothing.split(","))
     smoothing = (int(deg), ratio) if deg > 0 else None
     bh = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
-                                   warm_start=net, streams=args.streams, warm_smoothing=smoothing)
+                                   warm_start=net, streams=args.streams, warm_smoothing=smoothing,
+                                   inner_restart=args.inner_restart)
     t_tune = time.perf_counter()
     if args.tiling:
         ty, xc, zpt = (int(v) for v in args.tiling.split(","))
@@ -213,7 +216,7 @@ def main():
     if net is not None:
         # the same batched mixed solver from zero on the same cells (after the timed region), for the gain
         bz = pkg.BatchedHomogenization(shapes, emin=emin, tol=tol, maxit=40000, precision=args.precision,
-                                       streams=args.streams)
+                                       streams=args.streams, inner_restart=args.inner_restart)
         for i in range(len(set(shapes))):
             bz._plan.set_tiling(i, *bh._plan.get_tiling(i))
         bz.C_full(rho_d)
@@ -258,7 +261,7 @@ def main():
             "workload": f"{desc}, 6 unit strains, PCG to true rel. residual {tol:g}, emin {emin:g}, C 6x6, "
                         f"batched solver ({args.precision})" + (", learned warm start" if net else ""),
             "cells_per_step": ncell, "rhs": 6, "tol": tol, "precision": args.precision,
-            "streams": args.streams,
+            "streams": args.streams, "inner_restart": bool(args.inner_restart),
             "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
                        "ty_xc_zpt": tiling}, "tune_seconds": round(t_tune, 2),
             "learned_start": bool(net), "learned_start_fallback_steps": fallbacks,

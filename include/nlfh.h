@@ -81,6 +81,12 @@ int nlfh_cheb_smooth(nlfh_plan* plan, const double* s, const double* Es_host, in
  * No counterpart in the reference (which has no mixed precision). */
 int nlfh_plan_set_inner_tol(nlfh_plan* plan, double inner_tol);
 
+/* restart = 1: every FP32 inner solve of NLFH_PREC_MIXED restarts its PCG from p = z. restart = 0 (default):
+ * plans whose cells all take the FP32 direction iteration (cubic 16-128 cells) carry the last search direction
+ * of an inner solve over into the next, after the refinement step has replaced the recursive residual by the
+ * true one; other plans always restart. No counterpart in the reference (which has no mixed precision). */
+int nlfh_plan_set_inner_restart(nlfh_plan* plan, int restart);
+
 /* s = emin + (1 - emin) clip01(rho)^p over all cells' voxels (s nullable), and optionally
  * dsdrho = p (1 - emin) clip01(rho)^(p-1) inside [0, 1] (bounds included) and 0 outside, the
  * derivative of the reference's clip01 (nlfiber/linear.py:27-30).

@@ -24,10 +24,11 @@ from .linear import Plan, _device, _es_host, _ptr, _stream, check_grid
 
 class BatchedHomogenization:
     """C_full for a list of cells. `shapes`: list of (nx, ny, nz); `m` right-hand sides per cell
-    (the unit strains 0..m-1). precision: "fp64" or "mixed"."""
+    (the unit strains 0..m-1). precision: "fp64" or "mixed"; inner_restart=True restarts each FP32 inner
+    PCG of the mixed solver from p = z instead of carrying the search direction over."""
 
     def __init__(self, shapes, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-8, maxit=4000, precision="mixed", m=6,
-                 tune=False, warm_start=None, streams=2, warm_smoothing=(128, 900.0)):
+                 tune=False, warm_start=None, streams=2, warm_smoothing=(128, 900.0), inner_restart=False):
         self.shapes = [tuple(int(v) for v in s) for s in shapes]
         self.E, self.nu, self.emin, self.p, self.tol, self.maxit = E, nu, emin, p, tol, maxit
         if precision not in ("fp64", "mixed"):
@@ -37,7 +38,9 @@ class BatchedHomogenization:
         for sh in dict.fromkeys(self.shapes):
             check_grid(sh)
         self._dev = _device()
+        self.inner_restart = bool(inner_restart)
         self._plan = Plan(self.shapes, E, nu, m_layout=m)
+        self._plan.set_inner_restart(self.inner_restart)
         self._lib = _lib.load()
         self.nvox = [int(np.prod(s)) for s in self.shapes]
         self.off = np.concatenate([[0], np.cumsum(self.nvox)]).astype(np.int64)
@@ -73,7 +76,9 @@ class BatchedHomogenization:
             for g in self._groups:
                 shapes = [self.shapes[i] for i in g]
                 off = np.concatenate([[0], np.cumsum([self.nvox[i] for i in g])]).astype(np.int64)
-                self._subs.append((Plan(shapes, self.E, self.nu, m_layout=self.m), torch.cuda.Stream(), g, off))
+                sub = Plan(shapes, self.E, self.nu, m_layout=self.m)
+                sub.set_inner_restart(self.inner_restart)
+                self._subs.append((sub, torch.cuda.Stream(), g, off))
         distinct = list(dict.fromkeys(self.shapes))
         for plan, _, g, _ in self._subs:
             for j, sh in enumerate(dict.fromkeys(self.shapes[i] for i in g)):

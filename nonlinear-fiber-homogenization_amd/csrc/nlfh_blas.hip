@@ -158,9 +158,12 @@ __global__ void sub_mean_kernel(const VecArgs va, T* __restrict__ v, const doubl
   }
 }
 
-// Iterative refinement: r32 = r64 / scale ; d32 = 0   (scale per (cell, rhs))
+// Iterative refinement: r32 = r64 / scale ; d32 = 0   (scale per (cell, rhs)). With pdir, the search
+// direction an inner solve left behind (held as scale_prev * p_K by ir_accumulate_kernel) is divided by the
+// new scale too, which puts it in the units of the next inner system K d = r / ||r||.
 __global__ void ir_scale_down_kernel(const VecArgs va, const double* __restrict__ r64, float* __restrict__ r32,
-                                     float* __restrict__ d32, const double* __restrict__ scale) {
+                                     float* __restrict__ d32, const double* __restrict__ scale,
+                                     float* __restrict__ pdir) {
   const ChunkItem ch = va.chunks[blockIdx.x];
   if (va.done != nullptr && va.done[ch.cell]) return;
   const CellDesc cd = va.cells[ch.cell];
@@ -173,22 +176,48 @@ __global__ void ir_scale_down_kernel(const VecArgs va, const double* __restrict_
       const int64_t o = base + c * cd.nvox + i;
       r32[o] = (float)(r64[o] * inv);
       d32[o] = 0.0f;
+      if (pdir != nullptr) pdir[o] = (float)((double)pdir[o] * inv);
     }
 }
 
-// Iterative refinement: x64 += scale * d32
+// dst = src for the cells with flag[cell] != 0 (m fields of the plan's layout)
+template <typename T>
+__global__ void __launch_bounds__(256) copy_flagged_kernel(const VecArgs va, const int* __restrict__ flag,
+                                                           const T* __restrict__ src, T* __restrict__ dst) {
+  const ChunkItem ch = va.chunks[blockIdx.x];
+  if (!flag[ch.cell]) return;
+  const CellDesc cd = va.cells[ch.cell];
+  const int64_t base = cd.off_field + (int64_t)blockIdx.y * 3 * cd.nvox;
+  for (int c = 0; c < 3; ++c)
+    for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) dst[base + c * cd.nvox + i] = src[base + c * cd.nvox + i];
+}
+
+template <typename T>
+hipError_t launch_copy_flagged(const VecArgs& va, const int* flag, const T* src, T* dst, hipStream_t st) {
+  hipLaunchKernelGGL((copy_flagged_kernel<T>), dim3(va.nchunks, va.m), dim3(256), 0, st, va, flag, src, dst);
+  return hipGetLastError();
+}
+template hipError_t launch_copy_flagged<float>(const VecArgs&, const int*, const float*, float*, hipStream_t);
+template hipError_t launch_copy_flagged<double>(const VecArgs&, const int*, const double*, double*, hipStream_t);
+
+// Iterative refinement: x64 += scale * d32. With p0/p1, the last search direction of the inner solve,
+// p_K in P[K & 1] (K = it[cell], iteration k reads its direction from P[k & 1]), is kept in P[0] as
+// scale * p_K, its value in the units of the FP64 residual.
 __global__ void ir_accumulate_kernel(const VecArgs va, double* __restrict__ x64, const float* __restrict__ d32,
-                                     const double* __restrict__ scale) {
+                                     const double* __restrict__ scale, float* p0, const float* p1,
+                                     const int* __restrict__ it) {
   const ChunkItem ch = va.chunks[blockIdx.x];
   if (va.done != nullptr && va.done[ch.cell]) return;
   const CellDesc cd = va.cells[ch.cell];
   const int rhs = blockIdx.y;
   const double sc = scale[ch.cell * va.m + rhs];
   const int64_t base = cd.off_field + (int64_t)rhs * 3 * cd.nvox;
+  const float* src = (p0 != nullptr && (it[ch.cell] & 1)) ? p1 : p0;
   for (int c = 0; c < 3; ++c)
     for (int64_t i = ch.begin + threadIdx.x; i < ch.end; i += blockDim.x) {
       const int64_t o = base + c * cd.nvox + i;
       x64[o] += sc * (double)d32[o];
+      if (p0 != nullptr) p0[o] = (float)(sc * (double)src[o]);
     }
 }
 
@@ -559,13 +588,15 @@ hipError_t launch_sub_mean(const VecArgs& va, T* v, const double* mean, hipStrea
 template hipError_t launch_sub_mean<double>(const VecArgs&, double*, const double*, hipStream_t);
 
 hipError_t launch_ir_down(const VecArgs& va, const double* r64, float* r32, float* d32, const double* scale,
-                          hipStream_t st) {
-  hipLaunchKernelGGL(ir_scale_down_kernel, dim3(va.nchunks, va.m), dim3(256), 0, st, va, r64, r32, d32, scale);
+                          float* pdir, hipStream_t st) {
+  hipLaunchKernelGGL(ir_scale_down_kernel, dim3(va.nchunks, va.m), dim3(256), 0, st, va, r64, r32, d32, scale, pdir);
   return hipGetLastError();
 }
 
-hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, hipStream_t st) {
-  hipLaunchKernelGGL(ir_accumulate_kernel, dim3(va.nchunks, va.m), dim3(256), 0, st, va, x64, d32, scale);
+hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, float* p0,
+                         const float* p1, const int* it, hipStream_t st) {
+  hipLaunchKernelGGL(ir_accumulate_kernel, dim3(va.nchunks, va.m), dim3(256), 0, st, va, x64, d32, scale, p0, p1,
+                     it);
   return hipGetLastError();
 }
 

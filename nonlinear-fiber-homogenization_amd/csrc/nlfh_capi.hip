@@ -210,8 +210,10 @@ struct nlfh_plan {
   double* d_ir = nullptr;      // rn_true (ncells*M), bn (ncells*M), mean (ncells*M*3)
   int* d_outer_done = nullptr;
   int* d_maxed = nullptr;      // per cell: iteration budget exhausted
+  int* d_restart = nullptr;    // per cell: the next FP32 inner solve restarts from p = z
   double inner_tol = 1e-2;     // loosest relative tolerance of an FP32 inner solve
-  int* h_pinned = nullptr;     // ncells * 2 ints
+  bool inner_restart = false;  // restart every FP32 inner PCG from z_0 instead of carrying the direction over
+  int* h_pinned = nullptr;     // ncells * 3 ints
   double* h_pinned_d = nullptr;
   std::vector<void*> allocs;
   int64_t bytes = 0;
@@ -750,8 +752,13 @@ int get_graph(nlfh_plan* p, Work<T>& w, int m, hipGraphExec_t* out) {
 
 // PCG on the mean-free space from the state prepared in w (R = b - K X, B holds b for ||b||):
 // nlfiber/linear.py:173-203. Returns after every cell is done; iterations per cell in it_host.
+// resume (FP32 direction iteration only): P[0] already holds the first search direction (the previous inner
+// solve's last one, in this system's units), so the start computes r-hat_0 and r_0 . z_0 and stores z_0 aside
+// instead of taking p_0 = z_0.
+// restart_cells (device, nullable, with resume): cells flagged there take p_0 = z_0 after all.
 template <typename T>
-int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use_graph, int maxit) {
+int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use_graph, int maxit,
+           bool resume = false, const int* restart_cells = nullptr) {
   hipStream_t st = p->stream;
   w.cs.m = m;  // per-(cell, rhs) scalars use stride m
   VecArgs va = vec_args(p, m, nullptr);
@@ -766,10 +773,16 @@ int run_cg(nlfh_plan* p, Work<T>& w, int m, const T* bvec, int* it_host, int use
   const bool ln = lean<T>(p);
   const bool dir = ln && p->dir32;
   if (dir) {
-    a.z = w.P[0];  // p_0 = z_0
+    a.z = resume ? w.Z : w.P[0];  // p_0 = z_0 unless a direction is carried over
     a.rspec = w.rspec;  // r-hat_0
   }
+  if (resume && !dir) {
+    set_error("internal: only the FP32 direction iteration carries search directions over");
+    return NLFH_EINVAL;
+  }
   TRY(run_prec<T>(p, a, 0, ln ? 2 : 1, st));
+  if (resume && restart_cells != nullptr)
+    HIP_TRY(launch_copy_flagged<T>(vec_args(p, m, nullptr), restart_cells, w.Z, w.P[0], st));
   if (ln) {
     HIP_TRY(launch_cg_init(w.cs, p->ncells, p->d_part_a, (int)p->planes.size(), p->d_plane_range, p->d_part_bdot,
                            (int)p->lines32.size(), p->d_line_range32, p->d_part_b, (int)p->chunks.size(),
@@ -1127,11 +1140,12 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->alloc(&p->d_ir, (size_t)ncells * M * 5))) return fail(rc);
   if ((rc = p->alloc(&p->d_outer_done, ncells))) return fail(rc);
   if ((rc = p->alloc(&p->d_maxed, ncells))) return fail(rc);
-  if (hipHostMalloc((void**)&p->h_pinned, sizeof(int) * 2 * ncells, hipHostMallocDefault) != hipSuccess) {
+  if ((rc = p->alloc(&p->d_restart, ncells))) return fail(rc);
+  if (hipHostMalloc((void**)&p->h_pinned, sizeof(int) * 3 * ncells, hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     return fail(NLFH_ENOMEM);
   }
-  if (hipHostMalloc((void**)&p->h_pinned_d, sizeof(double) * ncells * M * 2, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&p->h_pinned_d, sizeof(double) * ncells * M * 4, hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     return fail(NLFH_ENOMEM);
   }
@@ -1318,6 +1332,12 @@ int solve_fp64(nlfh_plan* p, int m, const double* u0, double tol, int maxit, int
 // FP32 PCG steps inside FP64 iterative refinement: x += K^-1 r solved in FP32 from the scaled true
 // residual r = b - K x (FP64), until ||b - K x|| <= tol ||b|| per right-hand side (true FP64
 // residual). Scalars of the inner PCG (dots, alpha, beta) are FP64.
+// With the FP32 direction iteration the inner solves do not restart: each refinement step replaces the
+// recursive residual by the true one and the next inner PCG continues along the last search direction of the
+// previous one (rescaled to the new right-hand side r / ||r||; its first alpha is r.z / p.Ap with the new
+// residual). A restart from p = z would discard the Krylov information gathered so far and cost a few
+// percent more iterations per refinement step (residual replacement: van der Vorst and Ye, SIAM J. Sci.
+// Comput. 22 (2000); reliable updates in mixed-precision CG: Clark et al., Comput. Phys. Commun. 181 (2010)).
 int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, int* it_out, int* conv_out,
                 double* relres_out) {
   hipStream_t st = p->stream;
@@ -1346,6 +1366,10 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
   v.cs.skip = p->d_outer_done;
   std::vector<int> total(nc, 0), its(nc, 0), stall(nc, 0);
   std::vector<double> rn_prev((size_t)nk, 0.0);
+  const bool keep = p->dir32 && !p->inner_restart;
+  // per cell: the last inner solve continued a carried direction; the next one restarts
+  std::vector<int> resumed(nc, 0), restart_next(nc, 0);
+  const double* h_rin = p->h_pinned_d + 2 * nk;  // last inner solve: recursive ||r||, then ||b|| (= ||r / ||r||||)
   int* h = p->h_pinned;
   for (int outer = 0; outer < 200; ++outer) {
     // true residual r = b - K x = -(K x + load), its norms and the outer convergence flags
@@ -1356,17 +1380,35 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
                          p->d_outer_done, p->d_maxed, p->d_params + nc, p->inner_tol, 0.5, st));
     HIP_TRY(hipMemcpyAsync(h, p->d_outer_done, nc * sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipMemcpyAsync(p->h_pinned_d, rn_true, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+    if (keep && outer > 0) {
+      HIP_TRY(hipMemcpyAsync(p->h_pinned_d + 2 * nk, v.cs.rn, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipMemcpyAsync(p->h_pinned_d + 3 * nk, v.cs.bn, nk * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipStreamSynchronize(st));
     std::vector<int> odone(h, h + nc);  // h is reused by run_cg
     // stall: a refinement step that did not reduce some right-hand side's true residual by 10%
-    // (FP32 inner solves that stopped helping) ends the cell unconverged
+    // (FP32 inner solves that stopped helping) ends the cell unconverged; after an inner solve that continued a
+    // carried direction the cell first gets one more step, restarted from p = z
+    // drift: a true residual above 1.25x what the inner recursion reached (||r_prev|| times its recursive
+    // relative residual) and above twice the inner tolerance's reduction means the FP32 recursion has drifted
+    // from the true residual; the carried direction then belongs to the drifted problem and the next inner
+    // solve restarts (right-hand sides the recursion took far below the tolerance may sit at their FP32 floor)
     bool stall_any = false;
     for (int c = 0; c < nc; ++c) {
-      bool stalled = false;
+      bool stalled = false, drift = false;
       for (int t = 0; t < m; ++t) {
         const double rn = p->h_pinned_d[c * m + t];
         if (outer > 0 && !odone[c] && !(rn < 0.9 * rn_prev[c * m + t])) stalled = true;
+        if (keep && outer > 0 && !odone[c]) {
+          const double pred = rn_prev[c * m + t] * h_rin[c * m + t] / std::max(h_rin[nk + c * m + t], 1e-300);
+          if (!(rn <= std::max(1.25 * pred, 2.0 * p->inner_tol * rn_prev[c * m + t]))) drift = true;
+        }
         rn_prev[c * m + t] = rn;
+      }
+      restart_next[c] = drift ? 1 : 0;
+      if (stalled && !odone[c] && resumed[c]) {
+        stalled = false;
+        restart_next[c] = 1;
       }
       if (stalled && !odone[c]) {
         stall[c] = 1;
@@ -1387,10 +1429,24 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
     }
     if (all) break;
     progress("refinement step %d: %d of %d cells done", outer, ndone, nc);
-    // inner FP32 solve of K d = r / ||r|| from d = 0
-    HIP_TRY(launch_ir_down(vec_args(p, m, p->d_outer_done), w.R, v.R, v.X, rn_true, st));
-    TRY(run_cg<float>(p, v, m, v.R, its.data(), 1, maxit));
-    HIP_TRY(launch_ir_acc(vec_args(p, m, p->d_outer_done), w.X, v.X, rn_true, st));
+    // inner FP32 solve of K d = r / ||r|| from d = 0 (continuing the previous direction after the first step)
+    bool resume = false, any_restart = false;
+    for (int c = 0; c < nc; ++c) {
+      resumed[c] = keep && outer > 0 && !odone[c] && !restart_next[c];
+      resume = resume || resumed[c];
+      any_restart = any_restart || (!odone[c] && !resumed[c]);
+    }
+    if (resume && any_restart) {
+      int* hr = p->h_pinned + 2 * nc;  // pinned: the copy may run after the host moves on
+      for (int c = 0; c < nc; ++c) hr[c] = resumed[c] ? 0 : 1;
+      HIP_TRY(hipMemcpyAsync(p->d_restart, hr, nc * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    progress("refinement step %d: %d inner solves continue their direction", outer,
+             (int)std::count(resumed.begin(), resumed.end(), 1));
+    HIP_TRY(launch_ir_down(vec_args(p, m, p->d_outer_done), w.R, v.R, v.X, rn_true, resume ? v.P[0] : nullptr, st));
+    TRY(run_cg<float>(p, v, m, v.R, its.data(), 1, maxit, resume, any_restart ? p->d_restart : nullptr));
+    HIP_TRY(launch_ir_acc(vec_args(p, m, p->d_outer_done), w.X, v.X, rn_true, keep ? v.P[0] : nullptr, v.P[1],
+                          v.cs.it, st));
     int* maxed = p->h_pinned + nc;  // pinned: the copy below may run after this loop iteration
     bool any_max = false;
     for (int c = 0; c < nc; ++c) {
@@ -1621,6 +1677,13 @@ int nlfh_plan_set_inner_tol(nlfh_plan* p, double inner_tol) {
   CHECK_ARG(p != nullptr, "plan is null");
   CHECK_ARG(inner_tol > 0.0 && inner_tol < 1.0, "inner_tol must be in (0, 1)");
   p->inner_tol = inner_tol;
+  return NLFH_OK;
+}
+
+int nlfh_plan_set_inner_restart(nlfh_plan* p, int restart) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(restart == 0 || restart == 1, "restart must be 0 or 1");
+  p->inner_restart = restart != 0;
   return NLFH_OK;
 }
 

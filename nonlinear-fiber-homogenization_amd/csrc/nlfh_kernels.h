@@ -168,9 +168,12 @@ hipError_t launch_sum_partials(const double* part, int nitems, const int* range,
                                const CellDesc* cells, int out_stride, double* out, hipStream_t st);
 template <typename T>
 hipError_t launch_sub_mean(const VecArgs& va, T* v, const double* mean, hipStream_t st);
+template <typename T>
+hipError_t launch_copy_flagged(const VecArgs& va, const int* flag, const T* src, T* dst, hipStream_t st);
 hipError_t launch_ir_down(const VecArgs& va, const double* r64, float* r32, float* d32, const double* scale,
-                          hipStream_t st);
-hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, hipStream_t st);
+                          float* pdir, hipStream_t st);
+hipError_t launch_ir_acc(const VecArgs& va, double* x64, const float* d32, const double* scale, float* p0,
+                         const float* p1, const int* it, hipStream_t st);
 hipError_t launch_cg_init(const CgState& cs, int ncells, const double* pa, int np, const int* pr, const double* pz,
                           int nz, const int* zr, const double* pb, int nch, const int* cr, hipStream_t st);
 hipError_t launch_cg_alpha(const CgState& cs, int ncells, const double* pmv, int nmv, const int* mr, hipStream_t st);

@@ -190,6 +190,11 @@ class Plan:
     def set_tiling(self, shape_index, ty, xc, zpt):
         _lib.check(self._lib.nlfh_plan_set_tiling(self.handle, shape_index, ty, xc, zpt))
 
+    def set_inner_restart(self, restart):
+        """Mixed precision: restart each FP32 inner PCG from p = z (True) or carry the last search direction
+        over the refinement steps (False, the default; cubic 16-128 cells only, other plans restart)."""
+        _lib.check(self._lib.nlfh_plan_set_inner_restart(self.handle, 1 if restart else 0))
+
     def get_tiling(self, shape_index):
         ty, xc, zpt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _lib.check(self._lib.nlfh_plan_get_tiling(self.handle, shape_index, ctypes.byref(ty), ctypes.byref(xc),
@@ -287,13 +292,14 @@ class LinearHomogenization:
     """Linear homogenization on a fixed grid. `shape` is (nx, ny, nz) or (n, n) for plane strain.
 
     Extra keywords (not in the reference): precision = "fp64" (the reference's PCG, default) or
-    "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule); warm_start = a
+    "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule; inner_restart=True
+    restarts each inner PCG from p = z instead of carrying the search direction over); warm_start = a
     `learned.WarmStartNet` (or the path of its weights) whose Ritz-projected prediction starts
     `C_full` when no u0 is given, with a plain zero-start solve as the fallback if that solve does
     not converge."""
 
     def __init__(self, shape, E=1.0, nu=0.3, emin=1e-3, p=3.0, dtype=torch.float64, tol=1e-9, maxit=4000,
-                 precision="fp64", warm_start=None, warm_smoothing=(128, 900.0)):
+                 precision="fp64", warm_start=None, warm_smoothing=(128, 900.0), inner_restart=False):
         shape = tuple(int(n) for n in shape)
         self.plane = len(shape) == 2
         self.shape = shape + (1,) if self.plane else shape
@@ -311,6 +317,7 @@ class LinearHomogenization:
         self.Ke = torch.as_tensor(sum(B[g].T @ D @ B[g] for g in range(8)) / 8.0, device=self._dev)
         self.G = torch.as_tensor(sum(D @ B[g] for g in range(8)) / 8.0, device=self._dev)
         self._plan = Plan([self.shape], E, nu)
+        self._plan.set_inner_restart(inner_restart)
         self._lib = _lib.load()
         self.last_info = {}
         self.correctors = lambda s, Es, u0: _CorrectorsFn.apply(_as_f64(s, self._dev), self, Es, u0)

@@ -56,6 +56,37 @@ def test_mixed_precision_true_residual_and_C(gpu_pkg):
     assert np.abs(np.asarray(C) - Cr).max() <= 1e-6 * np.abs(Cr).max()
 
 
+def test_carried_directions_match_fp64_and_save_iterations(gpu_pkg):
+    """Mixed precision with the search direction carried over the refinement steps (the default) against
+    inner solves restarted from p = z and against plain FP64 PCG: every cell converges to the true FP64
+    residual without a fallback, C agrees with the FP64 C to 1e-9 of its largest entry, and no cell needs
+    more inner iterations than with restarts (measured: 637/701/654/279 carried, 680/727/686/297 restarted,
+    585/691/634/272 FP64; the 32^3 cells need at most 5 % more than FP64 PCG)."""
+    cells = [gpu_pkg.cells.random_cell(n, seed) for n, seed in ((16, 21), (32, 22), (32, 23), (16, 24))]
+    rhos = [c[0] for c in cells]
+    shapes = [r.shape for r in rhos]
+    res = {}
+    for name, kw in (("carry", dict(precision="mixed")), ("restart", dict(precision="mixed", inner_restart=True)),
+                     ("fp64", dict(precision="fp64"))):
+        bh = gpu_pkg.BatchedHomogenization(shapes, emin=1e-4, tol=1e-8, streams=1, **kw)
+        C, _, its = bh.C_full(rhos)
+        assert all(bh.last_info["converged"]) and bh.last_info["fallback_fp64"] == [], name
+        assert np.asarray(bh.last_info["relres"]).max() <= 1e-8, name
+        res[name] = (np.asarray(C), np.asarray(its))
+        print(name, list(its), flush=True)
+    C64, it64 = res["fp64"]
+    for name in ("carry", "restart"):
+        C, _ = res[name]
+        for i in range(len(rhos)):
+            assert np.abs(C[i] - C64[i]).max() <= 1e-9 * np.abs(C64[i]).max(), (name, i)
+    it_c, it_r = res["carry"][1], res["restart"][1]
+    assert np.all(it_c <= it_r), (it_c, it_r)
+    assert it_c.sum() <= 0.97 * it_r.sum(), (it_c, it_r)
+    for i, r in enumerate(rhos):
+        if r.shape[0] == 32:
+            assert it_c[i] <= 1.05 * it64[i], (i, it_c[i], it64[i])
+
+
 def test_mixed_unconverged_cells_fall_back_to_fp64(gpu_pkg):
     """Cells the mixed-precision solve leaves unconverged are solved again by plain FP64 PCG from zero
     (iteration cap forced from the host): the fallback cell's correctors are exactly those of an FP64
