@@ -18,4 +18,4 @@ from .linear import (  # noqa: F401
 __version__ = "0.1.0"
 from .batch import BatchedHomogenization  # noqa: E402,F401
 from . import learned  # noqa: E402,F401
-from . import density, fiber, mech  # noqa: E402,F401
+from . import density, fiber, mech, topology  # noqa: E402,F401

@@ -320,6 +320,19 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     if (PUPDATE && owns && xc >= 1) store_dir(p0 + roff, un);
     x_stage(std::integral_constant<int, 0>{}, un);
   }
+  // Wave-uniform offsets of the planes the step loop touches, advanced by one plane per step with a single
+  // wrap instead of recomputed from the plane index (two wraps each way) three times per step: the x-plane
+  // prefetched for the next step, x0 + min(s + 2, xc); the stiffness layer, x0 + min(s, xc - 1); the node plane
+  // stored, x0 - 2 + s; CHEB: the plane of x_prev and 1 / diag, x0 - 1 + min(s, xc).
+  const unsigned cell_span = (unsigned)nx * (unsigned)plane;
+  unsigned off_ld = plane_off(x0 + (2 < xc ? 2 : xc));
+  unsigned off_sv = plane_off(x0);
+  unsigned off_st = plane_off(x0 - 2);
+  unsigned off_xq = plane_off(x0 - 1);
+  auto advance = [&](unsigned& o) {
+    o += (unsigned)plane;
+    o = o >= cell_span ? o - cell_span : o;
+  };
   // Drain the prologue's loads here (they were issued just after the ones x_stage waited for): the
   // step loop's entry then carries no outstanding load, and its steady-state waits cover only the
   // plane prefetched one step earlier instead of the whole queue at the top of every step.
@@ -334,13 +347,13 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     // a cache hit): x-plane x0 + s + 2 (consumed by part (c) of step s + 1) and the stiffness of
     // element layer L_{s+1} (part (b) of step s + 1)
     {
-      load_row(plane_off(x0 + (s + 2 < xc ? s + 2 : xc)), nxt_u[PS], nxt_p[PS]);
-      const unsigned so = plane_off(x0 + (s < xc - 1 ? s : xc - 1)) + roff_ld;
+      load_row(off_ld, nxt_u[PS], nxt_p[PS]);
+      const unsigned so = off_sv + roff_ld;
 #pragma unroll
       for (int q = 0; q < ZPT; ++q) sv[PS][q] = load_s(so + q);
       if constexpr (CHEB) {
         // node plane x0 - 1 + s, stored by part (a) of step s + 1 (the last steps repeat plane x0 + xc - 1)
-        const unsigned xo = plane_off(x0 - 1 + (s < xc + 1 ? s : xc)) + roff_ld;
+        const unsigned xo = off_xq + roff_ld;
 #pragma unroll
         for (int q = 0; q < ZPT; ++q) {
 #pragma unroll
@@ -352,7 +365,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
     // (a) finish layer L_{s-1}: node plane L_{s-1} = x0 - 2 + s is stored for s >= 2
     if (s >= 1 && owns) {
       const T* UYp = lds + 2 * buf_sz + lbase - rowstep;
-      const unsigned oo = plane_off(x0 - 2 + s) + roff;
+      const unsigned oo = off_st + roff;
       T dsum = T(0);  // float32: the step's p . Ap terms summed in float, added to the double total once
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -481,6 +494,10 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
       if (PUPDATE && owns && s + 1 < xc) store_dir(plane_off(x0 + s + 1) + roff, un);
       x_stage(std::integral_constant<int, PQ>{}, un);
     }
+    if (s + 2 < xc) advance(off_ld);
+    if (s < xc - 1) advance(off_sv);
+    advance(off_st);
+    if (CHEB && s < xc) advance(off_xq);
     __syncthreads();
     return false;
   };

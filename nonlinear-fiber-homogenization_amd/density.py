@@ -10,11 +10,13 @@ one block. `rho` is differentiable in torch autograd with the reference's implic
 Three-dimensional cells only (the reference's d = 2 path is not on the GPU).
 """
 import ctypes
+import itertools
 
 import numpy as np
 import torch
 
 from . import _lib
+from . import topology
 
 
 def _ptr(t):
@@ -96,6 +98,32 @@ class DensityMap:
         _lib.check(self._lib.nlfh_dmap_rho(self._h, _ptr(p), float(beta), _ptr(rho), ctypes.byref(eta), _stream()))
         return eta.value
 
+    def binary(self, phi, up=1, v=None, remove_floating=True):
+        """Binary cell of side up * 2 n_o (nlfiber/density.py:96-121): the smoothed field u (on the GPU), upsampled
+        periodically by `up` and symmetrised again, thresholded at its (1 - v) quantile; with remove_floating, the
+        pieces that do not span the cell are dropped and the kept solid grows back to volume v through the void
+        voxels touching it, highest u first (whole orbits at a time, since u is exactly symmetric), until it is
+        within one voxel of v. Host arrays (numpy bool)."""
+        v = self.v if v is None else float(v)
+        u = self.smooth(phi).cpu().numpy()
+        if up > 1:
+            u = exact_symmetrize_host(topology.upsample_periodic(u, up))
+        b = u > np.quantile(u, 1.0 - v)
+        if remove_floating:
+            for _ in range(500):
+                b = topology.remove_floating(b)
+                missing = v - b.mean()
+                if missing * b.size < 1 or not b.any():
+                    break
+                front = topology.dilate_periodic(b) & ~b
+                cand = np.sort(u[front])[::-1]
+                if cand.size == 0:
+                    break
+                k = min(max(int(round(missing * b.size)), 1), cand.size)
+                b = b | (front & (u >= cand[k - 1]))
+            b = topology.remove_floating(b)
+        return b
+
     def rho_vjp(self, phi, beta, rbar):
         """(d rho / d phi)^T rbar for rows rbar (k, n, n, n) -> (k, n_o, n_o, n_o)."""
         p = self._phi(phi)
@@ -108,3 +136,26 @@ class DensityMap:
             _lib.check(self._lib.nlfh_dmap_rho_vjp(self._h, _ptr(p), float(beta), _ptr(blk), blk.shape[0],
                                                    _ptr(out[i:i + 3]), _stream()))
         return out
+
+
+def exact_symmetrize_host(full):
+    """Cubic (Oh) symmetrisation of a full cell (2 n_o)^3 in which every orbit gets one value exactly
+    (nlfiber/symmetry.py:61-66): the 8 mirror images averaged axis by axis (x, then y, then z), the octant averaged
+    over the 6 axis permutations (accumulated in itertools order), each octant entry replaced by the entry at
+    its sorted index, and the octant mirrored back out. numpy float64; the GPU density map does the same for
+    its own cells."""
+    x = np.asarray(full, dtype=np.float64)
+    for ax in range(3):
+        h = x.shape[ax] // 2
+        lo = np.take(x, np.arange(h), axis=ax)
+        hi = np.flip(np.take(x, np.arange(h, 2 * h), axis=ax), axis=ax)
+        x = 0.5 * (lo + hi)
+    acc = 0
+    for perm in itertools.permutations(range(3)):
+        acc = acc + np.transpose(x, perm)
+    octant = acc / 6
+    srt = np.sort(np.indices(octant.shape).reshape(3, -1), axis=0)
+    octant = octant[srt[0], srt[1], srt[2]].reshape(octant.shape)
+    for ax in range(3):
+        octant = np.concatenate([octant, np.flip(octant, axis=ax)], axis=ax)
+    return octant

@@ -5,8 +5,8 @@ c = (C11, C12, C44) / E, with Jacobian rows through the density map and the corr
 `Fiber.value` and `Fiber.value_jac` keep the reference's signatures and warm start (the correctors of
 the last solve start the next one, nlfiber/fiber.py:41-56). The density map, the 2-RHS PCG, the
 per-element energies and every vector-Jacobian product run in libnlfh.so; the three Jacobian rows are
-one batched density-map VJP of the three rho-gradients of c. `binary_check` needs the reference's
-topology module (floating-piece removal on binary cells), which is not on the GPU path.
+one batched density-map VJP of the three rho-gradients of c. `binary_check` binarizes on the host
+(DensityMap.binary, periodic connectivity in `topology`) and homogenizes the binary cell on the GPU.
 """
 import itertools
 
@@ -66,8 +66,22 @@ class Fiber:
         return self._lh_bin[shape]
 
     def binary_check(self, phi, up=1, full=False):
-        raise NotImplementedError("binary_check needs the topology module (floating-piece removal), which is "
-                                  "outside the GPU path")
+        """Binarize phi (upsampled by `up`), drop floating pieces and homogenize the binary cell
+        (nlfiber/fiber.py:67-81). Returns dict(vol, cell, c) and, with full=True, C (6x6), c and the cubic
+        leakage from the full C."""
+        b = self.dm.binary(phi, up=up)
+        lh = self.lh_for(b.shape)
+        rho = torch.as_tensor(b, dtype=torch.float64, device=self.dm._dev)
+        out = dict(vol=float(b.mean()), cell=b)
+        if full:
+            C, _, _ = lh.C_full(rho)
+            C = np.asarray(C)
+            out.update(C=C, c=cubic_invariants(C) / lh.E, leak=cubic_leakage(C))
+        else:
+            c, _, _ = lh.invariants(rho)
+            out["c"] = np.asarray(c.detach().cpu() if isinstance(c, torch.Tensor) else c)
+        self.solves += 1
+        return out
 
 
 def rel_error(c, target):

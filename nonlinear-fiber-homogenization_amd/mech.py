@@ -22,8 +22,9 @@ Kept from the reference (nlfiber/mech.py):
     (mech.py:454-515).
 Differences of form, not of result: the tangent is applied matrix-free (`tangent` returns the state it is
 taken at instead of 81 numbers per Gauss point); z, w and Fbar are torch / numpy float64 arrays instead of
-JAX arrays; 3x3 results (the returned Fbar, mean_stress) and toec's C and T are numpy arrays; toec is not
-differentiable in rho (the reference's is, through chi).
+JAX arrays; 3x3 results (the returned Fbar, mean_stress) are numpy arrays; toec's C and T are numpy arrays, or
+torch tensors differentiable in chi and cell.rho when either requires grad (the reference's is differentiable
+in rho through chi and the cell's stiffness; cubic_toec_t is the traceable cubic_toec).
 """
 import ctypes
 import math
@@ -32,7 +33,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .linear import LinearHomogenization, VOIGT, lame
+from .linear import NODES, LinearHomogenization, VOIGT, dN_ref, lame
 
 STALL_RTOL = 1e-6
 
@@ -624,41 +625,121 @@ def rank_one_min(A, m=1000, plane=False):
 
 
 # --- third-order elastic constants (mech.py:454-515) -----------------------------------------------------------
+_PAIRS = [(I, J) for I in range(6) for J in range(I, 6)]
+_TRIPLES = [(I, J, K) for I in range(6) for J in range(I, 6) for K in range(J, 6)]
+
+
+def _toec_kernel(cell, chi_t):
+    """The 77 energy derivatives at F = I from `nlfh_fs_toec`: d2f_IJ over the pairs I <= J, then d3f_IJK over the
+    triples I <= J <= K, with the cell's own stiffness scale and interpolation factor."""
+    out = torch.empty(77, dtype=torch.float64, device=cell._dev)
+    _lib.check(cell._lib.nlfh_fs_toec(cell._shape_c, _ptr(cell._sg), cell._mat.ctypes.data_as(_dp), cell._flags,
+                                      _ptr(chi_t), _ptr(out), _ptr(cell._work), _stream()))
+    return out
+
+
+def toec_values_t(cell, chi, sw, gam):
+    """The kernel's 77 values as a torch function of chi (6 fields (N, 3)), sw = wq s_e and gamma_e (N each): per
+    Gauss point H_I = e_I + grad chi_I, d2f_IJ = sum_e sw_e sum_g D2[H_I, H_J] with the linear tangent
+    D2[a, b] = lam tr a tr b + mu (a : b + a : b^T) (the interpolation drops out at F = I), and, neo-Hookean only,
+    d3f_IJK = sum_e sw_e gamma_e^3 sum_g D3[H_I, H_J, H_K] with
+    D3[a, b, c] = -mu (tr abc + tr acb) - lam (tr a tr bc + tr b tr ac + tr c tr ab). The same closed forms as the
+    kernel (whose values it reproduces to round-off); toec differentiates it for its backward pass."""
+    dev = chi.device
+    x = chi.reshape((6,) + cell.shape + (3,))
+    nod = torch.stack([torch.roll(x, shifts=tuple(-int(o) for o in off), dims=(1, 2, 3)) for off in NODES], dim=1)
+    nod = nod.reshape(6, 8, -1, 3)
+    dn = torch.as_tensor(dN_ref(), dtype=torch.float64, device=dev)
+    E = torch.stack([torch.as_tensor(voigt_tensor(I), dtype=torch.float64, device=dev) for I in range(6)])
+    lam, mu = cell.lam, cell.mu
+    neo = cell.material == "neohooke"
+    w3 = sw * gam ** 3 if neo else None
+    d2 = torch.zeros((6, 6), dtype=torch.float64, device=dev)
+    d3 = torch.zeros((6, 6, 6), dtype=torch.float64, device=dev)
+    for g in range(8):
+        H = torch.einsum("ianc,ad->incd", nod, dn[g]) + E[:, None]
+        tr = H.diagonal(dim1=-2, dim2=-1).sum(-1)
+        inner = torch.einsum("incd,jncd->ijn", H, H) + torch.einsum("incd,jndc->ijn", H, H)
+        d2 = d2 + torch.einsum("ijn,n->ij", lam * tr[:, None] * tr[None] + mu * inner, sw)
+        if neo:
+            prod = torch.einsum("incd,jnde->ijnce", H, H)
+            trp = prod.diagonal(dim1=-2, dim2=-1).sum(-1)
+            t3 = torch.einsum("ijncd,kndc->ijkn", prod, H)
+            D3 = -mu * (t3 + t3.transpose(1, 2)) - lam * (tr[:, None, None] * trp[None] + tr[None, :, None] *
+                                                         trp[:, None, :] + tr[None, None, :] * trp[:, :, None])
+            d3 = d3 + torch.einsum("ijkn,n->ijk", D3, w3)
+    pi = torch.tensor(_PAIRS, device=dev)
+    ti = torch.tensor(_TRIPLES, device=dev)
+    return torch.cat([d2[pi[:, 0], pi[:, 1]], d3[ti[:, 0], ti[:, 1], ti[:, 2]]])
+
+
+class _ToecFn(torch.autograd.Function):
+    """The 77 derivatives from the kernel; the backward differentiates toec_values_t (the same closed forms) in
+    chi, sw and gamma."""
+
+    @staticmethod
+    def forward(ctx, chi_flat, sw, gam, cell):
+        ctx.cell = cell
+        ctx.save_for_backward(chi_flat, sw, gam)
+        return _toec_kernel(cell, chi_flat.detach().contiguous())
+
+    @staticmethod
+    def backward(ctx, gv):
+        chi_flat, sw, gam = ctx.saved_tensors
+        with torch.enable_grad():
+            xs = [t.detach().requires_grad_(True) for t in (chi_flat, sw, gam)]
+            v = toec_values_t(ctx.cell, *xs)
+            grads = torch.autograd.grad(v, xs, gv, allow_unused=True)
+        return (*grads, None)
+
+
+def _geometric_X():
+    """X[:, J, K] = voigt(e_J e_K): the Voigt vector of the product of two unit Voigt strains."""
+    X = np.zeros((6, 6, 6))
+    for J in range(6):
+        for K in range(6):
+            X[:, J, K] = to_voigt(voigt_tensor(J) @ voigt_tensor(K))
+    return X
+
+
+_SYM3 = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+def _toec_assemble_t(v):
+    """C (6x6) and T = f''' - 3 Sym C X (6x6x6) from the 77 values, in torch (differentiable)."""
+    dev = v.device
+    idx2 = np.zeros((6, 6), dtype=np.int64)
+    for q, (I, J) in enumerate(_PAIRS):
+        idx2[I, J] = idx2[J, I] = q
+    idx3 = np.zeros((6, 6, 6), dtype=np.int64)
+    for q, (I, J, K) in enumerate(_TRIPLES):
+        for a, b, c in {(I, J, K), (I, K, J), (J, I, K), (J, K, I), (K, I, J), (K, J, I)}:
+            idx3[a, b, c] = 21 + q
+    C = v[torch.as_tensor(idx2, device=dev)]
+    T3 = v[torch.as_tensor(idx3, device=dev)]
+    CX = torch.einsum("ia,ajk->ijk", C, torch.as_tensor(_geometric_X(), dtype=torch.float64, device=dev))
+    sym = sum(CX.permute(*p) for p in _SYM3) / 6.0
+    return C, T3 - 3.0 * sym
+
+
 def toec(cell, chi):
     """C (6x6) and Lagrangian-strain third-order constants T (6x6x6) of the cell from the linear correctors chi
     (6, nx, ny, nz, 3) of unit Voigt strains (same stiffness scale and E, nu) (mech.py:454-490): with
     f(s) = <psi(I + sum_I s_I (e_I + grad chi_I))>, which equals Wbar to O(s^4), C_IJ = f''_IJ and
     T_IJK = f'''_IJK - 3 Sym C[e_I, e_J e_K]. The second and third derivatives come from one kernel pass
-    (`nlfh_fs_toec`, closed-form derivatives of the energy at F = I); numpy arrays (not differentiable)."""
-    chi_t = torch.as_tensor(chi, dtype=torch.float64, device=cell._dev).reshape(6, -1).contiguous()
+    (`nlfh_fs_toec`, closed-form derivatives of the energy at F = I). numpy arrays; when chi or cell.rho requires
+    grad (and grad mode is on), torch tensors on the cell's device, differentiable in both (the reference's toec
+    is differentiable in rho through chi and the cell's stiffness, mech.py:458-462)."""
+    chi_t = torch.as_tensor(chi, dtype=torch.float64, device=cell._dev).reshape(6, -1)
     if chi_t.shape[1] != 3 * cell.N:
         raise ValueError(f"chi must have shape (6,) + {cell.shape + (3,)}")
-    out = torch.empty(77, dtype=torch.float64, device=cell._dev)
-    _lib.check(cell._lib.nlfh_fs_toec(cell._shape_c, _ptr(cell._sg), cell._mat.ctypes.data_as(_dp), cell._flags,
-                                      _ptr(chi_t), _ptr(out), _ptr(cell._work), _stream()))
-    v = out.cpu().numpy()
-    C = np.zeros((6, 6))
-    T3 = np.zeros((6, 6, 6))
-    q = 0
-    for I in range(6):
-        for J in range(I, 6):
-            C[I, J] = C[J, I] = v[q]
-            q += 1
-    for I in range(6):
-        for J in range(I, 6):
-            for K in range(J, 6):
-                for a, b, c in {(I, J, K), (I, K, J), (J, I, K), (J, K, I), (K, I, J), (K, J, I)}:
-                    T3[a, b, c] = v[q]
-                q += 1
-    # geometric part: 3 Sym_IJK C(e_I, voigt(e_J e_K))
-    X = np.zeros((6, 6, 6))
-    for J in range(6):
-        for K in range(6):
-            X[:, J, K] = to_voigt(voigt_tensor(J) @ voigt_tensor(K))
-    CX = np.einsum("ia,ajk->ijk", C, X)
-    perms = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
-    sym = sum(np.transpose(CX, p) for p in perms) / 6.0
-    return C, T3 - 3.0 * sym
+    if torch.is_grad_enabled() and (chi_t.requires_grad or cell.rho.requires_grad):
+        sw = (cell.wq * cell.scale_of(cell.rho)).reshape(-1)
+        gam = cell.gamma_of(cell.rho)
+        gam = torch.ones_like(sw) if gam is None else gam.reshape(-1)
+        return _toec_assemble_t(_ToecFn.apply(chi_t, sw, gam, cell))
+    C, T = _toec_assemble_t(_toec_kernel(cell, chi_t.detach().contiguous()).cpu())
+    return C.numpy(), T.numpy()
 
 
 CUBIC_TOEC = {
@@ -674,5 +755,15 @@ CUBIC_TOEC = {
 def cubic_toec(T):
     """The six cubic third-order constants (C111, C112, C123, C144, C155, C456), each the mean of its equivalent
     entries (Voigt indices 1..6 = xx yy zz yz xz xy) (mech.py:493-515)."""
-    T = np.asarray(T)
+    T = np.asarray(T.detach().cpu() if isinstance(T, torch.Tensor) else T)
     return {k: float(np.mean([T[i] for i in idx])) for k, idx in CUBIC_TOEC.items()}
+
+
+def cubic_toec_t(T):
+    """cubic_toec for a torch T, differentiable: dict of 0-d tensors (the reference's cubic_toec_jnp,
+    mech.py:504-506)."""
+    T = torch.as_tensor(T)
+    return {k: torch.stack([T[i] for i in idx]).mean() for k, idx in CUBIC_TOEC.items()}
+
+
+cubic_toec_jnp = cubic_toec_t  # the reference's name for the traceable form

@@ -6,8 +6,11 @@ vector-Jacobian products against these functions; the product never imports them
 Follows nlfiber/symmetry.py:26-102 (sym_perm, mirror, fold, exact_symmetrize, free entries) and
 nlfiber/density.py:23-91 (hat filter, periodic convolution, eta by bisection + Newton, implicit-derivative
 threshold, DensityMap.smooth / rho), in three dimensions. `rho_vjp` is the hand-derived adjoint of
-`rho` (pinned by central differences in tests/test_oracle.py).
+`rho` (pinned by central differences in tests/test_oracle.py). `binary` restates DensityMap.binary
+(nlfiber/density.py:96-148) with the periodic components of nlfiber/topology.py:8-65 found by a plain
+breadth-first search over the 26-neighbourhood (`label_bfs`, pinned on hand-made cells in tests/test_oracle.py).
 """
+from collections import deque
 import itertools
 
 import numpy as np
@@ -146,3 +149,81 @@ def sum_mirror(full):
         b = np.flip(np.take(x, np.arange(n_o, 2 * n_o), axis=ax), ax)
         x = a + b
     return x
+
+
+# --- binary cells: nlfiber/density.py:96-148, nlfiber/topology.py:8-65 ---------------------------------------
+def label_bfs(b, face_only=False):
+    """Periodic connected components by breadth-first search (26-neighbourhood, or 6 with face_only), numbered
+    1..n in the order of their first voxel in C order. Returns (labels, n)."""
+    b = np.asarray(b, dtype=bool)
+    shp = b.shape
+    offs = [o for o in itertools.product((-1, 0, 1), repeat=b.ndim) if any(o)]
+    if face_only:
+        offs = [o for o in offs if sum(abs(t) for t in o) == 1]
+    lab = np.zeros(shp, dtype=np.int64)
+    n = 0
+    for start in zip(*np.nonzero(b)):
+        if lab[start]:
+            continue
+        n += 1
+        lab[start] = n
+        q = deque([start])
+        while q:
+            cur = q.popleft()
+            for o in offs:
+                nb = tuple((c + t) % m for c, t, m in zip(cur, o, shp))
+                if b[nb] and not lab[nb]:
+                    lab[nb] = n
+                    q.append(nb)
+    return lab, n
+
+
+def spans(lab, k):
+    m = lab == k
+    for ax in range(m.ndim):
+        if not np.all(m.any(axis=tuple(a for a in range(m.ndim) if a != ax))):
+            return False
+    return True
+
+
+def drop_floating(b):
+    lab, n = label_bfs(b)
+    keep = np.array([False] + [spans(lab, k) for k in range(1, n + 1)])
+    return keep[lab]
+
+
+def upsample(c, f):
+    """Periodic multilinear resampling, one axis at a time (nlfiber/density.py:132-148)."""
+    out = np.asarray(c, dtype=np.float64)
+    for ax in range(out.ndim):
+        n = out.shape[ax]
+        m = int(round(n * f))
+        x = (np.arange(m) + 0.5) * n / m - 0.5
+        i0 = np.floor(x).astype(int)
+        w = (x - i0).reshape([m if a == ax else 1 for a in range(out.ndim)])
+        out = (1 - w) * np.take(out, i0 % n, axis=ax) + w * np.take(out, (i0 + 1) % n, axis=ax)
+    return out
+
+
+def binary(phi, r, v, up=1):
+    """Binary cell: smoothed u (upsampled and re-symmetrised when up > 1) above its (1 - v) quantile, floating
+    pieces removed, the kept solid grown back to v through its void neighbours (highest u first)."""
+    u = smooth(phi, r)
+    if up > 1:
+        u = exact_symmetrize(upsample(u, up))
+    b = u > np.quantile(u, 1.0 - v)
+    for _ in range(500):
+        b = drop_floating(b)
+        missing = v - b.mean()
+        if missing * b.size < 1 or not b.any():
+            break
+        front = b.copy()
+        for ax in range(b.ndim):
+            front = front | np.roll(front, 1, axis=ax) | np.roll(front, -1, axis=ax)
+        front &= ~b
+        cand = np.sort(u[front])[::-1]
+        if cand.size == 0:
+            break
+        k = min(max(int(round(missing * b.size)), 1), cand.size)
+        b = b | (front & (u >= cand[k - 1]))
+    return drop_floating(b)

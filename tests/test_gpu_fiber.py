@@ -109,3 +109,31 @@ def test_identity_filter_radius(gpu_pkg):
     dm = DensityMap(4, r=1.0, v=0.3)
     np.testing.assert_allclose(dm.smooth(phi).cpu().numpy(), DR.smooth(phi, 1.0), rtol=0, atol=1e-12)
     assert abs(dm.rho(phi, 4.0).mean().item() - 0.3) < 1e-12
+
+
+@pytest.mark.parametrize("n_o,up", [(8, 1), (6, 2)])
+def test_binary_check_matches_oracle(gpu_pkg, n_o, up):
+    """Fiber.binary_check (nlfiber/fiber.py:67-81): the binary cell is the oracle's (same voxels), it has
+    one spanning solid piece, its volume is within one voxel of v, every orbit is kept or dropped whole, and
+    the GPU's C of it equals the FP64 oracle's C."""
+    from nlfh_amd.fiber import Fiber
+    from nlfh_amd.linear import cubic_invariants
+    x = 2 * np.pi * (np.arange(n_o) + 0.5) / (2 * n_o)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    phi = -(np.cos(X) + np.cos(Y) + np.cos(Z)) + 0.5 * _phi(n_o, 40)  # a perturbed Schwarz P octant
+    fib = Fiber(n_o, v=0.3, r=1.5)
+    out = fib.binary_check(phi, up=up, full=True)
+    b = out["cell"]
+    ref = DR.binary(phi, 1.5, 0.3, up=up)
+    assert b.shape == (2 * n_o * up,) * 3 and np.array_equal(b, ref)
+    lab, n = gpu_pkg.topology.periodic_label(b)
+    assert n == 1 and gpu_pkg.topology.spanning(lab, 1)
+    # grown back to v (whole orbits, so it may pass v by a few orbits of up to 48 voxels)
+    assert (0.3 - out["vol"]) * b.size < 1.0 and out["vol"] < 0.32
+    assert np.array_equal(b, b[::-1]) and np.array_equal(b, np.transpose(b, (1, 2, 0)))
+    lr = O.LinearRef(b.shape, emin=1e-3, tol=1e-9)
+    Cr, _, _ = lr.C_full(b.astype(np.float64))
+    assert np.abs(out["C"] - Cr).max() < 1e-7 * np.abs(Cr).max()
+    assert np.allclose(out["c"], cubic_invariants(Cr), rtol=1e-7, atol=0)
+    c_only = fib.binary_check(phi, up=up)["c"]
+    assert np.allclose(c_only, out["c"], rtol=1e-7, atol=0)

@@ -367,3 +367,61 @@ def test_energy_at_another_density(M):
     np.testing.assert_array_equal(a.gradient(w, Fb, rho2)[1], b.gradient(w, Fb)[1])
     mix = M.Mixed(a, free=[])
     assert mix.Ez(w.reshape(-1), Fb, rho2) == b.energy(w, Fb)
+
+
+@pytest.mark.parametrize("material,wls", [("neohooke", (500.0, 0.01)), ("neohooke", None), ("linear", None)])
+def test_toec_closed_form_matches_kernel(M, material, wls):
+    """toec's backward differentiates toec_values_t: its 77 values equal the kernel's to round-off."""
+    import torch
+    rng = np.random.default_rng(21)
+    rho = rng.uniform(0.1, 1.0, (4, 5, 3))
+    chi = torch.as_tensor(0.3 * rng.standard_normal((6,) + rho.shape + (3,)), device="cuda")
+    cell = M.Cell(rho, material=material, wls=wls)
+    v = M._toec_kernel(cell, chi.reshape(6, -1).contiguous())
+    sw = (cell.wq * cell.scale_of(cell.rho)).reshape(-1)
+    gam = cell.gamma_of(cell.rho)
+    gam = torch.ones_like(sw) if gam is None else gam.reshape(-1)
+    vt = M.toec_values_t(cell, chi.reshape(6, -1), sw, gam)
+    assert _rel(vt.cpu(), v.cpu()) < 1e-12
+
+
+def test_toec_loss_gradient_matches_central_differences(M, gpu_pkg):
+    """The reference's test_toec_loss_gradient (tests/test_pipeline.py:112-119) with its ToecTarget loss
+    (nlfiber/steer/losses.py:103-124): L(rho) = (C111 + 1)^2 + C456^2 from toec of the cell with the linear
+    correctors chi(rho); the autograd gradient (through toec, the corrector adjoint and the stiffness scale)
+    against central differences along d = randn * rho (1 - rho)."""
+    import torch
+    from nlfh_amd.density import DensityMap
+    dm = DensityMap(4, r=1.5, v=0.3)
+    phi = np.random.default_rng(30).standard_normal((4, 4, 4))
+    rho0 = dm.rho(phi, 4.0).cpu().numpy()
+    lh = gpu_pkg.LinearHomogenization(rho0.shape, E=1.0, nu=0.3, emin=1e-3, p=3.0, tol=1e-12)
+    zeros = torch.zeros((6,) + rho0.shape + (3,), dtype=torch.float64, device="cuda")
+
+    def loss(rho):
+        s = lh.stiffness_scale(rho)
+        chi = lh.correctors(s, np.eye(6), zeros)
+        cell = M.Cell(rho, E=1.0, nu=0.3, wls=(500.0, 0.01), emin=1e-3, p=3.0)
+        _, T = M.toec(cell, chi)
+        vals = M.cubic_toec_jnp(T)
+        return (vals["C111"] + 1.0) ** 2 + vals["C456"] ** 2
+
+    r = torch.tensor(rho0, device="cuda", requires_grad=True)
+    L = loss(r)
+    (g,) = torch.autograd.grad(L, r)
+    d = np.random.default_rng(1).standard_normal(rho0.shape) * rho0 * (1 - rho0)
+    h = 1e-5
+    with torch.no_grad():
+        fd = (float(loss(torch.tensor(rho0 + h * d, device="cuda"))) -
+              float(loss(torch.tensor(rho0 - h * d, device="cuda")))) / (2 * h)
+    gd = float((g.cpu().numpy() * d).sum())
+    assert abs(gd - fd) < 1e-3 * abs(fd) + 1e-8, (gd, fd)
+    # without grad the same call returns numpy arrays equal to the tensors' values
+    with torch.no_grad():
+        cell = M.Cell(rho0, E=1.0, nu=0.3, wls=(500.0, 0.01), emin=1e-3, p=3.0)
+        chi = lh.correctors(lh.stiffness_scale(torch.tensor(rho0, device="cuda")), np.eye(6), zeros)
+        Cn, Tn = M.toec(cell, chi)
+    assert isinstance(Tn, np.ndarray)
+    Ct, Tt = M.toec(M.Cell(r, E=1.0, nu=0.3, wls=(500.0, 0.01), emin=1e-3, p=3.0), chi)
+    np.testing.assert_allclose(Tt.detach().cpu().numpy(), Tn, rtol=0, atol=1e-13 * np.abs(Tn).max())
+    np.testing.assert_allclose(Ct.detach().cpu().numpy(), Cn, rtol=0, atol=1e-13 * np.abs(Cn).max())

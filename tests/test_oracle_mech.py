@@ -134,3 +134,32 @@ def test_toec_derivatives_second_order_is_linear_C_and_symmetric():
     np.testing.assert_allclose(C2, C, atol=1e-9 * np.abs(C).max())
     np.testing.assert_allclose(T3, np.transpose(T3, (1, 0, 2)), atol=1e-7 * np.abs(T3).max())
     np.testing.assert_allclose(T3, np.transpose(T3, (0, 2, 1)), atol=1e-7 * np.abs(T3).max())
+
+
+@pytest.mark.parametrize("material,wls", [("neohooke", (500.0, 0.01)), ("neohooke", None), ("linear", None)])
+def test_toec_closed_form_of_the_backward_matches_oracle(pkg, material, wls):
+    """The torch closed form that toec's backward differentiates (mech.toec_values_t; CPU here) against the
+    oracle's second derivatives (explicit tangent) and third derivatives (central differences of it)."""
+    import types
+
+    import torch
+    rng = np.random.default_rng(21)
+    rho = rng.uniform(0.1, 1.0, (4, 5, 3))
+    chi = 0.3 * rng.standard_normal((6,) + rho.shape + (3,))
+    C2, T3 = M.toec_derivatives(M.CellRef(rho, material=material, wls=wls), chi)
+    lam, mu = pkg.mech.lame(1.0, 0.3)
+    cell = types.SimpleNamespace(shape=rho.shape, lam=lam, mu=mu, material=material)
+    s = 1e-3 + (1 - 1e-3) * rho ** 3
+    gam = np.ones_like(rho)
+    if wls is not None:
+        b, r0 = wls
+        gam = (np.tanh(b * r0) + np.tanh(b * (rho ** 3 - r0))) / (np.tanh(b * r0) + np.tanh(b * (1 - r0)))
+    v = pkg.mech.toec_values_t(cell, torch.tensor(chi).reshape(6, -1), torch.tensor(s / (8 * rho.size)).reshape(-1),
+                               torch.tensor(gam).reshape(-1)).numpy()
+    C, T = pkg.mech._toec_assemble_t(torch.tensor(v))
+    assert np.abs(C.numpy() - C2).max() < 1e-13 * np.abs(C2).max()
+    d3 = np.zeros((6, 6, 6))
+    for q, (I, J, K) in enumerate(pkg.mech._TRIPLES):
+        for t in {(I, J, K), (I, K, J), (J, I, K), (J, K, I), (K, I, J), (K, J, I)}:
+            d3[t] = v[21 + q]
+    assert np.abs(d3 - T3).max() <= 1e-6 * max(np.abs(T3).max(), 1e-300)
