@@ -54,6 +54,14 @@ int nlfh_plan_destroy(nlfh_plan* plan);
 int nlfh_plan_set_tiling(nlfh_plan* plan, int shape_index, int ty, int xc, int zpt);
 int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* xc, int* zpt);
 
+/* Block shape of the FP32 pass B (the x transforms, Kinv and inverse x transforms of the preconditioner) of a
+ * cubic 16-128 shape: pass_b_positions (ky, kz) positions per block, 0 or the default (256 threads; 16 x 16 threads
+ * at 128), half or double of it. get returns the value in use and the default (0 for shapes on the generic passes,
+ * whose line length is not tunable). No counterpart in the reference (a launch-shape choice of
+ * LinearHomogenization.precond, nlfiber/linear.py:167-170). */
+int nlfh_plan_set_fft_tiling(nlfh_plan* plan, int shape_index, int pass_b_positions);
+int nlfh_plan_get_fft_tiling(const nlfh_plan* plan, int shape_index, int* pass_b_positions, int* pass_b_default);
+
 /* Energy-optimal (Ritz-Galerkin) start from predicted correctors: for every cell and right-hand side
  * I, u_out_I = sum_J a_IJ u_in_J with a_I minimising the PCG energy 1/2 u.Ku + f_I.u over the span
  * of the cell's m predicted fields (G a_I = -(f_I . u_in_J)_J, G_JK = u_in_J . K u_in_K, f = load).

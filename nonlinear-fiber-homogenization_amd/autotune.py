@@ -1,5 +1,6 @@
-"""Autotuner for the matvec tiling (ty node rows per tile, xc x-planes per chunk, zpt z values per
-lane) of each grid shape of a plan.
+"""Autotuner for the launch shapes of each grid shape of a plan: the matvec tiling (ty node rows per tile, xc
+x-planes per chunk, zpt z values per lane) and, for the FP32 preconditioner of cubic 16-128 cells, the block shape
+of pass B ((ky, kz) positions per block: the default, half or double of it; `tune_fft`).
 
 Candidates are every valid combination from small sets; each is timed on the device with HIP events
 over repeated launches of the operator K(s) u on m right-hand sides (the same kernel family the
@@ -57,11 +58,11 @@ def batch_cells(shape):
     return max(1, min(64, (1 << 21) // nvox))
 
 
-def _key(shape, precision, m, batched):
-    return f"{tuple(shape)}|{precision}|{m}" + ("|batch" if batched else "")
+def _key(shape, precision, m, batched, kind=""):
+    return f"{tuple(shape)}|{precision}|{m}" + (f"|{kind}" if kind else "") + ("|batch" if batched else "")
 
 
-def _packaged(shape, precision, m, dev, batched=False):
+def _packaged(shape, precision, m, dev, batched=False, kind=""):
     """Tiling measured on MI355X (gfx950) and shipped with the package (tune_table.py writes it)."""
     try:
         arch = torch.cuda.get_device_properties(dev).gcnArchName
@@ -74,7 +75,7 @@ def _packaged(shape, precision, m, dev, batched=False):
             table = json.load(f)
     except (OSError, ValueError):
         return None
-    e = table.get(_key(shape, precision, m, batched)) or table.get(_key(shape, precision, m, False))
+    e = table.get(_key(shape, precision, m, batched, kind)) or table.get(_key(shape, precision, m, False, kind))
     return tuple(e["best"]) if e else None
 
 
@@ -168,5 +169,78 @@ def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, ver
     plan.set_tiling(shape_index, *best)
     if use_cache:
         cache[key] = {"best": list(best), "us": table[0][1]}
+        _save_cache(cache)
+    return best, table
+
+
+def time_precond(plan, r, z, m, reps):
+    """Microseconds per FP32 nlfh_precond over the plan's cells (HIP events, current stream)."""
+    lib = _lib.load()
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    args = (plan.handle, ctypes.c_void_p(r.data_ptr()), ctypes.c_void_p(z.data_ptr()), m, 1, sp)
+    for _ in range(2):
+        _lib.check(lib.nlfh_precond(*args))
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        _lib.check(lib.nlfh_precond(*args))
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e3
+
+
+def fft_candidates(default):
+    """Pass B positions per block to try: the default, half and double of it."""
+    return [default, default // 2, 2 * default]
+
+
+def tune_fft(plan, shape_index=0, m=6, reps=10, use_cache=True, verbose=False, batched=False):
+    """Pick the FP32 pass B block shape of distinct shape `shape_index` of `plan` (cubic 16-128 shapes; others are
+    left alone) by timing nlfh_precond in FP32 (on a plan of batch_cells(shape) cells with `batched`) and set it.
+    Returns (best positions per block or None, [(positions, microseconds), ...])."""
+    distinct = []
+    for s in plan.shapes:
+        if s not in distinct:
+            distinct.append(s)
+    shape = distinct[shape_index]
+    _, default = plan.get_fft_tiling(shape_index)
+    if default == 0:
+        return None, []
+    batched = batched and batch_cells(shape) > 1
+    dev = torch.device("cuda", torch.cuda.current_device())
+    key = f"{torch.cuda.get_device_name(dev)}|" + _key(shape, "f32", m, batched, "fftb")
+    cache = _load_cache() if use_cache else {}
+    if use_cache and key in cache:
+        best = int(cache[key]["best"][0])
+        plan.set_fft_tiling(shape_index, best)
+        return best, []
+    if use_cache:
+        best = _packaged(shape, "f32", m, dev, batched, "fftb")
+        if best is not None:
+            plan.set_fft_tiling(shape_index, int(best[0]))
+            return int(best[0]), []
+    from .linear import Plan
+    nb = batch_cells(shape) if batched else 1
+    tplan = Plan([shape] * nb, 1.0, 0.3, m_layout=m)
+    nvox = int(np.prod(shape))
+    r = torch.randn(nb * m * 3 * nvox, dtype=torch.float32, device=dev)
+    z = torch.empty_like(r)
+    table = []
+    for cand in fft_candidates(default):
+        try:
+            tplan.set_fft_tiling(0, cand)
+        except _lib.NlfhError:
+            continue
+        t = time_precond(tplan, r, z, m, reps)
+        table.append((cand, t))
+        if verbose:
+            print(f"  pass B positions per block {cand:4d}: {t:9.2f} us", flush=True)
+    table.sort(key=lambda x: x[1])
+    best = table[0][0]
+    plan.set_fft_tiling(shape_index, best)
+    if use_cache:
+        cache[key] = {"best": [best], "us": table[0][1]}
         _save_cache(cache)
     return best, table

@@ -85,11 +85,15 @@ class BatchedHomogenization:
                 want = self._plan.get_tiling(distinct.index(sh))
                 if plan.get_tiling(j) != want:
                     plan.set_tiling(j, *want)
+                want_b = self._plan.get_fft_tiling(distinct.index(sh))[0]
+                if plan.get_fft_tiling(j)[0] != want_b:
+                    plan.set_fft_tiling(j, want_b)
         return self._subs
 
     def autotune(self, precision=None):
         """Tile every distinct shape with the autotuner and apply it: shapes that several cells of one
-        stream group share take the batched tiling, the others the single-cell one."""
+        stream group share take the batched tiling, the others the single-cell one. Mixed-precision batches also
+        take the tuned FP32 pass B block shape of their cubic 16-128 shapes."""
         from . import autotune as at
         distinct = []
         for s in self.shapes:
@@ -102,6 +106,10 @@ class BatchedHomogenization:
             tmp = Plan([s], self.E, self.nu)
             best, _ = at.tune(tmp, 0, m=self.m, precision=prec, batched=per_group >= 2)
             self._plan.set_tiling(i, *best)
+            if prec == "f32":
+                bb, _ = at.tune_fft(tmp, 0, m=self.m, batched=per_group >= 2)
+                if bb is not None:
+                    self._plan.set_fft_tiling(i, bb)
 
     def pack(self, rhos):
         """Concatenate the cells' densities into one device array (cell-major)."""

@@ -90,6 +90,7 @@ struct PrecGroup {
   int plane_begin, plane_count;
   int line_begin[2], line_count[2];  // [0] float64 line items, [1] float32
   int vec;  // every cell's field and spectrum offsets allow 16-byte vector access (staged passes)
+  int lpb32 = 0;  // FP32 pass B positions per block on the fast path (0: the default)
 };
 
 template <typename T>
@@ -434,6 +435,42 @@ int build_mv_items(nlfh_plan* p) {
 }
 
 template <typename T>
+size_t line_len_for(int nx, int64_t npos);
+
+// FP32 line items of pass B (per shape group: the fast path's positions per block, lpb32 or the default, or the
+// generic length) with their per-cell ranges; `rebuild` frees the device tables of an earlier build
+int build_lines32(nlfh_plan* p, bool rebuild) {
+  p->lines32.clear();
+  std::vector<int> lrange32(2 * p->ncells);
+  for (PrecGroup& g : p->pgroups) {
+    g.line_begin[1] = (int)p->lines32.size();
+    for (int c = 0; c < p->ncells; ++c) {
+      if (p->cell_shape[c] != g.shape) continue;
+      const CellDesc& cd = p->cells[c];
+      const int64_t npos = (int64_t)cd.ny * cd.nzh;
+      const int len = g.fast ? (g.lpb32 > 0 ? g.lpb32 : fast_line_len(g.fast, 4)) : (int)line_len_for<float>(cd.nx, npos);
+      lrange32[2 * c] = (int)p->lines32.size();
+      for (int64_t q = 0; q < npos; q += len)
+        p->lines32.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
+      lrange32[2 * c + 1] = (int)p->lines32.size();
+    }
+    g.line_count[1] = (int)p->lines32.size() - g.line_begin[1];
+  }
+  if (rebuild) {
+    for (void** q : {(void**)&p->d_lines32, (void**)&p->d_line_range32}) {
+      if (*q) {
+        (void)hipFree(*q);
+        p->allocs.erase(std::find(p->allocs.begin(), p->allocs.end(), *q));
+        *q = nullptr;
+      }
+    }
+  }
+  TRY(p->upload(&p->d_lines32, p->lines32));
+  TRY(p->upload(&p->d_line_range32, lrange32));
+  return NLFH_OK;
+}
+
+template <typename T>
 size_t line_len_for(int nx, int64_t npos) {
   const size_t budget = 48 * 1024;
   size_t len = budget / (3 * (size_t)nx * sizeof(cplx<T>));
@@ -639,7 +676,8 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
       const Shape& sh = p->shapes[g.shape];
       const cplx<T>* tw = tw_ptr<T>(p) + p->lens[sh.len[0]].tw_off;
       const T* symp = symsoa_ptr<T>(p) + p->symsoa_off[g.shape];
-      HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec, st));
+      HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec,
+                                  sizeof(T) == 4 ? g.lpb32 : 0, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
       if (dot >= 2 || update == 1 || update == 4 || a.rspec != nullptr) return NLFH_EUNSUPPORTED;
@@ -1025,7 +1063,7 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
     shape_len.push_back(sh.len[2]);
   }
   // plane and line items grouped by shape (a cell's items stay contiguous), chunk items per cell
-  std::vector<int> prange(2 * ncells), crange, lrange32(2 * ncells);
+  std::vector<int> prange(2 * ncells), crange;
   for (int sidx = 0; sidx < (int)p->shapes.size(); ++sidx) {
     const Shape& sh = p->shapes[sidx];
     PrecGroup g{};
@@ -1042,21 +1080,15 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
         for (int x = 0; x < cd.nx; ++x) p->planes.push_back({c, comp, x, 0});
       prange[2 * c + 1] = (int)p->planes.size();
       const int64_t npos = (int64_t)cd.ny * cd.nzh;
-      for (int prec = 0; prec < 2; ++prec) {
-        const int len = g.fast ? fast_line_len(g.fast, prec == 0 ? 8 : 4)
-                               : (int)(prec == 0 ? line_len_for<double>(cd.nx, npos) : line_len_for<float>(cd.nx, npos));
-        std::vector<LineItem>& lines = prec == 0 ? p->lines64 : p->lines32;
-        if (prec == 1) lrange32[2 * c] = (int)lines.size();
-        for (int64_t q = 0; q < npos; q += len) lines.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
-        if (prec == 1) lrange32[2 * c + 1] = (int)lines.size();
-      }
+      const int len = g.fast ? fast_line_len(g.fast, 8) : (int)line_len_for<double>(cd.nx, npos);
+      for (int64_t q = 0; q < npos; q += len)
+        p->lines64.push_back({c, (int)q, (int)std::min<int64_t>(len, npos - q), 0});
     }
     g.vec = 1;
     for (int c = 0; c < ncells; ++c)
       if (p->cell_shape[c] == sidx && (p->cells[c].off_field % 4 != 0 || p->cells[c].off_spec % 2 != 0)) g.vec = 0;
     g.plane_count = (int)p->planes.size() - g.plane_begin;
     g.line_count[0] = (int)p->lines64.size() - g.line_begin[0];
-    g.line_count[1] = (int)p->lines32.size() - g.line_begin[1];
     p->pgroups.push_back(g);
   }
   for (int c = 0; c < ncells; ++c) {
@@ -1112,10 +1144,9 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_symsoa32, soa32))) return fail(rc);
   if ((rc = p->upload(&p->d_planes, p->planes))) return fail(rc);
   if ((rc = p->upload(&p->d_lines64, p->lines64))) return fail(rc);
-  if ((rc = p->upload(&p->d_lines32, p->lines32))) return fail(rc);
+  if ((rc = build_lines32(p, false))) return fail(rc);
   if ((rc = p->upload(&p->d_chunks, p->chunks))) return fail(rc);
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
-  if ((rc = p->upload(&p->d_line_range32, lrange32))) return fail(rc);
   p->lean32 = true;
   p->dir32 = true;
   for (const PrecGroup& g : p->pgroups) {
@@ -1131,7 +1162,15 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->alloc(&p->d_part_mv, (size_t)M * max_mv))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_a, (size_t)M * p->planes.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_c, (size_t)M * p->planes.size()))) return fail(rc);
-  if ((rc = p->alloc(&p->d_part_bdot, (size_t)M * std::max<size_t>(1, p->lines32.size())))) return fail(rc);
+  // pass B partials: sized for the smallest positions per block the FP32 passes offer (half the default)
+  int64_t max_lines32 = 0;
+  for (int c = 0; c < ncells; ++c) {
+    const PrecGroup& g = p->pgroups[p->cell_shape[c]];
+    const int64_t npos = (int64_t)p->cells[c].ny * p->cells[c].nzh;
+    const int64_t len = g.fast ? std::max(1, fast_line_len(g.fast, 4) / 2) : (int64_t)line_len_for<float>(p->cells[c].nx, npos);
+    max_lines32 += (npos + len - 1) / len;
+  }
+  if ((rc = p->alloc(&p->d_part_bdot, (size_t)M * std::max<int64_t>(1, max_lines32)))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_b, (size_t)3 * M * p->chunks.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_part_e, (size_t)36 * p->chunks.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_strain, (size_t)6 * kMaxRhs))) return fail(rc);
@@ -1194,6 +1233,43 @@ int nlfh_plan_set_tiling(nlfh_plan* p, int shape_index, int ty, int xc, int zpt)
   p->graphs.clear();
   p->graph_src.clear();
   return build_mv_items(p);
+}
+
+int nlfh_plan_set_fft_tiling(nlfh_plan* p, int shape_index, int pass_b_positions) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
+  PrecGroup* grp = nullptr;
+  for (PrecGroup& g : p->pgroups)
+    if (g.shape == shape_index) grp = &g;
+  CHECK_ARG(grp != nullptr, "shape has no cells");
+  if (!grp->fast) {
+    CHECK_ARG(pass_b_positions == 0, "the generic passes have no tunable block shape");
+    return NLFH_OK;
+  }
+  CHECK_ARG(fast_pass_b_ok(grp->fast, pass_b_positions),
+            "pass_b_positions must be 0, the default, or half or double the default");
+  grp->lpb32 = pass_b_positions == fast_line_len(grp->fast, 4) ? 0 : pass_b_positions;
+  // cached graphs bake the line tables: drop them
+  HIP_TRY(hipStreamSynchronize(p->stream));
+  for (auto& kv : p->graphs) (void)hipGraphExecDestroy(kv.second);
+  for (auto& kv : p->graph_src) (void)hipGraphDestroy(kv.second);
+  p->graphs.clear();
+  p->graph_src.clear();
+  return build_lines32(p, true);
+}
+
+int nlfh_plan_get_fft_tiling(const nlfh_plan* p, int shape_index, int* pass_b_positions, int* pass_b_default) {
+  CHECK_ARG(p != nullptr, "plan is null");
+  CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
+  for (const PrecGroup& g : p->pgroups) {
+    if (g.shape != shape_index) continue;
+    const int def = g.fast ? fast_line_len(g.fast, 4) : 0;
+    if (pass_b_positions) *pass_b_positions = g.lpb32 > 0 ? g.lpb32 : def;
+    if (pass_b_default) *pass_b_default = def;
+    return NLFH_OK;
+  }
+  set_error("shape has no cells");
+  return NLFH_EINVAL;
 }
 
 int nlfh_plan_get_tiling(const nlfh_plan* p, int shape_index, int* ty, int* xc, int* zpt) {

@@ -855,13 +855,22 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
 // DOTB: also the partial r . z of the line item from the spectra (Parseval over the half spectrum:
 // weight 1 at kz = 0 and kz = N/2, 2 elsewhere; z-hat = Kinv r-hat / N is what the inverse transform
 // turns into z), so pass C need not read r.
-template <typename T, int N, bool DOTB>
-__global__ void __launch_bounds__((FastCfg<T, N>::BTB)) prec_b_fast(const PrecArgs<T> a, const cplx<T>* __restrict__ twg,
-                                                                  const T* __restrict__ symp) {
+// LPBX: (ky, kz) positions per block when not the default C::LPB (a launch shape the autotuner picks; the plan's
+// line items then hold LPBX positions each), with L * LPBX threads.
+template <typename T, int N, int LPBX>
+constexpr int pass_b_threads() {
+  return LPBX > 0 ? FastCfg<T, N>::L * LPBX : FastCfg<T, N>::BTB;
+}
+
+template <typename T, int N, bool DOTB, int LPBX = 0>
+__global__ void __launch_bounds__((pass_b_threads<T, N, LPBX>())) prec_b_fast(const PrecArgs<T> a,
+                                                                             const cplx<T>* __restrict__ twg,
+                                                                             const T* __restrict__ symp) {
   __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   double dacc = 0.0;
   using C = FastCfg<T, N>;
-  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, LPB = C::LPB, BT = C::BTB;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH;
+  constexpr int LPB = LPBX > 0 ? LPBX : C::LPB, BT = pass_b_threads<T, N, LPBX>();
   extern __shared__ double smem_d[];
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* xch = tw + N;  // [XS][LPB], one component at a time
@@ -959,9 +968,16 @@ size_t fast_lds_plane() {
   return ((size_t)N + (size_t)N * C::NH + (C::XIN ? 0 : (size_t)C::LPR * C::XS) + N) * sizeof(cplx<T>);
 }
 template <typename T, int N>
-size_t fast_lds_line() {
+size_t fast_lds_line(int lpb) {
   using C = FastCfg<T, N>;
-  return ((size_t)N + (size_t)C::XS * C::LPB) * sizeof(cplx<T>);
+  return ((size_t)N + (size_t)C::XS * (lpb > 0 ? lpb : C::LPB)) * sizeof(cplx<T>);
+}
+
+// pass B positions per block other than the default that the FP32 passes offer (half and double of it)
+template <int N>
+constexpr bool pass_b_alt_ok(int lpb) {
+  return lpb == FastCfg<float, N>::LPB / 2 ||
+         (lpb == 2 * FastCfg<float, N>::LPB && FastCfg<float, N>::L * lpb <= 1024);
 }
 
 // update: 0 none, 2 r -= alpha Ap and x += alpha p in pass A; 1 (r only) and 4 (spectral residual) on the
@@ -970,9 +986,11 @@ size_t fast_lds_line() {
 // direction, 4 as 3 with the paired iterate step of odd iterations (staged pass only, see prec_c_staged)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                                int update, int dot, int vec, hipStream_t st) {
+                                int update, int dot, int vec, int lpb, hipStream_t st) {
   using C = FastCfg<T, N>;
-  const size_t lp = fast_lds_plane<T, N>(), ll = fast_lds_line<T, N>();
+  if (lpb == C::LPB) lpb = 0;
+  if (lpb != 0 && (sizeof(T) != 4 || !pass_b_alt_ok<N>(lpb))) return hipErrorInvalidValue;
+  const size_t lp = fast_lds_plane<T, N>(), ll = fast_lds_line<T, N>(lpb);
   dim3 gp(nplanes, a.m, 1), gl(nlines, a.m, 1);
   auto big = [](const void* f, size_t b) {
     if (b > 65536) (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b);
@@ -980,9 +998,17 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
 #define NLFH_A(KERN, U)                                                                 \
   big((const void*)KERN<T, N, U>, lp);                                                \
   hipLaunchKernelGGL((KERN<T, N, U>), gp, dim3(C::BT), lp, st, a, tw);
+#define NLFH_B1(D, X)                                                                   \
+  big((const void*)prec_b_fast<T, N, D, X>, ll);                                      \
+  hipLaunchKernelGGL((prec_b_fast<T, N, D, X>), gl, dim3(pass_b_threads<T, N, X>()), ll, st, a, tw, symp);
 #define NLFH_B(D)                                                                       \
-  big((const void*)prec_b_fast<T, N, D>, ll);                                         \
-  hipLaunchKernelGGL((prec_b_fast<T, N, D>), gl, dim3(C::BTB), ll, st, a, tw, symp);
+  if constexpr (sizeof(T) == 4) {                                                     \
+    if (lpb == C::LPB / 2) { NLFH_B1(D, C::LPB / 2) }                                  \
+    else if (lpb == 2 * C::LPB) { NLFH_B1(D, (C::L * 2 * C::LPB <= 1024 ? 2 * C::LPB : C::LPB)) }  \
+    else { NLFH_B1(D, 0) }                                                           \
+  } else {                                                                            \
+    NLFH_B1(D, 0)                                                                     \
+  }
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
       if (update == 4) { NLFH_A(prec_a_staged, 4) }
@@ -1015,6 +1041,7 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   else { NLFH_B(false) }
 #undef NLFH_A
 #undef NLFH_B
+#undef NLFH_B1
   if (dot == 1) {
     big((const void*)prec_c_fast<T, N, true>, lp);
     hipLaunchKernelGGL((prec_c_fast<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
@@ -1048,22 +1075,34 @@ int fast_line_len(int n, int elem_bytes) {
   }
 }
 
+bool fast_pass_b_ok(int n, int lpb) {
+  if (lpb == 0) return true;
+  switch (n) {
+    case 16: return lpb == FastCfg<float, 16>::LPB || pass_b_alt_ok<16>(lpb);
+    case 32: return lpb == FastCfg<float, 32>::LPB || pass_b_alt_ok<32>(lpb);
+    case 48: return lpb == FastCfg<float, 48>::LPB || pass_b_alt_ok<48>(lpb);
+    case 64: return lpb == FastCfg<float, 64>::LPB || pass_b_alt_ok<64>(lpb);
+    case 128: return lpb == FastCfg<float, 128>::LPB || pass_b_alt_ok<128>(lpb);
+    default: return false;
+  }
+}
+
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, int vec, hipStream_t st) {
+                            int update, int dot, int vec, int lpb, hipStream_t st) {
   switch (n) {
-    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
-    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
-    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
-    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
-    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, vec, st);
+    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
+    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
+    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
+    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
+    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template hipError_t launch_prec_fast<double>(int, const PrecArgs<double>&, int, int, const cplx<double>*,
-                                             const double*, int, int, int, hipStream_t);
+                                             const double*, int, int, int, int, hipStream_t);
 template hipError_t launch_prec_fast<float>(int, const PrecArgs<float>&, int, int, const cplx<float>*, const float*,
-                                            int, int, int, hipStream_t);
+                                            int, int, int, int, hipStream_t);
 
 }  // namespace nlfh

@@ -139,9 +139,12 @@ hipError_t launch_prec(const PrecArgs<T>& a, int nplanes, int nlines, int update
 bool fast_fft_size(int n);
 bool fast_staged(int n, int elem_bytes);  // the staged passes A and C (one round of lines) fit this length
 int fast_line_len(int n, int elem_bytes);
+// FP32 pass B of the fast path: lpb (ky, kz) positions per block is the default (0 or its value), or half or
+// double of it
+bool fast_pass_b_ok(int n, int lpb);
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, int vec, hipStream_t st);
+                            int update, int dot, int vec, int lpb, hipStream_t st);
 hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, double* ds, int64_t n, double emin,
                                   double p, hipStream_t st);
 hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);

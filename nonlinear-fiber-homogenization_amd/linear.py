@@ -190,6 +190,17 @@ class Plan:
     def set_tiling(self, shape_index, ty, xc, zpt):
         _lib.check(self._lib.nlfh_plan_set_tiling(self.handle, shape_index, ty, xc, zpt))
 
+    def set_fft_tiling(self, shape_index, pass_b_positions):
+        """FP32 pass B block shape of a cubic 16-128 shape: (ky, kz) positions per block (0 or the default, half or
+        double of it)."""
+        _lib.check(self._lib.nlfh_plan_set_fft_tiling(self.handle, shape_index, int(pass_b_positions)))
+
+    def get_fft_tiling(self, shape_index):
+        """(positions per block in use, the default) of the FP32 pass B; (0, 0) for the generic passes."""
+        v, d = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.nlfh_plan_get_fft_tiling(self.handle, shape_index, ctypes.byref(v), ctypes.byref(d)))
+        return v.value, d.value
+
     def set_inner_restart(self, restart):
         """Mixed precision: restart each FP32 inner PCG from p = z (True) or carry the last search direction
         over the refinement steps (False, the default; cubic 16-128 cells only, other plans restart)."""

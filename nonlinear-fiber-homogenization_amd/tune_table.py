@@ -1,10 +1,12 @@
 """Measure the matvec tiling table shipped with the package (data/tiling_gfx950.json): for every
 shape and precision, the autotuner's sweep (no cache) repeated `--repeat` times; the tiling with the
 lowest median time wins. `--batched` measures the entries of batched plans (a plan of
-autotune.batch_cells(shape) cells per sweep). Entries are merged into the existing table.
+autotune.batch_cells(shape) cells per sweep). `--fft` measures the FP32 pass B block shape of the
+preconditioner instead (autotune.tune_fft; keys "...|fftb"). Entries are merged into the existing table.
 
     python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 32 48 64 128
     python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 32 48 64 --precisions f32 --batched
+    python nonlinear-fiber-homogenization_amd/tune_table.py --shapes 16 32 48 64 128 --fft [--batched]
 """
 import argparse
 import json
@@ -21,6 +23,7 @@ def main():
     ap.add_argument("--m", type=int, default=6)
     ap.add_argument("--repeat", type=int, default=3)
     ap.add_argument("--batched", action="store_true")
+    ap.add_argument("--fft", action="store_true", help="the FP32 pass B block shape instead of the matvec tiling")
     ap.add_argument("--out", default=os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                                    "tiling_gfx950.json"))
     args = ap.parse_args()
@@ -36,6 +39,21 @@ def main():
         table = {}
     for n in args.shapes:
         shape = (n, n, n)
+        if args.fft:
+            runs = {}
+            for _ in range(args.repeat):
+                plan = Plan([shape], 1.0, 0.3)
+                _, tab = at.tune_fft(plan, 0, m=args.m, reps=10, use_cache=False, batched=args.batched)
+                for cand, t in tab:
+                    runs.setdefault(cand, []).append(t)
+            med = sorted((float(np.median(v)), c) for c, v in runs.items())
+            best_t, best = med[0]
+            key = at._key(shape, "f32", args.m, args.batched, "fftb")
+            table[key] = {"best": [best], "us": round(best_t, 2), "runner_up": [[c] for _, c in med[1:]]}
+            if args.batched:
+                table[key]["cells"] = at.batch_cells(shape)
+            print(key, best, f"{best_t:.1f} us", {c: round(t, 1) for t, c in med}, flush=True)
+            continue
         for prec in args.precisions:
             runs = {}
             for _ in range(args.repeat):

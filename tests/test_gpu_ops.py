@@ -161,3 +161,38 @@ def test_host_parameters_are_captured_at_call_time(gpu_pkg):
         torch.cuda.synchronize()
         assert torch.equal(C.cpu(), C_ref)
         assert torch.equal(f, f_ref)
+
+
+@pytest.mark.parametrize("n", [16, 32, 48, 64, 128])
+def test_precond_fp32_pass_b_block_shapes(gpu_pkg, n):
+    """Every FP32 pass B block shape the autotuner may pick (the default, half and double of its (ky, kz)
+    positions per block) gives the same preconditioner, against the oracle; unsupported values are refused."""
+    import ctypes
+    shape = (n, n, n)
+    rng = np.random.default_rng(60 + n)
+    m = 2 if n < 128 else 1
+    r = rng.standard_normal((m,) + shape + (3,))
+    lh = gpu_pkg.LinearHomogenization(shape)
+    plan = lh._plan
+    cur, default = plan.get_fft_tiling(0)
+    assert cur == default > 0
+    rp = gpu_pkg.to_planar(r, shape).float().contiguous()
+    lib = gpu_pkg.load_library()
+    ref = O.LinearRef(shape).precond(r) if n <= 64 else None
+    outs = []
+    for lpb in (default, default // 2, 2 * default):
+        plan.set_fft_tiling(0, lpb)
+        assert plan.get_fft_tiling(0) == (lpb, default)
+        z = torch.empty_like(rp)
+        assert lib.nlfh_precond(plan.handle, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(z.data_ptr()), m, 1,
+                                ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
+        got = gpu_pkg.from_planar(z).double().cpu().numpy()
+        if ref is not None:
+            assert _rel(got, ref) < 1e-5
+        outs.append(got)
+    # the x transforms of a position do not depend on which block holds it: identical results
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    with pytest.raises(gpu_pkg.NlfhError):
+        plan.set_fft_tiling(0, 3)
+    plan.set_fft_tiling(0, 0)
+    assert plan.get_fft_tiling(0) == (default, default)

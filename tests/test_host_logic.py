@@ -13,29 +13,39 @@ def _entries():
     for key, e in table.items():
         parts = key.split("|")
         shape = ast.literal_eval(parts[0])
-        yield key, tuple(shape), parts[1], int(parts[2]), len(parts) == 4 and parts[3] == "batch", e
+        kind = parts[3] if len(parts) > 3 and parts[3] != "batch" else ""
+        yield key, tuple(shape), parts[1], int(parts[2]), parts[-1] == "batch", kind, e
 
 
 def test_packaged_tilings_are_valid_candidates(pkg):
     """Every tiling the package ships is one the autotuner could have picked for that shape (tile rows,
-    x-chunk and z values per lane that fit the 512-thread block and the LDS of the FP64 kernels)."""
+    x-chunk and z values per lane that fit the 512-thread block and the LDS of the FP64 kernels; for the FP32
+    pass B entries, the three positions per block of autotune.fft_candidates around the default)."""
     from nlfh_amd import autotune as at
-    n = 0
-    for key, shape, prec, m, batched, e in _entries():
+    n = nb = 0
+    for key, shape, prec, m, batched, kind, e in _entries():
         assert prec in ("f32", "f64") and m == 6, key
+        if kind == "fftb":
+            assert prec == "f32" and shape[0] == shape[1] == shape[2] and shape[0] in (16, 32, 48, 64, 128), key
+            tried = sorted([e["best"][0]] + [r[0] for r in e.get("runner_up", [])])
+            d = tried[1]
+            assert tried == sorted(at.fft_candidates(d)) and d in (16, 32, 64), key
+            nb += 1
+            continue
+        assert kind == "", key
         assert tuple(e["best"]) in at.candidates(shape, 8), key
         for ru in e.get("runner_up", []):
             assert tuple(ru) in at.candidates(shape, 8), key
         n += 1
-    assert n >= 10
+    assert n >= 10 and nb >= 9
 
 
 def test_batched_keys_only_for_shapes_that_batch(pkg):
     """Batched entries exist only for shapes whose representative batch has several cells (a 128^3 cell fills
     the GPU alone and keeps its single-cell entry), and the key format is the autotuner's."""
     from nlfh_amd import autotune as at
-    for key, shape, prec, m, batched, e in _entries():
-        assert key == at._key(shape, prec, m, batched)
+    for key, shape, prec, m, batched, kind, e in _entries():
+        assert key == at._key(shape, prec, m, batched, kind)
         if batched:
             assert at.batch_cells(shape) > 1, key
             assert e.get("cells") in (None, at.batch_cells(shape)), key
