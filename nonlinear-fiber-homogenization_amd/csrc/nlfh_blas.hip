@@ -252,7 +252,7 @@ __global__ void cg_init_kernel(CgState st, const double* part_a, int nplanes, co
   if (threadIdx.x == 0) {
     st.it[cell] = 0;
     const int skip = st.skip != nullptr ? st.skip[cell] : 0;
-    st.done[cell] = (skip || notconv == 0 || st.maxit[0] <= 0) ? 1 : 0;
+    st.done[cell] = (skip || notconv == 0 || st.maxit[cell] <= 0) ? 1 : 0;
   }
 }
 
@@ -293,7 +293,7 @@ __global__ void cg_beta_kernel(CgState st, const double* part_a, int nplanes, co
   if (threadIdx.x == 0) {
     const int it = st.it[cell] + 1;
     st.it[cell] = it;
-    if (notconv == 0 || it >= st.maxit[0]) st.done[cell] = 1;
+    if (notconv == 0 || it >= st.maxit[cell]) st.done[cell] = 1;
   }
 }
 

@@ -207,14 +207,14 @@ struct nlfh_plan {
   float* d_dinv32 = nullptr;   // Chebyshev smoothing: 1 / diag K(s) per voxel
   double* d_strain = nullptr;
   double* d_params = nullptr;  // per cell: [0, ncells) FP64 tolerance, [ncells, 2 ncells) inner FP32 tolerance
-  int* d_iparams = nullptr;    // maxit64, maxit32
+  int* d_iparams = nullptr;    // per cell: iteration cap of the FP64 PCG [0, ncells), of an FP32 inner PCG [ncells, 2 ncells)
   double* d_ir = nullptr;      // rn_true (ncells*M), bn (ncells*M), mean (ncells*M*3)
   int* d_outer_done = nullptr;
   int* d_maxed = nullptr;      // per cell: iteration budget exhausted
   int* d_restart = nullptr;    // per cell: the next FP32 inner solve restarts from p = z
   double inner_tol = 1e-2;     // loosest relative tolerance of an FP32 inner solve
   bool inner_restart = false;  // restart every FP32 inner PCG from z_0 instead of carrying the direction over
-  int* h_pinned = nullptr;     // ncells * 3 ints
+  int* h_pinned = nullptr;     // ncells * 4 ints
   double* h_pinned_d = nullptr;
   std::vector<void*> allocs;
   int64_t bytes = 0;
@@ -543,7 +543,7 @@ int get_ready32(nlfh_plan* p) {
       p->w32.cs.alpha_prev = p->w32.st + 5 * p->ncells * p->M;
     }
     p->w32.cs.tol = p->d_params + p->ncells;
-    p->w32.cs.maxit = p->d_iparams + 1;
+    p->w32.cs.maxit = p->d_iparams + p->ncells;
     p->w32_ready = true;
   }
   return NLFH_OK;
@@ -895,10 +895,7 @@ int set_strain(nlfh_plan* p, const double* Es_host, int m, hipStream_t st) {
 }
 
 int set_maxit(nlfh_plan* p, int maxit, hipStream_t st) {
-  SmallParams v{};
-  v.i[0] = maxit;
-  v.i[1] = maxit;
-  HIP_TRY(launch_set_small(v, 2, nullptr, p->d_iparams, st));
+  HIP_TRY(launch_fill<int>(p->d_iparams, 2 * (int64_t)p->ncells, maxit, st));
   return NLFH_OK;
 }
 
@@ -1175,12 +1172,12 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->alloc(&p->d_part_e, (size_t)36 * p->chunks.size()))) return fail(rc);
   if ((rc = p->alloc(&p->d_strain, (size_t)6 * kMaxRhs))) return fail(rc);
   if ((rc = p->alloc(&p->d_params, 2 * (size_t)ncells))) return fail(rc);
-  if ((rc = p->alloc(&p->d_iparams, 2))) return fail(rc);
+  if ((rc = p->alloc(&p->d_iparams, 2 * (size_t)ncells))) return fail(rc);
   if ((rc = p->alloc(&p->d_ir, (size_t)ncells * M * 5))) return fail(rc);
   if ((rc = p->alloc(&p->d_outer_done, ncells))) return fail(rc);
   if ((rc = p->alloc(&p->d_maxed, ncells))) return fail(rc);
   if ((rc = p->alloc(&p->d_restart, ncells))) return fail(rc);
-  if (hipHostMalloc((void**)&p->h_pinned, sizeof(int) * 3 * ncells, hipHostMallocDefault) != hipSuccess) {
+  if (hipHostMalloc((void**)&p->h_pinned, sizeof(int) * 4 * ncells, hipHostMallocDefault) != hipSuccess) {
     set_error("hipHostMalloc failed");
     return fail(NLFH_ENOMEM);
   }
@@ -1443,8 +1440,9 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
   std::vector<int> total(nc, 0), its(nc, 0), stall(nc, 0);
   std::vector<double> rn_prev((size_t)nk, 0.0);
   const bool keep = p->dir32 && !p->inner_restart;
-  // per cell: the last inner solve continued a carried direction; the next one restarts
-  std::vector<int> resumed(nc, 0), restart_next(nc, 0);
+  // per cell: the last inner solve continued a carried direction; the next one restarts; the cell restarts for the
+  // rest of the solve (a continued solve hit its iteration cap)
+  std::vector<int> resumed(nc, 0), restart_next(nc, 0), sticky(nc, 0), cap(nc, maxit);
   const double* h_rin = p->h_pinned_d + 2 * nk;  // last inner solve: recursive ||r||, then ||b|| (= ||r / ||r||||)
   int* h = p->h_pinned;
   for (int outer = 0; outer < 200; ++outer) {
@@ -1508,7 +1506,7 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
     // inner FP32 solve of K d = r / ||r|| from d = 0 (continuing the previous direction after the first step)
     bool resume = false, any_restart = false;
     for (int c = 0; c < nc; ++c) {
-      resumed[c] = keep && outer > 0 && !odone[c] && !restart_next[c];
+      resumed[c] = keep && outer > 0 && !odone[c] && !restart_next[c] && !sticky[c];
       resume = resume || resumed[c];
       any_restart = any_restart || (!odone[c] && !resumed[c]);
     }
@@ -1516,6 +1514,16 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
       int* hr = p->h_pinned + 2 * nc;  // pinned: the copy may run after the host moves on
       for (int c = 0; c < nc; ++c) hr[c] = resumed[c] ? 0 : 1;
       HIP_TRY(hipMemcpyAsync(p->d_restart, hr, nc * sizeof(int), hipMemcpyHostToDevice, st));
+    }
+    // a continued inner solve gets at most twice the cell's inner iterations so far (at least 256), plus 64: a
+    // continuation that fails to converge is cut short, and the cell restarts from then on
+    if (keep && outer > 0) {
+      int* hc = p->h_pinned + 3 * nc;
+      for (int c = 0; c < nc; ++c) {
+        cap[c] = resumed[c] ? std::min(maxit, std::max(2 * total[c], 256) + 64) : maxit;
+        hc[c] = cap[c];
+      }
+      HIP_TRY(hipMemcpyAsync(p->d_iparams + nc, hc, nc * sizeof(int), hipMemcpyHostToDevice, st));
     }
     progress("refinement step %d: %d inner solves continue their direction", outer,
              (int)std::count(resumed.begin(), resumed.end(), 1));
@@ -1526,6 +1534,7 @@ int solve_mixed(nlfh_plan* p, int m, const double* u0, double tol, int maxit, in
     int* maxed = p->h_pinned + nc;  // pinned: the copy below may run after this loop iteration
     bool any_max = false;
     for (int c = 0; c < nc; ++c) {
+      if (!odone[c] && resumed[c] && its[c] >= cap[c]) sticky[c] = 1;
       if (!odone[c]) total[c] += its[c];
       maxed[c] = total[c] >= maxit ? 1 : 0;
       any_max = any_max || maxed[c];

@@ -108,7 +108,7 @@ struct CgState {
   int* it;        // [cell]
   int* done;      // [cell]
   const double* tol;  // [cell]: relative tolerance of each cell
-  const int* maxit;   // [1]
+  const int* maxit;   // [cell]: iteration cap
   const int* skip;    // nullable [cell]: cells that take no iterations (converged outer loop)
   int m;
 };
