@@ -158,8 +158,8 @@ def main():
     t_tune = time.perf_counter() - t_tune
     tiling = {f"{sh[0]}x{sh[1]}x{sh[2]}": bh._plan.get_tiling(i)
               for i, sh in enumerate(dict.fromkeys(shapes))}
-    pass_b = {f"{sh[0]}x{sh[1]}x{sh[2]}": list(bh._plan.get_fft_tiling(i))
-              for i, sh in enumerate(dict.fromkeys(shapes))}
+    fft_tiling = {f"{sh[0]}x{sh[1]}x{sh[2]}": list(bh._plan.get_fft_tiling(i))
+                  for i, sh in enumerate(dict.fromkeys(shapes))}
     rho_d = bh.pack(rhos)
 
     if args.precond_only:
@@ -221,7 +221,8 @@ def main():
                                        streams=args.streams, inner_restart=args.inner_restart)
         for i in range(len(set(shapes))):
             bz._plan.set_tiling(i, *bh._plan.get_tiling(i))
-            bz._plan.set_fft_tiling(i, bh._plan.get_fft_tiling(i)[0])
+            fb, _, fa = bh._plan.get_fft_tiling(i)
+            bz._plan.set_fft_tiling(i, fb, fa)
         bz.C_full(rho_d)
         torch.cuda.synchronize()
         tz = time.perf_counter()
@@ -267,7 +268,8 @@ def main():
             "streams": args.streams, "inner_restart": bool(args.inner_restart),
             "tiling": {"source": "given" if args.tiling else ("default" if args.no_tune else "autotuned"),
                        "ty_xc_zpt": tiling,
-                       "fft_pass_b_positions_default": pass_b}, "tune_seconds": round(t_tune, 2),
+                       "fft_pass_b_positions_default_pass_a_planes": fft_tiling},
+            "tune_seconds": round(t_tune, 2),
             "learned_start": bool(net), "learned_start_fallback_steps": fallbacks,
             "learned_start_smoothing": list(smoothing) if (net and smoothing) else None,
             "fp64_fallback_cells": fallback_fp64[-1] if fallback_fp64 else [],

@@ -54,13 +54,15 @@ int nlfh_plan_destroy(nlfh_plan* plan);
 int nlfh_plan_set_tiling(nlfh_plan* plan, int shape_index, int ty, int xc, int zpt);
 int nlfh_plan_get_tiling(const nlfh_plan* plan, int shape_index, int* ty, int* xc, int* zpt);
 
-/* Block shape of the FP32 pass B (the x transforms, Kinv and inverse x transforms of the preconditioner) of a
- * cubic 16-128 shape: pass_b_positions (ky, kz) positions per block, 0 or the default (256 threads; 16 x 16 threads
- * at 128), half or double of it. get returns the value in use and the default (0 for shapes on the generic passes,
- * whose line length is not tunable). No counterpart in the reference (a launch-shape choice of
- * LinearHomogenization.precond, nlfiber/linear.py:167-170). */
-int nlfh_plan_set_fft_tiling(nlfh_plan* plan, int shape_index, int pass_b_positions);
-int nlfh_plan_get_fft_tiling(const nlfh_plan* plan, int shape_index, int* pass_b_positions, int* pass_b_default);
+/* Block shapes of the FP32 preconditioner passes of a cubic 16-128 shape: pass_b_positions (ky, kz) positions per
+ * block of pass B (the x transforms, Kinv and inverse x transforms), 0 or the default (256 threads; 16 x 16 threads
+ * at 128), half or double of it; pass_a_planes x-planes per block of the spectral pass A of the FP32 inner iteration
+ * (1, or 2: the second plane's loads in flight during the first's transforms). get returns the values in use and
+ * the pass B default (0 for shapes on the generic passes, whose launch shapes are not tunable). No counterpart in
+ * the reference (launch-shape choices of LinearHomogenization.precond, nlfiber/linear.py:167-170). */
+int nlfh_plan_set_fft_tiling(nlfh_plan* plan, int shape_index, int pass_b_positions, int pass_a_planes);
+int nlfh_plan_get_fft_tiling(const nlfh_plan* plan, int shape_index, int* pass_b_positions, int* pass_b_default,
+                             int* pass_a_planes);
 
 /* Energy-optimal (Ritz-Galerkin) start from predicted correctors: for every cell and right-hand side
  * I, u_out_I = sum_J a_IJ u_in_J with a_I minimising the PCG energy 1/2 u.Ku + f_I.u over the span

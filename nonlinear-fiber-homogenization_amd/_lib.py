@@ -39,8 +39,8 @@ SIGNATURES = [
     ("nlfh_plan_get_tiling", ctypes.c_int, [_vp, ctypes.c_int, _c_int_p, _c_int_p, _c_int_p]),
     ("nlfh_plan_set_inner_tol", ctypes.c_int, [_vp, ctypes.c_double]),
     ("nlfh_plan_set_inner_restart", ctypes.c_int, [_vp, ctypes.c_int]),
-    ("nlfh_plan_set_fft_tiling", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
-    ("nlfh_plan_get_fft_tiling", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
+    ("nlfh_plan_set_fft_tiling", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    ("nlfh_plan_get_fft_tiling", ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, _vp]),
     ("nlfh_stiffness_scale", ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_double, _vp, _vp, _vp]),
     ("nlfh_apply_K", ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _vp]),
     ("nlfh_load", ctypes.c_int, [_vp, _vp, _c_double_p, ctypes.c_int, _vp, _vp]),

@@ -1,6 +1,7 @@
 """Autotuner for the launch shapes of each grid shape of a plan: the matvec tiling (ty node rows per tile, xc
-x-planes per chunk, zpt z values per lane) and, for the FP32 preconditioner of cubic 16-128 cells, the block shape
-of pass B ((ky, kz) positions per block: the default, half or double of it; `tune_fft`).
+x-planes per chunk, zpt z values per lane) and, for the FP32 preconditioner of cubic 16-128 cells, the block shapes
+of passes B and A ((ky, kz) positions per block of pass B: the default, half or double of it; x-planes per block of
+the spectral pass A: 1 or 2; `tune_fft`).
 
 Candidates are every valid combination from small sets; each is timed on the device with HIP events
 over repeated launches of the operator K(s) u on m right-hand sides (the same kernel family the
@@ -191,36 +192,69 @@ def time_precond(plan, r, z, m, reps):
     return e0.elapsed_time(e1) / reps * 1e3
 
 
+def time_inner_iterations(plan, s, m, iters):
+    """Microseconds per FP32 inner PCG iteration of a mixed solve over the plan's cells (HIP events around one solve
+    with the iteration cap `iters`, which ends unconverged): the spectral pass A only runs there."""
+    lib = _lib.load()
+    dev = s.device
+    nc = len(plan.shapes)
+    st = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(st.cuda_stream)
+    u = torch.empty(int(s.numel()) * 3 * m, dtype=torch.float64, device=dev)
+    es = (ctypes.c_double * (6 * m))(*[1.0 if j == i else 0.0 for i in range(m) for j in range(6)])
+    it = (ctypes.c_int * nc)()
+    cv = (ctypes.c_int * nc)()
+    rr = (ctypes.c_double * (nc * m))()
+
+    def solve():
+        _lib.check(lib.nlfh_solve(plan.handle, ctypes.c_void_p(s.data_ptr()), es, m, None, ctypes.c_void_p(u.data_ptr()),
+                                  1e-14, iters, 1, it, cv, rr, sp))
+
+    solve()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    solve()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / max(1, min(it))
+
+
 def fft_candidates(default):
-    """Pass B positions per block to try: the default, half and double of it."""
-    return [default, default // 2, 2 * default]
+    """(pass B positions per block, pass A planes per block) to try: the default, half and double positions, each
+    with one and two planes."""
+    return [(b, q) for b in (default, default // 2, 2 * default) for q in (1, 2)]
 
 
 def tune_fft(plan, shape_index=0, m=6, reps=10, use_cache=True, verbose=False, batched=False):
-    """Pick the FP32 pass B block shape of distinct shape `shape_index` of `plan` (cubic 16-128 shapes; others are
-    left alone) by timing nlfh_precond in FP32 (on a plan of batch_cells(shape) cells with `batched`) and set it.
-    Returns (best positions per block or None, [(positions, microseconds), ...])."""
+    """Pick the FP32 preconditioner block shapes of distinct shape `shape_index` of `plan` (cubic 16-128 shapes;
+    others are left alone) and set them: pass B's positions per block by timing nlfh_precond in FP32, then pass A's
+    planes per block by timing FP32 inner iterations of a capped mixed solve (the spectral pass A runs only there),
+    on a plan of batch_cells(shape) cells with `batched`. Returns ((positions, planes) or None, [(candidate,
+    microseconds), ...])."""
     distinct = []
     for s in plan.shapes:
         if s not in distinct:
             distinct.append(s)
     shape = distinct[shape_index]
-    _, default = plan.get_fft_tiling(shape_index)
+    _, default, _ = plan.get_fft_tiling(shape_index)
     if default == 0:
         return None, []
     batched = batched and batch_cells(shape) > 1
     dev = torch.device("cuda", torch.cuda.current_device())
     key = f"{torch.cuda.get_device_name(dev)}|" + _key(shape, "f32", m, batched, "fftb")
     cache = _load_cache() if use_cache else {}
-    if use_cache and key in cache:
-        best = int(cache[key]["best"][0])
-        plan.set_fft_tiling(shape_index, best)
+    if use_cache and key in cache and len(cache[key]["best"]) == 2:
+        best = tuple(int(v) for v in cache[key]["best"])
+        plan.set_fft_tiling(shape_index, *best)
         return best, []
     if use_cache:
         best = _packaged(shape, "f32", m, dev, batched, "fftb")
-        if best is not None:
-            plan.set_fft_tiling(shape_index, int(best[0]))
-            return int(best[0]), []
+        if best is not None and len(best) == 2:
+            best = tuple(int(v) for v in best)
+            plan.set_fft_tiling(shape_index, *best)
+            return best, []
     from .linear import Plan
     nb = batch_cells(shape) if batched else 1
     tplan = Plan([shape] * nb, 1.0, 0.3, m_layout=m)
@@ -228,19 +262,31 @@ def tune_fft(plan, shape_index=0, m=6, reps=10, use_cache=True, verbose=False, b
     r = torch.randn(nb * m * 3 * nvox, dtype=torch.float32, device=dev)
     z = torch.empty_like(r)
     table = []
-    for cand in fft_candidates(default):
+    for cand in sorted({c[0] for c in fft_candidates(default)}, key=lambda b: (b != default, b)):
         try:
-            tplan.set_fft_tiling(0, cand)
+            tplan.set_fft_tiling(0, cand, 1)
         except _lib.NlfhError:
             continue
         t = time_precond(tplan, r, z, m, reps)
-        table.append((cand, t))
+        table.append(((cand, 1), t))
         if verbose:
-            print(f"  pass B positions per block {cand:4d}: {t:9.2f} us", flush=True)
+            print(f"  pass B positions per block {cand:4d}: {t:9.2f} us per application", flush=True)
     table.sort(key=lambda x: x[1])
-    best = table[0][0]
-    plan.set_fft_tiling(shape_index, best)
+    b_best = table[0][0][0]
+    # pass A planes per block: whole FP32 inner iterations with the chosen pass B
+    s = torch.rand(nb * nvox, dtype=torch.float64, device=dev) * 0.99 + 0.01
+    iters = 64
+    t_it = []
+    for q in (1, 2):
+        tplan.set_fft_tiling(0, b_best, q)
+        t = time_inner_iterations(tplan, s, m, iters)
+        t_it.append(((b_best, q), t))
+        if verbose:
+            print(f"  pass A planes per block {q}: {t:9.2f} us per inner iteration", flush=True)
+    t_it.sort(key=lambda x: x[1])
+    best = t_it[0][0]
+    plan.set_fft_tiling(shape_index, *best)
     if use_cache:
-        cache[key] = {"best": [best], "us": table[0][1]}
+        cache[key] = {"best": list(best), "us_per_iteration": t_it[0][1]}
         _save_cache(cache)
-    return best, table
+    return best, table + t_it

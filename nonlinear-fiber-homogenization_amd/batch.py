@@ -85,9 +85,10 @@ class BatchedHomogenization:
                 want = self._plan.get_tiling(distinct.index(sh))
                 if plan.get_tiling(j) != want:
                     plan.set_tiling(j, *want)
-                want_b = self._plan.get_fft_tiling(distinct.index(sh))[0]
-                if plan.get_fft_tiling(j)[0] != want_b:
-                    plan.set_fft_tiling(j, want_b)
+                wb, _, wa = self._plan.get_fft_tiling(distinct.index(sh))
+                hb, _, ha = plan.get_fft_tiling(j)
+                if (hb, ha) != (wb, wa):
+                    plan.set_fft_tiling(j, wb, wa)
         return self._subs
 
     def autotune(self, precision=None):
@@ -109,7 +110,7 @@ class BatchedHomogenization:
             if prec == "f32":
                 bb, _ = at.tune_fft(tmp, 0, m=self.m, batched=per_group >= 2)
                 if bb is not None:
-                    self._plan.set_fft_tiling(i, bb)
+                    self._plan.set_fft_tiling(i, *bb)
 
     def pack(self, rhos):
         """Concatenate the cells' densities into one device array (cell-major)."""

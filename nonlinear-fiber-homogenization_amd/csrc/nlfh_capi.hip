@@ -91,6 +91,7 @@ struct PrecGroup {
   int line_begin[2], line_count[2];  // [0] float64 line items, [1] float32
   int vec;  // every cell's field and spectrum offsets allow 16-byte vector access (staged passes)
   int lpb32 = 0;  // FP32 pass B positions per block on the fast path (0: the default)
+  int ppa32 = 1;  // x-planes per block of the FP32 spectral pass A (1 or 2)
 };
 
 template <typename T>
@@ -671,13 +672,14 @@ int run_prec(nlfh_plan* p, PrecArgs<T>& a, int update, int dot, hipStream_t st) 
   for (const PrecGroup& g : p->pgroups) {
     if (g.plane_count == 0) continue;
     a.plane_base = g.plane_begin;
+    a.plane_count = g.plane_count;
     a.line_base = g.line_begin[pr];
     if (g.fast) {
       const Shape& sh = p->shapes[g.shape];
       const cplx<T>* tw = tw_ptr<T>(p) + p->lens[sh.len[0]].tw_off;
       const T* symp = symsoa_ptr<T>(p) + p->symsoa_off[g.shape];
       HIP_TRY(launch_prec_fast<T>(g.fast, a, g.plane_count, g.line_count[pr], tw, symp, update, dot, g.vec,
-                                  sizeof(T) == 4 ? g.lpb32 : 0, st));
+                                  sizeof(T) == 4 ? g.lpb32 : 0, sizeof(T) == 4 ? g.ppa32 : 1, st));
     } else {
       // the generic passes update r and x together and take r . z in pass C (lean mode never gets here)
       if (dot >= 2 || update == 1 || update == 4 || a.rspec != nullptr) return NLFH_EUNSUPPORTED;
@@ -1232,17 +1234,19 @@ int nlfh_plan_set_tiling(nlfh_plan* p, int shape_index, int ty, int xc, int zpt)
   return build_mv_items(p);
 }
 
-int nlfh_plan_set_fft_tiling(nlfh_plan* p, int shape_index, int pass_b_positions) {
+int nlfh_plan_set_fft_tiling(nlfh_plan* p, int shape_index, int pass_b_positions, int pass_a_planes) {
   CHECK_ARG(p != nullptr, "plan is null");
   CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
   PrecGroup* grp = nullptr;
   for (PrecGroup& g : p->pgroups)
     if (g.shape == shape_index) grp = &g;
   CHECK_ARG(grp != nullptr, "shape has no cells");
+  CHECK_ARG(pass_a_planes == 1 || pass_a_planes == 2, "pass_a_planes must be 1 or 2");
   if (!grp->fast) {
-    CHECK_ARG(pass_b_positions == 0, "the generic passes have no tunable block shape");
+    CHECK_ARG(pass_b_positions == 0 && pass_a_planes == 1, "the generic passes have no tunable block shape");
     return NLFH_OK;
   }
+  grp->ppa32 = pass_a_planes;  // every fast cell has an even nx: a block's two planes share cell and component
   CHECK_ARG(fast_pass_b_ok(grp->fast, pass_b_positions),
             "pass_b_positions must be 0, the default, or half or double the default");
   grp->lpb32 = pass_b_positions == fast_line_len(grp->fast, 4) ? 0 : pass_b_positions;
@@ -1255,7 +1259,8 @@ int nlfh_plan_set_fft_tiling(nlfh_plan* p, int shape_index, int pass_b_positions
   return build_lines32(p, true);
 }
 
-int nlfh_plan_get_fft_tiling(const nlfh_plan* p, int shape_index, int* pass_b_positions, int* pass_b_default) {
+int nlfh_plan_get_fft_tiling(const nlfh_plan* p, int shape_index, int* pass_b_positions, int* pass_b_default,
+                             int* pass_a_planes) {
   CHECK_ARG(p != nullptr, "plan is null");
   CHECK_ARG(shape_index >= 0 && shape_index < (int)p->shapes.size(), "shape_index out of range");
   for (const PrecGroup& g : p->pgroups) {
@@ -1263,6 +1268,7 @@ int nlfh_plan_get_fft_tiling(const nlfh_plan* p, int shape_index, int* pass_b_po
     const int def = g.fast ? fast_line_len(g.fast, 4) : 0;
     if (pass_b_positions) *pass_b_positions = g.lpb32 > 0 ? g.lpb32 : def;
     if (pass_b_default) *pass_b_default = def;
+    if (pass_a_planes) *pass_a_planes = g.ppa32;
     return NLFH_OK;
   }
   set_error("shape has no cells");

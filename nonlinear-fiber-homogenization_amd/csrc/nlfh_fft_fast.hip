@@ -353,10 +353,15 @@ struct StageCfg {
 // block from the matvec's partials with the fixed-order sum and guard of cg_alpha_kernel (bitwise its
 // value), and the block of plane 0, component 0 stores it for pass C. Without UPD 4, the plane's spectrum
 // goes to a.rspec when that is set (the start of such an iteration), else to a.spec.
-template <typename T, int N, int UPD>
+// PPB (spectral update only): x-planes per block. With 2, a block transforms two consecutive x-planes of one
+// (cell, component) in turn and issues the loads of the second (Ap and the residual spectrum) as soon as the first
+// is staged in LDS, so they are in flight during the first plane's transforms (the passes are LDS-bound to four
+// blocks per CU, which otherwise leaves their loads to the blocks' own load phases).
+template <typename T, int N, int UPD, int PPB = 1>
 __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   constexpr bool UPDATE = UPD == 1 || UPD == 2;
   constexpr bool SPEC = UPD == 4;
+  static_assert(PPB == 1 || (PPB == 2 && SPEC), "two planes per block: spectral update only");
   using C = FastCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
   constexpr int RS = StageCfg<T, N>::RS;
@@ -370,17 +375,17 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   cplx<T>* xch = plane + N * NH;
   cplx<T>* dcn = xch + C::LPR * C::XS;
   T* rp = reinterpret_cast<T*>(xch);
-  const int item = a.plane_base + blockIdx.x;
-  const PlaneItem pi = a.planes[item];
-  if (a.done != nullptr && a.done[pi.cell]) return;
-  const CellDesc cd = a.cells[pi.cell];
+  const int item0 = a.plane_base + (int)blockIdx.x * PPB;
+  if (PPB > 1 && (int)blockIdx.x * PPB >= a.plane_count) return;
+  const PlaneItem pi0 = a.planes[item0];
+  if (a.done != nullptr && a.done[pi0.cell]) return;
+  const CellDesc cd = a.cells[pi0.cell];
   const int rhs = blockIdx.y;
   const int tid = threadIdx.x;
   // twiddles: loaded into a register here, stored to LDS with the plane (one wait for all loads)
   static_assert(BT >= N, "one twiddle per thread");
   cplx<T> twv{};
   if (tid < N) twv = twg[tid];
-  const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   using V = typename VT::type;
   constexpr int NV = N * N / VN;
   // without the real-space update (UPD 0 and 4) a thread's whole share of the plane is loaded into
@@ -388,53 +393,70 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   // the LDS store that follows it); they are issued ahead of the other loads, which are consumed later
   constexpr bool BATCH = !UPDATE;
   constexpr int LPT = BATCH ? (NV + BT - 1) / BT : 1;
-  const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + fo);
-  V lbuf[LPT];
-  if constexpr (BATCH) {
+  // SPEC: the plane's residual spectrum is loaded with it, into registers, so that its latency passes
+  // during the transforms (it is consumed by the final update)
+  constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
+  constexpr int NVS = N * NH / CPV;
+  constexpr int RPT = SPEC ? (NVS + BT - 1) / BT : 1;
+  V lbuf[PPB][LPT];
+  V rpre[PPB][RPT];
+  auto field_off = [&](const PlaneItem& q) {
+    return cd.off_field + ((int64_t)rhs * 3 + q.comp) * cd.nvox + (int64_t)q.x * N * N;
+  };
+  auto spec_ptr = [&](const PlaneItem& q) {
+    return reinterpret_cast<V*>((a.rspec != nullptr ? a.rspec : a.spec) + cd.off_spec +
+                                ((int64_t)rhs * 3 + q.comp) * cd.nspec + (int64_t)q.x * N * NH);
+  };
+  auto issue = [&](int q, const PlaneItem& pq) {
+    if constexpr (BATCH) {
+      const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + field_off(pq));
 #pragma unroll
-    for (int j = 0; j < LPT; ++j) {
-      const int i = tid + j * BT;
-      if (i < NV) lbuf[j] = Rv[i];
+      for (int j = 0; j < LPT; ++j) {
+        const int i = tid + j * BT;
+        if (i < NV) lbuf[q][j] = Rv[i];
+      }
     }
-  }
+    if constexpr (SPEC) {
+      const V* rs = spec_ptr(pq);
+#pragma unroll
+      for (int j = 0; j < RPT; ++j) {
+        const int i = tid + j * BT;
+        if (i < NVS) rpre[q][j] = rs[i];
+      }
+    }
+  };
+  issue(0, pi0);
   T alpha = T(0);
-  if (UPDATE) alpha = (T)a.alpha[pi.cell * a.m + rhs];
+  if (UPDATE) alpha = (T)a.alpha[pi0.cell * a.m + rhs];
   __shared__ double alpha_sh;
   if (SPEC && tid < 64) {
-    const int k = pi.cell * a.m + rhs;
-    const double pAp = wave_sum(a.part_mv, a.nmv, rhs, a.mv_range, pi.cell);
+    const int k = pi0.cell * a.m + rhs;
+    const double pAp = wave_sum(a.part_mv, a.nmv, rhs, a.mv_range, pi0.cell);
     const double al = a.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
     if (tid == 0) {
       alpha_sh = al;
-      if (pi.x == 0 && pi.comp == 0) {  // no other block of this pass reads the stored values
+      if (pi0.x == 0 && pi0.comp == 0) {  // no other block of this pass reads the stored values
         a.alpha_prev_out[k] = a.alpha_out[k];
         a.alpha_out[k] = al;
       }
     }
   }
-  double acc = 0.0;
-  // SPEC: this plane's residual spectrum is loaded now, into registers, so that its latency passes
-  // during the transforms (it is consumed by the final update)
-  constexpr int CPV = 16 / sizeof(cplx<T>);  // complex values per vector (2 float, 1 double)
-  constexpr int NVS = N * NH / CPV;
-  constexpr int RPT = SPEC ? (NVS + BT - 1) / BT : 1;
-  V* const rsp = reinterpret_cast<V*>((a.rspec != nullptr ? a.rspec : a.spec) + cd.off_spec +
-                                      ((int64_t)rhs * 3 + pi.comp) * cd.nspec + (int64_t)pi.x * N * NH);
-  V rpre[RPT];
-  if constexpr (SPEC) {
 #pragma unroll
-    for (int j = 0; j < RPT; ++j) {
-      const int i = tid + j * BT;
-      if (i < NVS) rpre[j] = rsp[i];
-    }
-  }
+  for (int q = 0; q < PPB; ++q) {
+  const int item = item0 + q;
+  if (PPB > 1 && q > 0 && (int)blockIdx.x * PPB + q >= a.plane_count) break;
+  const PlaneItem pi = q == 0 ? pi0 : a.planes[item];
+  const int64_t fo = field_off(pi);
+  const V* Rv = reinterpret_cast<const V*>((SPEC ? a.Ap : a.r) + fo);
+  V* const rsp = spec_ptr(pi);
+  double acc = 0.0;
   // load (and update) the plane with 16-byte vectors into rp
   {
     V* ROv = reinterpret_cast<V*>(a.r_out + fo);
     const V* APv = reinterpret_cast<const V*>(a.Ap + fo);
     V* Xv = reinterpret_cast<V*>(a.x + fo);
     const V* Pv = reinterpret_cast<const V*>(a.p + fo);
-    if (tid < N) tw[tid] = twv;
+    if (q == 0 && tid < N) tw[tid] = twv;
     // vector i of the plane into the staged real plane (and its norm)
     auto put = [&](int i, const V& rv) {
       const int e = i * VN;
@@ -450,7 +472,9 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
     if constexpr (BATCH) {
 #pragma unroll
       for (int j = 0; j < LPT; ++j)
-        if (tid + j * BT < NV) put(tid + j * BT, lbuf[j]);
+        if (tid + j * BT < NV) put(tid + j * BT, lbuf[q][j]);
+      // the next plane's loads go out now, to be in flight during this plane's transforms
+      if (q + 1 < PPB && (int)blockIdx.x * PPB + q + 1 < a.plane_count) issue(q + 1, a.planes[item + 1]);
     } else {
       for (int i = tid; i < NV; i += BT) {
         V rv = Rv[i];
@@ -581,7 +605,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
       for (int j = 0; j < RPT; ++j) {
         const int i = tid + j * BT;
         if (i >= NVS) break;
-        V rh = rpre[j];
+        V rh = rpre[q][j];
         const V ap = src[i];
         if constexpr (VN == 4) {
           rh.x -= alpha * ap.x; rh.y -= alpha * ap.y; rh.z -= alpha * ap.z; rh.w -= alpha * ap.w;
@@ -603,6 +627,8 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   }
   const double tsum = block_sum(acc, red);
   if (tid == 0 && a.part_a) a.part_a[(int64_t)rhs * a.nplanes + item] = tsum;
+  if (PPB > 1) __syncthreads();  // the next plane reuses the LDS
+  }
 }
 
 // DIR (FP32 inner iteration): instead of z, the new search direction p_{k+1} = z + beta_k p_k goes to
@@ -986,7 +1012,7 @@ constexpr bool pass_b_alt_ok(int lpb) {
 // direction, 4 as 3 with the paired iterate step of odd iterations (staged pass only, see prec_c_staged)
 template <typename T, int N>
 static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                                int update, int dot, int vec, int lpb, hipStream_t st) {
+                                int update, int dot, int vec, int lpb, int ppa, hipStream_t st) {
   using C = FastCfg<T, N>;
   if (lpb == C::LPB) lpb = 0;
   if (lpb != 0 && (sizeof(T) != 4 || !pass_b_alt_ok<N>(lpb))) return hipErrorInvalidValue;
@@ -1011,7 +1037,10 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   }
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update == 4) { NLFH_A(prec_a_staged, 4) }
+      if (update == 4 && ppa == 2 && sizeof(T) == 4) {
+        big((const void*)prec_a_staged<T, N, 4, 2>, lp);
+        hipLaunchKernelGGL((prec_a_staged<T, N, 4, 2>), dim3((nplanes + 1) / 2, a.m, 1), dim3(C::BT), lp, st, a, tw);
+      } else if (update == 4) { NLFH_A(prec_a_staged, 4) }
       else if (update == 1) { NLFH_A(prec_a_staged, 1) }
       else if (update) { NLFH_A(prec_a_staged, 2) }
       else { NLFH_A(prec_a_staged, 0) }
@@ -1089,20 +1118,21 @@ bool fast_pass_b_ok(int n, int lpb) {
 
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, int vec, int lpb, hipStream_t st) {
+                            int update, int dot, int vec, int lpb, int ppa, hipStream_t st) {
+  if (ppa != 1 && ppa != 2) return hipErrorInvalidValue;
   switch (n) {
-    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
-    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
-    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
-    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
-    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, st);
+    case 16: return launch_fast_n<T, 16>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, ppa, st);
+    case 32: return launch_fast_n<T, 32>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, ppa, st);
+    case 48: return launch_fast_n<T, 48>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, ppa, st);
+    case 64: return launch_fast_n<T, 64>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, ppa, st);
+    case 128: return launch_fast_n<T, 128>(a, nplanes, nlines, tw, symp, update, dot, vec, lpb, ppa, st);
     default: return hipErrorInvalidValue;
   }
 }
 
 template hipError_t launch_prec_fast<double>(int, const PrecArgs<double>&, int, int, const cplx<double>*,
-                                             const double*, int, int, int, int, hipStream_t);
+                                             const double*, int, int, int, int, int, hipStream_t);
 template hipError_t launch_prec_fast<float>(int, const PrecArgs<float>&, int, int, const cplx<float>*, const float*,
-                                            int, int, int, int, hipStream_t);
+                                            int, int, int, int, int, hipStream_t);
 
 }  // namespace nlfh

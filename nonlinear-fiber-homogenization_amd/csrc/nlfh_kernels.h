@@ -87,6 +87,7 @@ struct PrecArgs {
   int nlines;
   int rows_per_batch;    // row pairs per LDS batch in passes A and C
   int plane_base;        // first plane item of this launch
+  int plane_count;       // plane items of this launch
   int line_base;         // first line item of this launch
 };
 
@@ -142,9 +143,10 @@ int fast_line_len(int n, int elem_bytes);
 // FP32 pass B of the fast path: lpb (ky, kz) positions per block is the default (0 or its value), or half or
 // double of it
 bool fast_pass_b_ok(int n, int lpb);
+// ppa: x-planes per block of the FP32 spectral pass A (1 or 2)
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,
-                            int update, int dot, int vec, int lpb, hipStream_t st);
+                            int update, int dot, int vec, int lpb, int ppa, hipStream_t st);
 hipError_t launch_stiffness_scale(const double* rho, double* s64, float* s32, double* ds, int64_t n, double emin,
                                   double p, hipStream_t st);
 hipError_t launch_d2f(const double* in, float* out, int64_t n, hipStream_t st);

@@ -190,16 +190,19 @@ class Plan:
     def set_tiling(self, shape_index, ty, xc, zpt):
         _lib.check(self._lib.nlfh_plan_set_tiling(self.handle, shape_index, ty, xc, zpt))
 
-    def set_fft_tiling(self, shape_index, pass_b_positions):
-        """FP32 pass B block shape of a cubic 16-128 shape: (ky, kz) positions per block (0 or the default, half or
-        double of it)."""
-        _lib.check(self._lib.nlfh_plan_set_fft_tiling(self.handle, shape_index, int(pass_b_positions)))
+    def set_fft_tiling(self, shape_index, pass_b_positions, pass_a_planes=1):
+        """FP32 preconditioner block shapes of a cubic 16-128 shape: pass B (ky, kz) positions per block (0 or the
+        default, half or double of it) and x-planes per block of the spectral pass A (1 or 2)."""
+        _lib.check(self._lib.nlfh_plan_set_fft_tiling(self.handle, shape_index, int(pass_b_positions),
+                                                      int(pass_a_planes)))
 
     def get_fft_tiling(self, shape_index):
-        """(positions per block in use, the default) of the FP32 pass B; (0, 0) for the generic passes."""
-        v, d = ctypes.c_int(), ctypes.c_int()
-        _lib.check(self._lib.nlfh_plan_get_fft_tiling(self.handle, shape_index, ctypes.byref(v), ctypes.byref(d)))
-        return v.value, d.value
+        """(pass B positions per block in use, its default, pass A planes per block); the pass B values are 0 for
+        the generic passes."""
+        v, d, q = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.nlfh_plan_get_fft_tiling(self.handle, shape_index, ctypes.byref(v), ctypes.byref(d),
+                                                      ctypes.byref(q)))
+        return v.value, d.value, q.value
 
     def set_inner_restart(self, restart):
         """Mixed precision: restart each FP32 inner PCG from p = z (True) or carry the last search direction

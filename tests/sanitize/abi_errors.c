@@ -33,7 +33,7 @@ int main(void) {
   EXPECT(nlfh_cheb_smooth(NULL, NULL, Es, 1, NULL, NULL, 8, 0.1, 1.0, NULL, NULL) != 0, "smooth without plan");
   EXPECT(nlfh_ritz_start(NULL, NULL, Es, 1, NULL, NULL, NULL, NULL) != 0, "ritz without plan");
   EXPECT(nlfh_plan_set_inner_restart(NULL, 0) != 0, "inner restart without plan");
-  EXPECT(nlfh_plan_set_fft_tiling(NULL, 0, 32) != 0, "fft tiling without plan");
+  EXPECT(nlfh_plan_set_fft_tiling(NULL, 0, 32, 1) != 0, "fft tiling without plan");
   /* host arithmetic: the spectral bound of the Jacobi-scaled operator (24x24 element matrix, power iteration) */
   const double b = nlfh_jacobi_bound(1.0, 0.3);
   EXPECT(b > 5.31 && b < 5.33, "jacobi bound at nu = 0.3");

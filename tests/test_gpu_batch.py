@@ -200,21 +200,23 @@ def test_mixed_precision_with_two_and_three_right_hand_sides(gpu_pkg):
     assert it3m <= 1.25 * it3d + 10
 
 
-def test_pass_b_block_shape_keeps_the_solve(gpu_pkg):
-    """A mixed-precision batch solved with every FP32 pass B block shape gives the same C to 1e-12 and the same
-    iteration counts to 2 (the block shape regroups the fixed-order r . z partial sums, so the scalars round
-    differently)."""
-    rhos = [gpu_pkg.cells.random_cell(n, seed)[0] for n, seed in ((32, 31), (32, 32), (16, 33))]
+def test_preconditioner_block_shapes_keep_the_solve(gpu_pkg):
+    """A mixed-precision batch solved with every FP32 pass B block shape and one or two pass A planes per block
+    gives the same C to 1e-12 and the same iteration counts to 2 (the pass B shape regroups the fixed-order r . z
+    partial sums, so the scalars round differently); the pass A planes alone change nothing (bitwise)."""
+    rhos = [gpu_pkg.cells.random_cell(n, seed)[0] for n, seed in ((32, 31), (32, 32), (16, 33), (64, 34))]
     shapes = [r.shape for r in rhos]
     res = []
-    for which in (0, 1, 2):
+    for which, planes in ((0, 1), (0, 2), (1, 1), (2, 2)):
         bh = gpu_pkg.BatchedHomogenization(shapes, emin=1e-4, tol=1e-8, streams=1)
-        for i in range(2):
-            _, d = bh._plan.get_fft_tiling(i)
-            bh._plan.set_fft_tiling(i, [d, d // 2, 2 * d][which])
-        C, _, its = bh.C_full(rhos)
+        for i in range(3):
+            _, d, _ = bh._plan.get_fft_tiling(i)
+            bh._plan.set_fft_tiling(i, [d, d // 2, 2 * d][which], planes)
+        C, u, its = bh.C_full(rhos)
         assert all(bh.last_info["converged"])
-        res.append((np.asarray(C), np.asarray(its)))
-    for C, its in res[1:]:
+        res.append((np.asarray(C), np.asarray(its), u))
+    assert np.array_equal(res[1][0], res[0][0]) and np.array_equal(res[1][1], res[0][1])
+    assert torch.equal(res[1][2], res[0][2])
+    for C, its, _ in res[2:]:
         assert np.abs(C - res[0][0]).max() <= 1e-12 * np.abs(res[0][0]).max()
         assert np.abs(its - res[0][1]).max() <= 2

@@ -174,15 +174,15 @@ def test_precond_fp32_pass_b_block_shapes(gpu_pkg, n):
     r = rng.standard_normal((m,) + shape + (3,))
     lh = gpu_pkg.LinearHomogenization(shape)
     plan = lh._plan
-    cur, default = plan.get_fft_tiling(0)
-    assert cur == default > 0
+    cur, default, planes = plan.get_fft_tiling(0)
+    assert cur == default > 0 and planes == 1
     rp = gpu_pkg.to_planar(r, shape).float().contiguous()
     lib = gpu_pkg.load_library()
     ref = O.LinearRef(shape).precond(r) if n <= 64 else None
     outs = []
     for lpb in (default, default // 2, 2 * default):
         plan.set_fft_tiling(0, lpb)
-        assert plan.get_fft_tiling(0) == (lpb, default)
+        assert plan.get_fft_tiling(0) == (lpb, default, 1)
         z = torch.empty_like(rp)
         assert lib.nlfh_precond(plan.handle, ctypes.c_void_p(rp.data_ptr()), ctypes.c_void_p(z.data_ptr()), m, 1,
                                 ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)) == 0
@@ -194,5 +194,7 @@ def test_precond_fp32_pass_b_block_shapes(gpu_pkg, n):
     assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
     with pytest.raises(gpu_pkg.NlfhError):
         plan.set_fft_tiling(0, 3)
-    plan.set_fft_tiling(0, 0)
-    assert plan.get_fft_tiling(0) == (default, default)
+    with pytest.raises(gpu_pkg.NlfhError):
+        plan.set_fft_tiling(0, default, 3)
+    plan.set_fft_tiling(0, 0, 2)
+    assert plan.get_fft_tiling(0) == (default, default, 2)

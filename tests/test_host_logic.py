@@ -27,9 +27,11 @@ def test_packaged_tilings_are_valid_candidates(pkg):
         assert prec in ("f32", "f64") and m == 6, key
         if kind == "fftb":
             assert prec == "f32" and shape[0] == shape[1] == shape[2] and shape[0] in (16, 32, 48, 64, 128), key
-            tried = sorted([e["best"][0]] + [r[0] for r in e.get("runner_up", [])])
+            tried = sorted(int(b) for b in e["pass_b_us"])
             d = tried[1]
-            assert tried == sorted(at.fft_candidates(d)) and d in (16, 32, 64), key
+            assert d in (16, 32, 64) and tried == sorted({c[0] for c in at.fft_candidates(d)}), key
+            assert tuple(e["best"]) in at.fft_candidates(d), key
+            assert sorted(int(q) for q in e["pass_a_us_per_iteration"]) == [1, 2], key
             nb += 1
             continue
         assert kind == "", key
