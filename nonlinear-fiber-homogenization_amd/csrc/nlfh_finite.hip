@@ -274,7 +274,33 @@ __host__ __device__ constexpr double tgrad(int j, int d) {
   return d == 0 ? ((j & 1) ? fy * fz : -fy * fz) : (d == 1 ? ((j & 2) ? fx * fz : -fx * fz) : ((j & 4) ? fx * fy : -fx * fy));
 }
 
-template <int MODE>
+// v[i ^ XM] inside each group of 32 lanes (XM < 32): ds_swizzle in its bit mode (and-mask 0x1f, or-mask 0,
+// xor-mask XM), which needs no lane-index arithmetic, unlike a bpermute
+template <int XM>
+__device__ __forceinline__ double swz_xor(double v) {
+  static_assert(XM > 0 && XM < 32, "swizzle partners lie in the same 32 lanes");
+  constexpr int kPattern = 0x1F | (XM << 10);
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_swizzle((int)(b & 0xffffffffll), kPattern);
+  const int hi = __builtin_amdgcn_ds_swizzle((int)(b >> 32), kPattern);
+  return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+}
+
+// sum over the 64 lanes of a wavefront, the same fixed xor butterfly in every lane
+__device__ __forceinline__ double wave_sum(double v) {
+  v += __shfl_xor(v, 32, 64);
+  v += swz_xor<16>(v);
+  v += swz_xor<8>(v);
+  v += swz_xor<4>(v);
+  v += swz_xor<2>(v);
+  v += swz_xor<1>(v);
+  return v;
+}
+
+// HASV: the product's direction field v is given (MODE >= 2); without it only dFbar enters dF. A template
+// parameter rather than a test of a.v inside the node loop, where the test made every load of v wait for all
+// loads in flight.
+template <int MODE, bool HASV = false>
 __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) {
   const int g = threadIdx.x & 7;
   // 32-bit indexing: the host limits cells to 24 N < 2^31 (the force buffer's extent)
@@ -319,7 +345,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         const double wv = a.w[nd * 3 + ci];
 #pragma unroll
         for (int dj = 0; dj < 3; ++dj) F[ci][dj] += wv * d[dj];
-        if (MODE >= 2 && a.v != nullptr) {
+        if constexpr (MODE >= 2 && HASV) {
           const double vv = a.v[nd * 3 + ci];
 #pragma unroll
           for (int dj = 0; dj < 3; ++dj) dF[ci][dj] += vv * d[dj];
@@ -359,9 +385,9 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
   if (MODE == 3) {
     // per-element sum over the 8 Gauss points (fixed butterfly); no block partials
     double t = acc[0];
-    t += __shfl_xor(t, 4, 8);
-    t += __shfl_xor(t, 2, 8);
-    t += __shfl_xor(t, 1, 8);
+    t += swz_xor<4>(t);
+    t += swz_xor<2>(t);
+    t += swz_xor<1>(t);
     if (live && g == 0) a.fe[e] = t;
     return;
   }
@@ -379,15 +405,15 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
 #pragma unroll
     for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) h4[q][ci] = contrib(q, ci) + __shfl_xor(contrib(q + 4, ci), 4, 8);
+      for (int ci = 0; ci < 3; ++ci) h4[q][ci] = contrib(q, ci) + swz_xor<4>(contrib(q + 4, ci));
     double h2[2][3];
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) h2[q][ci] = h4[q][ci] + __shfl_xor(h4[q + 2][ci], 2, 8);
+      for (int ci = 0; ci < 3; ++ci) h2[q][ci] = h4[q][ci] + swz_xor<2>(h4[q + 2][ci]);
     double h1[3];
 #pragma unroll
-    for (int ci = 0; ci < 3; ++ci) h1[ci] = h2[0][ci] + __shfl_xor(h2[1][ci], 1, 8);
+    for (int ci = 0; ci < 3; ++ci) h1[ci] = h2[0][ci] + swz_xor<1>(h2[1][ci]);
     if (live) {
 #pragma unroll
       for (int ci = 0; ci < 3; ++ci) a.fe[(g * 3 + ci) * n + e] = h1[ci];
@@ -401,9 +427,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
 #pragma unroll
   for (int q = 0; q < nq; ++q) {
     if (!((a.pmask >> q) & 1u)) continue;  // uniform over the launch
-    double v = acc[q];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const double v = wave_sum(acc[q]);
     if ((threadIdx.x & 63) == 0) redq[threadIdx.x >> 6][q] = v;
   }
   __syncthreads();
@@ -802,13 +826,15 @@ int fs_eval_impl(const int* shape, const double* sg, const double* mat, int flag
   const dim3 grid((unsigned)nblk);
   if (mode == 3) {  // per-element output straight into grad (N values)
     a.fe = grad;
-    hipLaunchKernelGGL(fs_element_kernel<3>, grid, dim3(kFsThreads), 0, st, a);
+    if (a.v != nullptr) hipLaunchKernelGGL((fs_element_kernel<3, true>), grid, dim3(kFsThreads), 0, st, a);
+    else hipLaunchKernelGGL((fs_element_kernel<3, false>), grid, dim3(kFsThreads), 0, st, a);
     HIP_TRY(hipGetLastError());
     return NLFH_OK;
   }
   if (mode == 0) hipLaunchKernelGGL(fs_element_kernel<0>, grid, dim3(kFsThreads), 0, st, a);
   else if (mode == 1) hipLaunchKernelGGL(fs_element_kernel<1>, grid, dim3(kFsThreads), 0, st, a);
-  else hipLaunchKernelGGL(fs_element_kernel<2>, grid, dim3(kFsThreads), 0, st, a);
+  else if (a.v != nullptr) hipLaunchKernelGGL((fs_element_kernel<2, true>), grid, dim3(kFsThreads), 0, st, a);
+  else hipLaunchKernelGGL((fs_element_kernel<2, false>), grid, dim3(kFsThreads), 0, st, a);
   HIP_TRY(hipGetLastError());
   if (mode >= 1) {
     hipLaunchKernelGGL(fs_gather_kernel, dim3((unsigned)gblk), dim3(kFsThreads), 0, st, a, grad);
