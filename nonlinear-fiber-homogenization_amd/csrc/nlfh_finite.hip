@@ -238,9 +238,35 @@ __device__ __forceinline__ void eff_P_dgamma(const double (&F)[3][3], const FsMa
     for (int j = 0; j < 3; ++j) Pg[i][j] = (Pn[i][j] - Pl[i][j]) + gam * (Dn[i][j] - Dl[i][j]);
 }
 
+// Division by a divisor fixed at launch, n / d for any 32-bit n, by a multiply-high and shifts (the round-up
+// method of Granlund and Montgomery, "Division by invariant integers using multiplication", PLDI 1994, sec. 4):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n), q = (t + ((n - t) >> s1)) >> s2 with
+// s1 = min(l, 1), s2 = max(l - 1, 0) (d = 1 gives m = 1, t = 0, q = n).
+struct FastDiv {
+  unsigned d = 1, m = 1;
+  int s1 = 0, s2 = 0;
+  FastDiv() = default;
+  explicit FastDiv(unsigned dv) : d(dv) {
+    int l = 0;
+    while ((1ull << l) < dv) ++l;
+    m = (unsigned)((((1ull << l) - dv) << 32) / dv + 1);
+    s1 = l < 1 ? l : 1;
+    s2 = l > 1 ? l - 1 : 0;
+  }
+  __host__ __device__ unsigned div(unsigned n) const {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned t = __umulhi(m, n);
+#else
+    const unsigned t = (unsigned)(((unsigned long long)m * n) >> 32);
+#endif
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
 struct FsArgs {
   int nx, ny, nz;
   int64_t n;
+  FastDiv dz, dy;    // division by nz and by ny (element index -> voxel coordinates)
   const double* sg;  // (4, N): wq s_e, gamma_e, wq ds_e/drho_e, dgamma_e/drho_e of every element (fs_prepare_kernel)
   FsMat m;
   double wq;
@@ -317,10 +343,15 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
 #pragma unroll
     for (int j = 0; j < 3; ++j) P[i][j] = 0.0;
   if (live) {
-    const int k = e % a.nz;
-    const int j = (e / a.nz) % a.ny;
-    const int i = e / (a.nz * a.ny);
+    const int row = (int)a.dz.div((unsigned)e);  // i ny + j
+    const int k = e - row * a.nz;
+    const int i = (int)a.dy.div((unsigned)row);
+    const int j = row - i * a.ny;
     const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
+    // offsets (in values) of the element's four node rows (x, y corner) and of its two z positions
+    const int r00 = row * a.nz * 3, r10 = (i1 * a.ny + j) * a.nz * 3;
+    const int r01 = (i * a.ny + j1) * a.nz * 3, r11 = (i1 * a.ny + j1) * a.nz * 3;
+    const int zoff[2] = {3 * k, 3 * k1};
     double F[3][3], dF[3][3];
 #pragma unroll
     for (int ci = 0; ci < 3; ++ci)
@@ -329,26 +360,37 @@ __global__ void __launch_bounds__(kFsThreads) fs_element_kernel(const FsArgs a) 
         F[ci][dj] = a.Fbar[ci * 3 + dj];
         dF[ci][dj] = MODE >= 2 ? a.dFbar[ci * 3 + dj] : 0.0;
       }
+    // this point's weight along each axis of a node on the low (index 0) or high (1) side of the element:
+    // kGaussSame when node and point sit on the same side
+    const double wx[2] = {(g & 1) ? kGaussOther : kGaussSame, (g & 1) ? kGaussSame : kGaussOther};
+    const double wy[2] = {(g & 2) ? kGaussOther : kGaussSame, (g & 2) ? kGaussSame : kGaussOther};
+    const double wz[2] = {(g & 4) ? kGaussOther : kGaussSame, (g & 4) ? kGaussSame : kGaussOther};
     // the loads go node by node, the same node for the 8 lanes of an element (one address per element and
-    // instruction); the lane's coefficient for node c is sigma_g (.) T_(c^g). Two nodes' loads in flight at
-    // a time: unrolling all eight holds 48 loaded values in registers and cuts the occupancy in half.
-#pragma unroll 2
-    for (int c = 0; c < 8; ++c) {
-      const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
-      const int nd = (ii * a.ny + jj) * a.nz + kk;
-      const int jl = c ^ g;
-      const double fx = (jl & 1) ? kGaussOther : kGaussSame, fy = (jl & 2) ? kGaussOther : kGaussSame,
-                   fz = (jl & 4) ? kGaussOther : kGaussSame;
-      const double d[3] = {((c & 1) ? fy : -fy) * fz, ((c & 2) ? fx : -fx) * fz, ((c & 4) ? fx : -fx) * fy};
+    // instruction). Node c = x + 2 y + 4 z (bits of its corner) has at this point the gradient
+    // {(x ? 1 : -1) wy[y] wz[z], (y ? 1 : -1) wx[x] wz[z], (z ? 1 : -1) wx[x] wy[y]}; the loop runs over the
+    // four (y, z) pairs of nodes x = 0, 1, whose y and z are uniform over the wavefront (scalar selects), with
+    // two nodes' loads in flight (unrolling all eight holds 48 loaded values in registers).
+#pragma unroll 1
+    for (int t = 0; t < 4; ++t) {
+      const bool hy = t & 1, hz = t >> 1;
+      const double fy = hy ? wy[1] : wy[0], fz = hz ? wz[1] : wz[0];
+      const double dyz = fy * fz;
+      const double fzs = hy ? fz : -fz, fys = hz ? fy : -fy;
+      const int zo = hz ? zoff[1] : zoff[0];
+      const int nd[2] = {(hy ? r01 : r00) + zo, (hy ? r11 : r10) + zo};
 #pragma unroll
-      for (int ci = 0; ci < 3; ++ci) {
-        const double wv = a.w[nd * 3 + ci];
+      for (int x = 0; x < 2; ++x) {
+        const double d[3] = {x ? dyz : -dyz, wx[x] * fzs, wx[x] * fys};
 #pragma unroll
-        for (int dj = 0; dj < 3; ++dj) F[ci][dj] += wv * d[dj];
-        if constexpr (MODE >= 2 && HASV) {
-          const double vv = a.v[nd * 3 + ci];
+        for (int ci = 0; ci < 3; ++ci) {
+          const double wv = a.w[nd[x] + ci];
 #pragma unroll
-          for (int dj = 0; dj < 3; ++dj) dF[ci][dj] += vv * d[dj];
+          for (int dj = 0; dj < 3; ++dj) F[ci][dj] += wv * d[dj];
+          if constexpr (MODE >= 2 && HASV) {
+            const double vv = a.v[nd[x] + ci];
+#pragma unroll
+            for (int dj = 0; dj < 3; ++dj) dF[ci][dj] += vv * d[dj];
+          }
         }
       }
     }
@@ -720,6 +762,8 @@ static int fs_setup(const int* shape, const double* mat, int flags, FsArgs& a) {
   a.ny = ny;
   a.nz = nz;
   a.n = (int64_t)nx * ny * nz;
+  a.dz = FastDiv((unsigned)nz);
+  a.dy = FastDiv((unsigned)ny);
   if (24 * a.n >= (int64_t)INT32_MAX) {  // the kernels index the (24, N) force buffer in 32 bits
     set_error("nlfh_fs: cell too large (24 N must stay below 2^31)");
     return NLFH_EINVAL;
