@@ -514,9 +514,10 @@ __global__ void __launch_bounds__(kFsThreads) fs_gather_kernel(const FsArgs a, d
   const int n = (int)a.n;  // 24 n < 2^31 (host-checked)
   const int nd = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
   if (nd >= n) return;
-  const int k = nd % a.nz;
-  const int j = (nd / a.nz) % a.ny;
-  const int i = nd / (a.nz * a.ny);
+  const int row = (int)a.dz.div((unsigned)nd);
+  const int k = nd - row * a.nz;
+  const int i = (int)a.dy.div((unsigned)row);
+  const int j = row - i * a.ny;
   const int i0 = i == 0 ? a.nx - 1 : i - 1, j0 = j == 0 ? a.ny - 1 : j - 1, k0 = k == 0 ? a.nz - 1 : k - 1;
   double o[3] = {0.0, 0.0, 0.0};
 #pragma unroll
@@ -599,9 +600,10 @@ __global__ void __launch_bounds__(kFsThreads) fs_toec_kernel(const FsArgs a, con
       for (int j = 0; j < 3; ++j) H[I][i][j] = 0.0;
   double ws = 0.0, wg3 = 0.0;
   if (live) {
-    const int k = e % a.nz;
-    const int j = (e / a.nz) % a.ny;
-    const int i = e / (a.nz * a.ny);
+    const int row = (int)a.dz.div((unsigned)e);
+    const int k = e - row * a.nz;
+    const int i = (int)a.dy.div((unsigned)row);
+    const int j = row - i * a.ny;
     const int i1 = i + 1 == a.nx ? 0 : i + 1, j1 = j + 1 == a.ny ? 0 : j + 1, k1 = k + 1 == a.nz ? 0 : k + 1;
     for (int c = 0; c < 8; ++c) {
       const int ii = (c & 1) ? i1 : i, jj = (c & 2) ? j1 : j, kk = (c & 4) ? k1 : k;
