@@ -256,8 +256,14 @@ int nlfh_fs_cg_update(int len, double* x_out, double* r_out, const double* x, co
 int nlfh_fs_cg_direction(int len, double* p_out, const double* z, const double* p, const double* rz_new,
                          const double* rz_old, void* stream);
 /* w (nx, ny, nz, 3) -= its per-component mean, in place: the w part of Mixed.proj (mech.py:236-239).
- * work: as for nlfh_fs_eval (its first 3 ceil(N / 256) + 3 doubles are overwritten). */
+ * work: as for nlfh_fs_eval (at most its first 3 ceil(N / 256) doubles are overwritten). */
 int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);
+/* The macro unknowns of Mixed (mech.py:183-199): k <= 9 free components of Fbar at the row-major positions idx
+ * (a host array). scatter: dst9 (device, 9 values) = the 3x3 direction with src[j] (device) at idx[j] (distinct)
+ * and zeros elsewhere, the dFbar of a Hessian product (Mixed.hvp_A, mech.py:227-233); gather: dst[j] = src9[idx[j]]
+ * (the free components of a stress sum, device to device). */
+int nlfh_fs_macro_scatter(int k, const int* idx, const double* src, double* dst9, void* stream);
+int nlfh_fs_macro_gather(int k, const int* idx, const double* src9, double* dst, void* stream);
 int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
                  double* work, void* stream);

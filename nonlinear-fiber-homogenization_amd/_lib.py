@@ -22,6 +22,8 @@ SIGNATURES = [
     ("nlfh_fs_work_doubles", ctypes.c_longlong, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("nlfh_fs_prepare", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _vp]),
     ("nlfh_fs_project", ctypes.c_int, [_c_int_p, _vp, _vp, _vp]),
+    ("nlfh_fs_macro_scatter", ctypes.c_int, [ctypes.c_int, _c_int_p, _vp, _vp, _vp]),
+    ("nlfh_fs_macro_gather", ctypes.c_int, [ctypes.c_int, _c_int_p, _vp, _vp, _vp]),
     ("nlfh_fs_cg_update", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("nlfh_fs_cg_direction", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("nlfh_fs_toec", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _vp, _vp, _vp]),

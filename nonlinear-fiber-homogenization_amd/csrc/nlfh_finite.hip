@@ -532,33 +532,65 @@ __global__ void __launch_bounds__(kFsThreads) fs_gather_kernel(const FsArgs a, d
   for (int d = 0; d < 3; ++d) out[nd * 3 + d] = o[d];
 }
 
-// Mean removal of an (N, 3) field: per-block component sums (one node per thread), their fixed-order
-// total, and the subtraction (mech.py:236-239, the w part of Mixed.proj).
+// Mean removal of an (N, 3) field in two launches (mech.py:236-239, the w part of Mixed.proj): per-block
+// component sums over a grid-stride sweep of at most kProjBlocks blocks (one node per thread and step), then a
+// subtraction pass whose every block first forms the three means from those partials, each component by one
+// wavefront in a fixed order (lane-strided sums, then a fixed butterfly), so all blocks subtract bitwise the
+// same values.
+constexpr int kProjBlocks = 128;
+
 __global__ void __launch_bounds__(kFsThreads) fs_sum3_kernel(const double* __restrict__ w, int n,
                                                              double* __restrict__ part) {
   __shared__ double red[kFsThreads / 64];
-  const int nd = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int nd = (int)blockIdx.x * kFsThreads + (int)threadIdx.x; nd < n; nd += (int)gridDim.x * kFsThreads) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] += w[nd * 3 + c];
+  }
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const double t = block_sum(nd < n ? w[nd * 3 + c] : 0.0, red);
-    if (threadIdx.x == 0) part[blockIdx.x * 3 + c] = t;
-    __syncthreads();
+    const double t = block_sum(acc[c], red);
+    if (threadIdx.x == 0) part[c * gridDim.x + blockIdx.x] = t;
   }
 }
 
-__global__ void __launch_bounds__(kFsThreads) fs_mean3_kernel(const double* __restrict__ part, int nblk, double inv_n,
-                                                              double* __restrict__ mean) {
-  __shared__ double red[kFsThreads / 64];
-  const int c = blockIdx.x;
-  double acc = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += kFsThreads) acc += part[b * 3 + c];
-  const double t = block_sum(acc, red);
-  if (threadIdx.x == 0) mean[c] = t * inv_n;
+__global__ void __launch_bounds__(kFsThreads) fs_sub3_kernel(double* __restrict__ w, int n,
+                                                             const double* __restrict__ part, int nb, double inv_n) {
+  __shared__ double mean[3];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wv < 3) {
+    double acc = 0.0;
+    for (int b = lane; b < nb; b += 64) acc += part[wv * nb + b];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) mean[wv] = acc * inv_n;
+  }
+  __syncthreads();
+  const int len = 3 * n;
+  for (int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x; i < len; i += (int)gridDim.x * kFsThreads)
+    w[i] -= mean[i % 3];
 }
 
-__global__ void __launch_bounds__(kFsThreads) fs_sub3_kernel(double* __restrict__ w, int n,
-                                                             const double* __restrict__ mean) {
-  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
-  if (i < 3 * n) w[i] -= mean[i % 3];
+// The macro unknowns of a mixed solve: k free components of Fbar at row-major positions idx (host values,
+// passed by value). scatter: dst (9) = the 3x3 matrix with src[j] at idx[j] and zeros elsewhere; gather:
+// dst[j] = src[idx[j]].
+struct FsIdx {
+  int k;
+  int i[9];
+};
+
+__global__ void fs_macro_scatter_kernel(const FsIdx ix, const double* __restrict__ src, double* __restrict__ dst) {
+  const int t = threadIdx.x;
+  if (t < 9) {
+    double v = 0.0;
+    for (int j = 0; j < ix.k; ++j)
+      if (ix.i[j] == t) v = src[j];
+    dst[t] = v;
+  }
+}
+
+__global__ void fs_macro_gather_kernel(const FsIdx ix, const double* __restrict__ src, double* __restrict__ dst) {
+  const int t = threadIdx.x;
+  if (t < ix.k) dst[t] = src[ix.i[t]];
 }
 
 // Third-order elastic constants of the cell from the linear correctors chi_I (6 fields (N, 3)), per
@@ -978,15 +1010,65 @@ int nlfh_fs_project(const int* shape, double* w, double* work, void* stream) {
     return NLFH_EINVAL;
   }
   const int n = (int)a.n;
-  const int nb = (n + kFsThreads - 1) / kFsThreads;
+  const int nodes_blk = (n + kFsThreads - 1) / kFsThreads;
+  const int nb = nodes_blk < kProjBlocks ? nodes_blk : kProjBlocks;  // partials: 3 nb <= 3 ceil(N / 256)
+  const int vals_blk = (3 * n + kFsThreads - 1) / kFsThreads;
+  const int ns = vals_blk < 4 * kProjBlocks ? vals_blk : 4 * kProjBlocks;
   double* part = work;
-  double* mean = work + 3 * (int64_t)nb;
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(fs_sum3_kernel, dim3(nb), dim3(kFsThreads), 0, st, w, n, part);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(fs_mean3_kernel, dim3(3), dim3(kFsThreads), 0, st, part, nb, 1.0 / n, mean);
+  hipLaunchKernelGGL(fs_sub3_kernel, dim3(ns), dim3(kFsThreads), 0, st, w, n, part, nb, 1.0 / n);
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL(fs_sub3_kernel, dim3((3 * n + kFsThreads - 1) / kFsThreads), dim3(kFsThreads), 0, st, w, n, mean);
+  return NLFH_OK;
+}
+
+static int fs_macro_idx(int k, const int* idx, bool distinct, FsIdx& ix, const char* who) {
+  if (k < 0 || k > 9 || (k > 0 && idx == nullptr)) {
+    set_error(std::string(who) + ": k must be in [0, 9] and idx given for k > 0");
+    return NLFH_EINVAL;
+  }
+  ix.k = k;
+  for (int j = 0; j < 9; ++j) ix.i[j] = 0;
+  for (int j = 0; j < k; ++j) {
+    if (idx[j] < 0 || idx[j] > 8) {
+      set_error(std::string(who) + ": idx values must be in [0, 8]");
+      return NLFH_EINVAL;
+    }
+    if (distinct)
+      for (int q = 0; q < j; ++q)
+        if (idx[q] == idx[j]) {
+          set_error(std::string(who) + ": idx values must be distinct");
+          return NLFH_EINVAL;
+        }
+    ix.i[j] = idx[j];
+  }
+  return NLFH_OK;
+}
+
+int nlfh_fs_macro_scatter(int k, const int* idx, const double* src, double* dst9, void* stream) {
+  FsIdx ix{};
+  const int rc = fs_macro_idx(k, idx, true, ix, "nlfh_fs_macro_scatter");
+  if (rc != NLFH_OK) return rc;
+  if (dst9 == nullptr || (k > 0 && src == nullptr)) {
+    set_error("nlfh_fs_macro_scatter: dst9 (and src for k > 0) are required");
+    return NLFH_EINVAL;
+  }
+  hipLaunchKernelGGL(fs_macro_scatter_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ix, src, dst9);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_macro_gather(int k, const int* idx, const double* src9, double* dst, void* stream) {
+  FsIdx ix{};
+  const int rc = fs_macro_idx(k, idx, false, ix, "nlfh_fs_macro_gather");
+  if (rc != NLFH_OK) return rc;
+  if (k == 0) return NLFH_OK;
+  if (src9 == nullptr || dst == nullptr) {
+    set_error("nlfh_fs_macro_gather: src9 and dst are required");
+    return NLFH_EINVAL;
+  }
+  hipLaunchKernelGGL(fs_macro_gather_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ix, src9, dst);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -293,6 +293,9 @@ class Mixed:
         self._flat = torch.tensor([3 * a + b for a, b in self.free], dtype=torch.long, device=self._dev)
         # the Hessian products need only the stress sums of the free components (scalar 1 + 3a + b)
         self._smask = sum(1 << (1 + 3 * a + b) for a, b in set(self.free))
+        # host copy of the flat indices for the macro scatter / gather kernels, and the dFbar of the products
+        self._flat_c = (ctypes.c_int * max(self.k, 1))(*[3 * a + b for a, b in self.free])
+        self._dF = torch.zeros(9, dtype=torch.float64, device=self._dev)
 
     # --- unknowns ----------------------------------------------------------------------------------------
     def z0(self, Fb):
@@ -345,14 +348,14 @@ class Mixed:
         """Hessian at the state A = (w, Fbar) applied to v (into `out` when given, a contiguous vector of
         nw + k values)."""
         w, F = A
-        dF = torch.zeros(9, dtype=torch.float64, device=self._dev)
-        if self.k:
-            dF.index_add_(0, self._flat, v[self.nw:])
+        lib = self.c._lib
+        if self.k:  # dFbar from the direction's macro part, on the device
+            _lib.check(lib.nlfh_fs_macro_scatter(self.k, self._flat_c, _ptr(v[self.nw:]), _ptr(self._dF), _stream()))
         if out is None:
             out = torch.empty(self.nw + self.k, dtype=torch.float64, device=self._dev)
-        _, sc = self.c._eval(w, F, 2, v[: self.nw], dF, out=out[: self.nw], scalars=self._smask)
-        if self.k:
-            out[self.nw:].copy_(sc[1:][self._flat])
+        _, sc = self.c._eval(w, F, 2, v[: self.nw], self._dF, out=out[: self.nw], scalars=self._smask)
+        if self.k:  # the free components of the stress increment sum
+            _lib.check(lib.nlfh_fs_macro_gather(self.k, self._flat_c, _ptr(sc[1:]), _ptr(out[self.nw:]), _stream()))
         return out
 
     def macro_diag(self, A, Fb):

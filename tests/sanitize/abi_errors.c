@@ -49,6 +49,14 @@ int main(void) {
   EXPECT(nlfh_fs_eval(sh, NULL, mat, 1, NULL, F, NULL, NULL, 7, NULL, NULL, NULL, NULL) != 0, "fs_eval mode");
   EXPECT(nlfh_fs_prepare(NULL, NULL, mat, 1, NULL, NULL) != 0, "fs_prepare nulls");
   EXPECT(nlfh_fs_project(sh, NULL, NULL, NULL) != 0, "fs_project nulls");
+  int midx[3] = {4, 9, 4};
+  double d9[9];
+  EXPECT(nlfh_fs_macro_scatter(10, midx, d9, d9, NULL) != 0, "macro scatter k > 9");
+  EXPECT(nlfh_fs_macro_scatter(2, midx, d9, d9, NULL) != 0, "macro scatter index out of range");
+  midx[1] = 0;
+  EXPECT(nlfh_fs_macro_scatter(3, midx, d9, d9, NULL) != 0, "macro scatter repeated index");
+  EXPECT(nlfh_fs_macro_gather(1, NULL, d9, d9, NULL) != 0, "macro gather without idx");
+  EXPECT(nlfh_fs_macro_gather(-1, midx, d9, d9, NULL) != 0, "macro gather k < 0");
   EXPECT(nlfh_fs_cg_update(0, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL, NULL) != 0, "cg update");
   /* network kernels: channel counts not multiples of 16 */
   EXPECT(nlfh_net_conv(NULL, 15, 0, NULL, 0, NULL, NULL, 8, 1, 16, 16, 1, 0, NULL, NULL, NULL) != 0, "conv args");

@@ -106,16 +106,43 @@ def test_small_strain_limit_is_the_linear_C(M, gpu_pkg):
     assert errs[0] < 5e-3 and errs[1] < 0.6 * errs[0]
 
 
-def test_projection_removes_the_component_means(M):
+@pytest.mark.parametrize("shape", [(7, 5, 3), (64, 64, 17), (40, 30, 45)])
+def test_projection_removes_the_component_means(M, shape):
+    # one block of partial sums, then grid-stride sweeps over more nodes than the partial-sum grid covers
     import torch
     rng = np.random.default_rng(9)
-    cell = M.Cell(rng.uniform(0.0, 1.0, (7, 5, 3)))
+    cell = M.Cell(rng.uniform(0.0, 1.0, shape))
     mix = M.Mixed(cell)
     v = rng.standard_normal(3 * cell.N + 2) + 0.3
     out = mix.proj(torch.as_tensor(v, device="cuda")).cpu().numpy()
     w = v[:-2].reshape(-1, 3)
-    np.testing.assert_allclose(out[:-2].reshape(-1, 3), w - w.mean(axis=0), atol=1e-14)
+    np.testing.assert_allclose(out[:-2].reshape(-1, 3), w - w.mean(axis=0), atol=1e-13)
+    assert np.abs(out[:-2].reshape(-1, 3).mean(axis=0)).max() < 1e-14
     assert out[-2:].tolist() == v[-2:].tolist()
+
+
+def test_macro_scatter_and_gather(M, gpu_pkg):
+    import ctypes
+    import torch
+    lib = gpu_pkg.load_library()
+    idx = [5, 0, 8]
+    ic = (ctypes.c_int * 3)(*idx)
+    src = torch.tensor([1.5, -2.0, 3.25], dtype=torch.float64, device="cuda")
+    dst = torch.full((9,), 7.0, dtype=torch.float64, device="cuda")
+    assert lib.nlfh_fs_macro_scatter(3, ic, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                     None) == 0
+    want = np.zeros(9)
+    want[idx] = [1.5, -2.0, 3.25]
+    assert dst.cpu().numpy().tolist() == want.tolist()
+    back = torch.zeros(3, dtype=torch.float64, device="cuda")
+    assert lib.nlfh_fs_macro_gather(3, ic, ctypes.c_void_p(dst.data_ptr()), ctypes.c_void_p(back.data_ptr()),
+                                    None) == 0
+    assert back.cpu().numpy().tolist() == [1.5, -2.0, 3.25]
+    assert lib.nlfh_fs_macro_scatter(0, None, None, ctypes.c_void_p(dst.data_ptr()), None) == 0
+    assert dst.cpu().numpy().tolist() == [0.0] * 9
+    dup = (ctypes.c_int * 2)(4, 4)
+    assert lib.nlfh_fs_macro_scatter(2, dup, ctypes.c_void_p(src.data_ptr()), ctypes.c_void_p(dst.data_ptr()),
+                                     None) != 0
 
 
 def test_tiled_cell_has_the_same_energy_density(M):
