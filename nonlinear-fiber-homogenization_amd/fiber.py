@@ -91,14 +91,16 @@ def rel_error(c, target):
 
 
 def sigma_min_normalized(J):
-    """Smallest singular value of the row-normalized Jacobian (nlfiber/fiber.py:96-100)."""
+    """Scale the Jacobian's rows to unit length and return its least singular value, a rank test
+    (nlfiber/fiber.py:96-100)."""
     Jf = np.asarray(J.detach().cpu() if isinstance(J, torch.Tensor) else J).reshape(J.shape[0], -1)
     Jf = Jf / np.linalg.norm(Jf, axis=1, keepdims=True)
     return float(np.linalg.svd(Jf, compute_uv=False).min())
 
 
 def gram_logdet(J, metric=None):
-    """log det(J G J^T) for Jacobian rows J (nlfiber/fiber.py:89-93); `metric` maps one row to G row."""
+    """The log-volume log det(J G J^T) spanned by the Jacobian's rows under the metric G (nlfiber/fiber.py:89-93);
+    `metric` applies G to one row (identity if None)."""
     Jt = J if isinstance(J, torch.Tensor) else torch.as_tensor(np.asarray(J))
     Jf = Jt.reshape(Jt.shape[0], -1)
     GJ = Jf if metric is None else torch.stack([torch.as_tensor(metric(r)).reshape(-1) for r in Jt])
@@ -106,8 +108,8 @@ def gram_logdet(J, metric=None):
 
 
 def random_start(n_o, d=3, seed=0, kmax=3, nterms=40):
-    """Random smooth symmetric pre-density (low-frequency cosine series), standardized
-    (nlfiber/fiber.py:103-121); a host array."""
+    """A seeded pre-density for starting a design: a sum of a few low-frequency cosines with the cell's symmetry,
+    shifted and scaled to zero mean and unit spread (nlfiber/fiber.py:103-121); a host array."""
     rng = np.random.default_rng(seed)
     n = 2 * n_o
     x = (np.arange(n_o) + 0.5) / n

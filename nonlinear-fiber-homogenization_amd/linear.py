@@ -34,7 +34,8 @@ GPTS = np.array([[x, y, z] for z in _g for y in _g for x in _g])
 
 
 def clip01(x):
-    """Clamp to [0, 1] with derivative 1 at the bounds (nlfiber/linear.py:27-30)."""
+    """rho limited to [0, 1]; its derivative is taken as 1 on the closed interval, bounds included
+    (nlfiber/linear.py:27-30)."""
     if isinstance(x, torch.Tensor):
         return torch.where(x > 1.0, torch.ones_like(x), torch.where(x < 0.0, torch.zeros_like(x), x))
     x = np.asarray(x)
@@ -42,7 +43,8 @@ def clip01(x):
 
 
 def dN_ref():
-    """Shape-function gradients of the unit Q1 element at the Gauss points: [gauss, node, dim]."""
+    """Gradients of the 8 trilinear shape functions of the unit cube, evaluated at the 2x2x2 Gauss points;
+    indexed [point, node, axis]."""
     d = np.zeros((8, 8, 3))
     for gi, (x, y, z) in enumerate(GPTS):
         for a, (i, j, k) in enumerate(NODES):
@@ -66,7 +68,8 @@ def iso_D(E, nu):
 
 
 def B_matrices():
-    """Strain-displacement matrices (8 Gauss points, 6, 24), engineering shear."""
+    """B matrices of the unit Q1 element at its 8 Gauss points, shape (8, 6, 24): Voigt strain (shears as
+    engineering strains) from the 24 nodal displacements."""
     dN = dN_ref()
     B = np.zeros((8, 6, 24))
     rows = {0: [(0, 0)], 1: [(1, 1)], 2: [(2, 2)], 3: [(1, 2), (2, 1)], 4: [(0, 2), (2, 0)], 5: [(0, 1), (1, 0)]}
@@ -303,7 +306,8 @@ class _CorrectorsFn(torch.autograd.Function):
 
 
 class LinearHomogenization:
-    """Linear homogenization on a fixed grid. `shape` is (nx, ny, nz) or (n, n) for plane strain.
+    """Small-strain homogenization of voxel cells of one grid size; `shape` is (nx, ny, nz), or (n, n) for a
+    plane-strain cell.
 
     Extra keywords (not in the reference): precision = "fp64" (the reference's PCG, default) or
     "mixed" (FP32 PCG inside FP64 iterative refinement, true-residual stopping rule; inner_restart=True
@@ -513,7 +517,9 @@ class LinearHomogenization:
 
     # --- homogenized stiffness (nlfiber/linear.py:220-264) ------------------------------------
     def bilinear(self, s, Es, u):
-        """C_IJ = mean_e s_e b_e(E_I + u_I, E_J + u_J) (host tensor, differentiable in s)."""
+        """The stiffness entries C_IJ: the element energy form of the total strains E_I + grad u_I and
+        E_J + grad u_J, weighted by s_e and averaged over the elements (a host tensor; torch can differentiate it
+        in s)."""
         s = self._s(s) if not isinstance(s, torch.Tensor) or s.device != self._dev else s
         if self.plane and s.dim() == 2:
             s = s[..., None]
@@ -548,7 +554,7 @@ class LinearHomogenization:
         return from_planar(out), coef.reshape(m, m)
 
     def C_full(self, rho, u0=None):
-        """6x6 C^H (all six unit strains). Returns (C, u, cg iterations)."""
+        """The full homogenized stiffness: one solve per unit strain, six in all. Returns (C, u, iterations)."""
         s = self.stiffness_scale(rho)
         Es = np.eye(6)
         if u0 is None and self.warm_start is not None and not self.plane:
@@ -576,11 +582,13 @@ class LinearHomogenization:
 
     @staticmethod
     def swap_xy(u):
-        """Corrector of E_yy from the corrector of E_xx of a cell symmetric under x<->y."""
+        """For a cell invariant under the exchange of x and y: the yy-strain corrector obtained by mapping the
+        xx-strain corrector through that exchange (field and components)."""
         return torch.swapaxes(u, -4, -3)[..., [1, 0, 2]]
 
     def invariants_from(self, s, u2):
-        """(C11, C12, C44) / E (C66 in 2D) from the xx and shear correctors u2."""
+        """C11, C12 and C44 over E (the in-plane shear C66 for a plane cell), assembled from the two correctors u2
+        of the xx and shear strains."""
         uyy = self.swap_xy(u2[0])
         u3 = torch.stack([u2[0], uyy, u2[1]])
         Es = np.stack([unit_strain(0), unit_strain(1), unit_strain(self._shear())])
@@ -588,8 +596,9 @@ class LinearHomogenization:
         return torch.stack([C[0, 0], C[1, 0], C[2, 2]]) / self.E
 
     def invariants(self, rho, u0=None):
-        """Normalized invariants of a symmetric cell. Returns (c, u2, cg iterations); c is
-        differentiable in rho (correctors held fixed, which is exact)."""
+        """Normalised cubic invariants of a cell with the x<->y symmetry, from two solves (xx and shear strain).
+        Returns (c, u2, iterations); torch can differentiate c in rho with the correctors held fixed, which is
+        exact because the correctors minimise the energy."""
         s = self.stiffness_scale(rho)
         u2, it = self.solve(s.detach(), self.sym_cases(), u0)
         return self.invariants_from(s, u2.detach()), u2, it
@@ -597,14 +606,16 @@ class LinearHomogenization:
 
 # --- host post-processing (nlfiber/linear.py:267-291) ------------------------------------------
 def cubic_invariants(C):
-    """(C11, C12, C44) as averages over the cubic-equivalent entries of a 6x6 C."""
+    """The three cubic constants of a 6x6 stiffness, each the mean of the entries that cubic symmetry makes equal:
+    C11 over the normal diagonal, C12 over the normal off-diagonal pairs, C44 over the shear diagonal."""
     C = np.asarray(C.detach().cpu() if isinstance(C, torch.Tensor) else C)
     return np.array([np.mean([C[0, 0], C[1, 1], C[2, 2]]), np.mean([C[0, 1], C[0, 2], C[1, 2]]),
                      np.mean([C[3, 3], C[4, 4], C[5, 5]])])
 
 
 def cubic_leakage(C):
-    """Relative distance of C from its cubic projection (full-tensor leakage)."""
+    """How far C is from cubic symmetry: the norm of C minus the cubic matrix built from its averaged constants,
+    over the norm of C, taken over the whole 6x6 matrix."""
     C = np.asarray(C.detach().cpu() if isinstance(C, torch.Tensor) else C)
     c11, c12, c44 = cubic_invariants(C)
     Cc = np.zeros((6, 6))
@@ -615,7 +626,8 @@ def cubic_leakage(C):
 
 
 def hashin_shtrikman(v, E=1.0, nu=0.3):
-    """HS upper bounds (K_HS, G_HS) of a porous isotropic solid at volume fraction v."""
+    """Hashin-Shtrikman upper bounds on bulk and shear modulus, (K_HS, G_HS), for an isotropic solid of volume
+    fraction v with empty pores."""
     lam, mu = lame(E, nu)
     K = lam + 2 * mu / 3
     K_hs = K + (1 - v) / (-1 / K + 3 * v / (3 * K + 4 * mu)) if v < 1 else K

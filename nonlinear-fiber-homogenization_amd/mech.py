@@ -77,7 +77,8 @@ def voigt_tensor(I):
 
 
 def to_voigt(M):
-    """Six-vector xx, yy, zz, yz, xz, xy of sym(M), shears doubled as engineering strains (mech.py:68-71)."""
+    """The symmetric part of a 3x3 matrix as a Voigt 6-vector in the order xx, yy, zz, yz, xz, xy, the three
+    off-diagonal entries counted twice (engineering shears) (mech.py:68-71)."""
     M = np.asarray(M, dtype=np.float64)
     S = 0.5 * (M + M.T)
     return np.array([S[0, 0], S[1, 1], S[2, 2], 2 * S[1, 2], 2 * S[0, 2], 2 * S[0, 1]])
@@ -137,7 +138,8 @@ class _MeanStressFn(torch.autograd.Function):
 
 
 class Cell:
-    """A periodic voxel cell. rho: design density (nx, ny, nz) or (n, n) in [0, 1] (mech.py:91-121)."""
+    """Density field on a periodic voxel grid with its material model (mech.py:91-121): rho holds one value in
+    [0, 1] per voxel, shaped (nx, ny, nz), or (n, n) for a plane cell."""
 
     def __init__(self, rho, E=1.0, nu=0.3, material="neohooke", wls=(500.0, 0.01), emin=1e-3, p=3.0):
         if material not in MATERIALS:
@@ -182,7 +184,7 @@ class Cell:
         return Cell(rho, self.E, self.nu, self.material, self.wls, self.emin, self.p)
 
     def tiled(self, k):
-        """k x k x k supercell (2D: k x k x 1) (mech.py:118-121)."""
+        """The cell repeated k times along each axis (along x and y only for a plane cell) (mech.py:118-121)."""
         reps = (k, k, 1) if self.shape[2] == 1 else (k, k, k)
         return Cell(self.rho.repeat(*reps), self.E, self.nu, self.material, self.wls, self.emin, self.p)
 
@@ -267,7 +269,7 @@ class Cell:
         return _MeanStressFn.apply(torch.as_tensor(w, dtype=torch.float64, device=self._dev), F, r, self)
 
     def precond_w(self, r, scale=1.0):
-        """Inverse symbol of the homogeneous linear operator with unit stiffness (mech.py:175-181). That
+        """The preconditioner's Fourier multiplier: the inverse of the homogeneous (s = 1) operator (mech.py:175-181). That
         operator is the linear path's K(1) divided by N (its energy carries wq = 1/(8N) instead of the
         element matrix's 1/8), so this is N times the linear preconditioner."""
         if self._lh is None:
@@ -277,8 +279,10 @@ class Cell:
 
 
 class Mixed:
-    """Mixed macroscopic control: Fbar components in `free` are unknowns with zero conjugate stress, the rest
-    are prescribed; free=[] is full control. Unknown z = (w, Fbar_free); solve min_z W(z) (mech.py:183-330)."""
+    """Equilibrium under partly prescribed macroscopic deformation (mech.py:183-330). The entries of Fbar listed
+    in `free` are solved for, with their average stress driven to zero; every other entry keeps its given value
+    (an empty `free` prescribes all nine). The unknown vector z stacks the periodic displacement w and the free
+    entries, and the solver minimises W over it."""
 
     def __init__(self, cell, free=((1, 1), (2, 2)), forcing="ew"):
         self.c = cell
@@ -359,7 +363,8 @@ class Mixed:
         return out
 
     def macro_diag(self, A, Fb):
-        """Diagonal of the Hessian in the macro unknowns (for the preconditioner) (mech.py:240-246)."""
+        """Hessian entries H_jj of the free macro entries j, one Hessian product per entry; the preconditioner
+        divides their residual components by them (mech.py:240-246)."""
         d = []
         for j in range(self.k):
             e = torch.zeros(self.nw + self.k, dtype=torch.float64, device=self._dev)
@@ -438,14 +443,15 @@ class Mixed:
         return float(torch.exp(torch.mean(torch.log(self.c.s))))
 
     def solve_linear(self, A, b, tol=1e-10, maxit=20000):
-        """Solve H x = b with the tangent at state A (mean-free in w). Returns (x, iterations, converged)."""
+        """Preconditioned CG on the Hessian at state A, the w part of every iterate kept at zero mean; returns the
+        solution, the iteration count and whether the tolerance was met."""
         dmac = self.macro_diag(A, None) if self.k else np.zeros(0)
         x, it, conv, _ = self._cg(A, b, dmac, self._sws(), tol, int(maxit))
         return x, int(it), bool(conv)
 
     # --- adjoint (mech.py:348-363) -------------------------------------------------------------------------
     def grad_rho(self, z, Fb, q, cg_tol=1e-10, cg_maxit=20000):
-        """Total derivative in rho of a scalar q(z, Fb, rho) evaluated at equilibrium z(rho) (mech.py:348-363).
+        """d/drho of q(z(rho), Fb, rho), with z(rho) the equilibrium for the density rho (mech.py:348-363).
         One adjoint solve with the Hessian, H lam = dq/dz, then the explicit part dq/drho at fixed z minus the
         density derivative of lam . grad_z W. q is a torch function of the device unknowns z and of the cell's
         density, for instance of the macro unknowns z[nw:] or of cell.mean_stress_t(*mix.unpack_t(z, Fb),
@@ -471,10 +477,12 @@ class Mixed:
 
     # --- stability (mech.py:365-409) ------------------------------------------------------------------------
     def lowest_eigs(self, z, Fb, k=4, X=None, tol=1e-5, maxiter=2000):
-        """Lowest eigenvalues of D^-1/2 H D^-1/2 (D the Hessian diagonal at the undeformed state, macro unknowns
-        included), rigid translations excluded; same inertia as H (Sylvester). scipy's LOBPCG drives it on the
-        host, each block product being one matrix-free Hessian product per column on the device.
-        Returns (values, vectors in z coordinates, iterations)."""
+        """The smallest eigenvalues of the Hessian after a symmetric diagonal scaling, D^-1/2 H D^-1/2, with D the
+        Hessian's diagonal at F = I over all unknowns including the macro entries, on the space orthogonal to
+        rigid translations. A congruence keeps the signs of the eigenvalues (Sylvester's law of inertia), so a
+        negative value means H is indefinite. scipy's LOBPCG runs on the host and calls one device Hessian
+        product per block column. Returns the eigenvalues, their eigenvectors mapped back to z, and LOBPCG's
+        iteration count."""
         from scipy.sparse.linalg import LinearOperator, lobpcg
         Fb = np.asarray(Fb, dtype=np.float64)
         A = self.tangent(torch.as_tensor(z, dtype=torch.float64, device=self._dev), Fb)
@@ -582,7 +590,8 @@ class Mixed:
 
 # --- effective tangent and strong ellipticity (mech.py:411-451) ---------------------------------------------------
 def effective_tangent(cell, w, Fbar, cg_tol=1e-10):
-    """A_abcd = d Pbar_ab / d Fbar_cd at the equilibrium (w, Fbar) under full control (mech.py:411-427): for each
+    """Homogenized tangent A_abcd = d Pbar_ab / d Fbar_cd of the equilibrium (w, Fbar), all of Fbar prescribed
+    (mech.py:411-427): for each
     unit direction E_cd, the displacement response dw solves H_ww dw = -H_wF E_cd, and A[:, :, c, d] is the stress
     part of the Hessian product with (dw, E_cd). 9 linear solves; a numpy (3, 3, 3, 3) array."""
     full = Mixed(cell, free=[])
@@ -610,9 +619,9 @@ def fibonacci_sphere(m):
 
 
 def rank_one_min(A, m=1000, plane=False):
-    """min over unit n of the smallest eigenvalue of the acoustic tensor Q_ik = A_ijkl n_j n_l (symmetrised);
-    strong ellipticity is lost when it is <= 0. 2D cells: n in the x-y plane. Returns (value, n)
-    (mech.py:437-451)."""
+    """Ellipticity margin of a tangent A (mech.py:437-451): over unit directions n (in the x-y plane for a plane
+    cell), the least eigenvalue of the symmetrised Q_ik = A_ijkl n_j n_l, and the n that attains it. A value
+    <= 0 marks a direction in which the material can form a localised band."""
     if plane:
         th = np.linspace(0.0, np.pi, m, endpoint=False)
         ns = np.stack([np.cos(th), np.sin(th), np.zeros_like(th)], axis=1)
@@ -726,7 +735,8 @@ def _toec_assemble_t(v):
 
 
 def toec(cell, chi):
-    """C (6x6) and Lagrangian-strain third-order constants T (6x6x6) of the cell from the linear correctors chi
+    """Second- and third-order elastic constants of the cell, C (6x6) and T (6x6x6) in Lagrangian strain, from the
+    linear correctors chi
     (6, nx, ny, nz, 3) of unit Voigt strains (same stiffness scale and E, nu) (mech.py:454-490): with
     f(s) = <psi(I + sum_I s_I (e_I + grad chi_I))>, which equals Wbar to O(s^4), C_IJ = f''_IJ and
     T_IJK = f'''_IJK - 3 Sym C[e_I, e_J e_K]. The second and third derivatives come from one kernel pass
@@ -756,8 +766,9 @@ CUBIC_TOEC = {
 
 
 def cubic_toec(T):
-    """The six cubic third-order constants (C111, C112, C123, C144, C155, C456), each the mean of its equivalent
-    entries (Voigt indices 1..6 = xx yy zz yz xz xy) (mech.py:493-515)."""
+    """Reduce T (6x6x6, Voigt order xx yy zz yz xz xy) to the six independent constants of cubic symmetry,
+    C111, C112, C123, C144, C155 and C456, averaging the entries that cubic symmetry makes equal
+    (mech.py:493-515)."""
     T = np.asarray(T.detach().cpu() if isinstance(T, torch.Tensor) else T)
     return {k: float(np.mean([T[i] for i in idx])) for k, idx in CUBIC_TOEC.items()}
 

@@ -31,8 +31,8 @@ def _offsets(d, connectivity):
 
 
 def periodic_label(b, connectivity=None):
-    """Connected components of a periodic binary array (nlfiber/topology.py:8-46). Returns (labels, n) with
-    labels 1..n and 0 for background."""
+    """Label the solid pieces of a binary array whose faces wrap around (nlfiber/topology.py:8-46). Returns
+    (labels, n): labels 1..n for the pieces, 0 on empty voxels."""
     b = np.asarray(b, dtype=bool)
     d = b.ndim
     idx = -np.ones(b.shape, dtype=np.int64)
@@ -62,8 +62,8 @@ def periodic_label(b, connectivity=None):
 
 
 def spanning(labels, k):
-    """True when component k meets every slice along every axis, so that it can carry load across the cell
-    (nlfiber/topology.py:49-56)."""
+    """Whether component k has a voxel in each plane normal to each axis, i.e. it runs through the cell in all
+    directions and can transmit force between opposite faces (nlfiber/topology.py:49-56)."""
     m = np.asarray(labels) == k
     for ax in range(m.ndim):
         other = tuple(a for a in range(m.ndim) if a != ax)
