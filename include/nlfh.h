@@ -264,6 +264,23 @@ int nlfh_fs_project(const int* shape, double* w, double* work, void* stream);
  * (the free components of a stress sum, device to device). */
 int nlfh_fs_macro_scatter(int k, const int* idx, const double* src, double* dst9, void* stream);
 int nlfh_fs_macro_gather(int k, const int* idx, const double* src9, double* dst, void* stream);
+/* The CG of Mixed._cg (mech.py:266-283) with its dot products folded into the neighbouring passes. part:
+ * nlfh_fs_dot_part_doubles() doubles of device memory. nlfh_fs_project_dot projects w like nlfh_fs_project and
+ * writes to part[0..128) the partial sums of y . w over the (N, 3) part (w after the projection); the consumer
+ * below adds them in a fixed order, plus the k macro terms (the last k entries of both vectors).
+ * nlfh_fs_cg_update_parts: pAp from pap_part (the partials of p . Ap) and the macro terms of p and Ap, stored to
+ * pap_out; alpha = rz / pAp (0 if pAp <= 0), x_out = x + alpha p, r_out = r - alpha Ap, and rn2_part
+ * (ceil(len / 256) doubles) the partials of |r_out|^2. nlfh_fs_cg_direction_parts: rz_new = r . z from rz_part
+ * (the partials of r . z) and the macro terms, p_out = z + (rz_new / rz_old) p (rz_old = 0 taken as 1),
+ * rn2_out = |r|^2 from rn2_part; rz_new must not be rz_old (every block reads rz_old). */
+long long nlfh_fs_dot_part_doubles(void);
+int nlfh_fs_project_dot(const int* shape, double* w, const double* y, double* part, void* stream);
+int nlfh_fs_cg_update_parts(int len, int k, double* x_out, double* r_out, const double* x, const double* r,
+                            const double* p, const double* Ap, const double* rz, const double* pap_part,
+                            double* pap_out, double* rn2_part, void* stream);
+int nlfh_fs_cg_direction_parts(int len, int k, double* p_out, const double* z, const double* p, const double* r,
+                               const double* rz_old, const double* rz_part, double* rz_new, const double* rn2_part,
+                               double* rn2_out, void* stream);
 int nlfh_fs_eval(const int* shape, const double* sg, const double* mat, int flags, const double* w,
                  const double* Fbar, const double* v, const double* dFbar, int mode, double* grad, double* scalars,
                  double* work, void* stream);

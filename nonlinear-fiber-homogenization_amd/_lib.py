@@ -24,6 +24,10 @@ SIGNATURES = [
     ("nlfh_fs_project", ctypes.c_int, [_c_int_p, _vp, _vp, _vp]),
     ("nlfh_fs_macro_scatter", ctypes.c_int, [ctypes.c_int, _c_int_p, _vp, _vp, _vp]),
     ("nlfh_fs_macro_gather", ctypes.c_int, [ctypes.c_int, _c_int_p, _vp, _vp, _vp]),
+    ("nlfh_fs_dot_part_doubles", ctypes.c_longlong, []),
+    ("nlfh_fs_project_dot", ctypes.c_int, [_c_int_p, _vp, _vp, _vp, _vp]),
+    ("nlfh_fs_cg_update_parts", ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_vp] * 11),
+    ("nlfh_fs_cg_direction_parts", ctypes.c_int, [ctypes.c_int, ctypes.c_int] + [_vp] * 10),
     ("nlfh_fs_cg_update", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("nlfh_fs_cg_direction", ctypes.c_int, [ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("nlfh_fs_toec", ctypes.c_int, [_c_int_p, _vp, _c_double_p, ctypes.c_int, _vp, _vp, _vp, _vp]),

@@ -593,6 +593,113 @@ __global__ void fs_macro_gather_kernel(const FsIdx ix, const double* __restrict_
   if (t < ix.k) dst[t] = src[ix.i[t]];
 }
 
+// Dot products of the CG without a reduction launch of their own: the subtraction pass of a projection writes
+// kDotParts per-block partials of y . w (w after the subtraction, its (N, 3) part), and the kernel that needs the
+// value sums them itself, every block in the same fixed order (two values per lane, then a fixed butterfly), and
+// adds the k macro terms in order.
+constexpr int kDotParts = 128;
+
+__device__ __forceinline__ double dot_total(const double* __restrict__ part, const double* __restrict__ a,
+                                            const double* __restrict__ b, int k) {
+  const int lane = threadIdx.x & 63;
+  double acc = part[lane] + part[lane + 64];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+  for (int j = 0; j < k; ++j) acc += a[j] * b[j];
+  return acc;
+}
+
+__global__ void __launch_bounds__(kFsThreads) fs_sub3_dot_kernel(double* __restrict__ w, int n,
+                                                                 const double* __restrict__ part, int nb, double inv_n,
+                                                                 const double* __restrict__ y,
+                                                                 double* __restrict__ dpart) {
+  __shared__ double mean[3];
+  __shared__ double red[kFsThreads / 64];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wv < 3) {
+    double acc = 0.0;
+    for (int b = lane; b < nb; b += 64) acc += part[wv * nb + b];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) mean[wv] = acc * inv_n;
+  }
+  __syncthreads();
+  const int len = 3 * n;
+  double acc = 0.0;
+  for (int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x; i < len; i += (int)gridDim.x * kFsThreads) {
+    const double v = w[i] - mean[i % 3];
+    w[i] = v;
+    acc += y[i] * v;
+  }
+  const double t = block_sum(acc, red);
+  if (threadIdx.x == 0) dpart[blockIdx.x] = t;
+}
+
+// x_out = x + alpha p, r_out = r - alpha Ap with alpha = rz / pAp (0 if pAp <= 0), pAp from its partials and the
+// macro terms (the last k entries); block 0 stores pAp; per-block partials of |r_out|^2
+__global__ void __launch_bounds__(kFsThreads) fs_cg_update_parts_kernel(int len, int k, double* __restrict__ x_out,
+                                                                        double* __restrict__ r_out,
+                                                                        const double* __restrict__ x,
+                                                                        const double* __restrict__ r,
+                                                                        const double* __restrict__ p,
+                                                                        const double* __restrict__ Ap,
+                                                                        const double* __restrict__ rz,
+                                                                        const double* __restrict__ pap_part,
+                                                                        double* __restrict__ pap_out,
+                                                                        double* __restrict__ rn2_part) {
+  __shared__ double red[kFsThreads / 64];
+  __shared__ double al_sh;
+  if (threadIdx.x < 64) {
+    const double pa = dot_total(pap_part, p + (len - k), Ap + (len - k), k);
+    if (threadIdx.x == 0) {
+      al_sh = pa <= 0.0 ? 0.0 : *rz / pa;
+      if (blockIdx.x == 0) *pap_out = pa;
+    }
+  }
+  __syncthreads();
+  const double al = al_sh;
+  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  double v = 0.0;
+  if (i < len) {
+    x_out[i] = x[i] + al * p[i];
+    const double rn = r[i] - al * Ap[i];
+    r_out[i] = rn;
+    v = rn * rn;
+  }
+  const double t = block_sum(v, red);
+  if (threadIdx.x == 0) rn2_part[blockIdx.x] = t;
+}
+
+// p_out = z + beta p with beta = rz_new / rz_old (rz_old = 0 taken as 1), rz_new = r . z from its partials and the
+// macro terms; block 0 stores rz_new and |r|^2 summed from the update's per-block partials (fixed order)
+__global__ void __launch_bounds__(kFsThreads) fs_cg_direction_parts_kernel(int len, int k, double* __restrict__ p_out,
+                                                                           const double* __restrict__ z,
+                                                                           const double* __restrict__ p,
+                                                                           const double* __restrict__ r,
+                                                                           const double* __restrict__ rz_old,
+                                                                           const double* __restrict__ rz_part,
+                                                                           double* __restrict__ rz_new,
+                                                                           const double* __restrict__ rn2_part,
+                                                                           int nrn2, double* __restrict__ rn2_out) {
+  __shared__ double beta_sh;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (wv == 0) {
+    const double rz1 = dot_total(rz_part, r + (len - k), z + (len - k), k);
+    if (lane == 0) {
+      const double ro = *rz_old;
+      beta_sh = rz1 / (ro == 0.0 ? 1.0 : ro);
+      if (blockIdx.x == 0) *rz_new = rz1;
+    }
+  } else if (wv == 1 && blockIdx.x == 0) {
+    double acc = 0.0;
+    for (int b = lane; b < nrn2; b += 64) acc += rn2_part[b];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (lane == 0) *rn2_out = acc;
+  }
+  __syncthreads();
+  const double beta = beta_sh;
+  const int i = (int)blockIdx.x * kFsThreads + (int)threadIdx.x;
+  if (i < len) p_out[i] = z[i] + beta * p[i];
+}
+
 // Third-order elastic constants of the cell from the linear correctors chi_I (6 fields (N, 3)), per
 // Gauss point H_I = e_I + grad chi_I with e_I the symmetric Voigt unit strain (engineering shear: 1/2
 // off the diagonal), at F = I (toec, mech.py:454-490), with f(s) = <psi_eff(I + sum_I s_I H_I)>:
@@ -1069,6 +1176,62 @@ int nlfh_fs_macro_gather(int k, const int* idx, const double* src9, double* dst,
     return NLFH_EINVAL;
   }
   hipLaunchKernelGGL(fs_macro_gather_kernel, dim3(1), dim3(64), 0, (hipStream_t)stream, ix, src9, dst);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+long long nlfh_fs_dot_part_doubles(void) { return (long long)(kDotParts + 3 * kProjBlocks); }
+
+int nlfh_fs_project_dot(const int* shape, double* w, const double* y, double* part, void* stream) {
+  FsArgs a{};
+  const double mat[6] = {0, 0, 0, 1, 0, 0};
+  const int rc = fs_setup(shape, mat, 0, a);
+  if (rc != NLFH_OK) return rc;
+  if (w == nullptr || y == nullptr || part == nullptr) {
+    set_error("nlfh_fs_project_dot: w, y and part are required");
+    return NLFH_EINVAL;
+  }
+  const int n = (int)a.n;
+  const int nodes_blk = (n + kFsThreads - 1) / kFsThreads;
+  const int nb = nodes_blk < kProjBlocks ? nodes_blk : kProjBlocks;
+  double* sums = part + kDotParts;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fs_sum3_kernel, dim3(nb), dim3(kFsThreads), 0, st, w, n, sums);
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL(fs_sub3_dot_kernel, dim3(kDotParts), dim3(kFsThreads), 0, st, w, n, sums, nb, 1.0 / n, y, part);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_cg_update_parts(int len, int k, double* x_out, double* r_out, const double* x, const double* r,
+                            const double* p, const double* Ap, const double* rz, const double* pap_part,
+                            double* pap_out, double* rn2_part, void* stream) {
+  if (len < 1 || k < 0 || k > len || x_out == nullptr || r_out == nullptr || x == nullptr || r == nullptr ||
+      p == nullptr || Ap == nullptr || rz == nullptr || pap_part == nullptr || pap_out == nullptr ||
+      rn2_part == nullptr) {
+    set_error("nlfh_fs_cg_update_parts: a required pointer is null, len < 1 or k outside [0, len]");
+    return NLFH_EINVAL;
+  }
+  const int nb = (len + kFsThreads - 1) / kFsThreads;
+  hipLaunchKernelGGL(fs_cg_update_parts_kernel, dim3(nb), dim3(kFsThreads), 0, (hipStream_t)stream, len, k, x_out,
+                     r_out, x, r, p, Ap, rz, pap_part, pap_out, rn2_part);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_fs_cg_direction_parts(int len, int k, double* p_out, const double* z, const double* p, const double* r,
+                               const double* rz_old, const double* rz_part, double* rz_new, const double* rn2_part,
+                               double* rn2_out, void* stream) {
+  if (len < 1 || k < 0 || k > len || p_out == nullptr || z == nullptr || p == nullptr || r == nullptr ||
+      rz_old == nullptr || rz_part == nullptr || rz_new == nullptr || rn2_part == nullptr || rn2_out == nullptr ||
+      rz_new == rz_old) {
+    set_error("nlfh_fs_cg_direction_parts: a required pointer is null, rz_new is rz_old, len < 1 or k outside "
+              "[0, len]");
+    return NLFH_EINVAL;
+  }
+  const int nb = (len + kFsThreads - 1) / kFsThreads;
+  hipLaunchKernelGGL(fs_cg_direction_parts_kernel, dim3(nb), dim3(kFsThreads), 0, (hipStream_t)stream, len, k, p_out,
+                     z, p, r, rz_old, rz_part, rz_new, rn2_part, nb, rn2_out);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -398,8 +398,12 @@ class Mixed:
     def _cg(self, A, b, dmac, sws, tol, maxit):
         """Preconditioned CG on the mean-free space with the reference's negative-curvature exit: a
         non-positive p.Ap stops the loop, keeping x (or z0 = M b on the first iteration). The scalars stay on
-        the device (the x, r and p updates are two fused kernels); one host read per iteration fetches p.Ap
-        and |r|^2 for the loop test."""
+        the device: the projections of Ap and z leave the partial sums of p.Ap and r.z, which the x, r update
+        and the direction update (two fused kernels) sum themselves. The loop test of step k needs p.Ap and
+        |r|^2 on the host: they are copied into pinned memory behind an event, and step k + 1 is queued before
+        the host waits for them, so the GPU does not idle while the host decides and launches. Three buffer sets
+        rotate, so a step queued past the exit never touches the iterate the exit keeps; its results are
+        dropped."""
         proj = self.proj
         lib, c = self.c._lib, self.c
 
@@ -414,27 +418,54 @@ class Mixed:
         bn = float(torch.linalg.norm(b)) + 1e-300
         z0 = M(b)
         L = b.numel()
-        x, r, p = torch.zeros_like(b), b, z0
-        bufs = [torch.empty_like(b) for _ in range(6)]  # two sets of (x, r, p), alternated
-        rz = torch.dot(b, z0).reshape(1)
-        rn2 = torch.empty(1, dtype=torch.float64, device=self._dev)
-        it, done, conv = 0, False, False
+        sets = [[torch.empty_like(b) for _ in range(3)] for _ in range(3)]  # (x, r, p) of step k in set k % 3
+        npart = int(lib.nlfh_fs_dot_part_doubles())
+        part_pa = torch.empty(npart, dtype=torch.float64, device=self._dev)
+        part_rz = torch.empty(npart, dtype=torch.float64, device=self._dev)
+        rn2_part = torch.empty((L + 255) // 256, dtype=torch.float64, device=self._dev)
+        rzs = torch.empty(2, dtype=torch.float64, device=self._dev)  # r.z read by step k in rzs[k % 2]
+        rzs[0] = torch.dot(b, z0)
+        s2 = torch.empty((2, 2), dtype=torch.float64, device=self._dev)  # (p.Ap, |r|^2) of step k in s2[k % 2]
+        h2 = torch.empty((2, 2), dtype=torch.float64, pin_memory=True)
+        events = [torch.cuda.Event(), torch.cuda.Event()]
+        stream = torch.cuda.current_stream()
+        inputs = {0: (torch.zeros_like(b), b, z0)}  # (x, r, p) each step starts from
+
+        def launch(k):
+            x, r, p = inputs[k]
+            x1, r1, p1 = sets[k % 3]
+            rz, rz1 = rzs[k % 2:k % 2 + 1], rzs[(k + 1) % 2:(k + 1) % 2 + 1]
+            sk = s2[k % 2]
+            Ap = self.hvp_A(A, p, Apb)
+            _lib.check(lib.nlfh_fs_project_dot(c._shape_c, _ptr(Ap), _ptr(p), _ptr(part_pa), _stream()))
+            _lib.check(lib.nlfh_fs_cg_update_parts(L, self.k, _ptr(x1), _ptr(r1), _ptr(x), _ptr(r), _ptr(p), _ptr(Ap),
+                                                   _ptr(rz), _ptr(part_pa), _ptr(sk[0:1]), _ptr(rn2_part), _stream()))
+            z1 = self.precond(r1, dmac, sws, zb)
+            _lib.check(lib.nlfh_fs_project_dot(c._shape_c, _ptr(z1), _ptr(r1), _ptr(part_rz), _stream()))
+            _lib.check(lib.nlfh_fs_cg_direction_parts(L, self.k, _ptr(p1), _ptr(z1), _ptr(p), _ptr(r1), _ptr(rz),
+                                                      _ptr(part_rz), _ptr(rz1), _ptr(rn2_part), _ptr(sk[1:2]),
+                                                      _stream()))
+            h2[k % 2].copy_(sk, non_blocking=True)
+            events[k % 2].record(stream)
+            inputs[k + 1] = (x1, r1, p1)
+
+        it, done, conv, x = 0, False, False, None
+        if maxit > 0:
+            launch(0)
         while not done and it < maxit:
-            x1, r1, p1 = bufs[0:3] if it % 2 == 0 else bufs[3:6]
-            Ap = self._proj_(self.hvp_A(A, p, Apb))
-            pAp = torch.dot(p, Ap).reshape(1)
-            _lib.check(lib.nlfh_fs_cg_update(L, _ptr(x1), _ptr(r1), _ptr(x), _ptr(r), _ptr(p), _ptr(Ap), _ptr(rz),
-                                             _ptr(pAp), _ptr(rn2), _ptr(c._work), _stream()))
-            z1 = M(r1, zb)
-            rz1 = torch.dot(r1, z1).reshape(1)
-            _lib.check(lib.nlfh_fs_cg_direction(L, _ptr(p1), _ptr(z1), _ptr(p), _ptr(rz1), _ptr(rz), _stream()))
-            pAp_h, rn2_h = torch.cat([pAp, rn2]).tolist()
+            k = it
+            if k + 1 < maxit:
+                launch(k + 1)  # queued ahead; dropped if step k ends the loop
+            events[k % 2].synchronize()
+            pAp_h, rn2_h = h2[k % 2].tolist()
             neg = pAp_h <= 0
             conv1 = math.sqrt(max(rn2_h, 0.0)) <= tol * bn
-            if neg:
-                x1 = z0 if it == 0 else x
-            x, r, p, rz, it = x1, r1, p1, rz1, it + 1
+            x = (z0 if k == 0 else inputs[k][0]) if neg else inputs[k + 1][0]
+            inputs.pop(k - 1, None)
+            it = k + 1
             done, conv = neg or conv1, conv1 and not neg
+        if x is None:
+            x = torch.zeros_like(b)
         return x.clone(), it, conv, done and not conv
 
     def _sws(self):

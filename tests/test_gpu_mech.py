@@ -121,6 +121,51 @@ def test_projection_removes_the_component_means(M, shape):
     assert out[-2:].tolist() == v[-2:].tolist()
 
 
+@pytest.mark.parametrize("shape,k", [((7, 5, 3), 2), ((64, 64, 17), 0), ((40, 30, 45), 3)])
+def test_cg_steps_with_folded_dot_products(M, gpu_pkg, shape, k):
+    # project_dot + cg_update_parts + cg_direction_parts against the same CG step in numpy
+    import ctypes
+    import torch
+    lib = gpu_pkg.load_library()
+    rng = np.random.default_rng(21)
+    N = int(np.prod(shape))
+    L = 3 * N + k
+    vec = lambda: rng.standard_normal(L) + 0.2  # noqa: E731
+    x, r, p, Ap, z = vec(), vec(), vec(), vec(), vec()
+    rz_old = 0.7
+    d = {n: torch.as_tensor(v, device="cuda").clone() for n, v in dict(x=x, r=r, p=p, Ap=Ap, z=z).items()}
+    npart = int(lib.nlfh_fs_dot_part_doubles())
+    part = torch.empty(npart, dtype=torch.float64, device="cuda")
+    shp = (ctypes.c_int * 3)(*shape)
+    P = lambda t: ctypes.c_void_p(t.data_ptr())  # noqa: E731
+    assert lib.nlfh_fs_project_dot(shp, P(d["Ap"]), P(d["p"]), P(part), None) == 0
+    Apw = Ap[:3 * N].reshape(-1, 3)
+    Ap_p = np.concatenate([(Apw - Apw.mean(axis=0)).ravel(), Ap[3 * N:]])
+    np.testing.assert_allclose(d["Ap"].cpu().numpy(), Ap_p, rtol=0, atol=1e-13)
+    pAp = float(p @ Ap_p)
+    rz = torch.tensor([rz_old, -1.0], dtype=torch.float64, device="cuda")
+    s2 = torch.zeros(2, dtype=torch.float64, device="cuda")
+    x1, r1, p1 = (torch.empty(L, dtype=torch.float64, device="cuda") for _ in range(3))
+    rn2p = torch.empty((L + 255) // 256, dtype=torch.float64, device="cuda")
+    assert lib.nlfh_fs_cg_update_parts(L, k, P(x1), P(r1), P(d["x"]), P(d["r"]), P(d["p"]), P(d["Ap"]), P(rz[0:1]),
+                                       P(part), P(s2[0:1]), P(rn2p), None) == 0
+    al = rz_old / pAp if pAp > 0 else 0.0
+    assert abs(s2[0].item() - pAp) <= 1e-12 * abs(pAp) + 1e-12
+    np.testing.assert_allclose(x1.cpu().numpy(), x + al * p, rtol=0, atol=1e-12)
+    r1_h = r - al * Ap_p
+    np.testing.assert_allclose(r1.cpu().numpy(), r1_h, rtol=0, atol=1e-12)
+    assert lib.nlfh_fs_project_dot(shp, P(d["z"]), P(r1), P(part), None) == 0
+    zw = z[:3 * N].reshape(-1, 3)
+    z_p = np.concatenate([(zw - zw.mean(axis=0)).ravel(), z[3 * N:]])
+    rz1 = float(r1_h @ z_p)
+    assert lib.nlfh_fs_cg_direction_parts(L, k, P(p1), P(d["z"]), P(d["p"]), P(r1), P(rz[0:1]), P(part), P(rz[1:2]),
+                                          P(rn2p), P(s2[1:2]), None) == 0
+    assert abs(rz[1].item() - rz1) <= 1e-12 * abs(rz1) + 1e-12
+    want = z_p + (rz1 / rz_old) * p
+    np.testing.assert_allclose(p1.cpu().numpy(), want, rtol=0, atol=1e-12 * np.abs(want).max())
+    assert abs(s2[1].item() - float(r1_h @ r1_h)) <= 1e-12 * float(r1_h @ r1_h)
+
+
 def test_macro_scatter_and_gather(M, gpu_pkg):
     import ctypes
     import torch
