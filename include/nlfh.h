@@ -266,7 +266,7 @@ int nlfh_fs_macro_scatter(int k, const int* idx, const double* src, double* dst9
 int nlfh_fs_macro_gather(int k, const int* idx, const double* src9, double* dst, void* stream);
 /* The CG of Mixed._cg (mech.py:266-283) with its dot products folded into the neighbouring passes. part:
  * nlfh_fs_dot_part_doubles() doubles of device memory. nlfh_fs_project_dot projects w like nlfh_fs_project and
- * writes to part[0..128) the partial sums of y . w over the (N, 3) part (w after the projection); the consumer
+ * writes to part[0..256) the partial sums of y . w over the (N, 3) part (w after the projection); the consumer
  * below adds them in a fixed order, plus the k macro terms (the last k entries of both vectors).
  * nlfh_fs_cg_update_parts: pAp from pap_part (the partials of p . Ap) and the macro terms of p and Ap, stored to
  * pap_out; alpha = rz / pAp (0 if pAp <= 0), x_out = x + alpha p, r_out = r - alpha Ap, and rn2_part

@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_gather_kernel(const FsArgs a, d
 // subtraction pass whose every block first forms the three means from those partials, each component by one
 // wavefront in a fixed order (lane-strided sums, then a fixed butterfly), so all blocks subtract bitwise the
 // same values.
-constexpr int kProjBlocks = 128;
+constexpr int kProjBlocks = 512;
 
 __global__ void __launch_bounds__(kFsThreads) fs_sum3_kernel(const double* __restrict__ w, int n,
                                                              double* __restrict__ part) {
@@ -560,6 +560,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_sub3_kernel(double* __restrict_
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (wv < 3) {
     double acc = 0.0;
+#pragma unroll 8
     for (int b = lane; b < nb; b += 64) acc += part[wv * nb + b];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (lane == 0) mean[wv] = acc * inv_n;
@@ -597,12 +598,13 @@ __global__ void fs_macro_gather_kernel(const FsIdx ix, const double* __restrict_
 // kDotParts per-block partials of y . w (w after the subtraction, its (N, 3) part), and the kernel that needs the
 // value sums them itself, every block in the same fixed order (two values per lane, then a fixed butterfly), and
 // adds the k macro terms in order.
-constexpr int kDotParts = 128;
+constexpr int kDotParts = 256;
 
 __device__ __forceinline__ double dot_total(const double* __restrict__ part, const double* __restrict__ a,
                                             const double* __restrict__ b, int k) {
   const int lane = threadIdx.x & 63;
-  double acc = part[lane] + part[lane + 64];
+  static_assert(kDotParts == 256, "four partials per lane");
+  double acc = (part[lane] + part[lane + 64]) + (part[lane + 128] + part[lane + 192]);
   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
   for (int j = 0; j < k; ++j) acc += a[j] * b[j];
   return acc;
@@ -617,6 +619,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_sub3_dot_kernel(double* __restr
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (wv < 3) {
     double acc = 0.0;
+#pragma unroll 8
     for (int b = lane; b < nb; b += 64) acc += part[wv * nb + b];
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
     if (lane == 0) mean[wv] = acc * inv_n;
@@ -669,7 +672,8 @@ __global__ void __launch_bounds__(kFsThreads) fs_cg_update_parts_kernel(int len,
 }
 
 // p_out = z + beta p with beta = rz_new / rz_old (rz_old = 0 taken as 1), rz_new = r . z from its partials and the
-// macro terms; block 0 stores rz_new and |r|^2 summed from the update's per-block partials (fixed order)
+// macro terms; block 0 stores rz_new and |r|^2 summed from the update's per-block partials by all its threads
+// (fixed order: a thread's partials in turn, then the block sum)
 __global__ void __launch_bounds__(kFsThreads) fs_cg_direction_parts_kernel(int len, int k, double* __restrict__ p_out,
                                                                            const double* __restrict__ z,
                                                                            const double* __restrict__ p,
@@ -680,6 +684,7 @@ __global__ void __launch_bounds__(kFsThreads) fs_cg_direction_parts_kernel(int l
                                                                            const double* __restrict__ rn2_part,
                                                                            int nrn2, double* __restrict__ rn2_out) {
   __shared__ double beta_sh;
+  __shared__ double red[kFsThreads / 64];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (wv == 0) {
     const double rz1 = dot_total(rz_part, r + (len - k), z + (len - k), k);
@@ -688,11 +693,13 @@ __global__ void __launch_bounds__(kFsThreads) fs_cg_direction_parts_kernel(int l
       beta_sh = rz1 / (ro == 0.0 ? 1.0 : ro);
       if (blockIdx.x == 0) *rz_new = rz1;
     }
-  } else if (wv == 1 && blockIdx.x == 0) {
+  }
+  if (blockIdx.x == 0) {  // uniform over the block
     double acc = 0.0;
-    for (int b = lane; b < nrn2; b += 64) acc += rn2_part[b];
-    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) *rn2_out = acc;
+#pragma unroll 8
+    for (int b = threadIdx.x; b < nrn2; b += kFsThreads) acc += rn2_part[b];
+    const double t = block_sum(acc, red);
+    if (threadIdx.x == 0) *rn2_out = t;
   }
   __syncthreads();
   const double beta = beta_sh;

@@ -57,7 +57,7 @@ int main(void) {
   EXPECT(nlfh_fs_macro_scatter(3, midx, d9, d9, NULL) != 0, "macro scatter repeated index");
   EXPECT(nlfh_fs_macro_gather(1, NULL, d9, d9, NULL) != 0, "macro gather without idx");
   EXPECT(nlfh_fs_macro_gather(-1, midx, d9, d9, NULL) != 0, "macro gather k < 0");
-  EXPECT(nlfh_fs_dot_part_doubles() == 128 + 3 * 128, "dot partials size");
+  EXPECT(nlfh_fs_dot_part_doubles() == 256 + 3 * 512, "dot partials size");
   EXPECT(nlfh_fs_project_dot(sh, d9, NULL, d9, NULL) != 0, "project_dot without y");
   EXPECT(nlfh_fs_cg_update_parts(8, 9, d9, d9, d9, d9, d9, d9, d9, d9, d9, d9, NULL) != 0, "cg update parts k > len");
   EXPECT(nlfh_fs_cg_direction_parts(8, 0, d9, d9, d9, d9, d9, d9, d9, d9, d9, NULL) != 0, "cg direction rz_new == rz_old");
