@@ -1246,9 +1246,11 @@ int nlfh_plan_set_fft_tiling(nlfh_plan* p, int shape_index, int pass_b_positions
     CHECK_ARG(pass_b_positions == 0 && pass_a_planes == 1, "the generic passes have no tunable block shape");
     return NLFH_OK;
   }
-  grp->ppa32 = pass_a_planes;  // every fast cell has an even nx: a block's two planes share cell and component
   CHECK_ARG(fast_pass_b_ok(grp->fast, pass_b_positions),
             "pass_b_positions must be 0, the default, or half or double the default");
+  CHECK_ARG(pass_a_planes == 1 || fast_two_planes_ok(grp->fast),
+            "pass_a_planes must be 1 for this cell side (its staged pass A holds one plane per block)");
+  grp->ppa32 = pass_a_planes;  // every fast cell has an even nx: a block's two planes share cell and component
   grp->lpb32 = pass_b_positions == fast_line_len(grp->fast, 4) ? 0 : pass_b_positions;
   // cached graphs bake the line tables: drop them
   HIP_TRY(hipStreamSynchronize(p->stream));

@@ -339,12 +339,32 @@ struct Vec16<double> {
   __device__ static double get(const double2& v, int k) { return k == 0 ? v.x : v.y; }
 };
 
+// IP (FP32 at 128): 512-thread blocks over two rounds of lines, each line's exchange in its own slots of the plane
+// (a row pair or a column, as the FP64 128-point kernels do) and the real plane staged in the spectrum's own slots
+// (row y in row y's 2 NH values), so that a block needs the plane alone, 69 KiB, and two blocks share a CU; with
+// the separate exchange buffer (139 KiB) one 1024-thread block holds the CU, and its load and transform phases
+// cannot overlap another block's.
 template <typename T, int N>
 struct StageCfg {
-  static constexpr int RS = N + (sizeof(T) == 4 ? 4 : 2);
-  static constexpr bool fits = !FastCfg<T, N>::XIN && FastCfg<T, N>::LPR >= N / 2 &&
-                               (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>);
+  static constexpr bool IP = sizeof(T) == 4 && N == 128;
+  static constexpr int BT = IP ? 512 : FastCfg<T, N>::BT;
+  static constexpr int LPR = BT / FastCfg<T, N>::L;
+  static constexpr int RS = IP ? 2 * FastCfg<T, N>::NH : N + (sizeof(T) == 4 ? 4 : 2);
+  static constexpr bool fits = IP || (!FastCfg<T, N>::XIN && FastCfg<T, N>::LPR >= N / 2 &&
+                               (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>));
+  // waves per SIMD the compiler may plan for: the in-place form's LDS admits two 8-wave blocks per CU (4 per
+  // SIMD), so it gets their 128 registers; the others keep the default range
+  static constexpr int WMIN = IP ? 4 : 1, WMAX = IP ? 4 : 8;
+  static constexpr size_t lds_bytes =
+      IP ? ((size_t)N + (size_t)N * FastCfg<T, N>::NH + N) * sizeof(cplx<T>)
+         : ((size_t)N + (size_t)N * FastCfg<T, N>::NH + (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS + N) * sizeof(cplx<T>);
 };
+
+// exchange slot of element (k1, t) of a line inside the line's own N slots (in-place form of xslot)
+template <int N2>
+__device__ __forceinline__ int xslot_ip(int k1, int t) {
+  return k1 * N2 + ((t + k1) & (N2 - 1));
+}
 
 // UPD: 0 none; 1 r -= alpha Ap; 2 also x += alpha p; 4 spectral residual (FP32 inner iteration): the
 // plane of Ap is transformed and the residual's (z, y) spectrum takes the update in place,
@@ -358,12 +378,15 @@ struct StageCfg {
 // is staged in LDS, so they are in flight during the first plane's transforms (the passes are LDS-bound to four
 // blocks per CU, which otherwise leaves their loads to the blocks' own load phases).
 template <typename T, int N, int UPD, int PPB = 1>
-__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+__global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_waves_per_eu(StageCfg<T, N>::WMIN, StageCfg<T, N>::WMAX))) prec_a_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   constexpr bool UPDATE = UPD == 1 || UPD == 2;
   constexpr bool SPEC = UPD == 4;
   static_assert(PPB == 1 || (PPB == 2 && SPEC), "two planes per block: spectral update only");
   using C = FastCfg<T, N>;
-  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
+  using SC = StageCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = SC::BT, LPR = SC::LPR;
+  constexpr bool IP = SC::IP;
+  static_assert(!IP || PPB == 1, "one plane per block in the in-place form");
   constexpr int RS = StageCfg<T, N>::RS;
   static_assert(StageCfg<T, N>::fits, "staged pass A needs one round of lines and room for the plane");
   using VT = Vec16<T>;
@@ -372,9 +395,9 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* plane = tw + N;
-  cplx<T>* xch = plane + N * NH;
-  cplx<T>* dcn = xch + C::LPR * C::XS;
-  T* rp = reinterpret_cast<T*>(xch);
+  cplx<T>* xch = plane + N * NH;  // not used in the in-place form
+  cplx<T>* dcn = IP ? xch : xch + C::LPR * C::XS;
+  T* rp = IP ? reinterpret_cast<T*>(plane) : reinterpret_cast<T*>(xch);
   const int item0 = a.plane_base + (int)blockIdx.x * PPB;
   if (PPB > 1 && (int)blockIdx.x * PPB >= a.plane_count) return;
   const PlaneItem pi0 = a.planes[item0];
@@ -399,7 +422,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   constexpr int NVS = N * NH / CPV;
   constexpr int RPT = SPEC ? (NVS + BT - 1) / BT : 1;
   V lbuf[PPB][LPT];
-  V rpre[PPB][RPT];
+  V rpre[PPB][IP ? 1 : RPT];
   auto field_off = [&](const PlaneItem& q) {
     return cd.off_field + ((int64_t)rhs * 3 + q.comp) * cd.nvox + (int64_t)q.x * N * N;
   };
@@ -416,7 +439,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
         if (i < NV) lbuf[q][j] = Rv[i];
       }
     }
-    if constexpr (SPEC) {
+    if constexpr (SPEC && !IP) {
       const V* rs = spec_ptr(pq);
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {
@@ -502,8 +525,39 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   const int l = tid / L, t = tid - (tid / L) * L;
   __syncthreads();
 
-  // z: complex transform of row pairs (one round: line l = pair p)
+  // z: complex transform of row pairs (line l = pair p; two rounds in the in-place form, where a pair's exchange
+  // and its Z stay in the pair's own rows, which only the pair's line (one wavefront) reads and writes)
   {
+  if constexpr (IP) {
+  for (int p0 = 0; p0 < (IP ? NP : 1); p0 += LPR) {  // one pass over all pairs when not in place
+    const int p = IP ? p0 + l : l;
+    const bool act = p < NP && t < N2;
+    cplx<T> v[N1];
+    if (act) {
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) v[j1] = {rp[(2 * p) * RS + t + N2 * j1], rp[(2 * p + 1) * RS + t + N2 * j1]};
+    }
+    __syncthreads();  // rp is read completely before the exchange (same memory) is written
+    cplx<T>* xl = IP ? plane + (2 * p) * NH : xch + l * C::XS;
+    if (act) {
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1)
+        xl[IP ? xslot_ip<N2>(k1, t) : k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (p < NP && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[IP ? xslot_ip<N2>(t, j2) : t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+      cplx<T>* zp = plane + (2 * p) * NH;  // the pair's two rows hold Z[0..N-1] for now
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) zp[t + N1 * k2] = w[k2];
+    }
+    __syncthreads();
+  }
+  } else {
     const int p = l;
     const bool act = p < NP && t < N2;
     cplx<T> v[N1];
@@ -529,6 +583,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
       for (int k2 = 0; k2 < N2; ++k2) zp[t + N1 * k2] = w[k2];
     }
     __syncthreads();
+  }
     constexpr int ITEMS = NP * NH;
     constexpr int IPT = (ITEMS + BT - 1) / BT;
     cplx<T> A[IPT], B[IPT];
@@ -556,7 +611,41 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
   }
   __syncthreads();
 
-  // y: columns 1..NP-1 complex, columns 0 and NP (real) packed into line 0; results in place
+  // y: columns 1..NP-1 complex, columns 0 and NP (real) packed into line 0; results in place (the in-place form
+  // exchanges inside the column, stride NH)
+  if constexpr (IP) {
+  for (int c0 = 0; c0 < (IP ? NP : 1); c0 += LPR) {  // one pass over all columns when not in place
+    const int col = IP ? c0 + l : l;
+    const bool act = col < NP;
+    cplx<T>* xl = IP ? plane + col : xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        v[j1] = col == 0 ? cplx<T>{plane[y * NH].re, plane[y * NH + NP].re} : plane[y * NH + col];
+      }
+      Dft<T, N1, -1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1)
+        xl[IP ? xslot_ip<N2>(k1, t) * NH : k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], false);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[IP ? xslot_ip<N2>(t, j2) * NH : t * (N2 + 1) + j2];
+      Dft<T, N2, -1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int ky = t + N1 * k2;
+        if (col == 0) dcn[ky] = w[k2];
+        else plane[ky * NH + col] = w[k2];
+      }
+    }
+    __syncthreads();
+  }
+  } else {
   {
     const int col = l;
     const bool act = col < NP;
@@ -587,6 +676,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
     }
     __syncthreads();
   }
+  }
   // unpack the packed DC / Nyquist columns
   for (int ky = tid; ky < N; ky += BT) {
     const cplx<T> Z = dcn[ky], M = dcn[ky == 0 ? 0 : N - ky];
@@ -601,23 +691,50 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
     if constexpr (SPEC) {
       alpha = (T)alpha_sh;  // written before the barriers above
       double sacc = 0.0;
+      if constexpr (!IP) {
 #pragma unroll
-      for (int j = 0; j < RPT; ++j) {
-        const int i = tid + j * BT;
-        if (i >= NVS) break;
-        V rh = rpre[q][j];
-        const V ap = src[i];
-        if constexpr (VN == 4) {
-          rh.x -= alpha * ap.x; rh.y -= alpha * ap.y; rh.z -= alpha * ap.z; rh.w -= alpha * ap.w;
-        } else {
-          rh.x -= alpha * ap.x; rh.y -= alpha * ap.y;
+        for (int j = 0; j < RPT; ++j) {
+          const int i = tid + j * BT;
+          if (i >= NVS) break;
+          V rh = rpre[q][j];
+          const V ap = src[i];
+          if constexpr (VN == 4) {
+            rh.x -= alpha * ap.x; rh.y -= alpha * ap.y; rh.z -= alpha * ap.z; rh.w -= alpha * ap.w;
+          } else {
+            rh.x -= alpha * ap.x; rh.y -= alpha * ap.y;
+          }
+          dst[i] = rh;
+#pragma unroll
+          for (int c = 0; c < CPV; ++c) {
+            const int kz = (i * CPV + c) % NH;
+            const double re = (double)VT::get(rh, 2 * c), im = (double)VT::get(rh, 2 * c + 1);
+            sacc += (kz == 0 || 2 * kz == N ? 1.0 : 2.0) * (re * re + im * im);
+          }
         }
-        dst[i] = rh;
+      } else {
+        // in-place form: the residual spectrum is read here (not prefetched; the other block on the CU covers the
+        // latency), its loads issued together
+        const V* rs = spec_ptr(pi);
+        V rh[RPT];
 #pragma unroll
-        for (int c = 0; c < CPV; ++c) {
-          const int kz = (i * CPV + c) % NH;
-          const double re = (double)VT::get(rh, 2 * c), im = (double)VT::get(rh, 2 * c + 1);
-          sacc += (kz == 0 || 2 * kz == N ? 1.0 : 2.0) * (re * re + im * im);
+        for (int j = 0; j < RPT; ++j) {
+          const int i = tid + j * BT;
+          if (i < NVS) rh[j] = rs[i];
+        }
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+          const int i = tid + j * BT;
+          if (i < NVS) {
+            const V ap = src[i];
+            rh[j].x -= alpha * ap.x; rh[j].y -= alpha * ap.y; rh[j].z -= alpha * ap.z; rh[j].w -= alpha * ap.w;
+            dst[i] = rh[j];
+#pragma unroll
+            for (int c = 0; c < CPV; ++c) {
+              const int kz = (i * CPV + c) % NH;
+              const double re = (double)VT::get(rh[j], 2 * c), im = (double)VT::get(rh[j], 2 * c + 1);
+              sacc += (kz == 0 || 2 * kz == N ? 1.0 : 2.0) * (re * re + im * im);
+            }
+          }
         }
       }
       acc = sacc * (1.0 / ((double)N * (double)N));
@@ -639,10 +756,12 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_a_staged(const PrecA
 // both iterations, x += alpha_{k-1} p_{k-1} + alpha_k p_k, where p_{k-1} is the old content of a.z, read
 // by the same thread before it writes p_{k+1} there (x is read and written every second iteration).
 template <typename T, int N, bool DOT, int DIR = 0>
-__global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
+__global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_waves_per_eu(StageCfg<T, N>::WMIN, StageCfg<T, N>::WMAX))) prec_c_staged(const PrecArgs<T> a, const cplx<T>* __restrict__ twg) {
   static_assert(!(DOT && DIR), "the direction pass takes r . z from pass B");
   using C = FastCfg<T, N>;
-  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = C::BT;
+  using SC = StageCfg<T, N>;
+  constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = SC::BT, LPR = SC::LPR;
+  constexpr bool IP = SC::IP;
   constexpr int RS = StageCfg<T, N>::RS;
   static_assert(StageCfg<T, N>::fits, "staged pass C needs one round of lines and room for the plane");
   using VT = Vec16<T>;
@@ -652,8 +771,8 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   __shared__ double red[16];  // one per wavefront (up to 1024 threads)
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_d);
   cplx<T>* plane = tw + N;
-  cplx<T>* xch = plane + N * NH;
-  T* rp = reinterpret_cast<T*>(xch);
+  cplx<T>* xch = plane + N * NH;  // not used in the in-place form
+  T* rp = IP ? reinterpret_cast<T*>(plane) : reinterpret_cast<T*>(xch);
   const int item = a.plane_base + blockIdx.x;
   const PlaneItem pi = a.planes[item];
   if (a.done != nullptr && a.done[pi.cell]) return;
@@ -690,7 +809,9 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   const int64_t fo = cd.off_field + ((int64_t)rhs * 3 + pi.comp) * cd.nvox + (int64_t)pi.x * N * N;
   constexpr int NV = N * N / VN;
   constexpr int PPT = DIR ? (NV + BT - 1) / BT : 1;
-  constexpr int XPT = DIR == 2 ? PPT : 1;
+  // DIR 2 in the in-place form (twice the values per thread) reads x and p_{k-1} at the store instead
+  constexpr bool XQPRE = DIR == 2 && !IP;
+  constexpr int XPT = XQPRE ? PPT : 1;
   V ppre[PPT], xpre[XPT], qpre[XPT];
   if constexpr (DIR != 0) {
     const V* Pv = reinterpret_cast<const V*>(a.p + fo);
@@ -701,7 +822,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
       const int i = tid + j * BT;
       if (i < NV) {
         ppre[j] = Pv[i];
-        if constexpr (DIR == 2) {
+        if constexpr (XQPRE) {
           xpre[j] = Xv[i];
           qpre[j] = Qv[i];
         }
@@ -729,7 +850,51 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
   const int l = tid / L, t = tid - (tid / L) * L;
   __syncthreads();
 
-  // y inverse (one round); columns 0 and NP packed (their inverse transforms are real); in place
+  // y inverse; columns 0 and NP packed (their inverse transforms are real); in place (two rounds of columns in
+  // the in-place form, whose exchange stays in the column)
+  if constexpr (IP) {
+  for (int c0 = 0; c0 < (IP ? NP : 1); c0 += LPR) {  // one pass over all columns when not in place
+    const int col = IP ? c0 + l : l;
+    const bool act = col < NP;
+    cplx<T>* xl = IP ? plane + col : xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int y = t + N2 * j1;
+        if (col == 0) {
+          const cplx<T> A = plane[y * NH], B = plane[y * NH + NP];
+          v[j1] = {A.re - B.im, A.im + B.re};
+        } else {
+          v[j1] = plane[y * NH + col];
+        }
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1)
+        xl[IP ? xslot_ip<N2>(k1, t) * NH : k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    if (act && t < N1) {
+      cplx<T> w[N2];
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[IP ? xslot_ip<N2>(t, j2) * NH : t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int y = t + N1 * k2;
+        if (col == 0) {
+          plane[y * NH] = {w[k2].re, T(0)};
+          plane[y * NH + NP] = {w[k2].im, T(0)};
+        } else {
+          plane[y * NH + col] = w[k2];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  } else {
   {
     const int col = l;
     const bool act = col < NP;
@@ -770,7 +935,57 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     __syncthreads();
   }
 
-  // z inverse: row pairs, Z = A + i B with the Hermitian extension; outputs into rp
+  }
+  // z inverse: row pairs, Z = A + i B with the Hermitian extension; outputs into rp (in the in-place form the
+  // pair's exchange and its two real output rows stay in the pair's own rows)
+  if constexpr (IP) {
+  for (int p0 = 0; p0 < (IP ? NP : 1); p0 += LPR) {  // one pass over all pairs when not in place
+    const int p = IP ? p0 + l : l;
+    const bool act = p < NP;
+    cplx<T>* xl = IP ? plane + (2 * p) * NH : xch + l * C::XS;
+    if (act && t < N2) {
+      cplx<T> v[N1];
+      const cplx<T>* ra = plane + (2 * p) * NH;
+      const cplx<T>* rb = ra + NH;
+#pragma unroll
+      for (int j1 = 0; j1 < N1; ++j1) {
+        const int k = t + N2 * j1;
+        cplx<T> A, B;
+        if (k < NH) {
+          A = ra[k];
+          B = rb[k];
+        } else {
+          A = ra[N - k];
+          A.im = -A.im;
+          B = rb[N - k];
+          B.im = -B.im;
+        }
+        v[j1] = {A.re - B.im, A.im + B.re};
+      }
+      Dft<T, N1, 1>::run(v);
+#pragma unroll
+      for (int k1 = 0; k1 < N1; ++k1)
+        xl[IP ? xslot_ip<N2>(k1, t) : k1 * (N2 + 1) + t] = cmulw(v[k1], tw[t * k1], true);
+    }
+    __syncthreads();
+    cplx<T> w[N2];
+    if (act && t < N1) {
+#pragma unroll
+      for (int j2 = 0; j2 < N2; ++j2) w[j2] = xl[IP ? xslot_ip<N2>(t, j2) : t * (N2 + 1) + j2];
+      Dft<T, N2, 1>::run(w);
+    }
+    __syncthreads();  // the exchange is read completely before rp (same memory) is written
+    if (act && t < N1) {
+#pragma unroll
+      for (int k2 = 0; k2 < N2; ++k2) {
+        const int z = t + N1 * k2;
+        rp[(2 * p) * RS + z] = w[k2].re;
+        rp[(2 * p + 1) * RS + z] = w[k2].im;
+      }
+    }
+    __syncthreads();
+  }
+  } else {
   {
     const int p = l;
     const bool act = p < NP;
@@ -816,6 +1031,7 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
     }
     __syncthreads();
   }
+  }
   // store z (and r . z) with 16-byte vectors; DIR: p_new = z + beta p and x += alpha p instead
   double acc = 0.0;
   {
@@ -844,8 +1060,8 @@ __global__ void __launch_bounds__((FastCfg<T, N>::BT)) prec_c_staged(const PrecA
           zv.x += beta * pv.x; zv.y += beta * pv.y;
         }
         if constexpr (DIR == 2) {
-          const V qv = qpre[j];
-          V xv = xpre[j];
+          const V qv = XQPRE ? qpre[XQPRE ? j : 0] : reinterpret_cast<const V*>(a.z + fo)[i];
+          V xv = XQPRE ? xpre[XQPRE ? j : 0] : Xv[i];
           if constexpr (VN == 4) {
             xv.x += alpha_p * qv.x; xv.y += alpha_p * qv.y; xv.z += alpha_p * qv.z; xv.w += alpha_p * qv.w;
             xv.x += alpha * pv.x; xv.y += alpha * pv.y; xv.z += alpha * pv.z; xv.w += alpha * pv.w;
@@ -1037,28 +1253,36 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
   }
   if constexpr (StageCfg<T, N>::fits) {
     if (vec) {
-      if (update == 4 && ppa == 2 && sizeof(T) == 4) {
-        big((const void*)prec_a_staged<T, N, 4, 2>, lp);
-        hipLaunchKernelGGL((prec_a_staged<T, N, 4, 2>), dim3((nplanes + 1) / 2, a.m, 1), dim3(C::BT), lp, st, a, tw);
-      } else if (update == 4) { NLFH_A(prec_a_staged, 4) }
-      else if (update == 1) { NLFH_A(prec_a_staged, 1) }
-      else if (update) { NLFH_A(prec_a_staged, 2) }
-      else { NLFH_A(prec_a_staged, 0) }
+      using SC = StageCfg<T, N>;
+      const size_t ls = SC::lds_bytes;
+      const dim3 bs(SC::BT);
+#define NLFH_AS(U)                                                                      \
+  big((const void*)prec_a_staged<T, N, U>, ls);                                       \
+  hipLaunchKernelGGL((prec_a_staged<T, N, U>), gp, bs, ls, st, a, tw);
+#define NLFH_CS(D, DIRV)                                                                \
+  big((const void*)prec_c_staged<T, N, D, DIRV>, ls);                                 \
+  hipLaunchKernelGGL((prec_c_staged<T, N, D, DIRV>), gp, bs, ls, st, a, tw);
+      bool two = false;
+      if constexpr (sizeof(T) == 4 && !SC::IP) {
+        if (update == 4 && ppa == 2) {
+          two = true;
+          big((const void*)prec_a_staged<T, N, 4, 2>, ls);
+          hipLaunchKernelGGL((prec_a_staged<T, N, 4, 2>), dim3((nplanes + 1) / 2, a.m, 1), bs, ls, st, a, tw);
+        }
+      }
+      if (two) {}
+      else if (update == 4) { NLFH_AS(4) }
+      else if (update == 1) { NLFH_AS(1) }
+      else if (update) { NLFH_AS(2) }
+      else { NLFH_AS(0) }
       if (dot >= 2) { NLFH_B(true) }
       else { NLFH_B(false) }
-      if (dot == 1) {
-        big((const void*)prec_c_staged<T, N, true>, lp);
-        hipLaunchKernelGGL((prec_c_staged<T, N, true>), gp, dim3(C::BT), lp, st, a, tw);
-      } else if (dot == 3) {
-        big((const void*)prec_c_staged<T, N, false, 1>, lp);
-        hipLaunchKernelGGL((prec_c_staged<T, N, false, 1>), gp, dim3(C::BT), lp, st, a, tw);
-      } else if (dot == 4) {
-        big((const void*)prec_c_staged<T, N, false, 2>, lp);
-        hipLaunchKernelGGL((prec_c_staged<T, N, false, 2>), gp, dim3(C::BT), lp, st, a, tw);
-      } else {
-        big((const void*)prec_c_staged<T, N, false>, lp);
-        hipLaunchKernelGGL((prec_c_staged<T, N, false>), gp, dim3(C::BT), lp, st, a, tw);
-      }
+      if (dot == 1) { NLFH_CS(true, 0) }
+      else if (dot == 3) { NLFH_CS(false, 1) }
+      else if (dot == 4) { NLFH_CS(false, 2) }
+      else { NLFH_CS(false, 0) }
+#undef NLFH_AS
+#undef NLFH_CS
       return hipGetLastError();
     }
   }
@@ -1101,6 +1325,17 @@ int fast_line_len(int n, int elem_bytes) {
     case 48: return elem_bytes == 8 ? FastCfg<double, 48>::LPB : FastCfg<float, 48>::LPB;
     case 128: return elem_bytes == 8 ? FastCfg<double, 128>::LPB : FastCfg<float, 128>::LPB;
     default: return elem_bytes == 8 ? FastCfg<double, 64>::LPB : FastCfg<float, 64>::LPB;
+  }
+}
+
+bool fast_two_planes_ok(int n) {
+  switch (n) {
+    case 16: return StageCfg<float, 16>::fits && !StageCfg<float, 16>::IP;
+    case 32: return StageCfg<float, 32>::fits && !StageCfg<float, 32>::IP;
+    case 48: return StageCfg<float, 48>::fits && !StageCfg<float, 48>::IP;
+    case 64: return StageCfg<float, 64>::fits && !StageCfg<float, 64>::IP;
+    case 128: return StageCfg<float, 128>::fits && !StageCfg<float, 128>::IP;
+    default: return false;
   }
 }
 

@@ -138,11 +138,12 @@ template <typename T>
 hipError_t launch_prec(const PrecArgs<T>& a, int nplanes, int nlines, int update, int dot, size_t lds_plane,
                        size_t lds_line, hipStream_t st);
 bool fast_fft_size(int n);
-bool fast_staged(int n, int elem_bytes);  // the staged passes A and C (one round of lines) fit this length
+bool fast_staged(int n, int elem_bytes);  // the staged passes A and C (the plane in LDS) exist for this length
 int fast_line_len(int n, int elem_bytes);
 // FP32 pass B of the fast path: lpb (ky, kz) positions per block is the default (0 or its value), or half or
 // double of it
 bool fast_pass_b_ok(int n, int lpb);
+bool fast_two_planes_ok(int n);  // the FP32 staged pass A offers two x-planes per block at this length
 // ppa: x-planes per block of the FP32 spectral pass A (1 or 2)
 template <typename T>
 hipError_t launch_prec_fast(int n, const PrecArgs<T>& a, int nplanes, int nlines, const cplx<T>* tw, const T* symp,

@@ -139,7 +139,7 @@ def test_fp32_direction_iteration_matches_plain_fp32_schedule(gpu_pkg, cells):
     direction formed in pass C, residual kept as its (z, y) spectrum, iterate stepped every second
     iteration with the owed step of an even last iteration flushed) runs the same PCG recurrences as the
     plain FP32 schedule, which a plan takes as soon as one cell (here 12^3) has no register FFT (128^3 cells
-    run the staged passes with 1024-thread blocks). Per cell,
+    run the staged passes in place with 512-thread blocks). Per cell,
     the mixed solve needs at most 3 % more inner iterations than in the plain schedule (an inner solve that
     lost a step would need extra refinement steps; measured on MI355X: equal or up to 4 % fewer) and C
     agrees to 1e-12 of max|C|."""

@@ -166,7 +166,8 @@ def test_host_parameters_are_captured_at_call_time(gpu_pkg):
 @pytest.mark.parametrize("n", [16, 32, 48, 64, 128])
 def test_precond_fp32_pass_b_block_shapes(gpu_pkg, n):
     """Every FP32 pass B block shape the autotuner may pick (the default, half and double of its (ky, kz)
-    positions per block) gives the same preconditioner, against the oracle; unsupported values are refused."""
+    positions per block) gives the same preconditioner, against the oracle; unsupported values are refused (two
+    pass A planes per block too at 128, whose staged passes work in place with one plane per block)."""
     import ctypes
     shape = (n, n, n)
     rng = np.random.default_rng(60 + n)
@@ -196,5 +197,10 @@ def test_precond_fp32_pass_b_block_shapes(gpu_pkg, n):
         plan.set_fft_tiling(0, 3)
     with pytest.raises(gpu_pkg.NlfhError):
         plan.set_fft_tiling(0, default, 3)
-    plan.set_fft_tiling(0, 0, 2)
-    assert plan.get_fft_tiling(0) == (default, default, 2)
+    if n < 128:
+        plan.set_fft_tiling(0, 0, 2)
+        assert plan.get_fft_tiling(0) == (default, default, 2)
+    else:  # the in-place 128-point pass A holds one plane per block
+        with pytest.raises(gpu_pkg.NlfhError):
+            plan.set_fft_tiling(0, 0, 2)
+        assert plan.get_fft_tiling(0) == (2 * default, default, 1)
