@@ -193,6 +193,7 @@ struct nlfh_plan {
   double* d_part_bdot = nullptr;  // pass B spectral r . z partials (float32 lines)
   bool lean32 = false;            // every shape group on the register-FFT path: lean FP32 iteration
   bool dir32 = false;             // ... and on the staged passes: pass C forms p and steps x
+  bool alpha64 = false;           // every shape group on the staged FP64 passes: pass A forms alpha
   int* d_chunk_range = nullptr;
 
   // workspaces
@@ -727,8 +728,19 @@ int cg_iteration(nlfh_plan* p, Work<T>& w, int m, int cur, hipStream_t st) {
     TRY(run_matvec<T>(p, p_old, nullptr, nullptr, w.Ap, w.s, nullptr, p->d_part_mv, w.cs.done, m, 0, 0, 1, T(1), st));
   else
     TRY(run_matvec<T>(p, w.Z, p_old, p_new, w.Ap, w.s, w.cs.beta, p->d_part_mv, w.cs.done, m, 0, 1, 1, T(1), st));
-  if (!dir) HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
+  // FP64 with every group on the staged passes: pass A forms alpha from the matvec's partials (no alpha kernel)
+  const bool own_alpha = !dir && sizeof(T) == 8 && p->alpha64;
+  if (!dir && !own_alpha)
+    HIP_TRY(launch_cg_alpha(w.cs, p->ncells, p->d_part_mv, (int)p->mv.size(), p->d_mv_range, st));
   PrecArgs<T> a = prec_args<T>(p, m);
+  if (own_alpha) {
+    a.part_mv = p->d_part_mv;
+    a.mv_range = p->d_mv_range;
+    a.nmv = (int)p->mv.size();
+    a.rz = w.cs.rz;
+    a.alpha_out = w.cs.alpha;
+    a.alpha_prev_out = w.cs.alpha_prev;
+  }
   a.r = w.R;
   a.r_out = w.R;
   a.Ap = w.Ap;
@@ -1148,9 +1160,11 @@ int nlfh_plan_create(const int* shapes, int ncells, double E, double nu, int m_l
   if ((rc = p->upload(&p->d_plane_range, prange))) return fail(rc);
   p->lean32 = true;
   p->dir32 = true;
+  p->alpha64 = true;
   for (const PrecGroup& g : p->pgroups) {
     p->lean32 = p->lean32 && g.fast != 0;
     p->dir32 = p->dir32 && g.fast != 0 && fast_staged(g.fast, 4) && g.vec;
+    p->alpha64 = p->alpha64 && g.fast != 0 && fast_staged(g.fast, 8) && g.vec;
   }
   if ((rc = p->upload(&p->d_chunk_range, crange))) return fail(rc);
   if ((rc = build_mv_items(p))) return fail(rc);

@@ -450,8 +450,30 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
   };
   issue(0, pi0);
   T alpha = T(0);
-  if (UPDATE) alpha = (T)a.alpha[pi0.cell * a.m + rhs];
   __shared__ double alpha_sh;
+  if constexpr (UPDATE) {
+    // alpha from the matvec's p . Ap partials (a.part_mv set: no alpha kernel), with the fixed-order sum and guard
+    // of cg_alpha_kernel, so bitwise its value; the block of plane 0, component 0 stores it (and the one it
+    // replaces); else the value the alpha kernel stored
+    if (a.part_mv != nullptr) {
+      if (tid < 64) {
+        const int k = pi0.cell * a.m + rhs;
+        const double pAp = wave_sum(a.part_mv, a.nmv, rhs, a.mv_range, pi0.cell);
+        const double al = a.rz[k] / (pAp == 0.0 ? 1.0 : pAp);
+        if (tid == 0) {
+          alpha_sh = al;
+          if (pi0.x == 0 && pi0.comp == 0) {
+            if (a.alpha_prev_out != nullptr) a.alpha_prev_out[k] = a.alpha_out[k];
+            a.alpha_out[k] = al;
+          }
+        }
+      }
+      __syncthreads();
+      alpha = (T)alpha_sh;
+    } else {
+      alpha = (T)a.alpha[pi0.cell * a.m + rhs];
+    }
+  }
   if (SPEC && tid < 64) {
     const int k = pi0.cell * a.m + rhs;
     const double pAp = wave_sum(a.part_mv, a.nmv, rhs, a.mv_range, pi0.cell);
@@ -1287,7 +1309,9 @@ static hipError_t launch_fast_n(const PrecArgs<T>& a, int nplanes, int nlines, c
     }
   }
   // r-only and spectral updates, the direction pass and a kept residual spectrum: staged passes only
-  if (update == 1 || update == 4 || dot >= 3 || a.rspec != nullptr) return hipErrorInvalidValue;
+  // (and alpha formed from the matvec's partials: a.part_mv with an update)
+  if (update == 1 || update == 4 || dot >= 3 || a.rspec != nullptr || (update && a.part_mv != nullptr))
+    return hipErrorInvalidValue;
   if (update) { NLFH_A(prec_a_fast, true) }
   else { NLFH_A(prec_a_fast, false) }
   if (dot == 2) { NLFH_B(true) }
