@@ -339,22 +339,24 @@ struct Vec16<double> {
   __device__ static double get(const double2& v, int k) { return k == 0 ? v.x : v.y; }
 };
 
-// IP (FP32 at 128): 512-thread blocks over two rounds of lines, each line's exchange in its own slots of the plane
-// (a row pair or a column, as the FP64 128-point kernels do) and the real plane staged in the spectrum's own slots
-// (row y in row y's 2 NH values), so that a block needs the plane alone, 69 KiB, and two blocks share a CU; with
-// the separate exchange buffer (139 KiB) one 1024-thread block holds the CU, and its load and transform phases
-// cannot overlap another block's.
+// IP, the in-place form: each line's exchange in its own slots of the plane (a row pair or a column, as the FP64
+// 128-point kernels do) and the real plane staged in the spectrum's own slots (row y in row y's 2 NH values), so a
+// block needs the plane alone. FP32 at 128 (with B2: 512-thread blocks over two rounds of lines): 69 KiB, two blocks
+// per CU, where the separate exchange buffer (139 KiB) let one 1024-thread block hold the CU and its load and
+// transform phases could not overlap another block's. FP64 at 64: 36 KiB, four blocks per CU instead of two.
 template <typename T, int N>
 struct StageCfg {
-  static constexpr bool IP = sizeof(T) == 4 && N == 128;
-  static constexpr int BT = IP ? 512 : FastCfg<T, N>::BT;
+  static constexpr bool IP = (sizeof(T) == 4 && N == 128) || (sizeof(T) == 8 && N == 64);
+  // B2: two rounds of lines in 512-thread blocks, operands read where they are consumed (FP32 at 128)
+  static constexpr bool B2 = sizeof(T) == 4 && N == 128;
+  static constexpr int BT = B2 ? 512 : FastCfg<T, N>::BT;
   static constexpr int LPR = BT / FastCfg<T, N>::L;
   static constexpr int RS = IP ? 2 * FastCfg<T, N>::NH : N + (sizeof(T) == 4 ? 4 : 2);
   static constexpr bool fits = IP || (!FastCfg<T, N>::XIN && FastCfg<T, N>::LPR >= N / 2 &&
                                (size_t)N * RS * sizeof(T) <= (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS * sizeof(cplx<T>));
   // waves per SIMD the compiler may plan for: the in-place form's LDS admits two 8-wave blocks per CU (4 per
   // SIMD), so it gets their 128 registers; the others keep the default range
-  static constexpr int WMIN = IP ? 4 : 1, WMAX = IP ? 4 : 8;
+  static constexpr int WMIN = B2 ? 4 : 1, WMAX = B2 ? 4 : 8;
   static constexpr size_t lds_bytes =
       IP ? ((size_t)N + (size_t)N * FastCfg<T, N>::NH + N) * sizeof(cplx<T>)
          : ((size_t)N + (size_t)N * FastCfg<T, N>::NH + (size_t)FastCfg<T, N>::LPR * FastCfg<T, N>::XS + N) * sizeof(cplx<T>);
@@ -385,7 +387,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
   using C = FastCfg<T, N>;
   using SC = StageCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = SC::BT, LPR = SC::LPR;
-  constexpr bool IP = SC::IP;
+  constexpr bool IP = SC::IP, B2 = SC::B2;
   static_assert(!IP || PPB == 1, "one plane per block in the in-place form");
   constexpr int RS = StageCfg<T, N>::RS;
   static_assert(StageCfg<T, N>::fits, "staged pass A needs one round of lines and room for the plane");
@@ -422,7 +424,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
   constexpr int NVS = N * NH / CPV;
   constexpr int RPT = SPEC ? (NVS + BT - 1) / BT : 1;
   V lbuf[PPB][LPT];
-  V rpre[PPB][IP ? 1 : RPT];
+  V rpre[PPB][B2 ? 1 : RPT];
   auto field_off = [&](const PlaneItem& q) {
     return cd.off_field + ((int64_t)rhs * 3 + q.comp) * cd.nvox + (int64_t)q.x * N * N;
   };
@@ -439,7 +441,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
         if (i < NV) lbuf[q][j] = Rv[i];
       }
     }
-    if constexpr (SPEC && !IP) {
+    if constexpr (SPEC && !B2) {
       const V* rs = spec_ptr(pq);
 #pragma unroll
       for (int j = 0; j < RPT; ++j) {
@@ -713,7 +715,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
     if constexpr (SPEC) {
       alpha = (T)alpha_sh;  // written before the barriers above
       double sacc = 0.0;
-      if constexpr (!IP) {
+      if constexpr (!B2) {
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
           const int i = tid + j * BT;
@@ -783,7 +785,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
   using C = FastCfg<T, N>;
   using SC = StageCfg<T, N>;
   constexpr int N1 = C::N1, N2 = C::N2, NH = C::NH, NP = C::NP, L = C::L, BT = SC::BT, LPR = SC::LPR;
-  constexpr bool IP = SC::IP;
+  constexpr bool IP = SC::IP, B2 = SC::B2;
   constexpr int RS = StageCfg<T, N>::RS;
   static_assert(StageCfg<T, N>::fits, "staged pass C needs one round of lines and room for the plane");
   using VT = Vec16<T>;
@@ -832,7 +834,7 @@ __global__ void __launch_bounds__((StageCfg<T, N>::BT)) __attribute__((amdgpu_wa
   constexpr int NV = N * N / VN;
   constexpr int PPT = DIR ? (NV + BT - 1) / BT : 1;
   // DIR 2 in the in-place form (twice the values per thread) reads x and p_{k-1} at the store instead
-  constexpr bool XQPRE = DIR == 2 && !IP;
+  constexpr bool XQPRE = DIR == 2 && !B2;
   constexpr int XPT = XQPRE ? PPT : 1;
   V ppre[PPT], xpre[XPT], qpre[XPT];
   if constexpr (DIR != 0) {
