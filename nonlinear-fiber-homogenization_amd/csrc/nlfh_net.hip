@@ -197,19 +197,34 @@ __global__ void __launch_bounds__(256) conv3d_lds_kernel(const ConvArgs p) {
     }
     __syncthreads();
     const uint16_t* wc = p.w + c0 + kg * KL;
-    for (int t = 0; t < 27; ++t) {
-      int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-      if (p.flip) { dx = -dx; dy = -dy; dz = -dz; }
-      const int toff = ((dx * Bk::HY + dy) * Bk::HZ + dz) * KC;
-      Frag bw[NB];
+    // The weight fragments rotate through three slots, three taps ahead: tap t's products read slot t mod 3, which
+    // then takes tap t + 3's fragments, so a load has two taps' products to land; the loop runs the three taps of
+    // one z-line per pass (27 = 9 x 3), so the slot is a compile-time index.
+    Frag bw[3][NB];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
 #pragma unroll
       for (int j = 0; j < NB; ++j)
-        bw[j] = *reinterpret_cast<const Frag*>(wc + ((int64_t)t * p.coutp + j * 16 + row) * cin);
+        bw[k][j] = *reinterpret_cast<const Frag*>(wc + ((int64_t)k * p.coutp + j * 16 + row) * cin);
+#pragma unroll 1
+    for (int t0 = 0; t0 < 27; t0 += 3) {
 #pragma unroll
-      for (int mt = 0; mt < 4; ++mt) {
-        const Frag av = *reinterpret_cast<const Frag*>(lds + abase[mt] + toff);
+      for (int k = 0; k < 3; ++k) {
+        const int t = t0 + k;
+        int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = k - 1;
+        if (p.flip) { dx = -dx; dy = -dy; dz = -dz; }
+        const int toff = ((dx * Bk::HY + dy) * Bk::HZ + dz) * KC;
 #pragma unroll
-        for (int j = 0; j < NB; ++j) acc[mt][j] = KFrag<KC>::mfma(av, bw[j], acc[mt][j]);
+        for (int mt = 0; mt < 4; ++mt) {
+          const Frag av = *reinterpret_cast<const Frag*>(lds + abase[mt] + toff);
+#pragma unroll
+          for (int j = 0; j < NB; ++j) acc[mt][j] = KFrag<KC>::mfma(av, bw[k][j], acc[mt][j]);
+        }
+        if (t + 3 < 27) {
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            bw[k][j] = *reinterpret_cast<const Frag*>(wc + ((int64_t)(t + 3) * p.coutp + j * 16 + row) * cin);
+        }
       }
     }
     __syncthreads();
