@@ -25,7 +25,7 @@ from . import _lib
 
 TY_CHOICES = (1, 2, 4, 6, 8, 14, 30, 62)
 XC_CHOICES = (2, 4, 8, 16, 32, 64, 128)
-ZPT_CHOICES = (1, 2, 4)
+ZPT_CHOICES = (1, 2, 3, 4)
 
 
 def candidates(shape, elem_bytes=8):

@@ -64,8 +64,9 @@ __device__ __forceinline__ void modal_op(const T (&h)[3][8], T (&g)[3][8], const
   g[0][0] = T(0); g[1][0] = T(0); g[2][0] = T(0);
 }
 
-// z neighbours: a DPP wave rotation when a row is a whole wavefront (lane i <- i+1 with
-// wave_rol:1, i <- i-1 with wave_ror:1, checked on gfx950), ds_bpermute otherwise.
+// z neighbours by the kernel's DPP mode: 1, a row is a whole wavefront: a wave rotation (lane i <- i+1 with
+// wave_rol:1, i <- i-1 with wave_ror:1, checked on gfx950); 2, a row is one 16-lane DPP row (nz / zpt = 16): a row
+// rotation (lane i <- i+1 of its row with row_ror:15, i <- i-1 with row_ror:1); 0, ds_bpermute.
 // A wave rotation reads a valid source lane for every lane, so the instruction's `old` operand is
 // never used: mov_dpp leaves it undefined, which spares a register copy per rotation and lets the
 // compiler fold the rotation into the adds and subtracts that consume it.
@@ -94,6 +95,8 @@ __device__ __forceinline__ void settle(const T (&v)[K]) {
 constexpr int kWaitVmcnt0 = 0x0F70;
 constexpr int kDppWaveRol1 = 0x134;  // lane i receives lane i + 1 (mod 64)
 constexpr int kDppWaveRor1 = 0x13C;  // lane i receives lane i - 1 (mod 64)
+constexpr int kDppRowRor15 = 0x12F;  // lane i receives lane i + 1 of its 16-lane row (mod 16)
+constexpr int kDppRowRor1 = 0x121;   // lane i receives lane i - 1 of its 16-lane row (mod 16)
 
 // z butterflies of the last z value of a lane with the first of the next lane, one component, FP32, full
 // wavefront rows: h[mm] = next(a[mm]) + b[mm] (mm = 1..3; the sum of mode 0 is never used by the element
@@ -114,14 +117,16 @@ __device__ __forceinline__ void z_next_butterflies(const float (&a)[4], const fl
   h[0] = 0.0f;  // unused
 }
 
-template <typename T, bool DPP>
+template <typename T, int DPP>
 __device__ __forceinline__ T from_next(T v, int src) {
-  if constexpr (DPP) return dpp_rot<kDppWaveRol1>(v);
+  if constexpr (DPP == 1) return dpp_rot<kDppWaveRol1>(v);
+  else if constexpr (DPP == 2) return dpp_rot<kDppRowRor15>(v);
   else return __shfl(v, src, 64);
 }
-template <typename T, bool DPP>
+template <typename T, int DPP>
 __device__ __forceinline__ T from_prev(T v, int src) {
-  if constexpr (DPP) return dpp_rot<kDppWaveRor1>(v);
+  if constexpr (DPP == 1) return dpp_rot<kDppWaveRor1>(v);
+  else if constexpr (DPP == 2) return dpp_rot<kDppRowRor1>(v);
   else return __shfl(v, src, 64);
 }
 
@@ -149,13 +154,15 @@ __device__ __forceinline__ T& at(T* base, unsigned i) {
 // a 6-row tile share a CU (8 wavefronts per SIMD instead of 6; the standalone FP32 operator on 8 x 64^3
 // cells 106 -> 95 us).
 // Addressing: every global access is a wave-uniform base (cell, right-hand side, component, x-plane)
-// plus a 32-bit per-lane offset inside the plane; with a row per wavefront (DPP) the row index is
-// wave-uniform too.
+// plus a 32-bit per-lane offset inside the plane; with a row per wavefront (DPP mode 1) the row index is
+// wave-uniform too. In DPP modes 1 and 2 a row is a whole wavefront or a whole 16-lane DPP row, and the conditions
+// that mask the element work (active, erow) depend on the row alone, so a rotation only reads lanes of its own
+// active row.
 // S64: the stiffness is read in FP64 (a.s64) and rounded to T as it is loaded (the standalone FP32
 // operator, whose caller holds s in FP64; the solver converts s once per solve and reads a.s).
 // CHEB (STRAIN only): the owned node values become one step of the Chebyshev-Jacobi iteration for K u = -f,
 // u + c1 (u - x_prev) - c2 dinv (K u + f); x_prev and dinv of the plane stored at step s are loaded at step s - 1.
-template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, bool DPP, bool S64 = false, bool CHEB = false>
+template <typename T, int ZPT, bool STRAIN, bool PUPDATE, bool DOT, int DPP, bool S64 = false, bool CHEB = false>
 __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   static_assert(!CHEB || (STRAIN && !PUPDATE && !DOT && !S64), "the Chebyshev step is a strain-term launch");
   extern __shared__ double smem_d[];
@@ -167,13 +174,13 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
   const CellDesc cd = a.cells[it.cell];
   const int rhs = blockIdx.y;
   const int nx = cd.nx, ny = cd.ny, nz = cd.nz;
-  const int LZ = DPP ? 64 : a.LZ;
-  const int LZS = DPP ? 64 : a.LZS;
+  const int LZ = DPP == 1 ? 64 : (DPP == 2 ? 16 : a.LZ);
+  const int LZS = DPP == 1 ? 64 : (DPP == 2 ? 16 : a.LZS);
   const int ty = it.ty;
   const int RB = ty + 2;
   const int tid = threadIdx.x;
-  const int lz = DPP ? (tid & 63) : tid % LZS;
-  const int r = DPP ? __builtin_amdgcn_readfirstlane(tid >> 6) : tid / LZS;
+  const int lz = DPP == 1 ? (tid & 63) : (DPP == 2 ? (tid & 15) : tid % LZS);
+  const int r = DPP == 1 ? __builtin_amdgcn_readfirstlane(tid >> 6) : (DPP == 2 ? (tid >> 4) : tid / LZS);
   const bool active = r < RB && lz < LZ;
   const bool erow = r <= ty;  // computes element row yg
   const bool owns = active && r >= 1 && r <= ty;
@@ -417,7 +424,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
           Y[c][3][q] = d1 - d0;
         }
       // FP32 full-wavefront rows fold the rotation into the butterflies (z_next_butterflies)
-      constexpr bool kFoldNext = DPP && std::is_same<T, float>::value;
+      constexpr bool kFoldNext = DPP == 1 && std::is_same<T, float>::value;
       T Yn[3][4];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -515,7 +522,7 @@ __global__ void __launch_bounds__(512) matvec_kernel(const MatvecArgs<T> a) {
 
 // Host-side launcher -------------------------------------------------------------------------
 
-template <typename T, int ZPT, bool DPP>
+template <typename T, int ZPT, int DPP>
 static hipError_t launch_mv_zpt(const MatvecArgs<T>& a, int strain, int pupdate, int dot, int cheb, dim3 grid,
                                 int threads, size_t lds, hipStream_t st) {
   if (cheb) {
@@ -564,15 +571,20 @@ hipError_t launch_matvec(const MatvecArgs<T>& a, int nblk, int zpt, int max_ty, 
   const int threads = ((a.LZS * (max_ty + 2) + 63) / 64) * 64;
   const size_t lds = (size_t)3 * (max_ty + 2) * 6 * zpt * a.LZ * sizeof(T);
   dim3 grid(nblk, a.m, 1);
-  const bool dpp = a.LZ == 64;  // a row is a whole wavefront
-  switch (zpt * 2 + (dpp ? 1 : 0)) {
-    case 2: return launch_mv_zpt<T, 1, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 3: return launch_mv_zpt<T, 1, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 4: return launch_mv_zpt<T, 2, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 5: return launch_mv_zpt<T, 2, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 6: return launch_mv_zpt<T, 3, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 8: return launch_mv_zpt<T, 4, false>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
-    case 9: return launch_mv_zpt<T, 4, true>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+  // DPP mode: 1 when a row is a whole wavefront, 2 when it is one 16-lane DPP row, else 0 (ds_bpermute)
+  const int dpp = a.LZ == 64 ? 1 : (a.LZ == 16 && a.LZS == 16 ? 2 : 0);
+  switch (zpt * 4 + dpp) {
+    case 4: return launch_mv_zpt<T, 1, 0>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 5: return launch_mv_zpt<T, 1, 1>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 6: return launch_mv_zpt<T, 1, 2>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 8: return launch_mv_zpt<T, 2, 0>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 9: return launch_mv_zpt<T, 2, 1>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 10: return launch_mv_zpt<T, 2, 2>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 12: return launch_mv_zpt<T, 3, 0>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 14: return launch_mv_zpt<T, 3, 2>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 16: return launch_mv_zpt<T, 4, 0>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 17: return launch_mv_zpt<T, 4, 1>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
+    case 18: return launch_mv_zpt<T, 4, 2>(a, strain, pupdate, dot, cheb, grid, threads, lds, st);
     default: return hipErrorInvalidValue;
   }
 }

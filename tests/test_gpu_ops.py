@@ -55,6 +55,31 @@ def test_apply_K_fp32(gpu_pkg, shape):
     assert _rel(got, ref) < 2e-6
 
 
+@pytest.mark.parametrize("shape,tiling", [((16, 12, 16), (4, 2, 1)), ((10, 10, 32), (4, 4, 2)),
+                                          ((8, 10, 48), (3, 2, 3)), ((6, 9, 64), (6, 4, 4))])
+def test_apply_K_sixteen_lane_rows(gpu_pkg, shape, tiling):
+    """Tilings whose z rows are 16 lanes (nz / zpt = 16) take their z neighbours by rotations inside a 16-lane DPP
+    row: FP64 and FP32 against the oracle, with ragged y tiles and x chunks."""
+    import ctypes
+    lh = gpu_pkg.LinearHomogenization(shape)
+    lh._plan.set_tiling(0, *tiling)
+    assert lh._plan.get_tiling(0) == tiling
+    s, u = _rand(shape, 3)
+    ref = np.stack([O.LinearRef(shape).apply_K(s, x) for x in u])
+    assert _rel(lh.apply_K(s, u).cpu().numpy(), ref) < 1e-12
+    s, u = _rand(shape, 4, m=3)
+    up = gpu_pkg.to_planar(u, shape).float().contiguous()
+    out = torch.empty_like(up)
+    st = torch.tensor(s, device="cuda")
+    lib = gpu_pkg.load_library()
+    rc = lib.nlfh_apply_K(lh._plan.handle, ctypes.c_void_p(st.data_ptr()), ctypes.c_void_p(up.data_ptr()),
+                          ctypes.c_void_p(out.data_ptr()), 3, 1, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    got = gpu_pkg.from_planar(out).double().cpu().numpy()
+    ref = np.stack([O.LinearRef(shape).apply_K(s, x) for x in u])
+    assert _rel(got, ref) < 2e-6
+
+
 @pytest.mark.parametrize("shape", [(4, 4, 4), (3, 4, 5), (8, 8, 1), (12, 10, 6)])
 def test_load_and_residual(gpu_pkg, shape):
     s, u = _rand(shape, 3, m=6)
