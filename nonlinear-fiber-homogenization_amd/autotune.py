@@ -28,11 +28,14 @@ XC_CHOICES = (2, 4, 8, 16, 32, 64, 128)
 ZPT_CHOICES = (1, 2, 3, 4)
 
 
-def candidates(shape, elem_bytes=8):
+def candidates(shape, elem_bytes=8, precision="f32"):
+    """Tilings (ty, xc, zpt) for `shape` that fit a 512-thread block and the LDS of elem_bytes-wide kernels. FP64 plans
+    get no three-z-value tilings: the FP64 PCG's direction-update kernels need more than 256 registers there and spill
+    (the standalone operator the sweep times does not, so the sweep alone would pick them)."""
     nx, ny, nz = shape
     out = []
     for zpt in ZPT_CHOICES:
-        if nz % zpt or nz // zpt > 64:
+        if nz % zpt or nz // zpt > 64 or (precision == "f64" and zpt == 3):
             continue
         lz = nz // zpt
         lzs = 1
@@ -159,7 +162,7 @@ def tune(plan, shape_index=0, m=6, precision="f64", reps=10, use_cache=True, ver
     u = torch.randn(nb * m * 3 * nvox, dtype=tdt, device=dev)
     out = torch.empty_like(u)
     table = []
-    for cand in candidates(shape, 8):  # the plan's tiling must fit the float64 kernels too
+    for cand in candidates(shape, 8, precision):  # the plan's tiling must fit the float64 kernels too
         tplan.set_tiling(ti, *cand)
         t = time_matvec(tplan, s, u, out, m, dtype, reps)
         table.append((cand, t))

@@ -19,7 +19,8 @@ def _entries():
 
 def test_packaged_tilings_are_valid_candidates(pkg):
     """Every tiling the package ships is one the autotuner could have picked for that shape (tile rows,
-    x-chunk and z values per lane that fit the 512-thread block and the LDS of the FP64 kernels; for the FP32
+    x-chunk and z values per lane that fit the 512-thread block and the LDS of the FP64 kernels, no three z values per
+    lane for FP64 entries; for the FP32
     pass B entries, the three positions per block of autotune.fft_candidates around the default)."""
     from nlfh_amd import autotune as at
     n = nb = 0
@@ -35,9 +36,9 @@ def test_packaged_tilings_are_valid_candidates(pkg):
             nb += 1
             continue
         assert kind == "", key
-        assert tuple(e["best"]) in at.candidates(shape, 8), key
+        assert tuple(e["best"]) in at.candidates(shape, 8, prec), key
         for ru in e.get("runner_up", []):
-            assert tuple(ru) in at.candidates(shape, 8), key
+            assert tuple(ru) in at.candidates(shape, 8, prec), key
         n += 1
     assert n >= 10 and nb >= 9
 
