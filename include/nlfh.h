@@ -197,6 +197,11 @@ int nlfh_net_slice_add(const float* src, int cs, int c0, float* dst, long long n
 /* mean squared error: grad = 2 (pred - target) / len, per-block loss partials in loss_part[nparts] */
 int nlfh_net_mse(const float* pred, const float* target, float* grad, long long len, double* loss_part, int nparts,
                  void* stream);
+/* Learned-start read-out (no counterpart in the reference, whose network is not part of the path): dst[b][c][v] =
+   scale * out[b][v][c] - mean over v, float64, c < nch of the ldc-wide float32 head, channel-major per cell; part is
+   scratch of at least batch * 513 * nch doubles */
+int nlfh_net_readout(const float* out, int ldc, int nch, long long nvox, int batch, double scale, double* part,
+                     long long part_len, double* dst, void* stream);
 /* Adam step (bias-corrected, step >= 1, decoupled weight decay wd) on FP32 weights; refreshes the bf16 copy wh */
 int nlfh_net_adam(float* w, const float* g, float* m1, float* m2, unsigned short* wh, long long len, float lr,
                   float b1, float b2, float eps, int step, float wd, void* stream);

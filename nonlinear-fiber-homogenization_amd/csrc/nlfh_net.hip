@@ -621,6 +621,69 @@ __global__ void mse_kernel(const float* pred, const float* target, float* grad, 
   if (threadIdx.x == 0) loss_part[blockIdx.x] = t * inv_count;
 }
 
+// Read-out of the network head for the learned start: dst[b][c][v] = scale * out[b][v][c] - mean_v(scale * out[b][v][c])
+// in float64 for the channels c < nch of the ldc-wide float32 head, channel-major per cell (the planar field layout).
+// Three launches with fixed summation orders: per (cell, chunk of voxels) channel sums (8 voxel groups x 32 channel
+// lanes per block, four loads in flight, groups combined in order), per-cell means from the chunk sums in chunk
+// order, and the scaled, mean-free write (a thread per voxel).
+constexpr int kReadoutChunks = 512;
+__global__ void __launch_bounds__(256) readout_part_kernel(const float* out, int ldc, int nch, int64_t nvox,
+                                                           double* part) {
+  __shared__ double red[256];
+  const int b = blockIdx.y, chunk = blockIdx.x;
+  const int g = threadIdx.x >> 5, c = threadIdx.x & 31;
+  const int64_t len = (nvox + kReadoutChunks - 1) / kReadoutChunks;
+  const int64_t v0 = chunk * len, v1 = v0 + len < nvox ? v0 + len : nvox;
+  const float* o = out + (int64_t)b * nvox * ldc + c;
+  double s = 0.0;
+  if (c < nch) {
+    int64_t v = v0 + g;
+    for (; v + 24 < v1; v += 32) {
+      float x[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[k] = o[(v + 8 * k) * ldc];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) s += (double)x[k];
+    }
+    for (; v < v1; v += 8) s += (double)o[v * ldc];
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (g == 0 && c < nch) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += red[k * 32 + c];
+    part[((int64_t)b * kReadoutChunks + chunk) * nch + c] = t;
+  }
+}
+
+__global__ void __launch_bounds__(256) readout_mean_kernel(const double* part, int nch, int64_t nvox, double scale,
+                                                           double* mean) {
+  __shared__ double red[256];
+  const int b = blockIdx.x;
+  const int g = threadIdx.x >> 5, c = threadIdx.x & 31;
+  double s = 0.0;
+  if (c < nch)
+    for (int k = g; k < kReadoutChunks; k += 8) s += part[((int64_t)b * kReadoutChunks + k) * nch + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (g == 0 && c < nch) {
+    double t = 0.0;
+    for (int k = 0; k < 8; ++k) t += red[k * 32 + c];
+    mean[(int64_t)b * nch + c] = scale * t / (double)nvox;
+  }
+}
+
+__global__ void __launch_bounds__(256) readout_write_kernel(const float* out, int ldc, int nch, int64_t nvox,
+                                                            double scale, const double* mean, double* dst) {
+  const int b = blockIdx.y;
+  const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (v >= nvox) return;
+  const float* o = out + ((int64_t)b * nvox + v) * ldc;
+  double* d = dst + (int64_t)b * nch * nvox + v;
+  const double* mb = mean + (int64_t)b * nch;
+  for (int c = 0; c < nch; ++c) d[(int64_t)c * nvox] = (double)o[c] * scale - mb[c];
+}
+
 // Adam step on FP32 master weights; refreshes the BF16 copy. Weights laid out [27][coutp][cin];
 // entries with co >= cout stay zero (padding of the output channels).
 __global__ void adam_kernel(float* w, const float* g, float* m1, float* m2, uint16_t* wh, int64_t len, float lr,
@@ -827,6 +890,24 @@ int nlfh_net_mse(const float* pred, const float* target, float* grad, long long 
                  void* stream) {
   hipLaunchKernelGGL(mse_kernel, dim3(nparts), dim3(256), 0, (hipStream_t)stream, pred, target, grad, (int64_t)len,
                      1.0 / (double)len, loss_part);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_readout(const float* out, int ldc, int nch, long long nvox, int batch, double scale, double* part,
+                     long long part_len, double* dst, void* stream) {
+  if (!out || !part || !dst || nch < 1 || nch > 32 || ldc < nch || nvox < 1 || batch < 1 || batch > 65535 ||
+      part_len < (long long)batch * (kReadoutChunks + 1) * nch) {
+    set_error("nlfh_net_readout: invalid arguments (1 <= nch <= 32, ldc >= nch, part_len >= batch * 513 * nch)");
+    return NLFH_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  double* mean = part + (int64_t)batch * kReadoutChunks * nch;
+  hipLaunchKernelGGL(readout_part_kernel, dim3(kReadoutChunks, batch), dim3(256), 0, st, out, ldc, nch,
+                     (int64_t)nvox, part);
+  hipLaunchKernelGGL(readout_mean_kernel, dim3(batch), dim3(256), 0, st, part, nch, (int64_t)nvox, scale, mean);
+  hipLaunchKernelGGL(readout_write_kernel, dim3((unsigned)((nvox + 255) / 256), batch), dim3(256), 0, st, out, ldc,
+                     nch, (int64_t)nvox, scale, mean, dst);
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -252,8 +252,11 @@ class WarmStartNet:
             r = r.unsqueeze(0)
         n = r.shape[1]
         out = self.forward(self.input(r))
-        u = out[..., :NOUT].double() * n  # (B, n, n, n, 18)
-        B = u.shape[0]
-        u = u.reshape(B, n, n, n, 6, 3).permute(0, 4, 1, 2, 3, 5)
-        u = u - u.mean(dim=(2, 3, 4), keepdim=True)
-        return u
+        B = out.shape[0]
+        # head channels 0..17 (right-hand side x 3 + component) scaled by n, mean-free per channel, float64, written
+        # in the planar layout (B, 6, 3, n, n, n) by one read-out; returned as a (B, 6, n, n, n, 3) view of it
+        planar = torch.empty((B, 6, 3, n, n, n), dtype=torch.float64, device=self.dev)
+        part = torch.empty(B * 513 * NOUT, dtype=torch.float64, device=self.dev)
+        _lib.check(self.lib.nlfh_net_readout(_p(out), COUT_HEAD, NOUT, n ** 3, B, float(n), _p(part), part.numel(),
+                                             _p(planar), _st()))
+        return planar.permute(0, 1, 3, 4, 5, 2)

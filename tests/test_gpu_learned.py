@@ -2,6 +2,8 @@
 and every weight and bias gradient (BF16 storage, FP32 accumulation on the GPU; the reference rounds
 weights and activations to bfloat16 the same way, so the forward agrees to ~1e-2 and the gradients,
 which the GPU also stores in bfloat16 between layers, to a few 1e-2 of their largest entry)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -365,3 +367,27 @@ def test_trained_network_on_heldout_cells(gpu_pkg):
     assert max(errs) < 0.4
     assert max(ratios) < 0.97
     assert min(rough_ratios) > 1.0
+
+
+@pytest.mark.parametrize("B,n", [(3, 16), (2, 4)])
+def test_readout_matches_torch(gpu_pkg, B, n):
+    """nlfh_net_readout (head channels scaled by n, mean-free per cell and channel, float64 planar layout) against the
+    same steps in torch; n = 4 leaves most of the 512 voxel chunks empty."""
+    from nlfh_amd import learned as L
+    lib = gpu_pkg.load_library()
+    out = torch.randn((B, n, n, n, L.COUT_HEAD), dtype=torch.float32, device="cuda")
+    ref = out[..., :L.NOUT].double() * n
+    ref = ref.reshape(B, n, n, n, 6, 3).permute(0, 4, 1, 2, 3, 5)
+    ref = ref - ref.mean(dim=(2, 3, 4), keepdim=True)
+    dst = torch.empty((B, 6, 3, n, n, n), dtype=torch.float64, device="cuda")
+    part = torch.empty(B * 513 * L.NOUT, dtype=torch.float64, device="cuda")
+    rc = lib.nlfh_net_readout(ctypes.c_void_p(out.data_ptr()), L.COUT_HEAD, L.NOUT, n ** 3, B, float(n),
+                              ctypes.c_void_p(part.data_ptr()), part.numel(), ctypes.c_void_p(dst.data_ptr()),
+                              ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    assert rc == 0
+    got = dst.permute(0, 1, 3, 4, 5, 2)
+    assert float((got - ref).abs().max()) <= 1e-13 * float(ref.abs().max())
+    short = torch.empty(B * 513 * L.NOUT - 1, dtype=torch.float64, device="cuda")
+    assert lib.nlfh_net_readout(ctypes.c_void_p(out.data_ptr()), L.COUT_HEAD, L.NOUT, n ** 3, B, float(n),
+                                ctypes.c_void_p(short.data_ptr()), short.numel(), ctypes.c_void_p(dst.data_ptr()),
+                                None) != 0
