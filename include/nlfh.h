@@ -197,6 +197,9 @@ int nlfh_net_slice_add(const float* src, int cs, int c0, float* dst, long long n
 /* mean squared error: grad = 2 (pred - target) / len, per-block loss partials in loss_part[nparts] */
 int nlfh_net_mse(const float* pred, const float* target, float* grad, long long len, double* loss_part, int nparts,
                  void* stream);
+/* Learned-start input (no counterpart in the reference, whose network is not part of the path): x[v][0] =
+   bf16(2 rho[v] - 1) with rho rounded to float32, channels 1..15 zero (16 bf16 channels per voxel, x 16-byte aligned) */
+int nlfh_net_input(const double* rho, long long nvox, unsigned short* x, void* stream);
 /* Learned-start read-out (no counterpart in the reference, whose network is not part of the path): dst[b][c][v] =
    scale * out[b][v][c] - mean over v, float64, c < nch of the ldc-wide float32 head, channel-major per cell; part is
    scratch of at least batch * 513 * nch doubles */

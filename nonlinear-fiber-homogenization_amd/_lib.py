@@ -81,6 +81,7 @@ SIGNATURES = [
     ("nlfh_net_slice_add", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_longlong, ctypes.c_int,
                                           ctypes.c_int, _vp]),
     ("nlfh_net_mse", ctypes.c_int, [_vp, _vp, _vp, ctypes.c_longlong, _vp, ctypes.c_int, _vp]),
+    ("nlfh_net_input", ctypes.c_int, [_vp, ctypes.c_longlong, _vp, _vp]),
     ("nlfh_net_readout", ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_longlong, ctypes.c_int, ctypes.c_double, _vp,
                          ctypes.c_longlong, _vp, _vp]),
     ("nlfh_net_adam", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, ctypes.c_longlong, ctypes.c_float, ctypes.c_float,

@@ -621,6 +621,17 @@ __global__ void mse_kernel(const float* pred, const float* target, float* grad, 
   if (threadIdx.x == 0) loss_part[blockIdx.x] = t * inv_count;
 }
 
+// Network input: x[v][0] = bf16(2 rho[v] - 1) (rho rounded to float32 first, as the float32 path does), channels
+// 1..15 zero; two 16-byte stores per voxel.
+__global__ void input_kernel(const double* rho, int64_t nvox, uint4* x) {
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < nvox; v += (int64_t)gridDim.x * blockDim.x) {
+    const float r = (float)rho[v];
+    const uint16_t h = f2bf(2.0f * r - 1.0f);
+    x[2 * v] = make_uint4((uint32_t)h, 0u, 0u, 0u);
+    x[2 * v + 1] = make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 // Read-out of the network head for the learned start: dst[b][c][v] = scale * out[b][v][c] - mean_v(scale * out[b][v][c])
 // in float64 for the channels c < nch of the ldc-wide float32 head, channel-major per cell (the planar field layout).
 // Three launches with fixed summation orders: per (cell, chunk of voxels) channel sums (8 voxel groups x 32 channel
@@ -890,6 +901,17 @@ int nlfh_net_mse(const float* pred, const float* target, float* grad, long long 
                  void* stream) {
   hipLaunchKernelGGL(mse_kernel, dim3(nparts), dim3(256), 0, (hipStream_t)stream, pred, target, grad, (int64_t)len,
                      1.0 / (double)len, loss_part);
+  HIP_TRY(hipGetLastError());
+  return NLFH_OK;
+}
+
+int nlfh_net_input(const double* rho, long long nvox, unsigned short* x, void* stream) {
+  if (!rho || !x || nvox < 1 || (reinterpret_cast<uintptr_t>(x) & 15)) {
+    set_error("nlfh_net_input: invalid arguments (x 16-byte aligned, nvox >= 1)");
+    return NLFH_EINVAL;
+  }
+  hipLaunchKernelGGL(input_kernel, dim3(nb_for(nvox)), dim3(256), 0, (hipStream_t)stream, rho, (int64_t)nvox,
+                     reinterpret_cast<uint4*>(x));
   HIP_TRY(hipGetLastError());
   return NLFH_OK;
 }

@@ -95,12 +95,10 @@ class WarmStartNet:
 
     def input(self, rho):
         """(B, n, n, n) densities -> bf16 channels-last input with channel 0 = 2 rho - 1."""
-        rho = rho.to(self.dev, torch.float32)
+        rho = rho.to(self.dev, torch.float64).contiguous()
         B, n = rho.shape[0], rho.shape[1]
-        x = torch.zeros((B, n, n, n, 16), dtype=torch.float32, device=self.dev)
-        x[..., 0] = 2.0 * rho - 1.0
-        xh = torch.empty(x.shape, dtype=torch.int16, device=self.dev)
-        self._bf(x, xh)
+        xh = torch.empty((B, n, n, n, 16), dtype=torch.int16, device=self.dev)
+        _lib.check(self.lib.nlfh_net_input(_p(rho), rho.numel(), _p(xh), _st()))
         return xh
 
     def _conv(self, name, a, b, n, B, out_f=None, out_h=None):

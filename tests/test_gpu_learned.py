@@ -391,3 +391,18 @@ def test_readout_matches_torch(gpu_pkg, B, n):
     assert lib.nlfh_net_readout(ctypes.c_void_p(out.data_ptr()), L.COUT_HEAD, L.NOUT, n ** 3, B, float(n),
                                 ctypes.c_void_p(short.data_ptr()), short.numel(), ctypes.c_void_p(dst.data_ptr()),
                                 None) != 0
+
+
+def test_input_matches_float32_path(gpu_pkg):
+    """nlfh_net_input (bf16 channels-last input, channel 0 = 2 rho - 1 with rho rounded to float32) against the float32
+    construction in torch converted by nlfh_net_f2bf: bitwise."""
+    lib = gpu_pkg.load_library()
+    st = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rho = torch.rand((2, 8, 8, 8), dtype=torch.float64, device="cuda")
+    x = torch.zeros((2, 8, 8, 8, 16), dtype=torch.float32, device="cuda")
+    x[..., 0] = 2.0 * rho.float() - 1.0
+    ref = torch.empty(x.shape, dtype=torch.int16, device="cuda")
+    assert lib.nlfh_net_f2bf(ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(ref.data_ptr()), x.numel(), st) == 0
+    got = torch.full(x.shape, 7, dtype=torch.int16, device="cuda")
+    assert lib.nlfh_net_input(ctypes.c_void_p(rho.data_ptr()), rho.numel(), ctypes.c_void_p(got.data_ptr()), st) == 0
+    assert torch.equal(got, ref)
