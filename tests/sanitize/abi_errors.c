@@ -66,6 +66,10 @@ int main(void) {
   EXPECT(nlfh_net_conv(NULL, 15, 0, NULL, 0, NULL, NULL, 8, 1, 16, 16, 1, 0, NULL, NULL, NULL) != 0, "conv args");
   EXPECT(nlfh_net_wgrad(NULL, 16, NULL, 15, 0, NULL, 0, 8, 1, NULL, 0, NULL, NULL) != 0, "wgrad args");
   EXPECT(nlfh_net_bias_grad(NULL, NULL, 1, 16, 0, NULL, 0, NULL, NULL) != 0, "bias grad args");
+  EXPECT(nlfh_net_readout(NULL, 32, 18, 8, 1, 4.0, NULL, 0, NULL, NULL) != 0, "readout nulls");
+  EXPECT(nlfh_net_readout((const float*)d9, 16, 18, 8, 1, 4.0, d9, 1, d9, NULL) != 0 &&
+             strstr(nlfh_last_error(), "ldc >= nch"), "readout ldc < nch");
+  EXPECT(nlfh_net_input(NULL, 8, NULL, NULL) != 0, "input nulls");
   if (fails == 0) printf("abi_errors: all checks passed\n");
   return fails == 0 ? 0 : 1;
 }
